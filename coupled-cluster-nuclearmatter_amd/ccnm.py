@@ -1,0 +1,279 @@
+"""Python mirror of the engine's C ABI (include/ccgpu.h) over ctypes.
+
+The names follow the reference's hot-path routines (ccd_iter, t2_intermediate,
+diis_t1t2, ccd_energy_save, setup_t_amplitudes, pert_triples2) so host code and
+tests read like the reference's own driver.  All compute runs in the HIP library
+libccgpu.so next to this file; if that library is missing every call fails with
+an ImportError/RuntimeError: there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libccgpu.so")
+
+PNM, SNM = 0, 1
+INT_MINNESOTA = 0
+BLOCK_HHHH, BLOCK_HHHP, BLOCK_HHPP, BLOCK_HPHP, BLOCK_HPPP, BLOCK_PPPP = 1, 2, 3, 4, 5, 6
+PREC_FP64, PREC_FP32 = 0, 1
+
+
+class LatticeParams(C.Structure):
+    _fields_ = [("A", C.c_int), ("matter", C.c_int), ("rho", C.c_double), ("nmax", C.c_int),
+                ("twist", C.c_double * 3)]
+
+
+class Dims(C.Structure):
+    _fields_ = [("nsp", C.c_int), ("nholes", C.c_int), ("nparticles", C.c_int),
+                ("nchannels", C.c_int), ("nph_channels", C.c_int), ("max_npp", C.c_int),
+                ("max_nhh", C.c_int), ("t2_unique", C.c_longlong), ("t2_ordered", C.c_longlong),
+                ("vpppp_unique", C.c_longlong), ("ring_elems", C.c_longlong),
+                ("box", C.c_double), ("volume", C.c_double), ("kf", C.c_double)]
+
+
+class SolveOpts(C.Structure):
+    _fields_ = [("subspace", C.c_int), ("diis_step", C.c_int), ("tol", C.c_double),
+                ("maxit", C.c_int), ("precision", C.c_int), ("refine_every", C.c_int),
+                ("use_graph", C.c_int), ("verbose", C.c_int)]
+
+
+class SolveResult(C.Structure):
+    _fields_ = [("iterations", C.c_int), ("converged", C.c_int), ("e0", C.c_double * 2),
+                ("embpt2", C.c_double * 2), ("eccd", C.c_double * 2),
+                ("ecorr_last", C.c_double * 2), ("seconds", C.c_double)]
+
+    def as_dict(self):
+        z = lambda a: complex(a[0], a[1])
+        return {"iterations": self.iterations, "converged": bool(self.converged),
+                "e0": z(self.e0), "embpt2": z(self.embpt2), "eccd": z(self.eccd),
+                "ecorr": z(self.ecorr_last), "seconds": self.seconds}
+
+
+# every exported symbol with its argument types (kept in step with include/ccgpu.h)
+_I, _D, _P = C.c_int, C.c_double, C.c_void_p
+_IP, _DP = C.POINTER(C.c_int), C.POINTER(C.c_double)
+SIGNATURES = {
+    "ccgpu_plan": [C.POINTER(LatticeParams), C.POINTER(Dims)],
+    "ccgpu_create": [C.POINTER(LatticeParams), _IP],
+    "ccgpu_create_from_orbitals": [_I, _I, _IP, _IP, _IP, _IP, _IP, _DP, _DP, _DP, _DP, _D, _D, _IP],
+    "ccgpu_destroy": [_I],
+    "ccgpu_get_dims": [_I, C.POINTER(Dims)],
+    "ccgpu_get_orbitals": [_I, _IP, _IP, _IP, _IP, _IP, _DP],
+    "ccgpu_generate_interaction": [_I, _I],
+    "ccgpu_set_block": [_I, _I, _I, _IP, _IP, _I, _IP, _IP, _DP, _I],
+    "ccgpu_set_fock": [_I, _DP, _D, _D],
+    "ccgpu_finalize_setup": [_I],
+    "ccgpu_get_e0": [_I, _DP],
+    "ccgpu_get_fock_diag": [_I, _DP],
+    "ccgpu_mbpt2_start": [_I, _DP],
+    "ccgpu_residual": [_I, _I],
+    "ccgpu_update": [_I, _IP, _IP, _I, _I],
+    "ccgpu_energy": [_I, _DP],
+    "ccgpu_solve": [_I, C.POINTER(SolveOpts), C.POINTER(SolveResult)],
+    "ccgpu_triples": [_I, _DP],
+    "ccgpu_get_t2": [_I, _I, _IP, _IP, _IP, _IP, _DP],
+    "ccgpu_set_t2": [_I, _I, _IP, _IP, _IP, _IP, _DP],
+    "ccgpu_get_t2_unique": [_I, _DP],
+    "ccgpu_set_t2_unique": [_I, _DP],
+    "ccgpu_get_t2_labels": [_I, _IP],
+    "ccgpu_batch_solve": [_I, _IP, C.POINTER(SolveOpts), C.POINTER(SolveResult)],
+    "ccgpu_kernel_stats": [C.c_char_p, _I, _DP, C.POINTER(C.c_longlong), _I],
+    "ccgpu_last_error": [C.c_char_p, _I],
+}
+
+_lib = None
+
+
+def lib():
+    """Load libccgpu.so (built in-tree); raises if it was not built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is missing: run __graft_entry__.build() first")
+        L = C.CDLL(LIB_PATH)
+        for name, args in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.argtypes = args
+            fn.restype = C.c_int
+        _lib = L
+    return _lib
+
+
+def _check(status):
+    if status != 0:
+        buf = C.create_string_buffer(2048)
+        lib().ccgpu_last_error(buf, 2048)
+        raise RuntimeError(buf.value.decode())
+
+
+def _ip(a):
+    return a.ctypes.data_as(_IP)
+
+
+def _dp(a):
+    return a.ctypes.data_as(_DP)
+
+
+def plan(A, matter, rho, nmax, twist=(0.0, 0.0, 0.0)):
+    """Host-only lattice plan (no GPU): dimensions of the channel structure."""
+    p = LatticeParams(A, matter, rho, nmax, (C.c_double * 3)(*twist))
+    d = Dims()
+    _check(lib().ccgpu_plan(C.byref(p), C.byref(d)))
+    return {k: getattr(d, k) for k, _ in Dims._fields_}
+
+
+def default_opts(subspace=6, diis_step=6, tol=1e-6, maxit=100, precision=PREC_FP64,
+                 refine_every=0, use_graph=0, verbose=0):
+    return SolveOpts(subspace, diis_step, tol, maxit, precision, refine_every, use_graph, verbose)
+
+
+class Solver:
+    """One lattice solve on the GPU (ccd_iter and its callees)."""
+
+    def __init__(self, A=None, matter=PNM, rho=0.16, nmax=1, twist=(0.0, 0.0, 0.0), orbitals=None):
+        h = C.c_int(0)
+        if orbitals is None:
+            p = LatticeParams(A, matter, rho, nmax, (C.c_double * 3)(*twist))
+            _check(lib().ccgpu_create(C.byref(p), C.byref(h)))
+        else:
+            o = orbitals
+            arr_i = [np.ascontiguousarray(o[k], np.int32) for k in ("nx", "ny", "nz", "sz", "tz")]
+            arr_d = [np.ascontiguousarray(o[k], np.float64) for k in ("kx", "ky", "kz", "e")]
+            nsp = len(arr_d[3])
+            _check(lib().ccgpu_create_from_orbitals(nsp, int(o["A"]), *[_ip(a) for a in arr_i],
+                                                    *[_dp(a) for a in arr_d], float(o["box"]),
+                                                    float(o["volume"]), C.byref(h)))
+        self.h = h.value
+        self.dims = self.get_dims()
+
+    def close(self):
+        if self.h:
+            lib().ccgpu_destroy(self.h)
+            self.h = 0
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def get_dims(self):
+        d = Dims()
+        _check(lib().ccgpu_get_dims(self.h, C.byref(d)))
+        return {k: getattr(d, k) for k, _ in Dims._fields_}
+
+    def orbitals(self):
+        n = self.dims["nsp"]
+        out = {k: np.zeros(n, np.int32) for k in ("nx", "ny", "nz", "sz", "tz")}
+        e = np.zeros(n)
+        _check(lib().ccgpu_get_orbitals(self.h, *[_ip(out[k]) for k in ("nx", "ny", "nz", "sz", "tz")], _dp(e)))
+        out["e"] = e
+        return out
+
+    # setup
+    def generate_interaction(self, kind=INT_MINNESOTA):
+        _check(lib().ccgpu_generate_interaction(self.h, kind))
+
+    def set_block(self, kind, bra_p, bra_q, ket_r, ket_s, val):
+        """Reference-layout block: 1-based pair lists, val[nbra, nket] complex."""
+        bp, bq = np.ascontiguousarray(bra_p, np.int32), np.ascontiguousarray(bra_q, np.int32)
+        kr, ks = np.ascontiguousarray(ket_r, np.int32), np.ascontiguousarray(ket_s, np.int32)
+        v = np.asfortranarray(np.asarray(val, np.complex128))
+        _check(lib().ccgpu_set_block(self.h, kind, len(bp), _ip(bp), _ip(bq), len(kr), _ip(kr), _ip(ks),
+                                     v.ctypes.data_as(_DP), len(bp)))
+
+    def set_fock(self, fock, e0):
+        f = np.asfortranarray(np.asarray(fock, np.complex128))
+        _check(lib().ccgpu_set_fock(self.h, f.ctypes.data_as(_DP), float(np.real(e0)), float(np.imag(e0))))
+
+    def finalize_setup(self):
+        _check(lib().ccgpu_finalize_setup(self.h))
+
+    def e0(self):
+        x = np.zeros(2)
+        _check(lib().ccgpu_get_e0(self.h, _dp(x)))
+        return complex(x[0], x[1])
+
+    def fock_diag(self):
+        x = np.zeros(2 * self.dims["nsp"])
+        _check(lib().ccgpu_get_fock_diag(self.h, _dp(x)))
+        return x[0::2] + 1j * x[1::2]
+
+    # reference routine mirrors
+    def setup_t_amplitudes(self):
+        """MBPT2 start; returns the MBPT2 correlation energy."""
+        x = np.zeros(2)
+        _check(lib().ccgpu_mbpt2_start(self.h, _dp(x)))
+        return complex(x[0], x[1])
+
+    def t2_intermediate(self, precision=PREC_FP64):
+        _check(lib().ccgpu_residual(self.h, precision))
+
+    def diis_t1t2(self, count, nstep, subspace=6, diis_step=6):
+        c, n = C.c_int(count), C.c_int(nstep)
+        _check(lib().ccgpu_update(self.h, C.byref(c), C.byref(n), subspace, diis_step))
+        return c.value, n.value
+
+    def ccd_energy_save(self):
+        x = np.zeros(2)
+        _check(lib().ccgpu_energy(self.h, _dp(x)))
+        return complex(x[0], x[1])
+
+    def ccd_iter(self, **kw):
+        o = default_opts(**kw)
+        r = SolveResult()
+        _check(lib().ccgpu_solve(self.h, C.byref(o), C.byref(r)))
+        return r.as_dict()
+
+    def pert_triples2(self):
+        x = np.zeros(2)
+        _check(lib().ccgpu_triples(self.h, _dp(x)))
+        return complex(x[0], x[1])
+
+    # amplitudes
+    def get_t2(self, a, b, i, j):
+        a, b, i, j = (np.ascontiguousarray(x, np.int32) for x in (a, b, i, j))
+        out = np.zeros(2 * len(a))
+        _check(lib().ccgpu_get_t2(self.h, len(a), _ip(a), _ip(b), _ip(i), _ip(j), _dp(out)))
+        return out[0::2] + 1j * out[1::2]
+
+    def set_t2(self, a, b, i, j, t):
+        a, b, i, j = (np.ascontiguousarray(x, np.int32) for x in (a, b, i, j))
+        tt = np.ascontiguousarray(np.asarray(t, np.complex128))
+        _check(lib().ccgpu_set_t2(self.h, len(a), _ip(a), _ip(b), _ip(i), _ip(j), tt.ctypes.data_as(_DP)))
+
+    def t2_unique(self):
+        out = np.zeros(self.dims["t2_unique"], np.complex128)
+        _check(lib().ccgpu_get_t2_unique(self.h, out.ctypes.data_as(_DP)))
+        return out
+
+    def set_t2_unique(self, t):
+        tt = np.ascontiguousarray(np.asarray(t, np.complex128))
+        assert tt.size == self.dims["t2_unique"]
+        _check(lib().ccgpu_set_t2_unique(self.h, tt.ctypes.data_as(_DP)))
+
+    def t2_labels(self):
+        out = np.zeros((self.dims["t2_unique"], 4), np.int32)
+        _check(lib().ccgpu_get_t2_labels(self.h, _ip(out)))
+        return out
+
+
+def batch_solve(solvers, **kw):
+    o = default_opts(**kw)
+    hs = np.array([s.h for s in solvers], np.int32)
+    res = (SolveResult * len(solvers))()
+    _check(lib().ccgpu_batch_solve(len(solvers), _ip(hs), C.byref(o), res))
+    return [r.as_dict() for r in res]
+
+
+def kernel_stats():
+    names = C.create_string_buffer(4096)
+    avg = np.zeros(64)
+    cnt = np.zeros(64, np.int64)
+    k = lib().ccgpu_kernel_stats(names, 4096, _dp(avg), cnt.ctypes.data_as(C.POINTER(C.c_longlong)), 64)
+    nm = names.value.decode().split(";") if k else []
+    return {nm[x]: (float(avg[x]), int(cnt[x])) for x in range(k)}

@@ -1,0 +1,1176 @@
+// C-ABI of the CCD/(T) engine: host runtime (device arenas, work lists, the
+// iteration driver, DIIS solve, batching, hipGraph replay, timing).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <complex>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/ccgpu.h"
+#include "device.h"
+#include "plan.h"
+
+namespace ccgpu {
+// kernels.hip
+void launch_fill_blocks(const SolveDesc &d, double volume, int nch, int nphc, int max_npp, int max_nhh,
+                        int max_nhp, hipStream_t st);
+void launch_derive(const SolveDesc &d, int nch, int nphc, int max_npp, int max_nhh, int max_nph, int max_nhp,
+                   hipStream_t st);
+void launch_to_f32(const double2 *src, float2 *dst, int64_t n, hipStream_t st);
+void launch_mbpt2(const SolveDesc *dd, int nsolve, int64_t maxT, hipStream_t st);
+void launch_energy(const SolveDesc *dd, int nsolve, int slot, hipStream_t st);
+void launch_i7(const SolveDesc *dd, int nsolve, int maxA, hipStream_t st);
+void launch_i6(const SolveDesc *dd, int nsolve, int maxnp, hipStream_t st);
+void launch_i9(const SolveDesc *dd, int nsolve, int64_t maxvhh, hipStream_t st);
+void launch_resid_init(const SolveDesc *dd, int nsolve, int64_t maxT, hipStream_t st);
+void launch_ring_gather(const SolveDesc *dd, int nsolve, int64_t maxT, hipStream_t st);
+void launch_ring_scatter(const SolveDesc *dd, int nsolve, int64_t maxT, hipStream_t st);
+void launch_update(const SolveDesc *dd, int nsolve, int64_t maxT, const int *slot, hipStream_t st);
+void launch_gram(const SolveDesc *dd, int nsolve, const int *slots, int nstep, hipStream_t st);
+void launch_diis_axpy(const SolveDesc *dd, int nsolve, int64_t maxT, const int *slots, const double2 *coef,
+                      int nvec, hipStream_t st);
+void launch_gemm(const GemmGroup *groups, const GemmTile *tiles, int ntiles, const SolveDesc *dd,
+                 const int *gsolve, int variant, hipStream_t st);
+}  // namespace ccgpu
+
+using namespace ccgpu;
+
+namespace {
+
+thread_local std::string g_err;
+
+struct Error : std::runtime_error {
+  explicit Error(const std::string &m) : std::runtime_error(m) {}
+};
+
+std::string fmt(const char *f, ...) {
+  char buf[1024];
+  va_list ap;
+  va_start(ap, f);
+  vsnprintf(buf, sizeof buf, f, ap);
+  va_end(ap);
+  return buf;
+}
+
+#define HIPCHK(x)                                                                         \
+  do {                                                                                    \
+    hipError_t e_ = (x);                                                                  \
+    if (e_ != hipSuccess) throw Error(fmt("%s failed: %s", #x, hipGetErrorString(e_))); \
+  } while (0)
+
+template <typename T>
+T *dalloc(size_t n, std::vector<void *> &owned) {
+  void *p = nullptr;
+  if (n == 0) n = 1;
+  hipError_t e = hipMalloc(&p, n * sizeof(T));
+  if (e != hipSuccess)
+    throw Error(fmt("hipMalloc of %zu bytes failed: %s", n * sizeof(T), hipGetErrorString(e)));
+  owned.push_back(p);
+  return (T *)p;
+}
+
+template <typename T>
+T *dupload(const std::vector<T> &v, std::vector<void *> &owned) {
+  T *p = dalloc<T>(v.size(), owned);
+  if (!v.empty()) HIPCHK(hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+  return p;
+}
+
+struct GemmList {
+  std::vector<GemmGroup> groups;
+  std::vector<GemmTile> tiles;
+  std::vector<int> gsolve;
+  GemmGroup *d_groups = nullptr;
+  GemmTile *d_tiles = nullptr;
+  int *d_gsolve = nullptr;
+  double flops = 0;       // algorithmic real flops per launch
+  double bytes = 0;       // algorithmic bytes of the A operand per launch
+  void add(const GemmGroup &g, int solve, int bm = 64, int bn = 64) {
+    if (g.M <= 0 || g.N <= 0 || g.K <= 0) return;
+    int gi = (int)groups.size();
+    groups.push_back(g);
+    gsolve.push_back(solve);
+    for (int m = 0; m < g.M; m += bm)
+      for (int n = 0; n < g.N; n += bn) tiles.push_back(GemmTile{gi, m, n, 0});
+    flops += 8.0 * g.M * (double)g.N * g.K;
+  }
+  void upload(std::vector<void *> &owned) {
+    d_groups = dupload(groups, owned);
+    d_tiles = dupload(tiles, owned);
+    d_gsolve = dupload(gsolve, owned);
+  }
+};
+
+// XCD-aware reorder: tiles of one group stay on one of the 8 XCDs (blocks are
+// dealt round-robin to XCDs), so the shared operand of a channel is read from
+// one L2.  Groups are packed onto XCD bins by largest-first.
+void xcd_order(GemmList &L) {
+  const int nx = 8;
+  std::vector<std::vector<GemmTile>> bins(nx);
+  std::vector<double> load(nx, 0.0);
+  std::vector<std::vector<GemmTile>> per(L.groups.size());
+  for (auto &t : L.tiles) per[t.g].push_back(t);
+  std::vector<int> order(L.groups.size());
+  for (size_t i = 0; i < order.size(); ++i) order[i] = (int)i;
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return per[a].size() > per[b].size(); });
+  for (int g : order) {
+    int best = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+    for (auto &t : per[g]) bins[best].push_back(t);
+    load[best] += (double)per[g].size();
+  }
+  std::vector<GemmTile> out;
+  size_t maxlen = 0;
+  for (auto &b : bins) maxlen = std::max(maxlen, b.size());
+  for (size_t r = 0; r < maxlen; ++r)
+    for (int x = 0; x < nx; ++x)
+      if (r < bins[x].size()) out.push_back(bins[x][r]);
+  L.tiles.swap(out);
+}
+
+struct Solve {
+  Plan plan;
+  std::vector<void *> owned;
+  SolveDesc desc{};
+  SolveDesc *d_desc = nullptr;
+  hipStream_t stream = nullptr;
+  bool have_v = false, have_fock = false, derived = false, have_t = false;
+  double2 e0{0, 0};
+  // hhphp records of the drop-in path: key (i,b,j,a) -> <ib||ja>
+  std::unordered_map<uint64_t, std::complex<double>> hphp;
+  // DIIS state
+  int depth = 0;
+  std::vector<int> ring;  // logical column -> physical slot
+  int nstep = 0, count = 0;
+  int *d_slot = nullptr;   // [1] physical slot for k_update
+  int *d_slots = nullptr;  // [16] physical slots in logical order
+  double2 *d_coef = nullptr;
+  // work lists for the five GEMM families
+  GemmList g_x, g_pp, g_hh, g_i4, g_rt;
+  bool vpppp32 = false;
+  double2 *scal = nullptr;
+  ~Solve() {
+    for (void *p : owned) (void)hipFree(p);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+};
+
+std::mutex g_mu;
+std::map<int, std::unique_ptr<Solve>> g_solves;
+int g_next = 1;
+
+Solve &get(int h) {
+  auto it = g_solves.find(h);
+  if (it == g_solves.end()) throw Error(fmt("invalid handle %d", h));
+  return *it->second;
+}
+
+int gemm_variant_override() {
+  const char *e = getenv("CCGPU_GEMM");
+  if (e && std::string(e) == "valu") return 3;
+  return -1;
+}
+
+// ----- timing of kernel families -----
+struct Stat {
+  double ms = 0;
+  long long n = 0;
+};
+std::map<std::string, Stat> g_stats;
+std::vector<std::string> g_stat_order;
+
+struct Timer {
+  bool on;
+  hipStream_t st;
+  hipEvent_t a{}, b{};
+  std::string name;
+  Timer(bool on_, hipStream_t s, const char *nm) : on(on_), st(s), name(nm) {
+    if (on) {
+      (void)hipEventCreate(&a);
+      (void)hipEventCreate(&b);
+      (void)hipEventRecord(a, st);
+    }
+  }
+  ~Timer() {
+    if (!on) return;
+    (void)hipEventRecord(b, st);
+    (void)hipEventSynchronize(b);
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, a, b);
+    if (!g_stats.count(name)) g_stat_order.push_back(name);
+    g_stats[name].ms += ms;
+    g_stats[name].n += 1;
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+  }
+};
+
+void build_device(Solve &S) {
+  Plan &P = S.plan;
+  auto &o = S.owned;
+  SolveDesc &d = S.desc;
+  HIPCHK(hipStreamCreateWithFlags(&S.stream, hipStreamNonBlocking));
+  d.A = P.A; d.nsp = P.nsp; d.np = P.np; d.nch = P.nch; d.nphc = P.nphc;
+  d.nT = P.nT; d.nRt = P.nRt; d.nI4 = P.nI4; d.active = 1;
+  d.kx = dupload(P.orb.kx, o); d.ky = dupload(P.orb.ky, o); d.kz = dupload(P.orb.kz, o);
+  d.e = dupload(P.orb.e, o);
+  d.nx = dupload(P.orb.nx, o); d.ny = dupload(P.orb.ny, o); d.nz = dupload(P.orb.nz, o);
+  d.sz = dupload(P.orb.sz, o); d.tz = dupload(P.orb.tz, o);
+  d.partner = dupload(P.partner, o);
+  d.t_ch = dupload(P.t_ch, o);
+  d.npp = dupload(P.npp, o); d.nhh = dupload(P.nhh, o);
+  d.off_t = dupload(P.off_t, o); d.off_pp = dupload(P.off_pp, o); d.off_hh = dupload(P.off_hh, o);
+  d.off_vpp = dupload(P.off_vpp, o); d.off_vhh = dupload(P.off_vhh, o);
+  d.pp_a = dupload(P.pp_a, o); d.pp_b = dupload(P.pp_b, o);
+  d.hh_i = dupload(P.hh_i, o); d.hh_j = dupload(P.hh_j, o);
+  d.ob_rowB = dupload(P.ob_rowB, o); d.ob_rowA = dupload(P.ob_rowA, o);
+  d.ob_colJ = dupload(P.ob_colJ, o); d.ob_colI = dupload(P.ob_colI, o);
+  std::vector<DPair> pr(P.ppref.size()), hr(P.hhref.size());
+  for (size_t x = 0; x < pr.size(); ++x)
+    pr[x] = DPair{P.ppref[x].sign ? P.ppref[x].ch : -1, P.ppref[x].idx * 2 + (P.ppref[x].sign < 0)};
+  for (size_t x = 0; x < hr.size(); ++x)
+    hr[x] = DPair{P.hhref[x].sign ? P.hhref[x].ch : -1, P.hhref[x].idx * 2 + (P.hhref[x].sign < 0)};
+  d.ppref = dupload(pr, o);
+  d.hhref = dupload(hr, o);
+  d.nph = dupload(P.nph, o); d.nhp = dupload(P.nhp, o);
+  d.off_ph = dupload(P.off_ph, o); d.off_hp = dupload(P.off_hp, o);
+  d.off_rt = dupload(P.off_rt, o); d.off_i4 = dupload(P.off_i4, o); d.off_vt = dupload(P.off_vt, o);
+  d.ph_a = dupload(P.ph_a, o); d.ph_i = dupload(P.ph_i, o);
+  d.hp_j = dupload(P.hp_j, o); d.hp_b = dupload(P.hp_b, o);
+  d.ring4 = dupload(P.ring4, o);
+  d.T = dalloc<double2>(P.nT, o);
+  d.R = dalloc<double2>(P.nT, o);
+  d.R0 = dalloc<double2>(P.nT, o);
+  d.Vhhpp = dalloc<double2>(P.nT, o);
+  d.Vhhhh = dalloc<double2>(P.nVhh, o);
+  d.X = dalloc<double2>(2 * P.nVhh, o);
+  d.Vpppp = dalloc<double2>(P.nVpp, o);
+  d.Vpppp32 = nullptr;
+  d.Tt = dalloc<double2>(P.nRt, o);
+  d.Rt = dalloc<double2>(P.nRt, o);
+  d.Vt = dalloc<double2>(P.nRt, o);
+  d.I4 = dalloc<double2>(P.nI4, o);
+  d.W0 = dalloc<double2>(P.nI4, o);
+  d.fdiag = dalloc<double2>(P.nsp, o);
+  d.fpart = dalloc<double2>(P.nsp, o);
+  d.dh = dalloc<double2>(P.A, o);
+  d.I7off = dalloc<double2>(P.A, o);
+  d.dp = dalloc<double2>(P.np, o);
+  d.I6off = dalloc<double2>(P.np, o);
+  d.e0 = dalloc<double2>(1, o);
+  d.red = dalloc<double2>(512 * 36, o);
+  d.scal = dalloc<double2>(64, o);
+  d.hist = nullptr;
+  S.scal = d.scal;
+  HIPCHK(hipMemset(d.Tt, 0, P.nRt * sizeof(double2)));  // entries with a=b or i=j stay zero
+  HIPCHK(hipMemset(d.T, 0, P.nT * sizeof(double2)));
+  HIPCHK(hipMemset(d.Vpppp, 0, P.nVpp * sizeof(double2)));
+  HIPCHK(hipMemset(d.Vhhpp, 0, P.nT * sizeof(double2)));
+  HIPCHK(hipMemset(d.Vhhhh, 0, P.nVhh * sizeof(double2)));
+  HIPCHK(hipMemset(d.W0, 0, P.nI4 * sizeof(double2)));
+  S.d_desc = dalloc<SolveDesc>(1, o);
+  S.d_slot = dalloc<int>(1, o);
+  S.d_slots = dalloc<int>(16, o);
+  S.d_coef = dalloc<double2>(16, o);
+  HIPCHK(hipMemcpy(S.d_desc, &d, sizeof(SolveDesc), hipMemcpyHostToDevice));
+
+  // GEMM work lists
+  for (int c = 0; c < P.nch; ++c) {
+    int np_ = P.npp[c], nh = P.nhh[c];
+    GemmGroup g{};
+    // X = Vhhpp T
+    g = GemmGroup{d.Vhhpp + P.off_t[c], d.T + P.off_t[c], d.X + P.off_vhh[c], nullptr, nh, nh, np_, np_, nh, nh, 0, 1.0, 0.0};
+    S.g_x.add(g, 0);
+    // R += Vpppp T   (pp ladder, ccm_t2_eqn.f90:574-607)
+    g = GemmGroup{d.Vpppp + P.off_vpp[c], d.T + P.off_t[c], d.R + P.off_t[c], d.R + P.off_t[c], np_, nh, np_, np_, nh, nh, nh, 1.0, 1.0};
+    S.g_pp.add(g, 0);
+    // R += T I9      (hh ladder, ccm_t2_eqn.f90:614-647)
+    g = GemmGroup{d.T + P.off_t[c], d.X + P.nVhh + P.off_vhh[c], d.R + P.off_t[c], d.R + P.off_t[c], np_, nh, nh, nh, nh, nh, nh, 1.0, 1.0};
+    S.g_hh.add(g, 0);
+  }
+  for (int K = 0; K < P.nphc; ++K) {
+    int nph = P.nph[K], nhp = P.nhp[K];
+    // I4 = W0 + 1/2 Vt Tt (ccm_t2_eqn.f90:657-780)
+    GemmGroup g{d.Vt + P.off_vt[K], d.Tt + P.off_rt[K], d.I4 + P.off_i4[K], d.W0 + P.off_i4[K], nhp, nhp, nph, nph, nhp, nhp, nhp, 0.5, 1.0};
+    S.g_i4.add(g, 0);
+    // Rt = Tt I4 (ccm_t2_eqn.f90:995-1026)
+    g = GemmGroup{d.Tt + P.off_rt[K], d.I4 + P.off_i4[K], d.Rt + P.off_rt[K], nullptr, nph, nhp, nhp, nhp, nhp, nhp, 0, 1.0, 0.0};
+    S.g_rt.add(g, 0);
+  }
+  xcd_order(S.g_pp);
+  for (GemmList *L : {&S.g_x, &S.g_pp, &S.g_hh, &S.g_i4, &S.g_rt}) L->upload(o);
+  S.g_pp.bytes = 16.0 * (double)P.nVpp;
+}
+
+void alloc_history(Solve &S, int depth) {
+  if (depth < 2 || depth > 9) throw Error(fmt("DIIS subspace %d outside [2, 9]", depth));
+  if (S.depth >= depth) return;
+  // history has subspace columns (diis_setup, ccm_diis.f90:284-288)
+  S.desc.hist = dalloc<double2>((size_t)depth * S.plan.nT, S.owned);
+  S.depth = depth;
+  HIPCHK(hipMemcpy(S.d_desc, &S.desc, sizeof(SolveDesc), hipMemcpyHostToDevice));
+}
+
+void ensure_ready(Solve &S) {
+  if (!S.have_v) throw Error("no interaction: call ccgpu_generate_interaction or ccgpu_set_block first");
+  if (!S.derived) throw Error("setup not finalized: call ccgpu_finalize_setup");
+}
+
+// ---------- host complex linear solve (Gaussian elimination, partial pivoting
+// by |re|+|im| as the reference's ZGESV does) ----------
+std::vector<std::complex<double>> zsolve(std::vector<std::complex<double>> a, std::vector<std::complex<double>> b,
+                                         int n, int *info) {
+  *info = 0;
+  for (int k = 0; k < n; ++k) {
+    int piv = k;
+    double best = -1;
+    for (int i = k; i < n; ++i) {
+      double v = std::fabs(a[i * n + k].real()) + std::fabs(a[i * n + k].imag());
+      if (v > best) { best = v; piv = i; }
+    }
+    if (best == 0.0) { *info = k + 1; return b; }
+    if (piv != k) {
+      for (int j = 0; j < n; ++j) std::swap(a[k * n + j], a[piv * n + j]);
+      std::swap(b[k], b[piv]);
+    }
+    for (int i = k + 1; i < n; ++i) {
+      std::complex<double> f = a[i * n + k] / a[k * n + k];
+      a[i * n + k] = f;
+      for (int j = k + 1; j < n; ++j) a[i * n + j] -= f * a[k * n + j];
+      b[i] -= f * b[k];
+    }
+  }
+  for (int i = n - 1; i >= 0; --i) {
+    std::complex<double> s = b[i];
+    for (int j = i + 1; j < n; ++j) s -= a[i * n + j] * b[j];
+    b[i] = s / a[i * n + i];
+  }
+  return b;
+}
+
+// ---------- one residual for a set of solves sharing work lists ----------
+struct Batch {
+  std::vector<Solve *> solves;
+  SolveDesc *d_descs = nullptr;  // device array
+  std::vector<SolveDesc> h_descs;
+  GemmList g_x, g_pp, g_hh, g_i4, g_rt, g_pp32;
+  int64_t maxT = 0, maxVhh = 0;
+  int maxA = 0, maxNp = 0;
+  int *d_slot = nullptr, *d_slots = nullptr;
+  double2 *d_coef = nullptr;
+  double2 *d_scal = nullptr;  // nsolve * 64
+  std::vector<int> h_slot, h_slots;       // host staging of control arrays
+  std::vector<double2> h_coef, h_scal;
+  std::vector<void *> owned;
+  hipStream_t stream = nullptr;
+  hipGraph_t graph[2] = {nullptr, nullptr};
+  hipGraphExec_t gexec[2] = {nullptr, nullptr};
+  ~Batch() {
+    for (int q = 0; q < 2; ++q) {
+      if (gexec[q]) (void)hipGraphExecDestroy(gexec[q]);
+      if (graph[q]) (void)hipGraphDestroy(graph[q]);
+    }
+    for (void *p : owned) (void)hipFree(p);
+  }
+};
+
+void merge(GemmList &dst, const GemmList &src, int solve) {
+  int base = (int)dst.groups.size();
+  for (size_t i = 0; i < src.groups.size(); ++i) {
+    dst.groups.push_back(src.groups[i]);
+    dst.gsolve.push_back(solve);
+  }
+  for (auto t : src.tiles) {
+    t.g += base;
+    dst.tiles.push_back(t);
+  }
+  dst.flops += src.flops;
+  dst.bytes += src.bytes;
+}
+
+std::unique_ptr<Batch> make_batch(const std::vector<Solve *> &ss, hipStream_t st) {
+  auto B = std::make_unique<Batch>();
+  B->solves = ss;
+  B->stream = st;
+  int n = (int)ss.size();
+  B->d_scal = dalloc<double2>((size_t)n * 64, B->owned);
+  for (int s = 0; s < n; ++s) {
+    Solve &S = *ss[s];
+    SolveDesc d = S.desc;
+    d.active = 1;
+    d.scal = B->d_scal + (size_t)s * 64;
+    B->h_descs.push_back(d);
+    merge(B->g_x, S.g_x, s);
+    merge(B->g_pp, S.g_pp, s);
+    merge(B->g_hh, S.g_hh, s);
+    merge(B->g_i4, S.g_i4, s);
+    merge(B->g_rt, S.g_rt, s);
+    B->maxT = std::max(B->maxT, S.plan.nT);
+    B->maxVhh = std::max(B->maxVhh, S.plan.nVhh);
+    B->maxA = std::max(B->maxA, S.plan.A);
+    B->maxNp = std::max(B->maxNp, S.plan.np);
+  }
+  // FP32 pp ladder list reads the float copy of Vpppp
+  B->g_pp32 = B->g_pp;
+  for (size_t g = 0; g < B->g_pp32.groups.size(); ++g) {
+    int s = B->g_pp32.gsolve[g];
+    Solve &S = *ss[s];
+    const double2 *base = S.desc.Vpppp;
+    const double2 *a = (const double2 *)B->g_pp32.groups[g].A;
+    if (S.desc.Vpppp32) B->g_pp32.groups[g].A = S.desc.Vpppp32 + (a - base);
+  }
+  if (n > 1) {
+    xcd_order(B->g_pp);
+    xcd_order(B->g_pp32);
+  }
+  for (GemmList *L : {&B->g_x, &B->g_pp, &B->g_hh, &B->g_i4, &B->g_rt, &B->g_pp32}) L->upload(B->owned);
+  B->d_descs = dupload(B->h_descs, B->owned);
+  B->d_slot = dalloc<int>(n, B->owned);
+  B->d_slots = dalloc<int>((size_t)n * 16, B->owned);
+  B->d_coef = dalloc<double2>((size_t)n * 16, B->owned);
+  return B;
+}
+
+void set_active(Batch &B, int s, int active) {
+  B.h_descs[s].active = active;
+  HIPCHK(hipMemcpyAsync(&B.d_descs[s].active, &B.h_descs[s].active, sizeof(int), hipMemcpyHostToDevice, B.stream));
+  HIPCHK(hipStreamSynchronize(B.stream));
+}
+
+void residual_launches(Batch &B, int prec, bool timing) {
+  const int n = (int)B.solves.size();
+  hipStream_t st = B.stream;
+  int var64 = gemm_variant_override() >= 0 ? 3 : 0;
+  int var = prec == CCGPU_PREC_FP32 ? 1 : var64;
+  int varpp = prec == CCGPU_PREC_FP32 ? 2 : var64;
+  {
+    Timer t(timing, st, "gemm_x_vhhpp_t");
+    launch_gemm(B.g_x.d_groups, B.g_x.d_tiles, (int)B.g_x.tiles.size(), B.d_descs, B.g_x.d_gsolve, var, st);
+  }
+  {
+    Timer t(timing, st, "i7_i9_i6");
+    launch_i7(B.d_descs, n, B.maxA, st);
+    launch_i9(B.d_descs, n, B.maxVhh, st);
+    launch_i6(B.d_descs, n, B.maxNp, st);
+  }
+  {
+    Timer t(timing, st, "resid_init_onebody");
+    launch_resid_init(B.d_descs, n, B.maxT, st);
+  }
+  {
+    Timer t(timing, st, "gemm_pp_ladder");
+    GemmList &L = (prec == CCGPU_PREC_FP32) ? B.g_pp32 : B.g_pp;
+    launch_gemm(L.d_groups, L.d_tiles, (int)L.tiles.size(), B.d_descs, L.d_gsolve, varpp, st);
+  }
+  {
+    Timer t(timing, st, "gemm_hh_ladder");
+    launch_gemm(B.g_hh.d_groups, B.g_hh.d_tiles, (int)B.g_hh.tiles.size(), B.d_descs, B.g_hh.d_gsolve, var, st);
+  }
+  {
+    Timer t(timing, st, "ring_gather");
+    launch_ring_gather(B.d_descs, n, B.maxT, st);
+  }
+  {
+    Timer t(timing, st, "gemm_ring_i4");
+    launch_gemm(B.g_i4.d_groups, B.g_i4.d_tiles, (int)B.g_i4.tiles.size(), B.d_descs, B.g_i4.d_gsolve, var, st);
+  }
+  {
+    Timer t(timing, st, "gemm_ring_rt");
+    launch_gemm(B.g_rt.d_groups, B.g_rt.d_tiles, (int)B.g_rt.tiles.size(), B.d_descs, B.g_rt.d_gsolve, var, st);
+  }
+  {
+    Timer t(timing, st, "ring_scatter");
+    launch_ring_scatter(B.d_descs, n, B.maxT, st);
+  }
+}
+
+void check_launch() {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) throw Error(fmt("kernel launch failed: %s", hipGetErrorString(e)));
+}
+
+// Update of every active solve (diis_t1t2, ccm_diis.f90:74-249); all active solves
+// of a batch share count/nstep because they start together.
+void update_launches(Batch &B, int &count, int &nstep, int subspace, int diis_step, bool timing) {
+  const int n = (int)B.solves.size();
+  hipStream_t st = B.stream;
+  // history column handling: shift when full (ring buffer)
+  std::vector<int> &slot = B.h_slot, &slots = B.h_slots;
+  slot.assign(n, 0);
+  slots.assign((size_t)n * 16, 0);
+  for (int s = 0; s < n; ++s) {
+    Solve &S = *B.solves[s];
+    if (nstep > subspace) {
+      // drop the oldest column
+      std::rotate(S.ring.begin(), S.ring.begin() + 1, S.ring.begin() + subspace);
+    }
+    slot[s] = S.ring[std::min(nstep, subspace) - 1];
+  }
+  if (nstep > subspace) nstep = subspace;
+  HIPCHK(hipMemcpyAsync(B.d_slot, slot.data(), n * sizeof(int), hipMemcpyHostToDevice, st));
+  {
+    Timer t(timing, st, "update");
+    launch_update(B.d_descs, n, B.maxT, B.d_slot, st);
+  }
+  if (count != diis_step) return;
+  count = 0;
+  if (nstep < 2) return;
+  for (int s = 0; s < n; ++s)
+    for (int k = 0; k < nstep; ++k) slots[(size_t)s * 16 + k] = B.solves[s]->ring[k];
+  HIPCHK(hipMemcpyAsync(B.d_slots, slots.data(), slots.size() * sizeof(int), hipMemcpyHostToDevice, st));
+  std::vector<double2> &sc = B.h_scal;
+  sc.assign((size_t)n * 64, double2{0, 0});
+  {
+    Timer t(timing, st, "diis_gram");
+    launch_gram(B.d_descs, n, B.d_slots, nstep, st);
+    check_launch();
+    HIPCHK(hipMemcpyAsync(sc.data(), B.d_scal, sc.size() * sizeof(double2), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+  }
+  std::vector<double2> &coef = B.h_coef;
+  coef.assign((size_t)n * 16, double2{0, 0});
+  const int ne = nstep - 1;
+  for (int s = 0; s < n; ++s) {
+    if (!B.h_descs[s].active) continue;
+    std::vector<std::complex<double>> M((size_t)nstep * nstep, 0.0), rhs(nstep, 0.0);
+    int q = 0;
+    for (int p = 0; p < ne; ++p)
+      for (int p2 = p; p2 < ne; ++p2) {
+        double2 v = sc[(size_t)s * 64 + 4 + q++];
+        M[p * nstep + p2] = M[p2 * nstep + p] = std::complex<double>(v.x, v.y);
+      }
+    for (int p = 0; p < ne; ++p) M[p * nstep + ne] = M[ne * nstep + p] = -1.0;
+    M[ne * nstep + ne] = 0.0;
+    rhs[ne] = -1.0;
+    int info = 0;
+    auto sol = zsolve(M, rhs, nstep, &info);
+    if (info != 0) throw Error(fmt("DIIS matrix singular (info %d) in solve %d", info, s));
+    for (int p = 0; p < ne; ++p) coef[(size_t)s * 16 + p] = double2{sol[p].real(), sol[p].imag()};
+  }
+  HIPCHK(hipMemcpyAsync(B.d_coef, coef.data(), coef.size() * sizeof(double2), hipMemcpyHostToDevice, st));
+  {
+    Timer t(timing, st, "diis_axpy");
+    launch_diis_axpy(B.d_descs, n, B.maxT, B.d_slots, B.d_coef, ne, st);
+  }
+}
+
+std::vector<std::complex<double>> energies(Batch &B, bool timing) {
+  const int n = (int)B.solves.size();
+  {
+    Timer t(timing, B.stream, "energy");
+    launch_energy(B.d_descs, n, 0, B.stream);
+  }
+  check_launch();
+  std::vector<double2> &sc = B.h_scal;
+  sc.assign((size_t)n * 64, double2{0, 0});
+  HIPCHK(hipMemcpyAsync(sc.data(), B.d_scal, sc.size() * sizeof(double2), hipMemcpyDeviceToHost, B.stream));
+  HIPCHK(hipStreamSynchronize(B.stream));
+  std::vector<std::complex<double>> e(n);
+  for (int s = 0; s < n; ++s) e[s] = std::complex<double>(sc[(size_t)s * 64].x, sc[(size_t)s * 64].y);
+  return e;
+}
+
+// The ccd_iter loop (ccm_iter.f90:60-135) over a batch of solves.
+void run_batch(Batch &B, const ccgpu_solve_opts &op, std::vector<ccgpu_solve_result> &res) {
+  const int n = (int)B.solves.size();
+  const bool timing = getenv("CCGPU_TIMING") && atoi(getenv("CCGPU_TIMING")) != 0;
+  if (timing) { g_stats.clear(); g_stat_order.clear(); }
+  for (Solve *S : B.solves) {
+    alloc_history(*S, op.subspace);
+    S->ring.resize(op.subspace);
+    for (int k = 0; k < op.subspace; ++k) S->ring[k] = k;
+  }
+  // descs may have changed (history pointer)
+  for (int s = 0; s < n; ++s) {
+    B.h_descs[s].hist = B.solves[s]->desc.hist;
+    B.h_descs[s].active = 1;
+  }
+  HIPCHK(hipMemcpy(B.d_descs, B.h_descs.data(), n * sizeof(SolveDesc), hipMemcpyHostToDevice));
+  if (op.precision == CCGPU_PREC_FP32) {
+    for (Solve *S : B.solves)
+      if (!S->desc.Vpppp32) throw Error("FP32 ladder requested before the FP32 copy of Vpppp was made");
+  }
+  auto e1 = energies(B, false);
+  res.assign(n, ccgpu_solve_result{});
+  std::vector<std::complex<double>> dener(n, 4.0), elast = e1;
+  std::vector<int> iters(n, 0), conv(n, 0);
+  for (int s = 0; s < n; ++s) {
+    Solve &S = *B.solves[s];
+    res[s].e0[0] = S.e0.x; res[s].e0[1] = S.e0.y;
+    res[s].embpt2[0] = S.e0.x + e1[s].real();
+    res[s].embpt2[1] = S.e0.y + e1[s].imag();
+  }
+  int count = 0, nstep = 0;
+  auto t0 = std::chrono::steady_clock::now();
+  bool graph_ok = op.use_graph != 0 && !timing && gemm_variant_override() < 0;
+  int refine = op.refine_every > 0 ? op.refine_every : 0;
+  bool fp64_locked = false;
+  for (int it = 1; it <= op.maxit; ++it) {
+    bool any = false;
+    for (int s = 0; s < n; ++s) {
+      bool act = std::abs(dener[s]) > op.tol;
+      if (!act && B.h_descs[s].active) { set_active(B, s, 0); conv[s] = 1; }
+      any = any || act;
+    }
+    if (!any) break;
+    // precision of this iteration: FP32 contractions while far from convergence,
+    // FP64 on every refine-th iteration and once |dE| is within 100 tol
+    int prec = CCGPU_PREC_FP64;
+    if (op.precision == CCGPU_PREC_FP32 && !fp64_locked) {
+      prec = CCGPU_PREC_FP32;
+      if (refine && it % refine == 0) prec = CCGPU_PREC_FP64;
+    }
+    if (graph_ok) {
+      int q = prec == CCGPU_PREC_FP32 ? 1 : 0;
+      if (!B.gexec[q]) {
+        HIPCHK(hipStreamBeginCapture(B.stream, hipStreamCaptureModeThreadLocal));
+        residual_launches(B, prec, false);
+        HIPCHK(hipStreamEndCapture(B.stream, &B.graph[q]));
+        HIPCHK(hipGraphInstantiate(&B.gexec[q], B.graph[q], nullptr, nullptr, 0));
+      }
+      HIPCHK(hipGraphLaunch(B.gexec[q], B.stream));
+    } else {
+      residual_launches(B, prec, timing);
+    }
+    check_launch();
+    ++count;
+    ++nstep;
+    update_launches(B, count, nstep, op.subspace, op.diis_step, timing);
+    check_launch();
+    auto e2 = energies(B, timing);
+    double maxd = 0;
+    for (int s = 0; s < n; ++s) {
+      if (!B.h_descs[s].active) continue;
+      dener[s] = e2[s] - elast[s];
+      elast[s] = e2[s];
+      iters[s] = it;
+      maxd = std::max(maxd, std::abs(dener[s]));
+      if (op.verbose) printf("dener %5d %5d %16.8E  E %.12f\n", s, it, dener[s].real(), e2[s].real());
+    }
+    if (op.precision == CCGPU_PREC_FP32 && maxd <= 100.0 * op.tol) fp64_locked = true;
+  }
+  HIPCHK(hipStreamSynchronize(B.stream));
+  double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  for (int s = 0; s < n; ++s) {
+    if (std::abs(dener[s]) <= op.tol) conv[s] = 1;
+    Solve &S = *B.solves[s];
+    res[s].iterations = iters[s];
+    res[s].converged = conv[s];
+    res[s].ecorr_last[0] = elast[s].real();
+    res[s].ecorr_last[1] = elast[s].imag();
+    res[s].eccd[0] = S.e0.x + elast[s].real();
+    res[s].eccd[1] = S.e0.y + elast[s].imag();
+    res[s].seconds = secs;
+    S.have_t = true;
+  }
+  // leave every solve active for later single calls
+  for (int s = 0; s < n; ++s) set_active(B, s, 1);
+  HIPCHK(hipStreamSynchronize(B.stream));
+}
+
+std::unique_ptr<Batch> single_batch(Solve &S) {
+  std::vector<Solve *> v{&S};
+  return make_batch(v, S.stream);
+}
+
+// per-handle cached batch of one (for the step-by-step API)
+std::map<int, std::unique_ptr<Batch>> g_single;
+Batch &single(int h) {
+  auto it = g_single.find(h);
+  if (it != g_single.end()) return *it->second;
+  Solve &S = get(h);
+  g_single[h] = single_batch(S);
+  return *g_single[h];
+}
+void drop_single(int h) { g_single.erase(h); }
+
+template <typename F>
+int guarded(F &&f) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  try {
+    f();
+    return 0;
+  } catch (const std::exception &e) {
+    g_err = e.what();
+    return 1;
+  } catch (...) {
+    g_err = "unknown error";
+    return 1;
+  }
+}
+
+void fill_dims(const Plan &P, ccgpu_dims *d) {
+  d->nsp = P.nsp; d->nholes = P.A; d->nparticles = P.np;
+  d->nchannels = P.nch; d->nph_channels = P.nphc;
+  d->max_npp = P.max_npp; d->max_nhh = P.max_nhh;
+  d->t2_unique = P.nT; d->t2_ordered = P.t2_ordered; d->vpppp_unique = P.nVpp;
+  d->ring_elems = P.nRt;
+  d->box = P.orb.box; d->volume = P.orb.volume; d->kf = P.orb.kf;
+}
+
+int64_t t_index(const Plan &P, int a, int b, int i, int j, double &sign) {
+  sign = 0;
+  if (a < P.A || b < P.A || i >= P.A || j >= P.A || a >= P.nsp || b >= P.nsp || i < 0 || j < 0) return -1;
+  if (a == b || i == j) return -1;
+  PairRef p = P.pp(a, b), h = P.hh(i, j);
+  if (!p.sign || !h.sign || p.ch != h.ch) return -1;
+  sign = p.sign * h.sign;
+  return P.off_t[p.ch] + (int64_t)p.idx * P.nhh[p.ch] + h.idx;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ccgpu_last_error(char *buf, int len) {
+  if (buf && len > 0) {
+    strncpy(buf, g_err.c_str(), len - 1);
+    buf[len - 1] = 0;
+  }
+  return (int)g_err.size();
+}
+
+int ccgpu_plan(const ccgpu_lattice_params *p, ccgpu_dims *dims) {
+  return guarded([&] {
+    if (!p || !dims) throw Error("null argument");
+    Orbitals o = build_orbitals(p->A, p->matter, p->rho, p->nmax, p->twist);
+    Plan P;
+    build_plan(o, P);
+    fill_dims(P, dims);
+  });
+}
+
+int ccgpu_create(const ccgpu_lattice_params *p, int *handle) {
+  return guarded([&] {
+    if (!p || !handle) throw Error("null argument");
+    auto S = std::make_unique<Solve>();
+    Orbitals o = build_orbitals(p->A, p->matter, p->rho, p->nmax, p->twist);
+    build_plan(o, S->plan);
+    build_device(*S);
+    int h = g_next++;
+    g_solves[h] = std::move(S);
+    *handle = h;
+  });
+}
+
+int ccgpu_create_from_orbitals(int nsp, int nholes, const int *nx, const int *ny, const int *nz, const int *sz,
+                               const int *tz, const double *kx, const double *ky, const double *kz,
+                               const double *e, double box, double volume, int *handle) {
+  return guarded([&] {
+    if (nsp <= 0 || nholes <= 0 || nholes >= nsp) throw Error("invalid orbital counts");
+    Orbitals o;
+    o.nsp = nsp; o.A = nholes;
+    o.nx.assign(nx, nx + nsp); o.ny.assign(ny, ny + nsp); o.nz.assign(nz, nz + nsp);
+    o.sz.assign(sz, sz + nsp); o.tz.assign(tz, tz + nsp);
+    o.kx.assign(kx, kx + nsp); o.ky.assign(ky, ky + nsp); o.kz.assign(kz, kz + nsp);
+    o.e.assign(e, e + nsp);
+    o.box = box; o.volume = volume;
+    int nm = 0;
+    for (int x = 0; x < nsp; ++x) nm = std::max({nm, std::abs(o.nx[x]), std::abs(o.ny[x]), std::abs(o.nz[x])});
+    o.nmax = nm;
+    for (int x = 0; x < nsp; ++x)
+      if (std::abs(o.sz[x]) != 1 || std::abs(o.tz[x]) != 1) throw Error("sz and tz must be +-1");
+    auto S = std::make_unique<Solve>();
+    build_plan(o, S->plan);
+    build_device(*S);
+    int h = g_next++;
+    g_solves[h] = std::move(S);
+    *handle = h;
+  });
+}
+
+int ccgpu_destroy(int handle) {
+  return guarded([&] {
+    get(handle);
+    drop_single(handle);
+    g_solves.erase(handle);
+  });
+}
+
+int ccgpu_get_dims(int handle, ccgpu_dims *dims) {
+  return guarded([&] { fill_dims(get(handle).plan, dims); });
+}
+
+int ccgpu_get_orbitals(int handle, int *nx, int *ny, int *nz, int *sz, int *tz, double *e) {
+  return guarded([&] {
+    const Orbitals &o = get(handle).plan.orb;
+    for (int x = 0; x < o.nsp; ++x) {
+      if (nx) nx[x] = o.nx[x];
+      if (ny) ny[x] = o.ny[x];
+      if (nz) nz[x] = o.nz[x];
+      if (sz) sz[x] = o.sz[x];
+      if (tz) tz[x] = o.tz[x];
+      if (e) e[x] = o.e[x];
+    }
+  });
+}
+
+int ccgpu_generate_interaction(int handle, int kind) {
+  return guarded([&] {
+    Solve &S = get(handle);
+    if (kind != CCGPU_INT_MINNESOTA) throw Error(fmt("unknown interaction kind %d", kind));
+    const Plan &P = S.plan;
+    int max_nph = 0, max_nhp = 0;
+    for (int K = 0; K < P.nphc; ++K) { max_nph = std::max(max_nph, P.nph[K]); max_nhp = std::max(max_nhp, P.nhp[K]); }
+    launch_fill_blocks(S.desc, P.orb.volume, P.nch, P.nphc, P.max_npp, P.max_nhh, max_nhp, S.stream);
+    check_launch();
+    launch_derive(S.desc, P.nch, P.nphc, P.max_npp, P.max_nhh, max_nph, max_nhp, S.stream);
+    check_launch();
+    HIPCHK(hipMemcpyAsync(&S.e0, S.desc.e0, sizeof(double2), hipMemcpyDeviceToHost, S.stream));
+    HIPCHK(hipStreamSynchronize(S.stream));
+    S.have_v = S.have_fock = S.derived = true;
+  });
+}
+
+// Host staging of reference-layout blocks (drop-in path).
+int ccgpu_set_block(int handle, int kind, int nbra, const int *bra_p, const int *bra_q, int nket,
+                    const int *ket_r, const int *ket_s, const double *val, int ld) {
+  return guarded([&] {
+    Solve &S = get(handle);
+    const Plan &P = S.plan;
+    if (ld < nbra) throw Error("ld < nbra");
+    auto V = [&](int r, int c) {
+      const double *p = val + 2 * ((size_t)c * ld + r);
+      return double2{p[0], p[1]};
+    };
+    auto chk = [&](int x) {
+      if (x < 1 || x > P.nsp) throw Error(fmt("orbital index %d out of range", x));
+      return x - 1;
+    };
+    std::vector<std::pair<int64_t, double2>> up;
+    double2 *dst = nullptr;
+    switch (kind) {
+      case CCGPU_BLOCK_PPPP:
+      case CCGPU_BLOCK_HHPP:
+      case CCGPU_BLOCK_HHHH: {
+        for (int r = 0; r < nbra; ++r) {
+          int p = chk(bra_p[r]), q = chk(bra_q[r]);
+          if (p >= q) continue;
+          for (int c = 0; c < nket; ++c) {
+            int x = chk(ket_r[c]), y = chk(ket_s[c]);
+            if (x >= y) continue;
+            PairRef br, kr;
+            if (kind == CCGPU_BLOCK_PPPP) {
+              if (p < P.A || x < P.A) throw Error("pppp block with hole indices");
+              br = P.pp(p, q); kr = P.pp(x, y);
+              if (br.sign != 1 || kr.sign != 1 || br.ch != kr.ch) continue;
+              up.push_back({P.off_vpp[br.ch] + (int64_t)br.idx * P.npp[br.ch] + kr.idx, V(r, c)});
+              dst = S.desc.Vpppp;
+            } else if (kind == CCGPU_BLOCK_HHPP) {
+              if (p >= P.A || x < P.A) throw Error("hhpp block with wrong index classes");
+              br = P.hh(p, q); kr = P.pp(x, y);
+              if (br.sign != 1 || kr.sign != 1 || br.ch != kr.ch) continue;
+              up.push_back({P.off_t[br.ch] + (int64_t)br.idx * P.npp[br.ch] + kr.idx, V(r, c)});
+              dst = S.desc.Vhhpp;
+            } else {
+              if (p >= P.A || x >= P.A) throw Error("hhhh block with particle indices");
+              br = P.hh(p, q); kr = P.hh(x, y);
+              if (br.sign != 1 || kr.sign != 1 || br.ch != kr.ch) continue;
+              up.push_back({P.off_vhh[br.ch] + (int64_t)br.idx * P.nhh[br.ch] + kr.idx, V(r, c)});
+              dst = S.desc.Vhhhh;
+            }
+          }
+        }
+        break;
+      }
+      case CCGPU_BLOCK_HPHP: {
+        for (int r = 0; r < nbra; ++r) {
+          int i = chk(bra_p[r]), b = chk(bra_q[r]);
+          for (int c = 0; c < nket; ++c) {
+            int j = chk(ket_r[c]), a = chk(ket_s[c]);
+            uint64_t key = (((uint64_t)i * P.nsp + b) * P.nsp + j) * P.nsp + a;
+            double2 v = V(r, c);
+            S.hphp[key] = std::complex<double>(v.x, v.y);
+          }
+        }
+        break;
+      }
+      default:
+        throw Error(fmt("block kind %d not accepted by ccgpu_set_block", kind));
+    }
+    if (dst && !up.empty()) {
+      // scatter on the host into a staging copy of the whole array
+      int64_t n = (kind == CCGPU_BLOCK_PPPP) ? P.nVpp : (kind == CCGPU_BLOCK_HHPP ? P.nT : P.nVhh);
+      std::vector<double2> h(n);
+      HIPCHK(hipMemcpy(h.data(), dst, n * sizeof(double2), hipMemcpyDeviceToHost));
+      for (auto &kv : up) h[kv.first] = kv.second;
+      HIPCHK(hipMemcpy(dst, h.data(), n * sizeof(double2), hipMemcpyHostToDevice));
+    }
+    S.have_v = true;
+    S.derived = false;
+  });
+}
+
+int ccgpu_set_fock(int handle, const double *fock, double e0_re, double e0_im) {
+  return guarded([&] {
+    Solve &S = get(handle);
+    const Plan &P = S.plan;
+    std::vector<double2> fd(P.nsp), fp(P.nsp);
+    for (int p = 0; p < P.nsp; ++p) {
+      const double *x = fock + 2 * ((size_t)p * P.nsp + p);
+      fd[p] = double2{x[0], x[1]};
+      int q = P.partner[p];
+      if (q >= 0) {
+        const double *y = fock + 2 * ((size_t)q * P.nsp + p);  // f(p,q), column-major
+        fp[p] = double2{y[0], y[1]};
+      } else {
+        fp[p] = double2{0, 0};
+      }
+    }
+    HIPCHK(hipMemcpy(S.desc.fdiag, fd.data(), P.nsp * sizeof(double2), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(S.desc.fpart, fp.data(), P.nsp * sizeof(double2), hipMemcpyHostToDevice));
+    S.e0 = double2{e0_re, e0_im};
+    HIPCHK(hipMemcpy(S.desc.e0, &S.e0, sizeof(double2), hipMemcpyHostToDevice));
+    S.have_fock = true;
+  });
+}
+
+int ccgpu_finalize_setup(int handle) {
+  return guarded([&] {
+    Solve &S = get(handle);
+    const Plan &P = S.plan;
+    if (!S.have_v || !S.have_fock) throw Error("set V blocks and the Fock matrix first");
+    // W0 from the staged hphp records: W0((i,a),(j,b)) = -<ib||ja>
+    if (!S.hphp.empty()) {
+      std::vector<double2> w(P.nI4);
+      for (int K = 0; K < P.nphc; ++K)
+        for (int r = 0; r < P.nhp[K]; ++r)
+          for (int s = 0; s < P.nhp[K]; ++s) {
+            int i = P.hp_j[P.off_hp[K] + r], a = P.hp_b[P.off_hp[K] + r];
+            int j = P.hp_j[P.off_hp[K] + s], b = P.hp_b[P.off_hp[K] + s];
+            uint64_t key = (((uint64_t)i * P.nsp + b) * P.nsp + j) * P.nsp + a;
+            auto it = S.hphp.find(key);
+            std::complex<double> v = it == S.hphp.end() ? 0.0 : it->second;
+            w[P.off_i4[K] + (int64_t)r * P.nhp[K] + s] = double2{-v.real(), -v.imag()};
+          }
+      HIPCHK(hipMemcpy(S.desc.W0, w.data(), P.nI4 * sizeof(double2), hipMemcpyHostToDevice));
+    }
+    int max_nph = 0, max_nhp = 0;
+    for (int K = 0; K < P.nphc; ++K) { max_nph = std::max(max_nph, P.nph[K]); max_nhp = std::max(max_nhp, P.nhp[K]); }
+    launch_derive(S.desc, P.nch, P.nphc, P.max_npp, P.max_nhh, max_nph, max_nhp, S.stream);
+    check_launch();
+    HIPCHK(hipStreamSynchronize(S.stream));
+    S.derived = true;
+  });
+}
+
+int ccgpu_get_e0(int handle, double *e0) {
+  return guarded([&] {
+    Solve &S = get(handle);
+    e0[0] = S.e0.x;
+    e0[1] = S.e0.y;
+  });
+}
+
+int ccgpu_get_fock_diag(int handle, double *f) {
+  return guarded([&] {
+    Solve &S = get(handle);
+    HIPCHK(hipMemcpy(f, S.desc.fdiag, S.plan.nsp * sizeof(double2), hipMemcpyDeviceToHost));
+  });
+}
+
+int ccgpu_mbpt2_start(int handle, double *ecorr) {
+  return guarded([&] {
+    Solve &S = get(handle);
+    ensure_ready(S);
+    Batch &B = single(handle);
+    launch_mbpt2(B.d_descs, 1, S.plan.nT, B.stream);
+    check_launch();
+    auto e = energies(B, false);
+    S.have_t = true;
+    S.nstep = 0;
+    S.count = 0;
+    if (ecorr) { ecorr[0] = e[0].real(); ecorr[1] = e[0].imag(); }
+  });
+}
+
+int ccgpu_residual(int handle, int precision) {
+  return guarded([&] {
+    Solve &S = get(handle);
+    ensure_ready(S);
+    if (!S.have_t) throw Error("no amplitudes: call ccgpu_mbpt2_start or ccgpu_set_t2 first");
+    Batch &B = single(handle);
+    residual_launches(B, precision, false);
+    check_launch();
+    HIPCHK(hipStreamSynchronize(B.stream));
+  });
+}
+
+int ccgpu_update(int handle, int *count, int *nstep, int subspace, int diis_step) {
+  return guarded([&] {
+    Solve &S = get(handle);
+    ensure_ready(S);
+    Batch &B = single(handle);
+    if (S.depth < subspace) {
+      alloc_history(S, subspace);
+      B.h_descs[0].hist = S.desc.hist;
+      HIPCHK(hipMemcpy(B.d_descs, B.h_descs.data(), sizeof(SolveDesc), hipMemcpyHostToDevice));
+    }
+    if ((int)S.ring.size() != subspace) {
+      S.ring.resize(subspace);
+      for (int k = 0; k < subspace; ++k) S.ring[k] = k;
+    }
+    int c = *count, n = *nstep;
+    update_launches(B, c, n, subspace, diis_step, false);
+    check_launch();
+    HIPCHK(hipStreamSynchronize(B.stream));
+    *count = c;
+    *nstep = n;
+  });
+}
+
+int ccgpu_energy(int handle, double *ecorr) {
+  return guarded([&] {
+    Solve &S = get(handle);
+    ensure_ready(S);
+    Batch &B = single(handle);
+    auto e = energies(B, false);
+    ecorr[0] = e[0].real();
+    ecorr[1] = e[0].imag();
+  });
+}
+
+int ccgpu_solve(int handle, const ccgpu_solve_opts *opts, ccgpu_solve_result *res) {
+  return guarded([&] {
+    Solve &S = get(handle);
+    ensure_ready(S);
+    if (!S.have_t) throw Error("no amplitudes: call ccgpu_mbpt2_start or ccgpu_set_t2 first");
+    if (opts->precision == CCGPU_PREC_FP32 && !S.desc.Vpppp32) {
+      S.desc.Vpppp32 = dalloc<float2>(S.plan.nVpp, S.owned);
+      launch_to_f32(S.desc.Vpppp, S.desc.Vpppp32, S.plan.nVpp, S.stream);
+      check_launch();
+      HIPCHK(hipStreamSynchronize(S.stream));
+      HIPCHK(hipMemcpy(S.d_desc, &S.desc, sizeof(SolveDesc), hipMemcpyHostToDevice));
+      drop_single(handle);
+    }
+    Batch &B = single(handle);
+    std::vector<ccgpu_solve_result> r;
+    run_batch(B, *opts, r);
+    *res = r[0];
+  });
+}
+
+int ccgpu_batch_solve(int n, const int *handles, const ccgpu_solve_opts *opts, ccgpu_solve_result *results) {
+  return guarded([&] {
+    if (n <= 0) throw Error("empty batch");
+    std::vector<Solve *> ss;
+    for (int s = 0; s < n; ++s) {
+      Solve &S = get(handles[s]);
+      ensure_ready(S);
+      if (!S.have_t) throw Error(fmt("handle %d has no amplitudes", handles[s]));
+      if (opts->precision == CCGPU_PREC_FP32 && !S.desc.Vpppp32) {
+        S.desc.Vpppp32 = dalloc<float2>(S.plan.nVpp, S.owned);
+        launch_to_f32(S.desc.Vpppp, S.desc.Vpppp32, S.plan.nVpp, S.stream);
+        check_launch();
+        HIPCHK(hipStreamSynchronize(S.stream));
+        HIPCHK(hipMemcpy(S.d_desc, &S.desc, sizeof(SolveDesc), hipMemcpyHostToDevice));
+      }
+      ss.push_back(&S);
+    }
+    auto B = make_batch(ss, ss[0]->stream);
+    std::vector<ccgpu_solve_result> r;
+    run_batch(*B, *opts, r);
+    for (int s = 0; s < n; ++s) results[s] = r[s];
+  });
+}
+
+int ccgpu_get_t2(int handle, int n, const int *a, const int *b, const int *i, const int *j, double *t) {
+  return guarded([&] {
+    Solve &S = get(handle);
+    std::vector<double2> h(S.plan.nT);
+    HIPCHK(hipMemcpy(h.data(), S.desc.T, S.plan.nT * sizeof(double2), hipMemcpyDeviceToHost));
+    for (int x = 0; x < n; ++x) {
+      double sg;
+      int64_t u = t_index(S.plan, a[x], b[x], i[x], j[x], sg);
+      t[2 * x] = u < 0 ? 0.0 : sg * h[u].x;
+      t[2 * x + 1] = u < 0 ? 0.0 : sg * h[u].y;
+    }
+  });
+}
+
+int ccgpu_set_t2(int handle, int n, const int *a, const int *b, const int *i, const int *j, const double *t) {
+  return guarded([&] {
+    Solve &S = get(handle);
+    std::vector<double2> h(S.plan.nT);
+    HIPCHK(hipMemcpy(h.data(), S.desc.T, S.plan.nT * sizeof(double2), hipMemcpyDeviceToHost));
+    for (int x = 0; x < n; ++x) {
+      double sg;
+      int64_t u = t_index(S.plan, a[x], b[x], i[x], j[x], sg);
+      if (u < 0) continue;
+      h[u] = double2{sg * t[2 * x], sg * t[2 * x + 1]};
+    }
+    HIPCHK(hipMemcpy(S.desc.T, h.data(), S.plan.nT * sizeof(double2), hipMemcpyHostToDevice));
+    S.have_t = true;
+  });
+}
+
+int ccgpu_get_t2_unique(int handle, double *t) {
+  return guarded([&] {
+    Solve &S = get(handle);
+    HIPCHK(hipMemcpy(t, S.desc.T, S.plan.nT * sizeof(double2), hipMemcpyDeviceToHost));
+  });
+}
+
+int ccgpu_set_t2_unique(int handle, const double *t) {
+  return guarded([&] {
+    Solve &S = get(handle);
+    HIPCHK(hipMemcpy(S.desc.T, t, S.plan.nT * sizeof(double2), hipMemcpyHostToDevice));
+    S.have_t = true;
+  });
+}
+
+int ccgpu_get_t2_labels(int handle, int *abij) {
+  return guarded([&] {
+    const Plan &P = get(handle).plan;
+    for (int c = 0; c < P.nch; ++c)
+      for (int r = 0; r < P.npp[c]; ++r)
+        for (int s = 0; s < P.nhh[c]; ++s) {
+          int64_t u = P.off_t[c] + (int64_t)r * P.nhh[c] + s;
+          abij[4 * u + 0] = P.pp_a[P.off_pp[c] + r];
+          abij[4 * u + 1] = P.pp_b[P.off_pp[c] + r];
+          abij[4 * u + 2] = P.hh_i[P.off_hh[c] + s];
+          abij[4 * u + 3] = P.hh_j[P.off_hh[c] + s];
+        }
+  });
+}
+
+int ccgpu_triples(int handle, double *et) {
+  return guarded([&] {
+    (void)handle;
+    (void)et;
+    throw Error("ccgpu_triples: not available in this build");
+  });
+}
+
+int ccgpu_kernel_stats(char *names, int len, double *avg_ms, long long *launches, int max) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  std::string all;
+  int k = 0;
+  for (auto &nm : g_stat_order) {
+    if (k >= max) break;
+    const Stat &s = g_stats[nm];
+    if (avg_ms) avg_ms[k] = s.n ? s.ms / s.n : 0.0;
+    if (launches) launches[k] = s.n;
+    if (!all.empty()) all += ";";
+    all += nm;
+    ++k;
+  }
+  if (names && len > 0) {
+    strncpy(names, all.c_str(), len - 1);
+    names[len - 1] = 0;
+  }
+  return k;
+}
+
+}  // extern "C"

@@ -1,0 +1,64 @@
+// Device-side descriptors shared by the kernels and the host runtime.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace ccgpu {
+
+// Ordered pair reference packed for the device: ch = channel (-1 absent),
+// v = idx * 2 + (sign < 0).
+struct DPair {
+  int ch;
+  int v;
+};
+
+// Everything one solve's kernels read or write.  All pointers are device
+// pointers; arrays indexed by unique amplitude u have length nT.
+struct SolveDesc {
+  int A, nsp, np, nch, nphc;
+  int64_t nT, nRt, nI4;
+  int active;  // 0: converged, kernels skip this solve
+  // orbitals
+  const double *kx, *ky, *kz, *e;
+  const int *nx, *ny, *nz, *sz, *tz, *partner;
+  // channels
+  const int *t_ch, *npp, *nhh;
+  const int64_t *off_t, *off_pp, *off_hh, *off_vpp, *off_vhh;
+  const int *pp_a, *pp_b, *hh_i, *hh_j;
+  const int *ob_rowB, *ob_rowA, *ob_colJ, *ob_colI;
+  const DPair *ppref, *hhref;
+  // Pandya
+  const int *nph, *nhp;
+  const int64_t *off_ph, *off_hp, *off_rt, *off_i4, *off_vt;
+  const int *ph_a, *ph_i, *hp_j, *hp_b;
+  const int64_t *ring4;
+  // amplitudes and blocks
+  double2 *T, *R, *R0, *Vhhpp, *Vhhhh, *X, *Vpppp;
+  float2 *Vpppp32;
+  double2 *Tt, *Rt, *I4, *W0, *Vt;
+  // one-body
+  double2 *fdiag, *fpart;   // f(p,p) and f(p,partner(p))
+  double2 *dh, *I7off, *dp, *I6off;
+  double2 *e0;              // device E0
+  // DIIS history (depth x nT) and reductions
+  double2 *hist;
+  double2 *red;             // reduction partials
+  double2 *scal;            // small scalar outputs (energy, Gram)
+};
+
+// One member of a grouped complex GEMM: C = alpha * A * B + beta * C0.
+struct GemmGroup {
+  const void *A;
+  const double2 *B;
+  double2 *C;
+  const double2 *C0;
+  int M, N, K;
+  int lda, ldb, ldc, ldc0;
+  double alpha, beta;
+};
+
+struct GemmTile {
+  int g, m0, n0, pad;
+};
+
+}  // namespace ccgpu

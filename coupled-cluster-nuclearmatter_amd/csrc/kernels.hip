@@ -1,0 +1,824 @@
+// HIP kernels of the CCD/(T) engine for gfx950.
+//
+// Storage (per solve, all complex interleaved double2, row-major):
+//   T, R, R0 : per (K,Tz) channel a block Npp x Nhh of unique pairs (a<b, i<j)
+//   Vpppp    : Npp x Npp per channel       Vhhpp : Nhh x Npp per channel
+//   Vhhhh, X : Nhh x Nhh per channel       (X = Vhhpp T, I9 = X + Vhhhh)
+//   Tt, Rt   : Pandya blocks Nph x Nhp     I4, W0 : Nhp x Nhp    Vt : Nhp x Nph
+// Reference routines restated here are cited per kernel (paths relative to
+// deltafull-CCMkspace/ of the reference).
+#include "device.h"
+
+#include <algorithm>
+
+namespace ccgpu {
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+typedef float f4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ double2 cmul(double2 a, double2 b) {
+  return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ double2 cscale(double s, double2 a) { return make_double2(s * a.x, s * a.y); }
+__device__ __forceinline__ double2 cconj(double2 a) { return make_double2(a.x, -a.y); }
+__device__ __forceinline__ double2 cdiv(double2 a, double2 b) {
+  double d = b.x * b.x + b.y * b.y;
+  return make_double2((a.x * b.x + a.y * b.y) / d, (a.y * b.x - a.x * b.y) / d);
+}
+
+// ---------------------------------------------------------------------------
+// Interaction: Minnesota potential in momentum space, vmom_minnesota
+// (chiral_module_andreas_with_delta.f90:8980-9111).  The reference's
+// default-real literals are reproduced as their float32 values.
+// ---------------------------------------------------------------------------
+struct Orb {
+  const double *kx, *ky, *kz;
+  const int *sz, *tz;
+  double volume;
+};
+
+__device__ __forceinline__ double sig_dot_sig(int m1, int m2, int m3, int m4) {
+  // <m1 m2| s1.s2 |m3 m4> for m = +-1: m1*m2 on the diagonal, 2 for a spin exchange
+  double r = 0.0;
+  if (m1 == m3 && m2 == m4) r += (double)(m1 * m2);
+  if (m1 == m4 && m2 == m3 && m1 != m2) r += 2.0;
+  return r;
+}
+
+__device__ double minnesota_direct(const Orb &o, int p, int q, int r, int s) {
+  const double PI = 3.14159265358979;
+  const double v0r = 200.0, v0t = 178.0, v0s = (double)91.85f;
+  const double kr = (double)1.487f, kt = (double)0.639f, ks = (double)0.465f;
+  double qx = 0.5 * (o.kx[p] - o.kx[q]) - 0.5 * (o.kx[r] - o.kx[s]);
+  double qy = 0.5 * (o.ky[p] - o.ky[q]) - 0.5 * (o.ky[r] - o.ky[s]);
+  double qz = 0.5 * (o.kz[p] - o.kz[q]) - 0.5 * (o.kz[r] - o.kz[s]);
+  double q2 = qx * qx + qy * qy + qz * qz;
+  double pi15 = pow(PI, 1.5);
+  double vr = v0r * pi15 * exp(-q2 / (4.0 * kr)) / pow(kr, 1.5);
+  double vt = -v0t * pi15 * exp(-q2 / (4.0 * kt)) / pow(kt, 1.5);
+  double vs = -v0s * pi15 * exp(-q2 / (4.0 * ks)) / pow(ks, 1.5);
+  int m1 = o.sz[p], m2 = o.sz[q], m3 = o.sz[r], m4 = o.sz[s];
+  int t1 = o.tz[p], t2 = o.tz[q], t3 = o.tz[r], t4 = o.tz[s];
+  if (t1 + t2 != t3 + t4) return 0.0;
+  double ds = (m1 == m3 && m2 == m4) ? 1.0 : 0.0;
+  double dt = (t1 == t3 && t2 == t4) ? 1.0 : 0.0;
+  double ss = sig_dot_sig(m1, m2, m3, m4);
+  double tt = sig_dot_sig(t1, t2, t3, t4);
+  vr = vr * (3.0 * ds * dt - tt * ds - ss * dt - tt * ss) / 8.0;
+  vs = vs * (3.0 * ds * dt + tt * ds - 3.0 * ss * dt - tt * ss) / 16.0;
+  vt = vt * (3.0 * ds * dt - 3.0 * tt * ds + ss * dt - tt * ss) / 16.0;
+  return (vs + vr + vt) / o.volume;
+}
+
+// antisymmetrized <pq||rs> = v(p,q,r,s) - v(p,q,s,r); momentum conservation holds
+// for every element the callers request.
+__device__ __forceinline__ double2 vas(const Orb &o, int p, int q, int r, int s) {
+  return make_double2(minnesota_direct(o, p, q, r, s) - minnesota_direct(o, p, q, s, r), 0.0);
+}
+
+__device__ __forceinline__ Orb orb_of(const SolveDesc &d, double volume) {
+  Orb o;
+  o.kx = d.kx; o.ky = d.ky; o.kz = d.kz; o.sz = d.sz; o.tz = d.tz; o.volume = volume;
+  return o;
+}
+
+// V blocks of setup_channel_structures (ccm_iter.f90:819-1311), unique pairs.
+__global__ void k_fill_vpppp(SolveDesc d, double volume) {
+  const int c = blockIdx.y;
+  const Orb o = orb_of(d, volume);
+  const int n = d.npp[c];
+  const int64_t ne = (int64_t)n * n;
+  const int *pa = d.pp_a + d.off_pp[c], *pb = d.pp_b + d.off_pp[c];
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < ne; e += (int64_t)gridDim.x * blockDim.x) {
+    int r = (int)(e / n), s = (int)(e % n);
+    d.Vpppp[d.off_vpp[c] + e] = vas(o, pa[r], pb[r], pa[s], pb[s]);
+  }
+}
+
+__global__ void k_fill_vhhpp(SolveDesc d, double volume) {
+  const int c = blockIdx.y;
+  const Orb o = orb_of(d, volume);
+  const int np_ = d.npp[c], nh = d.nhh[c];
+  const int64_t ne = (int64_t)np_ * nh;
+  const int *pa = d.pp_a + d.off_pp[c], *pb = d.pp_b + d.off_pp[c];
+  const int *hi = d.hh_i + d.off_hh[c], *hj = d.hh_j + d.off_hh[c];
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < ne; e += (int64_t)gridDim.x * blockDim.x) {
+    int s = (int)(e / np_), r = (int)(e % np_);
+    d.Vhhpp[d.off_t[c] + e] = vas(o, hi[s], hj[s], pa[r], pb[r]);
+  }
+}
+
+__global__ void k_fill_vhhhh(SolveDesc d, double volume) {
+  const int c = blockIdx.y;
+  const Orb o = orb_of(d, volume);
+  const int nh = d.nhh[c];
+  const int64_t ne = (int64_t)nh * nh;
+  const int *hi = d.hh_i + d.off_hh[c], *hj = d.hh_j + d.off_hh[c];
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < ne; e += (int64_t)gridDim.x * blockDim.x) {
+    int s = (int)(e / nh), t = (int)(e % nh);
+    d.Vhhhh[d.off_vhh[c] + e] = vas(o, hi[s], hj[s], hi[t], hj[t]);
+  }
+}
+
+// W0((i,a),(j,b)) = -<ib||ja> (the recoupled hphp term of I4, ccm_t2_eqn.f90:740-780).
+__global__ void k_fill_w0(SolveDesc d, double volume) {
+  const int K = blockIdx.y;
+  const Orb o = orb_of(d, volume);
+  const int n = d.nhp[K];
+  const int64_t ne = (int64_t)n * n;
+  const int *hj = d.hp_j + d.off_hp[K], *hb = d.hp_b + d.off_hp[K];
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < ne; e += (int64_t)gridDim.x * blockDim.x) {
+    int r = (int)(e / n), s = (int)(e % n);
+    int i = hj[r], a = hb[r], j = hj[s], b = hb[s];
+    double2 v = vas(o, i, b, j, a);
+    d.W0[d.off_i4[K] + e] = make_double2(-v.x, -v.y);
+  }
+}
+
+// Fock matrix f_pq = t_p delta_pq + sum_h <ph||qh> for q in {p, partner(p)}
+// (normal_ordered_hamiltonian, ccm_main.f90:1586-1657).
+__global__ void k_fock(SolveDesc d, double volume) {
+  const Orb o = orb_of(d, volume);
+  int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= d.nsp) return;
+  double2 fd = make_double2(d.e[p], 0.0), fp = make_double2(0.0, 0.0);
+  int pp = d.partner[p];
+  for (int h = 0; h < d.A; ++h) {
+    fd = cadd(fd, vas(o, p, h, p, h));
+    if (pp >= 0) fp = cadd(fp, vas(o, p, h, pp, h));
+  }
+  d.fdiag[p] = fd;
+  d.fpart[p] = fp;
+}
+
+// E0 = sum_i t_ii + 1/2 sum_ij <ij||ij> (ccm_main.f90:1661-1684).  One block.
+__global__ void k_e0(SolveDesc d, double volume) {
+  const Orb o = orb_of(d, volume);
+  __shared__ double sre[256], sim[256];
+  double re = 0.0, im = 0.0;
+  for (int64_t e = threadIdx.x; e < (int64_t)d.A * d.A; e += blockDim.x) {
+    int i = (int)(e / d.A), j = (int)(e % d.A);
+    double2 v = vas(o, i, j, i, j);
+    re += 0.5 * v.x; im += 0.5 * v.y;
+  }
+  for (int i = threadIdx.x; i < d.A; i += blockDim.x) re += d.e[i];
+  sre[threadIdx.x] = re; sim[threadIdx.x] = im;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) { sre[threadIdx.x] += sre[threadIdx.x + w]; sim[threadIdx.x] += sim[threadIdx.x + w]; }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) d.e0[0] = make_double2(sre[0], sim[0]);
+}
+
+// R0(ab,ij) = conj(<ij||ab>): the V term of the residual (ccm_t2_eqn.f90:107-138),
+// stored in amplitude layout.
+__global__ void k_derive_r0(SolveDesc d) {
+  const int c = blockIdx.y;
+  const int np_ = d.npp[c], nh = d.nhh[c];
+  const int64_t ne = (int64_t)np_ * nh;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < ne; e += (int64_t)gridDim.x * blockDim.x) {
+    int r = (int)(e / nh), s = (int)(e % nh);
+    d.R0[d.off_t[c] + e] = cconj(d.Vhhpp[d.off_t[c] + (int64_t)s * np_ + r]);
+  }
+}
+
+__device__ __forceinline__ int pref_idx(DPair p) { return p.v >> 1; }
+__device__ __forceinline__ double pref_sign(DPair p) { return (p.v & 1) ? -1.0 : 1.0; }
+
+// Vt((i,a),(b,j)) = <ij||ab> gathered from Vhhpp (the amat gather, ccm_t2_eqn.f90:694-716).
+__global__ void k_derive_vt(SolveDesc d) {
+  const int K = blockIdx.y;
+  const int nhp = d.nhp[K], nph = d.nph[K];
+  const int64_t ne = (int64_t)nhp * nph;
+  const int *hj = d.hp_j + d.off_hp[K], *hb = d.hp_b + d.off_hp[K];
+  const int *pa = d.ph_a + d.off_ph[K], *pi = d.ph_i + d.off_ph[K];
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < ne; e += (int64_t)gridDim.x * blockDim.x) {
+    int r = (int)(e / nph), s = (int)(e % nph);
+    int i = hj[r], a = hb[r], b = pa[s], j = pi[s];
+    double2 v = make_double2(0.0, 0.0);
+    if (i != j && a != b) {
+      DPair h = d.hhref[(size_t)i * d.A + j];
+      DPair p = d.ppref[(size_t)(a - d.A) * d.np + (b - d.A)];
+      if (h.ch >= 0 && p.ch == h.ch) {
+        double sg = pref_sign(h) * pref_sign(p);
+        v = cscale(sg, d.Vhhpp[d.off_t[h.ch] + (int64_t)pref_idx(h) * d.npp[h.ch] + pref_idx(p)]);
+      }
+    }
+    d.Vt[d.off_vt[K] + e] = v;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// per-iteration kernels; blockIdx.y selects the solve of a batch
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void unique_labels(const SolveDesc &d, int64_t u, int &c, int &r, int &s) {
+  c = d.t_ch[u];
+  int64_t loc = u - d.off_t[c];
+  int nh = d.nhh[c];
+  r = (int)(loc / nh);
+  s = (int)(loc % nh);
+}
+
+// MBPT2 start t(ab,ij) = conj(<ij||ab>)/(f_ii+f_jj-f_aa-f_bb), zero denominators
+// skipped (setup_t_amplitudes, ccm_iter.f90:2351-2367).
+__global__ void k_mbpt2(const SolveDesc *descs) {
+  const SolveDesc &d = descs[blockIdx.y];
+  for (int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; u < d.nT; u += (int64_t)gridDim.x * blockDim.x) {
+    int c, r, s;
+    unique_labels(d, u, c, r, s);
+    int a = d.pp_a[d.off_pp[c] + r], b = d.pp_b[d.off_pp[c] + r];
+    int i = d.hh_i[d.off_hh[c] + s], j = d.hh_j[d.off_hh[c] + s];
+    double den = -d.fdiag[a].x - d.fdiag[b].x + d.fdiag[i].x + d.fdiag[j].x;
+    double2 t = make_double2(0.0, 0.0);
+    if (fabs(den) != 0.0) { double2 v = d.R0[u]; t = make_double2(v.x / den, v.y / den); }
+    d.T[u] = t;
+  }
+}
+
+// Deterministic block reduction of one complex value per thread.
+__device__ double2 block_sum(double2 v) {
+  __shared__ double sre[256], sim[256];
+  sre[threadIdx.x] = v.x; sim[threadIdx.x] = v.y;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) { sre[threadIdx.x] += sre[threadIdx.x + w]; sim[threadIdx.x] += sim[threadIdx.x + w]; }
+    __syncthreads();
+  }
+  double2 r = make_double2(sre[0], sim[0]);
+  __syncthreads();
+  return r;
+}
+
+// Correlation energy 1/4 sum_ordered t(ab,ij) <ij||ab> = sum_unique t * conj(R0)
+// (ccd_energy_save, ccm_energy.f90:31-48).  Pass 1: block partials.
+__global__ void k_energy_partial(const SolveDesc *descs) {
+  const SolveDesc &d = descs[blockIdx.y];
+  double2 acc = make_double2(0.0, 0.0);
+  for (int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; u < d.nT; u += (int64_t)gridDim.x * blockDim.x)
+    acc = cadd(acc, cmul(d.T[u], cconj(d.R0[u])));
+  double2 s = block_sum(acc);
+  if (threadIdx.x == 0) d.red[blockIdx.x] = s;
+}
+
+// Pass 2: fixed-order sum of nparts partials into scal[slot].
+__global__ void k_reduce_final(const SolveDesc *descs, int nparts, int stride, int nvals, int slot) {
+  const SolveDesc &d = descs[blockIdx.y];
+  for (int v = 0; v < nvals; ++v) {
+    double2 acc = make_double2(0.0, 0.0);
+    for (int p = threadIdx.x; p < nparts; p += blockDim.x) acc = cadd(acc, d.red[(int64_t)p * stride + v]);
+    double2 s = block_sum(acc);
+    if (threadIdx.x == 0) d.scal[slot + v] = s;
+  }
+}
+
+// I7(k,i) = f_ki + 1/2 sum_l sum_cd <kl||cd> t(cd,il) for k in {i, partner(i)}, read
+// from X = Vhhpp T (setup_I6I7_intermediate, ccm_t2_eqn.f90:1352-1392, 1507-1510).
+// dh(i) = I7(i,i) is the dynamic hole denominator (ccm_t2_eqn.f90:81-84).
+__global__ void k_i7(const SolveDesc *descs) {
+  const SolveDesc &d = descs[blockIdx.y];
+  if (!d.active) return;
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= 2 * d.A) return;
+  int i = t >> 1, which = t & 1;
+  int k = which ? d.partner[i] : i;
+  if (k < 0 || k >= d.A) {
+    if (which) d.I7off[i] = make_double2(0.0, 0.0);
+    return;
+  }
+  double2 acc = which ? d.fpart[k] : d.fdiag[i];
+  for (int l = 0; l < d.A; ++l) {
+    if (l == k || l == i) continue;
+    DPair pk = d.hhref[(size_t)k * d.A + l], pi = d.hhref[(size_t)i * d.A + l];
+    if (pk.ch < 0 || pk.ch != pi.ch) continue;
+    int c = pk.ch;
+    double2 x = d.X[d.off_vhh[c] + (int64_t)pref_idx(pk) * d.nhh[c] + pref_idx(pi)];
+    acc = cadd(acc, cscale(pref_sign(pk) * pref_sign(pi), x));
+  }
+  if (which) d.I7off[i] = acc; else d.dh[i] = acc;
+}
+
+// I6(a,c) = f_ac - 1/2 sum_d sum_kl <kl||dc> t(da,kl) for c in {a, partner(a)}
+// (ccm_t2_eqn.f90:1399-1441); dp(a) = I6(a,a) (ccm_t2_eqn.f90:86-89).
+// One wavefront per target; lanes stride over d, fixed butterfly reduction.
+__global__ __launch_bounds__(256) void k_i6(const SolveDesc *descs) {
+  const SolveDesc &d = descs[blockIdx.y];
+  if (!d.active) return;
+  const int lane = threadIdx.x & 63;
+  const int t = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (t >= 2 * d.np) return;
+  const int a = d.A + (t >> 1), which = t & 1;
+  const int c = which ? d.partner[a] : a;
+  if (c < d.A) {
+    if (which && lane == 0) d.I6off[a - d.A] = make_double2(0.0, 0.0);
+    return;
+  }
+  double re = 0.0, im = 0.0;
+  for (int dd = d.A + lane; dd < d.nsp; dd += 64) {
+    if (dd == a || dd == c) continue;
+    DPair pa = d.ppref[(size_t)(dd - d.A) * d.np + (a - d.A)];
+    DPair pc = d.ppref[(size_t)(dd - d.A) * d.np + (c - d.A)];
+    if (pa.ch < 0 || pc.ch != pa.ch) continue;
+    int ch = pa.ch, nh = d.nhh[ch];
+    const double2 *tr = d.T + d.off_t[ch] + (int64_t)pref_idx(pa) * nh;
+    const double2 *vr = d.R0 + d.off_t[ch] + (int64_t)pref_idx(pc) * nh;
+    double sr = 0.0, si = 0.0;
+    for (int s = 0; s < nh; ++s) {
+      double2 tv = tr[s], vv = vr[s];  // V(kl,dc) = conj(R0(dc,kl))
+      sr += vv.x * tv.x + vv.y * tv.y;
+      si += vv.x * tv.y - vv.y * tv.x;
+    }
+    double sg = pref_sign(pa) * pref_sign(pc);
+    re -= sg * sr;
+    im -= sg * si;
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    re += __shfl_xor(re, off);
+    im += __shfl_xor(im, off);
+  }
+  if (lane == 0) {
+    double2 f = which ? d.fpart[a] : d.fdiag[a];
+    double2 v = make_double2(f.x + re, f.y + im);
+    if (which) d.I6off[a - d.A] = v; else d.dp[a - d.A] = v;
+  }
+}
+
+// I9 = Vhhhh + 1/2 Vhhpp T (ccm_t2_eqn.f90:431-464, 558-560): X[0,n) holds
+// Vhhpp T (unique pairs absorb the 1/2), X[n,2n) receives I9.
+__global__ void k_i9(const SolveDesc *descs) {
+  const SolveDesc &d = descs[blockIdx.y];
+  if (!d.active) return;
+  int64_t n = d.off_vhh[d.nch];
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x)
+    d.X[n + e] = cadd(d.X[e], d.Vhhhh[e]);
+}
+
+// Residual start: R = conj(<ij||ab>) + P(ab) sum_c I6(b,c) t(ac,ij)
+//                 - P(ij) sum_k I7(k,j) t(ab,ik), diagonals removed
+// (ccm_t2_eqn.f90:81-211).  Owner-computes over unique elements.
+__global__ void k_resid_init(const SolveDesc *descs) {
+  const SolveDesc &d = descs[blockIdx.y];
+  if (!d.active) return;
+  for (int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; u < d.nT; u += (int64_t)gridDim.x * blockDim.x) {
+    int c, r, s;
+    unique_labels(d, u, c, r, s);
+    const int64_t o = d.off_t[c];
+    const int nh = d.nhh[c];
+    const int64_t pr = d.off_pp[c] + r, hs = d.off_hh[c] + s;
+    int a = d.pp_a[pr], b = d.pp_b[pr], i = d.hh_i[hs], j = d.hh_j[hs];
+    double2 acc = d.R0[u];
+    int vB = d.ob_rowB[pr];   // (a, partner(b))
+    if (vB >= 0) {
+      double2 tv = d.T[o + (int64_t)(vB >> 2) * nh + s];
+      acc = cadd(acc, cscale((double)((vB & 3) - 1), cmul(d.I6off[b - d.A], tv)));
+    }
+    int vA = d.ob_rowA[pr];   // (b, partner(a))
+    if (vA >= 0) {
+      double2 tv = d.T[o + (int64_t)(vA >> 2) * nh + s];
+      acc = csub(acc, cscale((double)((vA & 3) - 1), cmul(d.I6off[a - d.A], tv)));
+    }
+    int vJ = d.ob_colJ[hs];   // (i, partner(j))
+    if (vJ >= 0) {
+      double2 tv = d.T[o + (int64_t)r * nh + (vJ >> 2)];
+      acc = csub(acc, cscale((double)((vJ & 3) - 1), cmul(d.I7off[j], tv)));
+    }
+    int vI = d.ob_colI[hs];   // (j, partner(i))
+    if (vI >= 0) {
+      double2 tv = d.T[o + (int64_t)r * nh + (vI >> 2)];
+      acc = cadd(acc, cscale((double)((vI & 3) - 1), cmul(d.I7off[i], tv)));
+    }
+    d.R[u] = acc;
+  }
+}
+
+// Pandya gather T~((a,i),(j,b)) = t(ab,ij) (ccm_t2_eqn.f90:37-62).
+__global__ void k_ring_gather(const SolveDesc *descs) {
+  const SolveDesc &d = descs[blockIdx.y];
+  if (!d.active) return;
+  for (int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; u < d.nT; u += (int64_t)gridDim.x * blockDim.x) {
+    double2 t = d.T[u];
+    double2 m = make_double2(-t.x, -t.y);
+    const int64_t *p = d.ring4 + 4 * u;
+    d.Tt[p[0]] = t;
+    d.Tt[p[1]] = m;
+    d.Tt[p[2]] = m;
+    d.Tt[p[3]] = t;
+  }
+}
+
+// Ring scatter with P(ab)P(ij) (ccm_t2_eqn.f90:1133-1185), owner-computes.
+__global__ void k_ring_scatter(const SolveDesc *descs) {
+  const SolveDesc &d = descs[blockIdx.y];
+  if (!d.active) return;
+  for (int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; u < d.nT; u += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t *p = d.ring4 + 4 * u;
+    double2 v = csub(csub(cadd(d.Rt[p[0]], d.Rt[p[3]]), d.Rt[p[1]]), d.Rt[p[2]]);
+    d.R[u] = cadd(d.R[u], v);
+  }
+}
+
+// Jacobi step with mixing, t <- 0.75 t + 0.25 R/(dh_i+dh_j-dp_a-dp_b), and the
+// history write of diis_t1t2 (ccm_diis.f90:106-126).
+__global__ void k_update(const SolveDesc *descs, const int *slot) {
+  const SolveDesc &d = descs[blockIdx.y];
+  if (!d.active) return;
+  double2 *h = d.hist + (int64_t)slot[blockIdx.y] * d.nT;
+  for (int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; u < d.nT; u += (int64_t)gridDim.x * blockDim.x) {
+    int c, r, s;
+    unique_labels(d, u, c, r, s);
+    int a = d.pp_a[d.off_pp[c] + r], b = d.pp_b[d.off_pp[c] + r];
+    int i = d.hh_i[d.off_hh[c] + s], j = d.hh_j[d.off_hh[c] + s];
+    double2 den = csub(csub(cadd(d.dh[i], d.dh[j]), d.dp[a - d.A]), d.dp[b - d.A]);
+    double2 t = cadd(cscale(0.75, d.T[u]), cscale(0.25, cdiv(d.R[u], den)));
+    d.T[u] = t;
+    h[u] = t;
+  }
+}
+
+// DIIS Gram matrix B_pq = sum e_p e_q (no conjugation, ccm_diis.f90:134-147) of the
+// error vectors e_p = t_{p+1} - t_p over the ring-buffer history.  Up to 16
+// history vectors; partials of the upper triangle go to red[block*136 + idx].
+__global__ __launch_bounds__(256) void k_gram_partial(const SolveDesc *descs, const int *slots, int nstep) {
+  const SolveDesc &d = descs[blockIdx.y];
+  if (!d.active) return;
+  const int ne = nstep - 1;
+  double2 acc[36];
+  for (int q = 0; q < 36; ++q) acc[q] = make_double2(0.0, 0.0);
+  const int *sl = slots + blockIdx.y * 16;
+  for (int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; u < d.nT; u += (int64_t)gridDim.x * blockDim.x) {
+    double2 e[8];
+    double2 prev = d.hist[(int64_t)sl[0] * d.nT + u];
+    for (int p = 0; p < ne; ++p) {
+      double2 cur = d.hist[(int64_t)sl[p + 1] * d.nT + u];
+      e[p] = csub(cur, prev);
+      prev = cur;
+    }
+    int q = 0;
+    for (int p = 0; p < ne; ++p)
+      for (int p2 = p; p2 < ne; ++p2) {
+        acc[q] = cadd(acc[q], cmul(e[p], e[p2]));
+        ++q;
+      }
+  }
+  int q = 0;
+  for (int p = 0; p < ne; ++p)
+    for (int p2 = p; p2 < ne; ++p2) {
+      double2 s = block_sum(acc[q]);
+      if (threadIdx.x == 0) d.red[(int64_t)blockIdx.x * 36 + q] = s;
+      ++q;
+    }
+}
+
+// t = sum_p c_p t_p over the first nstep-1 history vectors (ccm_diis.f90:186-212).
+__global__ void k_diis_axpy(const SolveDesc *descs, const int *slots, const double2 *coef, int nvec) {
+  const SolveDesc &d = descs[blockIdx.y];
+  if (!d.active) return;
+  const int *sl = slots + blockIdx.y * 16;
+  const double2 *cf = coef + blockIdx.y * 16;
+  for (int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; u < d.nT; u += (int64_t)gridDim.x * blockDim.x) {
+    double2 acc = cmul(cf[0], d.hist[(int64_t)sl[0] * d.nT + u]);
+    for (int p = 1; p < nvec; ++p) acc = cadd(acc, cmul(cf[p], d.hist[(int64_t)sl[p] * d.nT + u]));
+    d.T[u] = acc;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Grouped complex GEMM on MFMA: C = alpha A B + beta C0, every group a different
+// channel (and solve).  256 threads = 4 wavefronts in a 2x2 grid; each wavefront
+// owns a (BM/2)x(BN/2) sub-tile of 16x16 MFMA blocks.  A and B are staged in LDS
+// as split real/imaginary planes; a complex multiply-accumulate is four real
+// MFMAs: re += ar br, re += (-ai) bi, im += ar bi, im += ai br.
+// ---------------------------------------------------------------------------
+template <typename AT>
+__device__ __forceinline__ double2 load_a(const AT *p);
+template <>
+__device__ __forceinline__ double2 load_a<double2>(const double2 *p) { return *p; }
+template <>
+__device__ __forceinline__ double2 load_a<float2>(const float2 *p) { float2 v = *p; return make_double2(v.x, v.y); }
+
+template <int BM, int BN, int BK, typename AT>
+__global__ __launch_bounds__(256) void k_zgemm_f64(const GemmGroup *__restrict__ groups,
+                                                   const GemmTile *__restrict__ tiles,
+                                                   const SolveDesc *__restrict__ descs,
+                                                   const int *__restrict__ gsolve) {
+  static_assert(BM == 64 && BN == 64 && BK == 16, "tile shape");
+  const GemmTile tl = tiles[blockIdx.x];
+  if (descs && !descs[gsolve[tl.g]].active) return;
+  const GemmGroup g = groups[tl.g];
+  const AT *A = (const AT *)g.A;
+  __shared__ double As[2][BK][BM + 2];
+  __shared__ double Bs[2][BK][BN + 2];
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int wm = w >> 1, wn = w & 1;
+  d4 cre[2][2], cim[2][2];
+  for (int x = 0; x < 2; ++x)
+    for (int y = 0; y < 2; ++y) { cre[x][y] = (d4){0, 0, 0, 0}; cim[x][y] = (d4){0, 0, 0, 0}; }
+  const int arow = tid >> 2, akq = (tid & 3) * 4;
+  const int bk = tid >> 4, bnq = (tid & 15) * 4;
+  for (int k0 = 0; k0 < g.K; k0 += BK) {
+    {
+      const int m = tl.m0 + arow;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int k = k0 + akq + q;
+        double2 v = make_double2(0.0, 0.0);
+        if (m < g.M && k < g.K) v = load_a<AT>(A + (int64_t)m * g.lda + k);
+        As[0][akq + q][arow] = v.x;
+        As[1][akq + q][arow] = v.y;
+      }
+      const int k = k0 + bk;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int n = tl.n0 + bnq + q;
+        double2 v = make_double2(0.0, 0.0);
+        if (k < g.K && n < g.N) v = g.B[(int64_t)k * g.ldb + n];
+        Bs[0][bk][bnq + q] = v.x;
+        Bs[1][bk][bnq + q] = v.y;
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int kk = 0; kk < BK; kk += 4) {
+      const int kr = kk + (lane >> 4), mi = lane & 15;
+      double ar[2], ai[2], br[2], bi[2];
+#pragma unroll
+      for (int x = 0; x < 2; ++x) {
+        ar[x] = As[0][kr][wm * 32 + x * 16 + mi];
+        ai[x] = As[1][kr][wm * 32 + x * 16 + mi];
+      }
+#pragma unroll
+      for (int y = 0; y < 2; ++y) {
+        br[y] = Bs[0][kr][wn * 32 + y * 16 + mi];
+        bi[y] = Bs[1][kr][wn * 32 + y * 16 + mi];
+      }
+#pragma unroll
+      for (int x = 0; x < 2; ++x)
+#pragma unroll
+        for (int y = 0; y < 2; ++y) {
+          cre[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[x], br[y], cre[x][y], 0, 0, 0);
+          cre[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(-ai[x], bi[y], cre[x][y], 0, 0, 0);
+          cim[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[x], bi[y], cim[x][y], 0, 0, 0);
+          cim[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai[x], br[y], cim[x][y], 0, 0, 0);
+        }
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int x = 0; x < 2; ++x)
+#pragma unroll
+    for (int y = 0; y < 2; ++y)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int m = tl.m0 + wm * 32 + x * 16 + (lane >> 4) * 4 + r;
+        const int n = tl.n0 + wn * 32 + y * 16 + (lane & 15);
+        if (m < g.M && n < g.N) {
+          double2 v = make_double2(g.alpha * cre[x][y][r], g.alpha * cim[x][y][r]);
+          if (g.beta != 0.0) {
+            double2 c0 = g.C0[(int64_t)m * g.ldc0 + n];
+            v.x += g.beta * c0.x;
+            v.y += g.beta * c0.y;
+          }
+          g.C[(int64_t)m * g.ldc + n] = v;
+        }
+      }
+}
+
+// Same contract with FP32 MFMA and FP32 accumulation: A is read as complex float
+// (stored) or double (rounded on load), B is rounded to float on load, and the
+// result is added to C0 in FP64.
+template <int BM, int BN, int BK, typename AT>
+__global__ __launch_bounds__(256) void k_zgemm_f32(const GemmGroup *__restrict__ groups,
+                                                   const GemmTile *__restrict__ tiles,
+                                                   const SolveDesc *__restrict__ descs,
+                                                   const int *__restrict__ gsolve) {
+  static_assert(BM == 64 && BN == 64 && BK == 16, "tile shape");
+  const GemmTile tl = tiles[blockIdx.x];
+  if (descs && !descs[gsolve[tl.g]].active) return;
+  const GemmGroup g = groups[tl.g];
+  const AT *A = (const AT *)g.A;
+  __shared__ float As[2][BK][BM + 4];
+  __shared__ float Bs[2][BK][BN + 4];
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int wm = w >> 1, wn = w & 1;
+  f4 cre[2][2], cim[2][2];
+  for (int x = 0; x < 2; ++x)
+    for (int y = 0; y < 2; ++y) { cre[x][y] = (f4){0, 0, 0, 0}; cim[x][y] = (f4){0, 0, 0, 0}; }
+  const int arow = tid >> 2, akq = (tid & 3) * 4;
+  const int bk = tid >> 4, bnq = (tid & 15) * 4;
+  for (int k0 = 0; k0 < g.K; k0 += BK) {
+    {
+      const int m = tl.m0 + arow;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int k = k0 + akq + q;
+        double2 v = make_double2(0.0, 0.0);
+        if (m < g.M && k < g.K) v = load_a<AT>(A + (int64_t)m * g.lda + k);
+        As[0][akq + q][arow] = (float)v.x;
+        As[1][akq + q][arow] = (float)v.y;
+      }
+      const int k = k0 + bk;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int n = tl.n0 + bnq + q;
+        double2 v = make_double2(0.0, 0.0);
+        if (k < g.K && n < g.N) v = g.B[(int64_t)k * g.ldb + n];
+        Bs[0][bk][bnq + q] = (float)v.x;
+        Bs[1][bk][bnq + q] = (float)v.y;
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int kk = 0; kk < BK; kk += 4) {
+      const int kr = kk + (lane >> 4), mi = lane & 15;
+      float ar[2], ai[2], br[2], bi[2];
+#pragma unroll
+      for (int x = 0; x < 2; ++x) {
+        ar[x] = As[0][kr][wm * 32 + x * 16 + mi];
+        ai[x] = As[1][kr][wm * 32 + x * 16 + mi];
+      }
+#pragma unroll
+      for (int y = 0; y < 2; ++y) {
+        br[y] = Bs[0][kr][wn * 32 + y * 16 + mi];
+        bi[y] = Bs[1][kr][wn * 32 + y * 16 + mi];
+      }
+#pragma unroll
+      for (int x = 0; x < 2; ++x)
+#pragma unroll
+        for (int y = 0; y < 2; ++y) {
+          cre[x][y] = __builtin_amdgcn_mfma_f32_16x16x4f32(ar[x], br[y], cre[x][y], 0, 0, 0);
+          cre[x][y] = __builtin_amdgcn_mfma_f32_16x16x4f32(-ai[x], bi[y], cre[x][y], 0, 0, 0);
+          cim[x][y] = __builtin_amdgcn_mfma_f32_16x16x4f32(ar[x], bi[y], cim[x][y], 0, 0, 0);
+          cim[x][y] = __builtin_amdgcn_mfma_f32_16x16x4f32(ai[x], br[y], cim[x][y], 0, 0, 0);
+        }
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int x = 0; x < 2; ++x)
+#pragma unroll
+    for (int y = 0; y < 2; ++y)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int m = tl.m0 + wm * 32 + x * 16 + (lane >> 4) * 4 + r;
+        const int n = tl.n0 + wn * 32 + y * 16 + (lane & 15);
+        if (m < g.M && n < g.N) {
+          double2 v = make_double2(g.alpha * (double)cre[x][y][r], g.alpha * (double)cim[x][y][r]);
+          if (g.beta != 0.0) {
+            double2 c0 = g.C0[(int64_t)m * g.ldc0 + n];
+            v.x += g.beta * c0.x;
+            v.y += g.beta * c0.y;
+          }
+          g.C[(int64_t)m * g.ldc + n] = v;
+        }
+      }
+}
+
+// Plain FP64 VALU GEMM with the same contract, kept as an independent check of
+// the MFMA lane layouts (selected with CCGPU_GEMM=valu).
+__global__ __launch_bounds__(256) void k_zgemm_valu(const GemmGroup *__restrict__ groups,
+                                                    const GemmTile *__restrict__ tiles,
+                                                    const SolveDesc *__restrict__ descs,
+                                                    const int *__restrict__ gsolve) {
+  const GemmTile tl = tiles[blockIdx.x];
+  if (descs && !descs[gsolve[tl.g]].active) return;
+  const GemmGroup g = groups[tl.g];
+  const double2 *A = (const double2 *)g.A;
+  __shared__ double2 As[16][64 + 1];
+  __shared__ double2 Bs[16][64 + 1];
+  const int tid = threadIdx.x;
+  const int tm = (tid >> 4) * 4, tn = (tid & 15) * 4;
+  double2 acc[4][4];
+  for (int x = 0; x < 4; ++x)
+    for (int y = 0; y < 4; ++y) acc[x][y] = make_double2(0.0, 0.0);
+  for (int k0 = 0; k0 < g.K; k0 += 16) {
+    for (int e = tid; e < 64 * 16; e += 256) {
+      int m = e >> 4, k = e & 15;
+      int gm = tl.m0 + m, gk = k0 + k;
+      As[k][m] = (gm < g.M && gk < g.K) ? A[(int64_t)gm * g.lda + gk] : make_double2(0.0, 0.0);
+      int kb = e >> 6, n = e & 63;
+      int gk2 = k0 + kb, gn = tl.n0 + n;
+      Bs[kb][n] = (gk2 < g.K && gn < g.N) ? g.B[(int64_t)gk2 * g.ldb + gn] : make_double2(0.0, 0.0);
+    }
+    __syncthreads();
+    for (int k = 0; k < 16; ++k)
+      for (int x = 0; x < 4; ++x)
+        for (int y = 0; y < 4; ++y) acc[x][y] = cadd(acc[x][y], cmul(As[k][tm + x], Bs[k][tn + y]));
+    __syncthreads();
+  }
+  for (int x = 0; x < 4; ++x)
+    for (int y = 0; y < 4; ++y) {
+      int m = tl.m0 + tm + x, n = tl.n0 + tn + y;
+      if (m < g.M && n < g.N) {
+        double2 v = cscale(g.alpha, acc[x][y]);
+        if (g.beta != 0.0) v = cadd(v, cscale(g.beta, g.C0[(int64_t)m * g.ldc0 + n]));
+        g.C[(int64_t)m * g.ldc + n] = v;
+      }
+    }
+}
+
+// Down-conversion of the stored pppp blocks for the FP32 ladder.
+__global__ void k_to_f32(const double2 *src, float2 *dst, int64_t n) {
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x) {
+    double2 v = src[e];
+    dst[e] = make_float2((float)v.x, (float)v.y);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// launch wrappers (host side)
+// ---------------------------------------------------------------------------
+void launch_fill_blocks(const SolveDesc &d, double volume, int nch, int nphc, int max_npp,
+                        int max_nhh, int max_nhp, hipStream_t st) {
+  auto gx = [](int64_t n) { return (unsigned)std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, 8192)); };
+  hipLaunchKernelGGL(k_fill_vpppp, dim3(gx((int64_t)max_npp * max_npp), nch), dim3(256), 0, st, d, volume);
+  hipLaunchKernelGGL(k_fill_vhhpp, dim3(gx((int64_t)max_npp * max_nhh), nch), dim3(256), 0, st, d, volume);
+  hipLaunchKernelGGL(k_fill_vhhhh, dim3(gx((int64_t)max_nhh * max_nhh), nch), dim3(256), 0, st, d, volume);
+  hipLaunchKernelGGL(k_fill_w0, dim3(gx((int64_t)max_nhp * max_nhp), nphc), dim3(256), 0, st, d, volume);
+  hipLaunchKernelGGL(k_fock, dim3((d.nsp + 255) / 256), dim3(256), 0, st, d, volume);
+  hipLaunchKernelGGL(k_e0, dim3(1), dim3(256), 0, st, d, volume);
+}
+
+void launch_derive(const SolveDesc &d, int nch, int nphc, int max_npp, int max_nhh, int max_nph,
+                   int max_nhp, hipStream_t st) {
+  auto gx = [](int64_t n) { return (unsigned)std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, 8192)); };
+  hipLaunchKernelGGL(k_derive_r0, dim3(gx((int64_t)max_npp * max_nhh), nch), dim3(256), 0, st, d);
+  hipLaunchKernelGGL(k_derive_vt, dim3(gx((int64_t)max_nhp * max_nph), nphc), dim3(256), 0, st, d);
+}
+
+void launch_to_f32(const double2 *src, float2 *dst, int64_t n, hipStream_t st) {
+  unsigned g = (unsigned)std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, 16384));
+  hipLaunchKernelGGL(k_to_f32, dim3(g), dim3(256), 0, st, src, dst, n);
+}
+
+// elementwise launch geometry: grid.x blocks of 256 per solve, grid.y = solves
+static inline dim3 egrid(int64_t n, int nsolve) {
+  return dim3((unsigned)std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, 2048)), nsolve);
+}
+
+void launch_mbpt2(const SolveDesc *dd, int nsolve, int64_t maxT, hipStream_t st) {
+  hipLaunchKernelGGL(k_mbpt2, egrid(maxT, nsolve), dim3(256), 0, st, dd);
+}
+
+// energy into scal[slot]; uses red[0..ENERGY_BLOCKS)
+constexpr int kRedBlocks = 512;
+void launch_energy(const SolveDesc *dd, int nsolve, int slot, hipStream_t st) {
+  hipLaunchKernelGGL(k_energy_partial, dim3(kRedBlocks, nsolve), dim3(256), 0, st, dd);
+  hipLaunchKernelGGL(k_reduce_final, dim3(1, nsolve), dim3(256), 0, st, dd, kRedBlocks, 1, 1, slot);
+}
+
+void launch_i7(const SolveDesc *dd, int nsolve, int maxA, hipStream_t st) {
+  hipLaunchKernelGGL(k_i7, dim3((2 * maxA + 127) / 128, nsolve), dim3(128), 0, st, dd);
+}
+void launch_i6(const SolveDesc *dd, int nsolve, int maxnp, hipStream_t st) {
+  hipLaunchKernelGGL(k_i6, dim3((2 * maxnp + 3) / 4, nsolve), dim3(256), 0, st, dd);
+}
+void launch_i9(const SolveDesc *dd, int nsolve, int64_t maxvhh, hipStream_t st) {
+  hipLaunchKernelGGL(k_i9, egrid(maxvhh, nsolve), dim3(256), 0, st, dd);
+}
+void launch_resid_init(const SolveDesc *dd, int nsolve, int64_t maxT, hipStream_t st) {
+  hipLaunchKernelGGL(k_resid_init, egrid(maxT, nsolve), dim3(256), 0, st, dd);
+}
+void launch_ring_gather(const SolveDesc *dd, int nsolve, int64_t maxT, hipStream_t st) {
+  hipLaunchKernelGGL(k_ring_gather, egrid(maxT, nsolve), dim3(256), 0, st, dd);
+}
+void launch_ring_scatter(const SolveDesc *dd, int nsolve, int64_t maxT, hipStream_t st) {
+  hipLaunchKernelGGL(k_ring_scatter, egrid(maxT, nsolve), dim3(256), 0, st, dd);
+}
+void launch_update(const SolveDesc *dd, int nsolve, int64_t maxT, const int *slot, hipStream_t st) {
+  hipLaunchKernelGGL(k_update, egrid(maxT, nsolve), dim3(256), 0, st, dd, slot);
+}
+void launch_gram(const SolveDesc *dd, int nsolve, const int *slots, int nstep, hipStream_t st) {
+  hipLaunchKernelGGL(k_gram_partial, dim3(kRedBlocks, nsolve), dim3(256), 0, st, dd, slots, nstep);
+  int ne = nstep - 1;
+  int nv = ne * (ne + 1) / 2;
+  hipLaunchKernelGGL(k_reduce_final, dim3(1, nsolve), dim3(256), 0, st, dd, kRedBlocks, 36, nv, 4);
+}
+void launch_diis_axpy(const SolveDesc *dd, int nsolve, int64_t maxT, const int *slots, const double2 *coef,
+                      int nvec, hipStream_t st) {
+  hipLaunchKernelGGL(k_diis_axpy, egrid(maxT, nsolve), dim3(256), 0, st, dd, slots, coef, nvec);
+}
+
+void launch_gemm(const GemmGroup *groups, const GemmTile *tiles, int ntiles, const SolveDesc *dd,
+                 const int *gsolve, int variant, hipStream_t st) {
+  if (ntiles <= 0) return;
+  switch (variant) {
+    case 0:
+      hipLaunchKernelGGL((k_zgemm_f64<64, 64, 16, double2>), dim3(ntiles), dim3(256), 0, st, groups, tiles, dd, gsolve);
+      break;
+    case 1:
+      hipLaunchKernelGGL((k_zgemm_f32<64, 64, 16, double2>), dim3(ntiles), dim3(256), 0, st, groups, tiles, dd, gsolve);
+      break;
+    case 2:
+      hipLaunchKernelGGL((k_zgemm_f32<64, 64, 16, float2>), dim3(ntiles), dim3(256), 0, st, groups, tiles, dd, gsolve);
+      break;
+    case 3:
+      hipLaunchKernelGGL(k_zgemm_valu, dim3(ntiles), dim3(256), 0, st, groups, tiles, dd, gsolve);
+      break;
+    default:
+      break;
+  }
+}
+
+}  // namespace ccgpu

@@ -1,0 +1,173 @@
+/*
+ * ccgpu — MI355X (gfx950) CCD/(T) amplitude engine for nuclear matter on the
+ * periodic momentum lattice.  C ABI: plain ints, doubles and pointers; complex
+ * numbers are interleaved (re, im) doubles, layout-identical to Fortran
+ * complex(c_double_complex) and hipDoubleComplex.
+ *
+ * Every function returns 0 on success and a nonzero status otherwise; the
+ * message of the last failure is available from ccgpu_last_error().  Handles
+ * are small integers; one host thread drives a handle at a time.
+ *
+ * Reference interfaces replaced (paths relative to deltafull-CCMkspace/ of the
+ * reference):
+ *   ccgpu_plan / ccgpu_create         allocate_sp_data + setup_sp_data
+ *                                     (ccm_main.f90:878-1197) and
+ *                                     setup_channel_structures /
+ *                                     setup_ph_channel_structures
+ *                                     (ccm_iter.f90:175-811, 1816-2212)
+ *   ccgpu_create_from_orbitals        the same channel setup, from the host's
+ *                                     own all_orbit table (ccm_modules.f90:546-614)
+ *   ccgpu_generate_interaction        the V-block loops of setup_channel_structures
+ *                                     (ccm_iter.f90:819-1311) plus
+ *                                     normal_ordered_hamiltonian Fock/E0
+ *                                     (ccm_main.f90:1586-1684)
+ *   ccgpu_set_block / ccgpu_set_fock  hand-over of the host's vnn_* blocks and
+ *                                     fock_mtx/e0 (ccm_modules.f90:1167-1171, 528)
+ *   ccgpu_mbpt2_start                 setup_t_amplitudes (ccm_iter.f90:2329-2405)
+ *   ccgpu_residual                    t2_intermediate(0) (ccm_t2_eqn.f90:4-1304)
+ *   ccgpu_update                      diis_t1t2 (ccm_diis.f90:74-249)
+ *   ccgpu_energy                      ccd_energy_save (ccm_energy.f90:4-71)
+ *   ccgpu_solve                       ccd_iter loop (ccm_iter.f90:1-170)
+ *   ccgpu_triples                     pert_triples2 (ccm_triples.f90:528-735)
+ *   ccgpu_get_t2 / ccgpu_set_t2       reads/writes of t2_ccm(ch)%val
+ *   ccgpu_batch_solve                 no reference counterpart: the reference
+ *                                     runs one twist per process
+ *                                     (ccm_main.f90:163-279)
+ */
+#ifndef CCGPU_H
+#define CCGPU_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* matter */
+#define CCGPU_PNM 0
+#define CCGPU_SNM 1
+
+/* interaction generators for ccgpu_generate_interaction */
+#define CCGPU_INT_MINNESOTA 0
+
+/* block kinds for ccgpu_set_block (reference block names) */
+#define CCGPU_BLOCK_HHHH 1
+#define CCGPU_BLOCK_HHHP 2
+#define CCGPU_BLOCK_HHPP 3
+#define CCGPU_BLOCK_HPHP 4
+#define CCGPU_BLOCK_HPPP 5
+#define CCGPU_BLOCK_PPPP 6
+
+/* precision modes of the pp/hh ladders and the ring contractions */
+#define CCGPU_PREC_FP64 0
+#define CCGPU_PREC_FP32 1
+
+typedef struct {
+  int A;            /* number of nucleons (holes) */
+  int matter;       /* CCGPU_PNM or CCGPU_SNM */
+  double rho;       /* density in fm^-3 */
+  int nmax;         /* |n_i| <= nmax per axis */
+  double twist[3];  /* twist angles theta in [0, pi] per axis */
+} ccgpu_lattice_params;
+
+typedef struct {
+  int nsp, nholes, nparticles;
+  int nchannels;          /* (K, Tz) channels with hh and pp pairs */
+  int nph_channels;       /* Pandya (particle-hole) channels */
+  int max_npp, max_nhh;   /* largest unique pair counts per channel */
+  long long t2_unique;    /* stored amplitudes (a<b, i<j) */
+  long long t2_ordered;   /* reference-shaped count (all ordered pairs) */
+  long long vpppp_unique; /* stored pppp elements (unique x unique) */
+  long long ring_elems;   /* sum over ph channels of Nph * Nhp */
+  double box, volume, kf;
+} ccgpu_dims;
+
+typedef struct {
+  int subspace;      /* DIIS history length (reference: 6) */
+  int diis_step;     /* DIIS extrapolation cadence (reference: 6) */
+  double tol;        /* stop when |dE| <= tol (MeV); reference 1e-6 */
+  int maxit;         /* iteration cap (reference: 100) */
+  int precision;     /* CCGPU_PREC_* */
+  int refine_every;  /* mixed precision: FP64 residual every k iterations */
+  int use_graph;     /* capture one iteration in a hipGraph and replay it */
+  int verbose;       /* print per-iteration lines like the reference */
+} ccgpu_solve_opts;
+
+typedef struct {
+  int iterations;
+  int converged;
+  double e0[2];          /* vacuum (HF) energy */
+  double embpt2[2];      /* E0 + MBPT2 correlation energy */
+  double eccd[2];        /* E0 + CCD correlation energy */
+  double ecorr_last[2];  /* last correlation energy */
+  double seconds;        /* wall time of the iteration loop */
+} ccgpu_solve_result;
+
+/* Host-only planning of a lattice: no GPU work.  Fills dims. */
+int ccgpu_plan(const ccgpu_lattice_params *p, ccgpu_dims *dims);
+
+/* Build basis + channels for a lattice and allocate device storage. */
+int ccgpu_create(const ccgpu_lattice_params *p, int *handle);
+
+/* Same, from an explicit orbital table (the first nholes orbitals are holes).
+ * k in fm^-1, e in MeV; sz and tz are +-1. */
+int ccgpu_create_from_orbitals(int nsp, int nholes, const int *nx, const int *ny,
+                               const int *nz, const int *sz, const int *tz,
+                               const double *kx, const double *ky, const double *kz,
+                               const double *e, double box, double volume, int *handle);
+
+int ccgpu_destroy(int handle);
+int ccgpu_get_dims(int handle, ccgpu_dims *dims);
+
+/* Orbital table of a handle (arrays of length nsp). */
+int ccgpu_get_orbitals(int handle, int *nx, int *ny, int *nz, int *sz, int *tz, double *e);
+
+/* Fill every V block, the Fock matrix and E0 on the GPU. */
+int ccgpu_generate_interaction(int handle, int kind);
+
+/* Hand over one reference-layout block: rows are pairs (bra_p[r], bra_q[r]),
+ * columns pairs (ket_r[c], ket_s[c]), 1-based orbital indices as in the
+ * reference lookup tables; val is column-major complex with leading dim ld.
+ * Only canonical orderings are read; the rest follows from antisymmetry. */
+int ccgpu_set_block(int handle, int kind, int nbra, const int *bra_p, const int *bra_q,
+                    int nket, const int *ket_r, const int *ket_s, const double *val, int ld);
+
+/* Fock matrix (nsp x nsp complex, column-major) and E0. */
+int ccgpu_set_fock(int handle, const double *fock, double e0_re, double e0_im);
+
+/* Derive the device-side intermediates (ring gathers, conj V) after set_block. */
+int ccgpu_finalize_setup(int handle);
+
+int ccgpu_get_e0(int handle, double *e0);     /* e0[2] */
+int ccgpu_get_fock_diag(int handle, double *f); /* nsp complex */
+
+int ccgpu_mbpt2_start(int handle, double *ecorr);                /* ecorr[2] */
+int ccgpu_residual(int handle, int precision);
+int ccgpu_update(int handle, int *count, int *nstep, int subspace, int diis_step);
+int ccgpu_energy(int handle, double *ecorr);                     /* ecorr[2] */
+int ccgpu_solve(int handle, const ccgpu_solve_opts *opts, ccgpu_solve_result *res);
+int ccgpu_triples(int handle, double *et);                       /* et[2] */
+
+/* Amplitudes t(ab,ij) for n orbital quadruples (0-based), any ordering. */
+int ccgpu_get_t2(int handle, int n, const int *a, const int *b, const int *i,
+                 const int *j, double *t);
+int ccgpu_set_t2(int handle, int n, const int *a, const int *b, const int *i,
+                 const int *j, const double *t);
+/* Whole unique amplitude vector (t2_unique complex) in storage order. */
+int ccgpu_get_t2_unique(int handle, double *t);
+int ccgpu_set_t2_unique(int handle, const double *t);
+/* Orbital quadruples (a,b,i,j) of the storage order, 4 ints per element. */
+int ccgpu_get_t2_labels(int handle, int *abij);
+
+/* Solve several handles together: every kernel launch covers all of them. */
+int ccgpu_batch_solve(int n, const int *handles, const ccgpu_solve_opts *opts,
+                      ccgpu_solve_result *results);
+
+/* Per-kernel timings (ms, averaged over the recorded launches) of the last
+ * solve; names are written as a ';'-separated list. */
+int ccgpu_kernel_stats(char *names, int len, double *avg_ms, long long *launches, int max);
+
+int ccgpu_last_error(char *buf, int len);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,125 @@
+"""Write the golden fixtures of tests/golden/ from the compiled reference.
+
+Runs only where the reference sources were compiled (oracle/Makefile builds
+oracle/_ref/ref_driver from the reference in place).  Each case runs the
+reference's own setup, CCD loop and (T) through oracle/ref_driver.f90 and stores
+the results as a compressed .npz: per-iteration correlation energies, E0, the
+final CCD / CCD(T) energies, and for the small cases the orbitals, Fock matrix,
+canonical V elements and converged T2 keyed by orbital index of the stored
+orbital table.
+
+Usage: python tests/golden/make_golden.py [case ...]
+"""
+import os
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+DRIVER = os.path.join(ROOT, "oracle", "_ref", "ref_driver")
+
+# name: (A, matter, rho, nmax, order, interaction, subspace, step, tol, maxit,
+#        twist, dump, triples, ranks)
+CASES = {
+    "pnm14_n1_nnlo": (14, "pnm", 0.16, 1, 3, "chiral", 6, 6, 1e-6, 100, (0, 0, 0), 1, 1, 1),
+    "pnm14_n1_minn": (14, "pnm", 0.16, 1, 3, "minnesota", 8, 8, 1e-8, 100, (0, 0, 0), 1, 1, 1),
+    "pnm14_n1_minn_twist": (14, "pnm", 0.16, 1, 3, "minnesota", 6, 6, 1e-6, 100,
+                            (0.3, 1.1, 2.0), 1, 1, 1),
+    "snm4_n1_nnlo": (4, "snm", 0.16, 1, 3, "chiral", 6, 6, 1e-6, 100, (0, 0, 0), 1, 1, 1),
+    "snm4_n1_minn": (4, "snm", 0.16, 1, 3, "minnesota", 6, 6, 1e-6, 100, (0, 0, 0), 1, 1, 1),
+    "pnm66_n2_minn": (66, "pnm", 0.16, 2, 3, "minnesota", 8, 8, 1e-8, 100, (0, 0, 0), 0, 0, 1),
+    "pnm66_n2_minn_ref": (66, "pnm", 0.16, 2, 3, "minnesota", 6, 6, 1e-6, 100, (0, 0, 0), 0, 0, 1),
+    "pnm66_n2_minn_twist": (66, "pnm", 0.16, 2, 3, "minnesota", 8, 8, 1e-8, 100,
+                            (0.7, 1.9, 2.6), 0, 0, 1),
+    "pnm66_n2_nnlo": (66, "pnm", 0.16, 2, 3, "chiral", 8, 8, 1e-8, 100, (0, 0, 0), 0, 0, 8),
+    "snm28_n2_minn": (28, "snm", 0.16, 2, 3, "minnesota", 8, 8, 1e-8, 100, (0, 0, 0), 0, 0, 1),
+}
+
+CCM_IN = """!Chiral order for Deltas(LO = 0,NLO=2,NNLO=3,N3LO=4) and cutoff
+{order}, 450
+! cE and cD 3nf parameters:
+ 0.017, 0.79
+! number of particles
+{A}
+! specify: pnm/snm, input type: density/kfermi
+{matter}, density
+! specify boundary conditions (PBC/TABC/TABCsp)
+PBC
+! dens/kf, ntwist,  nmax, my_twists
+{rho},1,{nmax},1
+! specify cluster approximation: CCD, CCD(T)
+CCD
+! tnf switch (T/F) and specify 3nf approximation: 0=tnf0b, 1=tnf1b, 2=tnf2b
+F, 0
+! 3nf cutoff(MeV),non-local reg. exp
+450, 3
+"""
+
+
+def run_case(name):
+    (A, matter, rho, nmax, order, inter, sub, step, tol, maxit, tw, dump, trip, ranks) = CASES[name]
+    with tempfile.TemporaryDirectory() as tmp:
+        inp = os.path.join(tmp, "ccm_in")
+        with open(inp, "w") as fh:
+            fh.write(CCM_IN.format(order=order, A=A, matter=matter, rho=rho, nmax=nmax))
+        cmd = [DRIVER, inp, tmp, inter, str(sub), str(step), repr(tol), str(maxit),
+               repr(tw[0]), repr(tw[1]), repr(tw[2]), str(dump), str(trip)]
+        if ranks > 1:
+            cmd = ["/opt/conda/bin/mpiexec", "-n", str(ranks)] + cmd
+        env = dict(os.environ, OMP_NUM_THREADS="1", MKL_NUM_THREADS="1")
+        res = subprocess.run(cmd, env=env, capture_output=True, text=True)
+        if res.returncode != 0:
+            raise RuntimeError(f"{name}: reference driver failed\n{res.stdout[-2000:]}\n{res.stderr[-2000:]}")
+        out = {"params": np.array([A, nmax, sub, step, maxit, order], np.int64),
+               "matter": np.array(matter), "interaction": np.array(inter),
+               "rho": np.float64(rho), "tol": np.float64(np.float32(tol)),
+               "twist": np.array(tw, np.float64)}
+        its = []
+        for line in open(os.path.join(tmp, "energies.txt")):
+            f = line.split()
+            if f[0] == "it":
+                its.append(complex(float(f[2]), float(f[3])))
+            elif f[0] in ("e0", "eccd", "eccdt"):
+                out[f[0]] = np.complex128(complex(float(f[1]), float(f[2])))
+            elif f[0] == "kf":
+                out["kf"] = np.float64(float(f[1]))
+            elif f[0] == "box":
+                out["box"] = np.float64(float(f[1]))
+                out["volume"] = np.float64(float(f[2]))
+            elif f[0] == "looptime":
+                out["looptime"] = np.float64(float(f[1]))
+        out["energies"] = np.array(its, np.complex128)
+        if dump:
+            with open(os.path.join(tmp, "orbitals.bin"), "rb") as fh:
+                nsp, na = np.frombuffer(fh.read(8), np.int32)
+                ints = np.frombuffer(fh.read(4 * 5 * nsp), np.int32).reshape(5, nsp)
+                dbl = np.frombuffer(fh.read(8 * 4 * nsp), np.float64).reshape(4, nsp)
+            out["orb_int"] = ints.copy()
+            out["orb_dbl"] = dbl.copy()
+            raw = np.fromfile(os.path.join(tmp, "fock.bin"), np.complex128)
+            out["fock"] = raw[: nsp * nsp].reshape(nsp, nsp).T.copy()
+            rec = np.dtype([("kind", "<i4"), ("p", "<i4"), ("q", "<i4"), ("r", "<i4"),
+                            ("s", "<i4"), ("v", "<c16")])
+            vb = np.fromfile(os.path.join(tmp, "vblocks.bin"), rec)
+            out["v_idx"] = np.stack([vb["kind"], vb["p"], vb["q"], vb["r"], vb["s"]], 1).astype(np.int16)
+            out["v_val"] = vb["v"].copy()
+            trec = np.dtype([("a", "<i4"), ("b", "<i4"), ("i", "<i4"), ("j", "<i4"), ("v", "<c16")])
+            t2 = np.fromfile(os.path.join(tmp, "t2.bin"), trec)
+            out["t2_idx"] = np.stack([t2["a"], t2["b"], t2["i"], t2["j"]], 1).astype(np.int16)
+            out["t2_val"] = t2["v"].copy()
+            if trip:
+                hrec = np.dtype([("k", "<i4"), ("d", "<i4"), ("c", "<i4"), ("b", "<i4"), ("v", "<c16")])
+                hp = np.fromfile(os.path.join(tmp, "vhppp.bin"), hrec)
+                out["hppp_idx"] = np.stack([hp["k"], hp["d"], hp["c"], hp["b"]], 1).astype(np.int16)
+                out["hppp_val"] = hp["v"].copy()
+    np.savez_compressed(os.path.join(HERE, name + ".npz"), **out)
+    print(name, "iterations", len(out["energies"]) - 1, "eccd", out.get("eccd"), "eccdt", out.get("eccdt"))
+
+
+if __name__ == "__main__":
+    names = sys.argv[1:] or list(CASES)
+    for n in names:
+        run_case(n)

@@ -1,0 +1,193 @@
+"""GPU parity: the HIP engine (through its C ABI) against the compiled reference's
+golden fixtures and against the oracle, on identical inputs."""
+import os
+
+import numpy as np
+import pytest
+
+import ccd_oracle as co
+from conftest import fixture_interaction, fixture_lattice, golden
+
+pytestmark = pytest.mark.gpu
+
+MINN_CASES = ["pnm14_n1_minn", "snm4_n1_minn", "pnm14_n1_minn_twist", "pnm66_n2_minn",
+              "pnm66_n2_minn_ref", "pnm66_n2_minn_twist", "snm28_n2_minn"]
+
+
+def params(g):
+    A, nmax, sub, step, maxit, order = (int(x) for x in g["params"])
+    return A, nmax, sub, step, maxit
+
+
+def step_loop(s, sub, step, tol, maxit, precision=0):
+    """ccd_iter (ccm_iter.f90:60-96) written against the step API."""
+    e1 = s.ccd_energy_save()
+    out = [e1]
+    count = nstep = 0
+    dener = 4.0
+    for it in range(1, maxit + 1):
+        if not abs(dener) > tol:
+            continue
+        s.t2_intermediate(precision)
+        count += 1
+        nstep += 1
+        count, nstep = s.diis_t1t2(count, nstep, sub, step)
+        e2 = s.ccd_energy_save()
+        dener = e2 - e1
+        e1 = e2
+        out.append(e2)
+    return np.array(out)
+
+
+def check_trajectory(en, ref, step, tol):
+    assert len(en) == len(ref), (len(en), len(ref))
+    scale = np.max(np.abs(ref))
+    k = min(step, len(ref))
+    assert np.max(np.abs(en[:k] - ref[:k])) < 1e-11 * scale
+    assert np.max(np.abs(en - ref)) < 1e-6 * scale
+    assert abs(en[-1] - ref[-1]) < 1e-9 + tol
+
+
+@pytest.mark.parametrize("name", MINN_CASES)
+def test_native_minnesota_against_reference(ccnm, name):
+    """Lattice, Minnesota V, Fock, E0, MBPT2 and the whole CCD trajectory built by
+    the engine itself versus the reference run on the same physical input."""
+    g = golden(name)
+    A, nmax, sub, step, maxit = params(g)
+    matter = ccnm.PNM if str(g["matter"]) == "pnm" else ccnm.SNM
+    s = ccnm.Solver(A=A, matter=matter, rho=float(g["rho"]), nmax=nmax, twist=tuple(g["twist"]))
+    assert abs(s.dims["box"] - float(g["box"])) < 1e-12
+    s.generate_interaction(ccnm.INT_MINNESOTA)
+    assert abs(s.e0() - g["e0"]) < 1e-10 * abs(g["e0"])
+    emp2 = s.setup_t_amplitudes()
+    assert abs(emp2 - g["energies"][0]) < 1e-11 * abs(g["energies"][0])
+    en = step_loop(s, sub, step, float(g["tol"]), maxit)
+    check_trajectory(en, g["energies"], step, float(g["tol"]))
+
+
+@pytest.mark.parametrize("name", ["pnm14_n1_nnlo", "snm4_n1_nnlo", "pnm14_n1_minn"])
+def test_dropin_reference_blocks(ccnm, name):
+    """Drop-in path: the reference's own orbitals, V blocks (chiral NNLO here) and
+    Fock matrix handed over through ccgpu_set_block / ccgpu_set_fock."""
+    g = golden(name)
+    A, nmax, sub, step, maxit = params(g)
+    oi, od = g["orb_int"], g["orb_dbl"]
+    orbs = {"nx": oi[0], "ny": oi[1], "nz": oi[2], "sz": oi[3], "tz": oi[4],
+            "kx": od[0], "ky": od[1], "kz": od[2], "e": od[3], "A": A,
+            "box": float(g["box"]), "volume": float(g["volume"])}
+    s = ccnm.Solver(orbitals=orbs)
+    idx = g["v_idx"].astype(np.int64)
+    kinds = {1: ccnm.BLOCK_HHHH, 3: ccnm.BLOCK_HHPP, 4: ccnm.BLOCK_HPHP, 6: ccnm.BLOCK_PPPP}
+    for k, kind in kinds.items():
+        sel = idx[:, 0] == k
+        bra = np.unique(idx[sel][:, 1:3], axis=0)
+        ket = np.unique(idx[sel][:, 3:5], axis=0)
+        bpos = {tuple(x): n for n, x in enumerate(bra)}
+        kpos = {tuple(x): n for n, x in enumerate(ket)}
+        val = np.zeros((len(bra), len(ket)), np.complex128)
+        for row, v in zip(idx[sel], g["v_val"][sel]):
+            val[bpos[(row[1], row[2])], kpos[(row[3], row[4])]] = v
+        s.set_block(kind, bra[:, 0], bra[:, 1], ket[:, 0], ket[:, 1], val)
+    s.set_fock(g["fock"], g["e0"])
+    s.finalize_setup()
+    emp2 = s.setup_t_amplitudes()
+    assert abs(emp2 - g["energies"][0]) < 1e-12 * abs(g["energies"][0])
+    en = step_loop(s, sub, step, float(g["tol"]), maxit)
+    check_trajectory(en, g["energies"], step, float(g["tol"]))
+    # converged amplitudes by orbital labels
+    t = g["t2_idx"].astype(np.int64) - 1
+    mine = s.get_t2(t[:, 0], t[:, 1], t[:, 2], t[:, 3])
+    assert np.max(np.abs(mine - g["t2_val"])) < 1e-6 * np.max(np.abs(g["t2_val"])) + float(g["tol"])
+
+
+def test_solve_equals_step_api_and_oracle(ccnm):
+    """ccgpu_solve (library-side loop) and the step API give the same energies,
+    and both agree with the oracle's ordered-pair restatement."""
+    s = ccnm.Solver(A=14, matter=ccnm.PNM, rho=0.16, nmax=1)
+    s.generate_interaction()
+    s.setup_t_amplitudes()
+    en = step_loop(s, 6, 6, 1e-8, 100)
+    s2 = ccnm.Solver(A=14, matter=ccnm.PNM, rho=0.16, nmax=1)
+    s2.generate_interaction()
+    s2.setup_t_amplitudes()
+    r = s2.ccd_iter(subspace=6, diis_step=6, tol=1e-8, maxit=100)
+    assert r["iterations"] == len(en) - 1
+    assert abs(r["ecorr"] - en[-1]) < 1e-12 * abs(en[-1])
+    lat = co.Lattice.build(14, "pnm", 0.16, 1)
+    cc = co.CCD(lat, co.MinnesotaInteraction(lat))
+    ref = np.array(cc.solve(6, 6, 1e-8, 100))
+    check_trajectory(en, ref, 6, 1e-8)
+    # amplitudes by labels against the oracle's converged T2
+    lab = s.t2_labels()
+    mine = s.t2_unique()
+    idx = cc.ch.flat_index(lab[:, 0], lab[:, 1], lab[:, 2], lab[:, 3])
+    assert np.all(idx >= 0)
+    theirs = cc.flat(cc.T)[idx]
+    assert np.max(np.abs(mine - theirs)) < 1e-8 * np.max(np.abs(theirs))
+
+
+def test_mfma_gemm_matches_valu_gemm(ccnm):
+    """The MFMA grouped GEMM against the plain VALU GEMM (independent lane
+    layout) on the PNM A=66, nmax=2 lattice."""
+    res = []
+    for mode in ("mfma", "valu"):
+        if mode == "valu":
+            os.environ["CCGPU_GEMM"] = "valu"
+        try:
+            s = ccnm.Solver(A=66, matter=ccnm.PNM, rho=0.16, nmax=2)
+            s.generate_interaction()
+            s.setup_t_amplitudes()
+            res.append(step_loop(s, 8, 8, 1e-8, 100))
+        finally:
+            os.environ.pop("CCGPU_GEMM", None)
+    a, b = res
+    assert len(a) == len(b)
+    assert np.max(np.abs(a - b)) < 1e-7 * np.max(np.abs(a))
+    assert abs(a[-1] - b[-1]) < 1e-10 * abs(a[-1])
+
+
+def test_graph_replay_matches_eager(ccnm):
+    out = []
+    for graph in (0, 1):
+        s = ccnm.Solver(A=66, matter=ccnm.PNM, rho=0.16, nmax=2)
+        s.generate_interaction()
+        s.setup_t_amplitudes()
+        out.append(s.ccd_iter(subspace=8, diis_step=8, tol=1e-8, maxit=100, use_graph=graph))
+    assert out[0]["iterations"] == out[1]["iterations"]
+    assert out[0]["eccd"] == out[1]["eccd"]
+
+
+def test_batch_matches_sequential(ccnm):
+    """Three twists in one batched launch sequence versus three separate solves."""
+    twists = [(0.0, 0.0, 0.0), (0.7, 1.9, 2.6), (2.1, 0.4, 1.3)]
+    seq = []
+    for tw in twists:
+        s = ccnm.Solver(A=66, matter=ccnm.PNM, rho=0.16, nmax=2, twist=tw)
+        s.generate_interaction()
+        s.setup_t_amplitudes()
+        seq.append(s.ccd_iter(subspace=8, diis_step=8, tol=1e-8, maxit=100))
+    ss = []
+    for tw in twists:
+        s = ccnm.Solver(A=66, matter=ccnm.PNM, rho=0.16, nmax=2, twist=tw)
+        s.generate_interaction()
+        s.setup_t_amplitudes()
+        ss.append(s)
+    bat = ccnm.batch_solve(ss, subspace=8, diis_step=8, tol=1e-8, maxit=100)
+    for a, b in zip(seq, bat):
+        assert a["iterations"] == b["iterations"]
+        assert abs(a["eccd"] - b["eccd"]) < 1e-11 * abs(a["eccd"])
+    g = golden("pnm66_n2_minn_twist")
+    assert abs(bat[1]["eccd"] - g["eccd"]) < 1e-8
+
+
+def test_fp32_mixed_precision_error(ccnm):
+    """FP32 ladders and ring with FP64 refinement: E/N within 1e-5 MeV of FP64."""
+    out = {}
+    for prec in (ccnm.PREC_FP64, ccnm.PREC_FP32):
+        s = ccnm.Solver(A=66, matter=ccnm.PNM, rho=0.16, nmax=2)
+        s.generate_interaction()
+        s.setup_t_amplitudes()
+        out[prec] = s.ccd_iter(subspace=8, diis_step=8, tol=1e-8, maxit=100, precision=prec, refine_every=4)
+    d = abs(out[0]["eccd"] - out[1]["eccd"]) / 66
+    assert out[1]["converged"]
+    assert d < 1e-5, d

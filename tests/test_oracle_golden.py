@@ -1,0 +1,91 @@
+"""Pins the oracle (oracle/ccd_oracle.py) to the compiled reference: every fixture
+in tests/golden/ was written by the reference's own routines
+(oracle/ref_driver.f90 + make_golden.py)."""
+import numpy as np
+import pytest
+
+import ccd_oracle as co
+from conftest import fixture_interaction, fixture_lattice, golden
+
+
+@pytest.mark.parametrize("name", ["pnm14_n1_nnlo", "snm4_n1_nnlo", "pnm14_n1_minn", "snm4_n1_minn",
+                                  "pnm14_n1_minn_twist"])
+def test_oracle_fock_e0_against_reference(name):
+    g = golden(name)
+    lat = fixture_lattice(g)
+    cc = co.CCD(lat, fixture_interaction(g, lat))
+    assert np.max(np.abs(cc.fock - g["fock"])) < 1e-11
+    assert abs(cc.e0 - g["e0"]) < 1e-10
+
+
+@pytest.mark.parametrize("name", ["pnm14_n1_nnlo", "snm4_n1_nnlo", "pnm14_n1_minn", "snm4_n1_minn",
+                                  "pnm14_n1_minn_twist"])
+def test_oracle_ccd_iterations_against_reference(name):
+    """Same V and Fock as the reference: every iteration's correlation energy.
+    Before the first DIIS step the agreement is at rounding level; the DIIS
+    solve is ill-conditioned (its coefficients reach 1e2 with alternating signs),
+    so rounding-level differences in the iterates grow to ~1e-7 relative after a
+    DIIS step and shrink again towards the fixed point."""
+    g = golden(name)
+    lat = fixture_lattice(g)
+    cc = co.CCD(lat, fixture_interaction(g, lat), fock=g["fock"], e0=g["e0"])
+    A, nmax, sub, step, maxit, order = (int(x) for x in g["params"])
+    en = np.array(cc.solve(sub, step, float(g["tol"]), maxit))
+    ref = g["energies"]
+    assert len(en) == len(ref)
+    k = min(step, len(ref))
+    assert np.max(np.abs(en[:k] - ref[:k])) < 1e-11 * np.max(np.abs(ref))
+    assert np.max(np.abs(en - ref)) < 1e-6 * np.max(np.abs(ref))
+    # the loop stops at |dE| <= tol, so the two fixed-point approaches may differ
+    # by up to tol
+    assert abs(cc.ecc - g["eccd"]) < 1e-9 + float(g["tol"])
+
+
+def test_oracle_minnesota_matches_reference_function():
+    """The oracle's own Minnesota restatement against the reference's
+    vmom_minnesota elements (stored V records of a Minnesota fixture)."""
+    for name in ("pnm14_n1_minn", "snm4_n1_minn", "pnm14_n1_minn_twist"):
+        g = golden(name)
+        lat = fixture_lattice(g)
+        mn = co.MinnesotaInteraction(lat)
+        idx = g["v_idx"].astype(np.int64)
+        v = mn(idx[:, 1] - 1, idx[:, 2] - 1, idx[:, 3] - 1, idx[:, 4] - 1)
+        assert np.max(np.abs(v - g["v_val"])) < 1e-14 * max(1.0, np.max(np.abs(g["v_val"])))
+
+
+def test_oracle_converged_t2_against_reference():
+    g = golden("pnm14_n1_minn")
+    lat = fixture_lattice(g)
+    cc = co.CCD(lat, fixture_interaction(g, lat), fock=g["fock"], e0=g["e0"])
+    cc.solve(8, 8, float(g["tol"]), 100)
+    t = g["t2_idx"].astype(np.int64) - 1
+    idx = cc.ch.flat_index(t[:, 0], t[:, 1], t[:, 2], t[:, 3])
+    mine = cc.flat(cc.T)[idx]
+    assert np.max(np.abs(mine - g["t2_val"])) < 1e-9 * np.max(np.abs(g["t2_val"]))
+
+
+def t_from_fixture(cc, g):
+    """Ordered amplitudes rebuilt from the reference's converged canonical T2."""
+    x = np.zeros(cc.ch.size, np.complex128)
+    t = g["t2_idx"].astype(np.int64) - 1
+    a, b, i, j = t[:, 0], t[:, 1], t[:, 2], t[:, 3]
+    for (p, q, r, s_, sg) in ((a, b, i, j, 1), (b, a, i, j, -1), (a, b, j, i, -1), (b, a, j, i, 1)):
+        idx = cc.ch.flat_index(p, q, r, s_)
+        x[idx[idx >= 0]] = sg * g["t2_val"][idx >= 0]
+    return cc.unflat(x)
+
+
+@pytest.mark.parametrize("name", ["snm4_n1_nnlo", "snm4_n1_minn", "pnm14_n1_minn_twist"])
+def test_oracle_triples_against_reference(name):
+    """(T) of the oracle on the reference's own converged T2 and V."""
+    g = golden(name)
+    lat = fixture_lattice(g)
+    vi = fixture_interaction(g, lat)
+    cc = co.CCD(lat, vi, fock=g["fock"], e0=g["e0"])
+    T = t_from_fixture(cc, g)
+    h = g["hppp_idx"].astype(np.int64) - 1
+    # vhppp records hold <cb||kd> keyed (k,d,c,b)
+    hv = co.TableInteraction(lat.nsp, [(h[:, 2], h[:, 3], h[:, 0], h[:, 1], g["hppp_val"])])
+    et = cc.triples(T, vhppp=hv)
+    ref = g["eccdt"] - g["eccd"]
+    assert abs(et - ref) < 1e-12 * max(1.0, abs(ref))
