@@ -32,7 +32,8 @@ class Dims(C.Structure):
                 ("nchannels", C.c_int), ("nph_channels", C.c_int), ("max_npp", C.c_int),
                 ("max_nhh", C.c_int), ("t2_unique", C.c_longlong), ("t2_ordered", C.c_longlong),
                 ("vpppp_unique", C.c_longlong), ("ring_elems", C.c_longlong),
-                ("box", C.c_double), ("volume", C.c_double), ("kf", C.c_double)]
+                ("box", C.c_double), ("volume", C.c_double), ("kf", C.c_double),
+                ("pp_ladder_flops", C.c_double), ("hh_ladder_flops", C.c_double), ("ring_flops", C.c_double)]
 
 
 class SolveOpts(C.Structure):
@@ -82,6 +83,7 @@ SIGNATURES = {
     "ccgpu_get_t2_labels": [_I, _IP],
     "ccgpu_batch_solve": [_I, _IP, C.POINTER(SolveOpts), C.POINTER(SolveResult)],
     "ccgpu_kernel_stats": [C.c_char_p, _I, _DP, C.POINTER(C.c_longlong), _I],
+    "ccgpu_debug_fetch": [_I, _I, _DP, C.c_longlong],
     "ccgpu_last_error": [C.c_char_p, _I],
 }
 
@@ -255,6 +257,14 @@ class Solver:
         tt = np.ascontiguousarray(np.asarray(t, np.complex128))
         assert tt.size == self.dims["t2_unique"]
         _check(lib().ccgpu_set_t2_unique(self.h, tt.ctypes.data_as(_DP)))
+
+    DEBUG = {"R": 0, "dh": 1, "I7off": 2, "dp": 3, "I6off": 4, "X": 5, "Rt": 6, "Tt": 7, "I4": 8,
+             "T": 9, "R0": 10, "fdiag": 11, "fpart": 12, "W0": 13, "Vt": 14}
+
+    def debug_fetch(self, what, n):
+        out = np.zeros(n, np.complex128)
+        _check(lib().ccgpu_debug_fetch(self.h, self.DEBUG[what], out.ctypes.data_as(_DP), n))
+        return out
 
     def t2_labels(self):
         out = np.zeros((self.dims["t2_unique"], 4), np.int32)

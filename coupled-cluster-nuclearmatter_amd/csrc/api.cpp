@@ -34,7 +34,7 @@ void launch_energy(const SolveDesc *dd, int nsolve, int slot, hipStream_t st);
 void launch_i7(const SolveDesc *dd, int nsolve, int maxA, hipStream_t st);
 void launch_i6(const SolveDesc *dd, int nsolve, int maxnp, hipStream_t st);
 void launch_i9(const SolveDesc *dd, int nsolve, int64_t maxvhh, hipStream_t st);
-void launch_resid_init(const SolveDesc *dd, int nsolve, int64_t maxT, hipStream_t st);
+void launch_resid_init(const SolveDesc *dd, int nsolve, int64_t maxT, bool onebody, hipStream_t st);
 void launch_ring_gather(const SolveDesc *dd, int nsolve, int64_t maxT, hipStream_t st);
 void launch_ring_scatter(const SolveDesc *dd, int nsolve, int64_t maxT, hipStream_t st);
 void launch_update(const SolveDesc *dd, int nsolve, int64_t maxT, const int *slot, hipStream_t st);
@@ -448,9 +448,17 @@ void set_active(Batch &B, int s, int active) {
   HIPCHK(hipStreamSynchronize(B.stream));
 }
 
+// CCGPU_TERMS (bit mask, default all): 1 one-body, 2 pp ladder, 4 hh ladder,
+// 8 ring.  Used by tests to compare single residual terms with the oracle.
+int term_mask() {
+  const char *e = getenv("CCGPU_TERMS");
+  return e ? atoi(e) : 15;
+}
+
 void residual_launches(Batch &B, int prec, bool timing) {
   const int n = (int)B.solves.size();
   hipStream_t st = B.stream;
+  const int terms = term_mask();
   int var64 = gemm_variant_override() >= 0 ? 3 : 0;
   int var = prec == CCGPU_PREC_FP32 ? 1 : var64;
   int varpp = prec == CCGPU_PREC_FP32 ? 2 : var64;
@@ -466,17 +474,18 @@ void residual_launches(Batch &B, int prec, bool timing) {
   }
   {
     Timer t(timing, st, "resid_init_onebody");
-    launch_resid_init(B.d_descs, n, B.maxT, st);
+    launch_resid_init(B.d_descs, n, B.maxT, (terms & 1) != 0, st);
   }
-  {
+  if (terms & 2) {
     Timer t(timing, st, "gemm_pp_ladder");
     GemmList &L = (prec == CCGPU_PREC_FP32) ? B.g_pp32 : B.g_pp;
     launch_gemm(L.d_groups, L.d_tiles, (int)L.tiles.size(), B.d_descs, L.d_gsolve, varpp, st);
   }
-  {
+  if (terms & 4) {
     Timer t(timing, st, "gemm_hh_ladder");
     launch_gemm(B.g_hh.d_groups, B.g_hh.d_tiles, (int)B.g_hh.tiles.size(), B.d_descs, B.g_hh.d_gsolve, var, st);
   }
+  if (!(terms & 8)) return;
   {
     Timer t(timing, st, "ring_gather");
     launch_ring_gather(B.d_descs, n, B.maxT, st);
@@ -717,6 +726,17 @@ void fill_dims(const Plan &P, ccgpu_dims *d) {
   d->t2_unique = P.nT; d->t2_ordered = P.t2_ordered; d->vpppp_unique = P.nVpp;
   d->ring_elems = P.nRt;
   d->box = P.orb.box; d->volume = P.orb.volume; d->kf = P.orb.kf;
+  double fpp = 0, fhh = 0, fr = 0;
+  for (int c = 0; c < P.nch; ++c) {
+    double np_ = P.npp[c], nh = P.nhh[c];
+    fpp += 8.0 * np_ * np_ * nh;
+    fhh += 8.0 * 2.0 * np_ * nh * nh;
+  }
+  for (int K = 0; K < P.nphc; ++K) {
+    double a = P.nph[K], b = P.nhp[K];
+    fr += 8.0 * 2.0 * a * b * b;
+  }
+  d->pp_ladder_flops = fpp; d->hh_ladder_flops = fhh; d->ring_flops = fr;
 }
 
 int64_t t_index(const Plan &P, int a, int b, int i, int j, double &sign) {
@@ -1150,6 +1170,22 @@ int ccgpu_triples(int handle, double *et) {
     (void)handle;
     (void)et;
     throw Error("ccgpu_triples: not available in this build");
+  });
+}
+
+int ccgpu_debug_fetch(int handle, int what, double *out, long long n) {
+  return guarded([&] {
+    Solve &S = get(handle);
+    const Plan &P = S.plan;
+    const SolveDesc &d = S.desc;
+    struct Arr { const void *p; long long n; };
+    Arr a[] = {{d.R, P.nT}, {d.dh, P.A}, {d.I7off, P.A}, {d.dp, P.np}, {d.I6off, P.np}, {d.X, 2 * P.nVhh},
+               {d.Rt, P.nRt}, {d.Tt, P.nRt}, {d.I4, P.nI4}, {d.T, P.nT}, {d.R0, P.nT}, {d.fdiag, P.nsp},
+               {d.fpart, P.nsp}, {d.W0, P.nI4}, {d.Vt, P.nRt}};
+    if (what < 0 || what >= (int)(sizeof a / sizeof a[0])) throw Error(fmt("unknown debug array %d", what));
+    if (n != a[what].n) throw Error(fmt("debug array %d has %lld elements, caller asked %lld", what, a[what].n, n));
+    HIPCHK(hipDeviceSynchronize());
+    HIPCHK(hipMemcpy(out, a[what].p, n * sizeof(double2), hipMemcpyDeviceToHost));
   });
 }
 

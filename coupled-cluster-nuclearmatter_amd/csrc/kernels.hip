@@ -358,7 +358,7 @@ __global__ void k_i9(const SolveDesc *descs) {
 // Residual start: R = conj(<ij||ab>) + P(ab) sum_c I6(b,c) t(ac,ij)
 //                 - P(ij) sum_k I7(k,j) t(ab,ik), diagonals removed
 // (ccm_t2_eqn.f90:81-211).  Owner-computes over unique elements.
-__global__ void k_resid_init(const SolveDesc *descs) {
+__global__ void k_resid_init(const SolveDesc *descs, int onebody) {
   const SolveDesc &d = descs[blockIdx.y];
   if (!d.active) return;
   for (int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; u < d.nT; u += (int64_t)gridDim.x * blockDim.x) {
@@ -369,6 +369,7 @@ __global__ void k_resid_init(const SolveDesc *descs) {
     const int64_t pr = d.off_pp[c] + r, hs = d.off_hh[c] + s;
     int a = d.pp_a[pr], b = d.pp_b[pr], i = d.hh_i[hs], j = d.hh_j[hs];
     double2 acc = d.R0[u];
+    if (!onebody) { d.R[u] = acc; continue; }
     int vB = d.ob_rowB[pr];   // (a, partner(b))
     if (vB >= 0) {
       double2 tv = d.T[o + (int64_t)(vB >> 2) * nh + s];
@@ -565,13 +566,15 @@ __global__ __launch_bounds__(256) void k_zgemm_f64(const GemmGroup *__restrict__
     }
     __syncthreads();
   }
+  // f64 16x16x4 result layout: register r of lane l holds row (l>>4) + 4r,
+  // column l&15 (the f32 16x16x4 variant uses row 4(l>>4) + r)
 #pragma unroll
   for (int x = 0; x < 2; ++x)
 #pragma unroll
     for (int y = 0; y < 2; ++y)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int m = tl.m0 + wm * 32 + x * 16 + (lane >> 4) * 4 + r;
+        const int m = tl.m0 + wm * 32 + x * 16 + (lane >> 4) + 4 * r;
         const int n = tl.n0 + wn * 32 + y * 16 + (lane & 15);
         if (m < g.M && n < g.N) {
           double2 v = make_double2(g.alpha * cre[x][y][r], g.alpha * cim[x][y][r]);
@@ -777,8 +780,8 @@ void launch_i6(const SolveDesc *dd, int nsolve, int maxnp, hipStream_t st) {
 void launch_i9(const SolveDesc *dd, int nsolve, int64_t maxvhh, hipStream_t st) {
   hipLaunchKernelGGL(k_i9, egrid(maxvhh, nsolve), dim3(256), 0, st, dd);
 }
-void launch_resid_init(const SolveDesc *dd, int nsolve, int64_t maxT, hipStream_t st) {
-  hipLaunchKernelGGL(k_resid_init, egrid(maxT, nsolve), dim3(256), 0, st, dd);
+void launch_resid_init(const SolveDesc *dd, int nsolve, int64_t maxT, bool onebody, hipStream_t st) {
+  hipLaunchKernelGGL(k_resid_init, egrid(maxT, nsolve), dim3(256), 0, st, dd, onebody ? 1 : 0);
 }
 void launch_ring_gather(const SolveDesc *dd, int nsolve, int64_t maxT, hipStream_t st) {
   hipLaunchKernelGGL(k_ring_gather, egrid(maxT, nsolve), dim3(256), 0, st, dd);

@@ -23,6 +23,7 @@ module ccgpu_iface
      integer(c_int) :: nsp, nholes, nparticles, nchannels, nph_channels, max_npp, max_nhh
      integer(c_long_long) :: t2_unique, t2_ordered, vpppp_unique, ring_elems
      real(c_double) :: box, volume, kf
+     real(c_double) :: pp_ladder_flops, hh_ladder_flops, ring_flops
   end type ccgpu_dims
 
   type, bind(C) :: ccgpu_solve_opts

@@ -78,6 +78,9 @@ typedef struct {
   long long vpppp_unique; /* stored pppp elements (unique x unique) */
   long long ring_elems;   /* sum over ph channels of Nph * Nhp */
   double box, volume, kf;
+  double pp_ladder_flops; /* 8 sum Npp^2 Nhh: real flops of one pp-ladder launch */
+  double hh_ladder_flops; /* 8 sum (Npp Nhh^2 + Nhh^2 Npp): I9 and hh ladder */
+  double ring_flops;      /* 8 sum (Nhp^2 Nph + Nph Nhp^2): both ring GEMMs */
 } ccgpu_dims;
 
 typedef struct {
@@ -160,6 +163,11 @@ int ccgpu_get_t2_labels(int handle, int *abij);
 /* Solve several handles together: every kernel launch covers all of them. */
 int ccgpu_batch_solve(int n, const int *handles, const ccgpu_solve_opts *opts,
                       ccgpu_solve_result *results);
+
+/* Copy one internal device array (what: 0 R, 1 dh, 2 I7off, 3 dp, 4 I6off,
+ * 5 X|I9, 6 Rt, 7 Tt, 8 I4, 9 T, 10 R0, 11 fdiag, 12 fpart, 13 W0, 14 Vt) of
+ * n complex elements to the host; for tests and diagnostics. */
+int ccgpu_debug_fetch(int handle, int what, double *out, long long n);
 
 /* Per-kernel timings (ms, averaged over the recorded launches) of the last
  * solve; names are written as a ';'-separated list. */

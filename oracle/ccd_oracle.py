@@ -389,7 +389,9 @@ class CCD:
         return I6, I7
 
     # ccm_t2_eqn.f90:4-1304
-    def residual(self, T):
+    def residual(self, T, terms=15):
+        """terms: bit mask of the contributions besides conj(V): 1 one-body,
+        2 pp ladder, 4 hh ladder, 8 ring (for term-by-term comparisons)."""
         lat, ch = self.lat, self.ch
         A = lat.A
         nC = len(ch.keys)
@@ -400,7 +402,7 @@ class CCD:
         np.fill_diagonal(I6, 0.0)
         R = [np.conj(self.v_hhpp[c]).T.copy() for c in range(nC)]
         # one-body terms with P(ab) and P(ij) (ccm_t2_eqn.f90:144-211)
-        for c in range(nC):
+        for c in range(nC if terms & 1 else 0):
             h, p = ch.hh[c], ch.pp[c]
             for row, (a, b) in enumerate(p):
                 s = np.zeros(len(h), np.complex128)
@@ -421,8 +423,12 @@ class CCD:
         # I9 and the two ladders (ccm_t2_eqn.f90:423-647)
         for c in range(nC):
             I9 = self.v_hhhh[c] + 0.5 * (self.v_hhpp[c] @ T[c])
-            R[c] += 0.5 * (self.v_pppp[c] @ T[c])
-            R[c] += 0.5 * (T[c] @ I9)
+            if terms & 2:
+                R[c] += 0.5 * (self.v_pppp[c] @ T[c])
+            if terms & 4:
+                R[c] += 0.5 * (T[c] @ I9)
+        if not terms & 8:
+            return R, d_h, d_p
         # ring (ccm_t2_eqn.f90:37-62, 657-1185)
         Tf = self.flat(T)
         Rf = self.flat(R)

@@ -35,7 +35,7 @@ CASES = {
     "pnm66_n2_minn_twist": (66, "pnm", 0.16, 2, 3, "minnesota", 8, 8, 1e-8, 100,
                             (0.7, 1.9, 2.6), 0, 0, 1),
     "pnm66_n2_nnlo": (66, "pnm", 0.16, 2, 3, "chiral", 8, 8, 1e-8, 100, (0, 0, 0), 0, 0, 8),
-    "snm28_n2_minn": (28, "snm", 0.16, 2, 3, "minnesota", 8, 8, 1e-8, 100, (0, 0, 0), 0, 0, 1),
+    "snm28_n2_minn": (28, "snm", 0.16, 2, 3, "minnesota", 8, 8, 1e-11, 300, (0, 0, 0), 0, 0, 1),
 }
 
 CCM_IN = """!Chiral order for Deltas(LO = 0,NLO=2,NNLO=3,N3LO=4) and cutoff
