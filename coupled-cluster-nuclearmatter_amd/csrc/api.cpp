@@ -32,7 +32,7 @@ void launch_to_f32(const double2 *src, float2 *dst, int64_t n, hipStream_t st);
 void launch_mbpt2(const SolveDesc *dd, int nsolve, int64_t maxT, hipStream_t st);
 void launch_energy(const SolveDesc *dd, int nsolve, int slot, hipStream_t st);
 void launch_i7(const SolveDesc *dd, int nsolve, int maxA, hipStream_t st);
-void launch_i6(const SolveDesc *dd, int nsolve, int maxnp, hipStream_t st);
+void launch_i6(const SolveDesc *dd, int nsolve, int maxnp, int64_t maxrows, hipStream_t st);
 void launch_i9(const SolveDesc *dd, int nsolve, int64_t maxvhh, hipStream_t st);
 void launch_resid_init(const SolveDesc *dd, int nsolve, int64_t maxT, bool onebody, hipStream_t st);
 void launch_ring_gather(const SolveDesc *dd, int nsolve, int64_t maxT, hipStream_t st);
@@ -43,6 +43,8 @@ void launch_diis_axpy(const SolveDesc *dd, int nsolve, int64_t maxT, const int *
                       int nvec, hipStream_t st);
 void launch_gemm(const GemmGroup *groups, const GemmTile *tiles, int ntiles, const SolveDesc *dd,
                  const int *gsolve, int variant, hipStream_t st);
+void launch_splitk_reduce(const GemmGroup *groups, const int *gidx, int n, int64_t maxmn, const SolveDesc *dd,
+                          const int *gsolve, hipStream_t st);
 }  // namespace ccgpu
 
 using namespace ccgpu;
@@ -88,44 +90,101 @@ T *dupload(const std::vector<T> &v, std::vector<void *> &owned) {
   return p;
 }
 
+// Work list of one GEMM family.  Tiles are classed by the column count of their
+// group: N <= 16 -> 128x16 tiles, N <= 32 -> 128x32, N <= 48 -> 64x48,
+// N <= 64 -> 64x64, N <= 80 -> 64x80, wider -> several 64x64 tiles, so narrow
+// amplitude blocks do not pay for padded MFMA columns.  tiles64 is a uniform
+// 64x64 tiling used by the VALU cross-check kernel.
 struct GemmList {
   std::vector<GemmGroup> groups;
-  std::vector<GemmTile> tiles;
+  static constexpr int NCLS = 5;
+  std::vector<GemmTile> tiles[NCLS];
+  std::vector<GemmTile> tiles64;
   std::vector<int> gsolve;
+  std::vector<int> split;        // groups reduced by k_splitk_reduce
+  std::vector<int64_t> poff;     // partial-buffer offset per group
+  int64_t psize = 0, maxmn = 0;
   GemmGroup *d_groups = nullptr;
-  GemmTile *d_tiles = nullptr;
-  int *d_gsolve = nullptr;
+  GemmTile *d_tiles[NCLS] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+  GemmTile *d_tiles64 = nullptr;
+  int *d_gsolve = nullptr, *d_split = nullptr;
   double flops = 0;       // algorithmic real flops per launch
   double bytes = 0;       // algorithmic bytes of the A operand per launch
-  void add(const GemmGroup &g, int solve, int bm = 64, int bn = 64) {
+  static int cls_of(int N) { return N <= 16 ? 0 : N <= 32 ? 1 : N <= 48 ? 2 : N <= 64 ? 3 : N <= 80 ? 4 : 3; }
+  static int rows_of(int c) { return c <= 1 ? 128 : 64; }
+  static int cols_of(int c) { static const int w[NCLS] = {16, 32, 48, 64, 80}; return w[c]; }
+  void add(GemmGroup g, int solve) {
     if (g.M <= 0 || g.N <= 0 || g.K <= 0) return;
     int gi = (int)groups.size();
+    // long-K, small-output products split K into 256-wide slices so that a
+    // channel spreads over many workgroups; partials are summed in fixed order
+    g.nsplit = 1;
+    if (g.K >= 512 && g.K > 8 * std::max(g.M, g.N)) g.nsplit = std::min(32, (g.K + 255) / 256);
+    int ksz = (g.K + g.nsplit - 1) / g.nsplit;
+    ksz = (ksz + 15) / 16 * 16;
+    g.nsplit = (g.K + ksz - 1) / ksz;
+    poff.push_back(g.nsplit > 1 ? psize : -1);
+    if (g.nsplit > 1) {
+      psize += (int64_t)g.nsplit * g.M * g.N;
+      split.push_back(gi);
+      maxmn = std::max<int64_t>(maxmn, (int64_t)g.M * g.N);
+    }
+    g.P = nullptr;
     groups.push_back(g);
     gsolve.push_back(solve);
-    for (int m = 0; m < g.M; m += bm)
-      for (int n = 0; n < g.N; n += bn) tiles.push_back(GemmTile{gi, m, n, 0});
+    int c = cls_of(g.N);
+    int bm = rows_of(c), bn = cols_of(c);
+    for (int ks = 0; ks < g.nsplit; ++ks)
+      for (int m = 0; m < g.M; m += bm)
+        for (int n = 0; n < g.N; n += bn)
+          tiles[c].push_back(GemmTile{gi, m, n, ks, ks * ksz, std::min(g.K, (ks + 1) * ksz), 0, 0});
+    for (int m = 0; m < g.M; m += 64)
+      for (int n = 0; n < g.N; n += 64) tiles64.push_back(GemmTile{gi, m, n, 0, 0, g.K, 0, 0});
     flops += 8.0 * g.M * (double)g.N * g.K;
   }
-  void upload(std::vector<void *> &owned) {
+  int ntiles() const {
+    size_t n = 0;
+    for (int c = 0; c < NCLS; ++c) n += tiles[c].size();
+    return (int)n;
+  }
+  // device partial buffer for the split groups, then upload
+  void upload(std::vector<void *> &owned, bool own_partials = true) {
+    if (own_partials && psize > 0) {
+      double2 *P = dalloc<double2>(psize, owned);
+      for (size_t g = 0; g < groups.size(); ++g)
+        if (poff[g] >= 0) groups[g].P = P + poff[g];
+    }
     d_groups = dupload(groups, owned);
-    d_tiles = dupload(tiles, owned);
+    for (int c = 0; c < NCLS; ++c) d_tiles[c] = dupload(tiles[c], owned);
+    d_tiles64 = dupload(tiles64, owned);
     d_gsolve = dupload(gsolve, owned);
+    d_split = dupload(split, owned);
   }
 };
+
+GemmGroup mkgroup(const void *A, const double2 *B, double2 *C, const double2 *C0, int M, int N, int K, int lda,
+                  int ldb, int ldc, int ldc0, double alpha, double beta) {
+  GemmGroup g{};
+  g.A = A; g.B = B; g.C = C; g.C0 = C0; g.P = nullptr;
+  g.M = M; g.N = N; g.K = K; g.lda = lda; g.ldb = ldb; g.ldc = ldc; g.ldc0 = ldc0;
+  g.nsplit = 1; g.alpha = alpha; g.beta = beta;
+  return g;
+}
 
 // XCD-aware reorder: tiles of one group stay on one of the 8 XCDs (blocks are
 // dealt round-robin to XCDs), so the shared operand of a channel is read from
 // one L2.  Groups are packed onto XCD bins by largest-first.
-void xcd_order(GemmList &L) {
+void xcd_order_list(std::vector<GemmTile> &tiles, size_t ngroups) {
   const int nx = 8;
   std::vector<std::vector<GemmTile>> bins(nx);
   std::vector<double> load(nx, 0.0);
-  std::vector<std::vector<GemmTile>> per(L.groups.size());
-  for (auto &t : L.tiles) per[t.g].push_back(t);
-  std::vector<int> order(L.groups.size());
+  std::vector<std::vector<GemmTile>> per(ngroups);
+  for (auto &t : tiles) per[t.g].push_back(t);
+  std::vector<int> order(ngroups);
   for (size_t i = 0; i < order.size(); ++i) order[i] = (int)i;
   std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return per[a].size() > per[b].size(); });
   for (int g : order) {
+    if (per[g].empty()) continue;
     int best = (int)(std::min_element(load.begin(), load.end()) - load.begin());
     for (auto &t : per[g]) bins[best].push_back(t);
     load[best] += (double)per[g].size();
@@ -136,7 +195,11 @@ void xcd_order(GemmList &L) {
   for (size_t r = 0; r < maxlen; ++r)
     for (int x = 0; x < nx; ++x)
       if (r < bins[x].size()) out.push_back(bins[x][r]);
-  L.tiles.swap(out);
+  tiles.swap(out);
+}
+
+void xcd_order(GemmList &L) {
+  for (int c = 0; c < GemmList::NCLS; ++c) xcd_order_list(L.tiles[c], L.groups.size());
 }
 
 struct Solve {
@@ -236,6 +299,11 @@ void build_device(Solve &S) {
   d.hh_i = dupload(P.hh_i, o); d.hh_j = dupload(P.hh_j, o);
   d.ob_rowB = dupload(P.ob_rowB, o); d.ob_rowA = dupload(P.ob_rowA, o);
   d.ob_colJ = dupload(P.ob_colJ, o); d.ob_colI = dupload(P.ob_colI, o);
+  d.nrows = (int64_t)P.pp_a.size();
+  d.row_ch = dupload(P.row_ch, o);
+  d.i6_dptr = dupload(P.i6_dptr, o); d.i6_didx = dupload(P.i6_didx, o);
+  d.i6_optr = dupload(P.i6_optr, o); d.i6_oidx = dupload(P.i6_oidx, o);
+  d.p6 = dalloc<double2>(3 * P.pp_a.size(), o);
   std::vector<DPair> pr(P.ppref.size()), hr(P.hhref.size());
   for (size_t x = 0; x < pr.size(); ++x)
     pr[x] = DPair{P.ppref[x].sign ? P.ppref[x].ch : -1, P.ppref[x].idx * 2 + (P.ppref[x].sign < 0)};
@@ -288,25 +356,19 @@ void build_device(Solve &S) {
   // GEMM work lists
   for (int c = 0; c < P.nch; ++c) {
     int np_ = P.npp[c], nh = P.nhh[c];
-    GemmGroup g{};
     // X = Vhhpp T
-    g = GemmGroup{d.Vhhpp + P.off_t[c], d.T + P.off_t[c], d.X + P.off_vhh[c], nullptr, nh, nh, np_, np_, nh, nh, 0, 1.0, 0.0};
-    S.g_x.add(g, 0);
+    S.g_x.add(mkgroup(d.Vhhpp + P.off_t[c], d.T + P.off_t[c], d.X + P.off_vhh[c], nullptr, nh, nh, np_, np_, nh, nh, 0, 1.0, 0.0), 0);
     // R += Vpppp T   (pp ladder, ccm_t2_eqn.f90:574-607)
-    g = GemmGroup{d.Vpppp + P.off_vpp[c], d.T + P.off_t[c], d.R + P.off_t[c], d.R + P.off_t[c], np_, nh, np_, np_, nh, nh, nh, 1.0, 1.0};
-    S.g_pp.add(g, 0);
+    S.g_pp.add(mkgroup(d.Vpppp + P.off_vpp[c], d.T + P.off_t[c], d.R + P.off_t[c], d.R + P.off_t[c], np_, nh, np_, np_, nh, nh, nh, 1.0, 1.0), 0);
     // R += T I9      (hh ladder, ccm_t2_eqn.f90:614-647)
-    g = GemmGroup{d.T + P.off_t[c], d.X + P.nVhh + P.off_vhh[c], d.R + P.off_t[c], d.R + P.off_t[c], np_, nh, nh, nh, nh, nh, nh, 1.0, 1.0};
-    S.g_hh.add(g, 0);
+    S.g_hh.add(mkgroup(d.T + P.off_t[c], d.X + P.nVhh + P.off_vhh[c], d.R + P.off_t[c], d.R + P.off_t[c], np_, nh, nh, nh, nh, nh, nh, 1.0, 1.0), 0);
   }
   for (int K = 0; K < P.nphc; ++K) {
     int nph = P.nph[K], nhp = P.nhp[K];
     // I4 = W0 + 1/2 Vt Tt (ccm_t2_eqn.f90:657-780)
-    GemmGroup g{d.Vt + P.off_vt[K], d.Tt + P.off_rt[K], d.I4 + P.off_i4[K], d.W0 + P.off_i4[K], nhp, nhp, nph, nph, nhp, nhp, nhp, 0.5, 1.0};
-    S.g_i4.add(g, 0);
+    S.g_i4.add(mkgroup(d.Vt + P.off_vt[K], d.Tt + P.off_rt[K], d.I4 + P.off_i4[K], d.W0 + P.off_i4[K], nhp, nhp, nph, nph, nhp, nhp, nhp, 0.5, 1.0), 0);
     // Rt = Tt I4 (ccm_t2_eqn.f90:995-1026)
-    g = GemmGroup{d.Tt + P.off_rt[K], d.I4 + P.off_i4[K], d.Rt + P.off_rt[K], nullptr, nph, nhp, nhp, nhp, nhp, nhp, 0, 1.0, 0.0};
-    S.g_rt.add(g, 0);
+    S.g_rt.add(mkgroup(d.Tt + P.off_rt[K], d.I4 + P.off_i4[K], d.Rt + P.off_rt[K], nullptr, nph, nhp, nhp, nhp, nhp, nhp, 0, 1.0, 0.0), 0);
   }
   xcd_order(S.g_pp);
   for (GemmList *L : {&S.g_x, &S.g_pp, &S.g_hh, &S.g_i4, &S.g_rt}) L->upload(o);
@@ -365,7 +427,7 @@ struct Batch {
   SolveDesc *d_descs = nullptr;  // device array
   std::vector<SolveDesc> h_descs;
   GemmList g_x, g_pp, g_hh, g_i4, g_rt, g_pp32;
-  int64_t maxT = 0, maxVhh = 0;
+  int64_t maxT = 0, maxVhh = 0, maxRows = 0;
   int maxA = 0, maxNp = 0;
   int *d_slot = nullptr, *d_slots = nullptr;
   double2 *d_coef = nullptr;
@@ -391,10 +453,18 @@ void merge(GemmList &dst, const GemmList &src, int solve) {
     dst.groups.push_back(src.groups[i]);
     dst.gsolve.push_back(solve);
   }
-  for (auto t : src.tiles) {
+  for (int c = 0; c < GemmList::NCLS; ++c)
+    for (auto t : src.tiles[c]) {
+      t.g += base;
+      dst.tiles[c].push_back(t);
+    }
+  for (auto t : src.tiles64) {
     t.g += base;
-    dst.tiles.push_back(t);
+    dst.tiles64.push_back(t);
   }
+  for (int g : src.split) dst.split.push_back(g + base);
+  for (size_t g = 0; g < src.groups.size(); ++g) dst.poff.push_back(-1);  // partial buffers stay the solve's
+  dst.maxmn = std::max(dst.maxmn, src.maxmn);
   dst.flops += src.flops;
   dst.bytes += src.bytes;
 }
@@ -418,6 +488,7 @@ std::unique_ptr<Batch> make_batch(const std::vector<Solve *> &ss, hipStream_t st
     merge(B->g_rt, S.g_rt, s);
     B->maxT = std::max(B->maxT, S.plan.nT);
     B->maxVhh = std::max(B->maxVhh, S.plan.nVhh);
+    B->maxRows = std::max<int64_t>(B->maxRows, (int64_t)S.plan.pp_a.size());
     B->maxA = std::max(B->maxA, S.plan.A);
     B->maxNp = std::max(B->maxNp, S.plan.np);
   }
@@ -434,7 +505,7 @@ std::unique_ptr<Batch> make_batch(const std::vector<Solve *> &ss, hipStream_t st
     xcd_order(B->g_pp);
     xcd_order(B->g_pp32);
   }
-  for (GemmList *L : {&B->g_x, &B->g_pp, &B->g_hh, &B->g_i4, &B->g_rt, &B->g_pp32}) L->upload(B->owned);
+  for (GemmList *L : {&B->g_x, &B->g_pp, &B->g_hh, &B->g_i4, &B->g_rt, &B->g_pp32}) L->upload(B->owned, false);
   B->d_descs = dupload(B->h_descs, B->owned);
   B->d_slot = dalloc<int>(n, B->owned);
   B->d_slots = dalloc<int>((size_t)n * 16, B->owned);
@@ -455,22 +526,32 @@ int term_mask() {
   return e ? atoi(e) : 15;
 }
 
+// One GEMM family: three launches (one per tile class), or the VALU check.
+void run_gemm(const GemmList &L, const Batch &B, int prec, bool a_f32, hipStream_t st) {
+  if (gemm_variant_override() == 3) {
+    launch_gemm(L.d_groups, L.d_tiles64, (int)L.tiles64.size(), B.d_descs, L.d_gsolve, 3, st);
+    return;
+  }
+  const int base = prec == CCGPU_PREC_FP64 ? 10 : (a_f32 ? 30 : 20);
+  const int nb[GemmList::NCLS] = {1, 2, 3, 4, 5};
+  for (int c = GemmList::NCLS - 1; c >= 0; --c)
+    launch_gemm(L.d_groups, L.d_tiles[c], (int)L.tiles[c].size(), B.d_descs, L.d_gsolve, base + nb[c], st);
+  launch_splitk_reduce(L.d_groups, L.d_split, (int)L.split.size(), L.maxmn, B.d_descs, L.d_gsolve, st);
+}
+
 void residual_launches(Batch &B, int prec, bool timing) {
   const int n = (int)B.solves.size();
   hipStream_t st = B.stream;
   const int terms = term_mask();
-  int var64 = gemm_variant_override() >= 0 ? 3 : 0;
-  int var = prec == CCGPU_PREC_FP32 ? 1 : var64;
-  int varpp = prec == CCGPU_PREC_FP32 ? 2 : var64;
   {
     Timer t(timing, st, "gemm_x_vhhpp_t");
-    launch_gemm(B.g_x.d_groups, B.g_x.d_tiles, (int)B.g_x.tiles.size(), B.d_descs, B.g_x.d_gsolve, var, st);
+    run_gemm(B.g_x, B, prec, false, st);
   }
   {
     Timer t(timing, st, "i7_i9_i6");
     launch_i7(B.d_descs, n, B.maxA, st);
     launch_i9(B.d_descs, n, B.maxVhh, st);
-    launch_i6(B.d_descs, n, B.maxNp, st);
+    launch_i6(B.d_descs, n, B.maxNp, B.maxRows, st);
   }
   {
     Timer t(timing, st, "resid_init_onebody");
@@ -478,12 +559,12 @@ void residual_launches(Batch &B, int prec, bool timing) {
   }
   if (terms & 2) {
     Timer t(timing, st, "gemm_pp_ladder");
-    GemmList &L = (prec == CCGPU_PREC_FP32) ? B.g_pp32 : B.g_pp;
-    launch_gemm(L.d_groups, L.d_tiles, (int)L.tiles.size(), B.d_descs, L.d_gsolve, varpp, st);
+    if (prec == CCGPU_PREC_FP32) run_gemm(B.g_pp32, B, prec, true, st);
+    else run_gemm(B.g_pp, B, prec, false, st);
   }
   if (terms & 4) {
     Timer t(timing, st, "gemm_hh_ladder");
-    launch_gemm(B.g_hh.d_groups, B.g_hh.d_tiles, (int)B.g_hh.tiles.size(), B.d_descs, B.g_hh.d_gsolve, var, st);
+    run_gemm(B.g_hh, B, prec, false, st);
   }
   if (!(terms & 8)) return;
   {
@@ -492,11 +573,11 @@ void residual_launches(Batch &B, int prec, bool timing) {
   }
   {
     Timer t(timing, st, "gemm_ring_i4");
-    launch_gemm(B.g_i4.d_groups, B.g_i4.d_tiles, (int)B.g_i4.tiles.size(), B.d_descs, B.g_i4.d_gsolve, var, st);
+    run_gemm(B.g_i4, B, prec, false, st);
   }
   {
     Timer t(timing, st, "gemm_ring_rt");
-    launch_gemm(B.g_rt.d_groups, B.g_rt.d_tiles, (int)B.g_rt.tiles.size(), B.d_descs, B.g_rt.d_gsolve, var, st);
+    run_gemm(B.g_rt, B, prec, false, st);
   }
   {
     Timer t(timing, st, "ring_scatter");
@@ -624,7 +705,10 @@ void run_batch(Batch &B, const ccgpu_solve_opts &op, std::vector<ccgpu_solve_res
   auto t0 = std::chrono::steady_clock::now();
   bool graph_ok = op.use_graph != 0 && !timing && gemm_variant_override() < 0;
   int refine = op.refine_every > 0 ? op.refine_every : 0;
-  bool fp64_locked = false;
+  // mixed precision: FP32 contractions (with DIIS) until max |dE| <= fp32_switch,
+  // then the DIIS history restarts and FP64 iterations finish the solve, so the
+  // final fixed point and the stopping test are FP64
+  bool fp32_phase = op.precision == CCGPU_PREC_FP32;
   for (int it = 1; it <= op.maxit; ++it) {
     bool any = false;
     for (int s = 0; s < n; ++s) {
@@ -633,10 +717,8 @@ void run_batch(Batch &B, const ccgpu_solve_opts &op, std::vector<ccgpu_solve_res
       any = any || act;
     }
     if (!any) break;
-    // precision of this iteration: FP32 contractions while far from convergence,
-    // FP64 on every refine-th iteration and once |dE| is within 100 tol
     int prec = CCGPU_PREC_FP64;
-    if (op.precision == CCGPU_PREC_FP32 && !fp64_locked) {
+    if (fp32_phase) {
       prec = CCGPU_PREC_FP32;
       if (refine && it % refine == 0) prec = CCGPU_PREC_FP64;
     }
@@ -667,7 +749,16 @@ void run_batch(Batch &B, const ccgpu_solve_opts &op, std::vector<ccgpu_solve_res
       maxd = std::max(maxd, std::abs(dener[s]));
       if (op.verbose) printf("dener %5d %5d %16.8E  E %.12f\n", s, it, dener[s].real(), e2[s].real());
     }
-    if (op.precision == CCGPU_PREC_FP32 && maxd <= 100.0 * op.tol) fp64_locked = true;
+    if (fp32_phase && maxd <= std::max(op.fp32_switch, 10.0 * op.tol)) {
+      fp32_phase = false;
+      count = 0;
+      nstep = 0;
+      for (Solve *S : B.solves)
+        for (int k = 0; k < op.subspace; ++k) S->ring[k] = k;
+      // an FP32 |dE| is no FP64 convergence test: force at least one FP64 step
+      for (int s = 0; s < n; ++s)
+        if (B.h_descs[s].active) dener[s] = std::max(std::abs(dener[s]), 2.0 * op.tol);
+    }
   }
   HIPCHK(hipStreamSynchronize(B.stream));
   double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();

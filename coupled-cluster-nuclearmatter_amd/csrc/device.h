@@ -26,6 +26,11 @@ struct SolveDesc {
   const int64_t *off_t, *off_pp, *off_hh, *off_vpp, *off_vhh;
   const int *pp_a, *pp_b, *hh_i, *hh_j;
   const int *ob_rowB, *ob_rowA, *ob_colJ, *ob_colI;
+  // I6 by rows: channel of each unique pp row, CSR lists of partials per particle
+  int64_t nrows;
+  const int *row_ch;
+  const int *i6_dptr, *i6_didx, *i6_optr, *i6_oidx;
+  double2 *p6;
   const DPair *ppref, *hhref;
   // Pandya
   const int *nph, *nhp;
@@ -47,18 +52,23 @@ struct SolveDesc {
 };
 
 // One member of a grouped complex GEMM: C = alpha * A * B + beta * C0.
+// With nsplit > 1 the K range is split over tiles that write raw partial
+// products to P (nsplit blocks of M x N); k_splitk_reduce then forms C.
 struct GemmGroup {
   const void *A;
   const double2 *B;
   double2 *C;
   const double2 *C0;
+  double2 *P;
   int M, N, K;
   int lda, ldb, ldc, ldc0;
+  int nsplit;
   double alpha, beta;
 };
 
 struct GemmTile {
-  int g, m0, n0, pad;
+  int g, m0, n0, ks;   // group, tile origin, split index
+  int kb, ke, pad0, pad1;  // k range of this tile
 };
 
 }  // namespace ccgpu

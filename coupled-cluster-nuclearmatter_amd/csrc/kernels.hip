@@ -10,6 +10,7 @@
 #include "device.h"
 
 #include <algorithm>
+#include <type_traits>
 
 namespace ccgpu {
 
@@ -277,19 +278,22 @@ __global__ void k_reduce_final(const SolveDesc *descs, int nparts, int stride, i
 // I7(k,i) = f_ki + 1/2 sum_l sum_cd <kl||cd> t(cd,il) for k in {i, partner(i)}, read
 // from X = Vhhpp T (setup_I6I7_intermediate, ccm_t2_eqn.f90:1352-1392, 1507-1510).
 // dh(i) = I7(i,i) is the dynamic hole denominator (ccm_t2_eqn.f90:81-84).
-__global__ void k_i7(const SolveDesc *descs) {
+__device__ __forceinline__ double2 wave_sum(double2 v);
+
+__global__ __launch_bounds__(256) void k_i7(const SolveDesc *descs) {
   const SolveDesc &d = descs[blockIdx.y];
   if (!d.active) return;
-  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int lane = threadIdx.x & 63;
+  const int t = blockIdx.x * 4 + (threadIdx.x >> 6);
   if (t >= 2 * d.A) return;
-  int i = t >> 1, which = t & 1;
-  int k = which ? d.partner[i] : i;
+  const int i = t >> 1, which = t & 1;
+  const int k = which ? d.partner[i] : i;
   if (k < 0 || k >= d.A) {
-    if (which) d.I7off[i] = make_double2(0.0, 0.0);
+    if (which && lane == 0) d.I7off[i] = make_double2(0.0, 0.0);
     return;
   }
-  double2 acc = which ? d.fpart[k] : d.fdiag[i];
-  for (int l = 0; l < d.A; ++l) {
+  double2 acc = make_double2(0.0, 0.0);
+  for (int l = lane; l < d.A; l += 64) {
     if (l == k || l == i) continue;
     DPair pk = d.hhref[(size_t)k * d.A + l], pi = d.hhref[(size_t)i * d.A + l];
     if (pk.ch < 0 || pk.ch != pi.ch) continue;
@@ -297,51 +301,84 @@ __global__ void k_i7(const SolveDesc *descs) {
     double2 x = d.X[d.off_vhh[c] + (int64_t)pref_idx(pk) * d.nhh[c] + pref_idx(pi)];
     acc = cadd(acc, cscale(pref_sign(pk) * pref_sign(pi), x));
   }
-  if (which) d.I7off[i] = acc; else d.dh[i] = acc;
+  acc = wave_sum(acc);
+  if (lane == 0) {
+    acc = cadd(acc, which ? d.fpart[k] : d.fdiag[i]);
+    if (which) d.I7off[i] = acc; else d.dh[i] = acc;
+  }
 }
 
 // I6(a,c) = f_ac - 1/2 sum_d sum_kl <kl||dc> t(da,kl) for c in {a, partner(a)}
 // (ccm_t2_eqn.f90:1399-1441); dp(a) = I6(a,a) (ccm_t2_eqn.f90:86-89).
-// One wavefront per target; lanes stride over d, fixed butterfly reduction.
-__global__ __launch_bounds__(256) void k_i6(const SolveDesc *descs) {
+// Pass 1, one unique pp row (x<y) per 16 lanes: with V(kl,dc) = conj(R0(dc,kl))
+// the row contributes -conj(R0_r).T_r to I6(x,x) and I6(y,y),
+// -s_B conj(R0_(x,ybar)).T_r to I6(y,ybar) and +s_A conj(R0_(y,xbar)).T_r to
+// I6(x,xbar).  Pass 2 sums each particle's list of partials in fixed order.
+__device__ __forceinline__ double2 cdotc(const double2 *v, const double2 *t, int n, int l16) {
+  double re = 0.0, im = 0.0;
+  for (int s = l16; s < n; s += 16) {
+    double2 a = v[s], b = t[s];
+    re += a.x * b.x + a.y * b.y;
+    im += a.x * b.y - a.y * b.x;
+  }
+  for (int off = 8; off > 0; off >>= 1) {
+    re += __shfl_xor(re, off, 16);
+    im += __shfl_xor(im, off, 16);
+  }
+  return make_double2(re, im);
+}
+
+__global__ __launch_bounds__(256) void k_i6_rows(const SolveDesc *descs) {
+  const SolveDesc &d = descs[blockIdx.y];
+  if (!d.active) return;
+  const int l16 = threadIdx.x & 15;
+  for (int64_t r = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 4; r < d.nrows;
+       r += ((int64_t)gridDim.x * blockDim.x) >> 4) {
+    const int c = d.row_ch[r], nh = d.nhh[c];
+    const int64_t base = d.off_t[c], loc = r - d.off_pp[c];
+    const double2 *tr = d.T + base + loc * nh;
+    double2 d0 = cdotc(d.R0 + base + loc * nh, tr, nh, l16);
+    double2 dB = make_double2(0.0, 0.0), dA = make_double2(0.0, 0.0);
+    const int vB = d.ob_rowB[r], vA = d.ob_rowA[r];
+    if (vB >= 0) dB = cscale((double)((vB & 3) - 1), cdotc(d.R0 + base + (int64_t)(vB >> 2) * nh, tr, nh, l16));
+    if (vA >= 0) dA = cscale((double)((vA & 3) - 1), cdotc(d.R0 + base + (int64_t)(vA >> 2) * nh, tr, nh, l16));
+    if (l16 == 0) {
+      d.p6[3 * r + 0] = make_double2(-d0.x, -d0.y);
+      d.p6[3 * r + 1] = make_double2(-dB.x, -dB.y);
+      d.p6[3 * r + 2] = dA;
+    }
+  }
+}
+
+// one wavefront per target; lanes take every 64th list entry, then a fixed
+// butterfly: deterministic for a given list
+__device__ __forceinline__ double2 wave_sum(double2 v) {
+  for (int off = 32; off > 0; off >>= 1) {
+    v.x += __shfl_xor(v.x, off);
+    v.y += __shfl_xor(v.y, off);
+  }
+  return v;
+}
+
+__global__ __launch_bounds__(256) void k_i6_reduce(const SolveDesc *descs) {
   const SolveDesc &d = descs[blockIdx.y];
   if (!d.active) return;
   const int lane = threadIdx.x & 63;
   const int t = blockIdx.x * 4 + (threadIdx.x >> 6);
   if (t >= 2 * d.np) return;
-  const int a = d.A + (t >> 1), which = t & 1;
-  const int c = which ? d.partner[a] : a;
-  if (c < d.A) {
-    if (which && lane == 0) d.I6off[a - d.A] = make_double2(0.0, 0.0);
+  const int p = t >> 1, which = t & 1, a = d.A + p;
+  if (which && d.partner[a] < d.A) {
+    if (lane == 0) d.I6off[p] = make_double2(0.0, 0.0);
     return;
   }
-  double re = 0.0, im = 0.0;
-  for (int dd = d.A + lane; dd < d.nsp; dd += 64) {
-    if (dd == a || dd == c) continue;
-    DPair pa = d.ppref[(size_t)(dd - d.A) * d.np + (a - d.A)];
-    DPair pc = d.ppref[(size_t)(dd - d.A) * d.np + (c - d.A)];
-    if (pa.ch < 0 || pc.ch != pa.ch) continue;
-    int ch = pa.ch, nh = d.nhh[ch];
-    const double2 *tr = d.T + d.off_t[ch] + (int64_t)pref_idx(pa) * nh;
-    const double2 *vr = d.R0 + d.off_t[ch] + (int64_t)pref_idx(pc) * nh;
-    double sr = 0.0, si = 0.0;
-    for (int s = 0; s < nh; ++s) {
-      double2 tv = tr[s], vv = vr[s];  // V(kl,dc) = conj(R0(dc,kl))
-      sr += vv.x * tv.x + vv.y * tv.y;
-      si += vv.x * tv.y - vv.y * tv.x;
-    }
-    double sg = pref_sign(pa) * pref_sign(pc);
-    re -= sg * sr;
-    im -= sg * si;
-  }
-  for (int off = 32; off > 0; off >>= 1) {
-    re += __shfl_xor(re, off);
-    im += __shfl_xor(im, off);
-  }
+  const int *ptr = which ? d.i6_optr : d.i6_dptr;
+  const int *idx = which ? d.i6_oidx : d.i6_didx;
+  double2 acc = make_double2(0.0, 0.0);
+  for (int q = ptr[p] + lane; q < ptr[p + 1]; q += 64) acc = cadd(acc, d.p6[idx[q]]);
+  acc = wave_sum(acc);
   if (lane == 0) {
-    double2 f = which ? d.fpart[a] : d.fdiag[a];
-    double2 v = make_double2(f.x + re, f.y + im);
-    if (which) d.I6off[a - d.A] = v; else d.dp[a - d.A] = v;
+    acc = cadd(acc, which ? d.fpart[a] : d.fdiag[a]);
+    if (which) d.I6off[p] = acc; else d.dp[p] = acc;
   }
 }
 
@@ -439,37 +476,41 @@ __global__ void k_update(const SolveDesc *descs, const int *slot) {
 }
 
 // DIIS Gram matrix B_pq = sum e_p e_q (no conjugation, ccm_diis.f90:134-147) of the
-// error vectors e_p = t_{p+1} - t_p over the ring-buffer history.  Up to 16
-// history vectors; partials of the upper triangle go to red[block*136 + idx].
-__global__ __launch_bounds__(256) void k_gram_partial(const SolveDesc *descs, const int *slots, int nstep) {
+// error vectors e_p = t_{p+1} - t_p over the ring-buffer history, NE = nstep-1
+// error vectors (at most 8).  Partials of the upper triangle go to
+// red[block*36 + idx].
+template <int NE>
+__global__ __launch_bounds__(256) void k_gram_partial(const SolveDesc *descs, const int *slots) {
   const SolveDesc &d = descs[blockIdx.y];
   if (!d.active) return;
-  const int ne = nstep - 1;
-  double2 acc[36];
-  for (int q = 0; q < 36; ++q) acc[q] = make_double2(0.0, 0.0);
+  constexpr int NP = NE * (NE + 1) / 2;
+  double2 acc[NP];
+#pragma unroll
+  for (int q = 0; q < NP; ++q) acc[q] = make_double2(0.0, 0.0);
   const int *sl = slots + blockIdx.y * 16;
   for (int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; u < d.nT; u += (int64_t)gridDim.x * blockDim.x) {
-    double2 e[8];
+    double2 e[NE];
     double2 prev = d.hist[(int64_t)sl[0] * d.nT + u];
-    for (int p = 0; p < ne; ++p) {
+#pragma unroll
+    for (int p = 0; p < NE; ++p) {
       double2 cur = d.hist[(int64_t)sl[p + 1] * d.nT + u];
       e[p] = csub(cur, prev);
       prev = cur;
     }
     int q = 0;
-    for (int p = 0; p < ne; ++p)
-      for (int p2 = p; p2 < ne; ++p2) {
+#pragma unroll
+    for (int p = 0; p < NE; ++p)
+#pragma unroll
+      for (int p2 = p; p2 < NE; ++p2) {
         acc[q] = cadd(acc[q], cmul(e[p], e[p2]));
         ++q;
       }
   }
-  int q = 0;
-  for (int p = 0; p < ne; ++p)
-    for (int p2 = p; p2 < ne; ++p2) {
-      double2 s = block_sum(acc[q]);
-      if (threadIdx.x == 0) d.red[(int64_t)blockIdx.x * 36 + q] = s;
-      ++q;
-    }
+#pragma unroll
+  for (int q = 0; q < NP; ++q) {
+    double2 s = block_sum(acc[q]);
+    if (threadIdx.x == 0) d.red[(int64_t)blockIdx.x * 36 + q] = s;
+  }
 }
 
 // t = sum_p c_p t_p over the first nstep-1 history vectors (ccm_diis.f90:186-212).
@@ -486,197 +527,28 @@ __global__ void k_diis_axpy(const SolveDesc *descs, const int *slots, const doub
 }
 
 // ---------------------------------------------------------------------------
-// Grouped complex GEMM on MFMA: C = alpha A B + beta C0, every group a different
-// channel (and solve).  256 threads = 4 wavefronts in a 2x2 grid; each wavefront
-// owns a (BM/2)x(BN/2) sub-tile of 16x16 MFMA blocks.  A and B are staged in LDS
-// as split real/imaginary planes; a complex multiply-accumulate is four real
-// MFMAs: re += ar br, re += (-ai) bi, im += ar bi, im += ai br.
+// Grouped complex GEMMs: C = alpha A B + beta C0, every group a different
+// channel (and solve).  A complex multiply-accumulate is four real MFMAs:
+// re += ar br, re += (-ai) bi, im += ar bi, im += ai br.
 // ---------------------------------------------------------------------------
+// Loads through global-address-space pointers: the compiler then emits
+// global_load (not flat_load) and can count outstanding loads precisely.
+typedef double gdouble2 __attribute__((ext_vector_type(2)));
+typedef float gfloat2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ double2 gload(const double2 *p) {
+  gdouble2 v = *(const __attribute__((address_space(1))) gdouble2 *)p;
+  return make_double2(v.x, v.y);
+}
+__device__ __forceinline__ float2 gload(const float2 *p) {
+  gfloat2 v = *(const __attribute__((address_space(1))) gfloat2 *)p;
+  return make_float2(v.x, v.y);
+}
 template <typename AT>
 __device__ __forceinline__ double2 load_a(const AT *p);
 template <>
-__device__ __forceinline__ double2 load_a<double2>(const double2 *p) { return *p; }
+__device__ __forceinline__ double2 load_a<double2>(const double2 *p) { return gload(p); }
 template <>
-__device__ __forceinline__ double2 load_a<float2>(const float2 *p) { float2 v = *p; return make_double2(v.x, v.y); }
-
-template <int BM, int BN, int BK, typename AT>
-__global__ __launch_bounds__(256) void k_zgemm_f64(const GemmGroup *__restrict__ groups,
-                                                   const GemmTile *__restrict__ tiles,
-                                                   const SolveDesc *__restrict__ descs,
-                                                   const int *__restrict__ gsolve) {
-  static_assert(BM == 64 && BN == 64 && BK == 16, "tile shape");
-  const GemmTile tl = tiles[blockIdx.x];
-  if (descs && !descs[gsolve[tl.g]].active) return;
-  const GemmGroup g = groups[tl.g];
-  const AT *A = (const AT *)g.A;
-  __shared__ double As[2][BK][BM + 2];
-  __shared__ double Bs[2][BK][BN + 2];
-  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-  const int wm = w >> 1, wn = w & 1;
-  d4 cre[2][2], cim[2][2];
-  for (int x = 0; x < 2; ++x)
-    for (int y = 0; y < 2; ++y) { cre[x][y] = (d4){0, 0, 0, 0}; cim[x][y] = (d4){0, 0, 0, 0}; }
-  const int arow = tid >> 2, akq = (tid & 3) * 4;
-  const int bk = tid >> 4, bnq = (tid & 15) * 4;
-  for (int k0 = 0; k0 < g.K; k0 += BK) {
-    {
-      const int m = tl.m0 + arow;
-#pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        const int k = k0 + akq + q;
-        double2 v = make_double2(0.0, 0.0);
-        if (m < g.M && k < g.K) v = load_a<AT>(A + (int64_t)m * g.lda + k);
-        As[0][akq + q][arow] = v.x;
-        As[1][akq + q][arow] = v.y;
-      }
-      const int k = k0 + bk;
-#pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        const int n = tl.n0 + bnq + q;
-        double2 v = make_double2(0.0, 0.0);
-        if (k < g.K && n < g.N) v = g.B[(int64_t)k * g.ldb + n];
-        Bs[0][bk][bnq + q] = v.x;
-        Bs[1][bk][bnq + q] = v.y;
-      }
-    }
-    __syncthreads();
-#pragma unroll
-    for (int kk = 0; kk < BK; kk += 4) {
-      const int kr = kk + (lane >> 4), mi = lane & 15;
-      double ar[2], ai[2], br[2], bi[2];
-#pragma unroll
-      for (int x = 0; x < 2; ++x) {
-        ar[x] = As[0][kr][wm * 32 + x * 16 + mi];
-        ai[x] = As[1][kr][wm * 32 + x * 16 + mi];
-      }
-#pragma unroll
-      for (int y = 0; y < 2; ++y) {
-        br[y] = Bs[0][kr][wn * 32 + y * 16 + mi];
-        bi[y] = Bs[1][kr][wn * 32 + y * 16 + mi];
-      }
-#pragma unroll
-      for (int x = 0; x < 2; ++x)
-#pragma unroll
-        for (int y = 0; y < 2; ++y) {
-          cre[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[x], br[y], cre[x][y], 0, 0, 0);
-          cre[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(-ai[x], bi[y], cre[x][y], 0, 0, 0);
-          cim[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[x], bi[y], cim[x][y], 0, 0, 0);
-          cim[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai[x], br[y], cim[x][y], 0, 0, 0);
-        }
-    }
-    __syncthreads();
-  }
-  // f64 16x16x4 result layout: register r of lane l holds row (l>>4) + 4r,
-  // column l&15 (the f32 16x16x4 variant uses row 4(l>>4) + r)
-#pragma unroll
-  for (int x = 0; x < 2; ++x)
-#pragma unroll
-    for (int y = 0; y < 2; ++y)
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int m = tl.m0 + wm * 32 + x * 16 + (lane >> 4) + 4 * r;
-        const int n = tl.n0 + wn * 32 + y * 16 + (lane & 15);
-        if (m < g.M && n < g.N) {
-          double2 v = make_double2(g.alpha * cre[x][y][r], g.alpha * cim[x][y][r]);
-          if (g.beta != 0.0) {
-            double2 c0 = g.C0[(int64_t)m * g.ldc0 + n];
-            v.x += g.beta * c0.x;
-            v.y += g.beta * c0.y;
-          }
-          g.C[(int64_t)m * g.ldc + n] = v;
-        }
-      }
-}
-
-// Same contract with FP32 MFMA and FP32 accumulation: A is read as complex float
-// (stored) or double (rounded on load), B is rounded to float on load, and the
-// result is added to C0 in FP64.
-template <int BM, int BN, int BK, typename AT>
-__global__ __launch_bounds__(256) void k_zgemm_f32(const GemmGroup *__restrict__ groups,
-                                                   const GemmTile *__restrict__ tiles,
-                                                   const SolveDesc *__restrict__ descs,
-                                                   const int *__restrict__ gsolve) {
-  static_assert(BM == 64 && BN == 64 && BK == 16, "tile shape");
-  const GemmTile tl = tiles[blockIdx.x];
-  if (descs && !descs[gsolve[tl.g]].active) return;
-  const GemmGroup g = groups[tl.g];
-  const AT *A = (const AT *)g.A;
-  __shared__ float As[2][BK][BM + 4];
-  __shared__ float Bs[2][BK][BN + 4];
-  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-  const int wm = w >> 1, wn = w & 1;
-  f4 cre[2][2], cim[2][2];
-  for (int x = 0; x < 2; ++x)
-    for (int y = 0; y < 2; ++y) { cre[x][y] = (f4){0, 0, 0, 0}; cim[x][y] = (f4){0, 0, 0, 0}; }
-  const int arow = tid >> 2, akq = (tid & 3) * 4;
-  const int bk = tid >> 4, bnq = (tid & 15) * 4;
-  for (int k0 = 0; k0 < g.K; k0 += BK) {
-    {
-      const int m = tl.m0 + arow;
-#pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        const int k = k0 + akq + q;
-        double2 v = make_double2(0.0, 0.0);
-        if (m < g.M && k < g.K) v = load_a<AT>(A + (int64_t)m * g.lda + k);
-        As[0][akq + q][arow] = (float)v.x;
-        As[1][akq + q][arow] = (float)v.y;
-      }
-      const int k = k0 + bk;
-#pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        const int n = tl.n0 + bnq + q;
-        double2 v = make_double2(0.0, 0.0);
-        if (k < g.K && n < g.N) v = g.B[(int64_t)k * g.ldb + n];
-        Bs[0][bk][bnq + q] = (float)v.x;
-        Bs[1][bk][bnq + q] = (float)v.y;
-      }
-    }
-    __syncthreads();
-#pragma unroll
-    for (int kk = 0; kk < BK; kk += 4) {
-      const int kr = kk + (lane >> 4), mi = lane & 15;
-      float ar[2], ai[2], br[2], bi[2];
-#pragma unroll
-      for (int x = 0; x < 2; ++x) {
-        ar[x] = As[0][kr][wm * 32 + x * 16 + mi];
-        ai[x] = As[1][kr][wm * 32 + x * 16 + mi];
-      }
-#pragma unroll
-      for (int y = 0; y < 2; ++y) {
-        br[y] = Bs[0][kr][wn * 32 + y * 16 + mi];
-        bi[y] = Bs[1][kr][wn * 32 + y * 16 + mi];
-      }
-#pragma unroll
-      for (int x = 0; x < 2; ++x)
-#pragma unroll
-        for (int y = 0; y < 2; ++y) {
-          cre[x][y] = __builtin_amdgcn_mfma_f32_16x16x4f32(ar[x], br[y], cre[x][y], 0, 0, 0);
-          cre[x][y] = __builtin_amdgcn_mfma_f32_16x16x4f32(-ai[x], bi[y], cre[x][y], 0, 0, 0);
-          cim[x][y] = __builtin_amdgcn_mfma_f32_16x16x4f32(ar[x], bi[y], cim[x][y], 0, 0, 0);
-          cim[x][y] = __builtin_amdgcn_mfma_f32_16x16x4f32(ai[x], br[y], cim[x][y], 0, 0, 0);
-        }
-    }
-    __syncthreads();
-  }
-#pragma unroll
-  for (int x = 0; x < 2; ++x)
-#pragma unroll
-    for (int y = 0; y < 2; ++y)
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int m = tl.m0 + wm * 32 + x * 16 + (lane >> 4) * 4 + r;
-        const int n = tl.n0 + wn * 32 + y * 16 + (lane & 15);
-        if (m < g.M && n < g.N) {
-          double2 v = make_double2(g.alpha * (double)cre[x][y][r], g.alpha * (double)cim[x][y][r]);
-          if (g.beta != 0.0) {
-            double2 c0 = g.C0[(int64_t)m * g.ldc0 + n];
-            v.x += g.beta * c0.x;
-            v.y += g.beta * c0.y;
-          }
-          g.C[(int64_t)m * g.ldc + n] = v;
-        }
-      }
-}
+__device__ __forceinline__ double2 load_a<float2>(const float2 *p) { float2 v = gload(p); return make_double2(v.x, v.y); }
 
 // Plain FP64 VALU GEMM with the same contract, kept as an independent check of
 // the MFMA lane layouts (selected with CCGPU_GEMM=valu).
@@ -719,6 +591,150 @@ __global__ __launch_bounds__(256) void k_zgemm_valu(const GemmGroup *__restrict_
         g.C[(int64_t)m * g.ldc + n] = v;
       }
     }
+}
+
+// Grouped complex GEMM with direct-to-register operand loads.  The streamed
+// operand A (one read per element) never touches LDS; B (a channel's amplitude
+// block, shared by every row tile of the channel) comes through L2.  The four
+// wavefronts of a workgroup stack along M, each owning MB 16-row blocks and the
+// workgroup's NB 16-column blocks.  The sum over k of one 16x16x4 MFMA is
+// permutation invariant, so lane group g = lane>>4 takes the k indices
+// k0+4g..k0+4g+3 of a 16-wide k chunk (one 64-byte read of A per lane, a full
+// 128-byte line per row across four lanes) and the four MFMAs of the chunk run
+// over those four k values.
+template <int MB, int NB, bool F32, typename AT>
+__global__ __launch_bounds__(256) void k_zgemm_dl(const GemmGroup *__restrict__ groups,
+                                                  const GemmTile *__restrict__ tiles,
+                                                  const SolveDesc *__restrict__ descs,
+                                                  const int *__restrict__ gsolve) {
+  typedef typename std::conditional<F32, float, double>::type real;
+  typedef typename std::conditional<F32, f4, d4>::type acc4;
+  const GemmTile tl = tiles[blockIdx.x];
+  if (descs && !descs[gsolve[tl.g]].active) return;
+  const GemmGroup g = groups[tl.g];
+  const AT *A = (const AT *)g.A;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int li = lane & 15, lg = lane >> 4;
+  const int mw = tl.m0 + w * (MB * 16);
+  acc4 cre[MB][NB], cim[MB][NB];
+#pragma unroll
+  for (int x = 0; x < MB; ++x)
+#pragma unroll
+    for (int y = 0; y < NB; ++y) {
+      cre[x][y] = (acc4){0, 0, 0, 0};
+      cim[x][y] = (acc4){0, 0, 0, 0};
+    }
+  double2 a0[MB][4], b0[NB][4], a1[MB][4], b1[NB][4];
+  const int kb = tl.kb, ke = min(tl.ke, g.K);
+  // branch-free loads: out-of-range indices are clamped to a valid element and
+  // the value is zeroed, so all loads of a chunk issue back to back
+  auto load = [&](double2 (&a)[MB][4], double2 (&b)[NB][4], int k0) {
+#pragma unroll
+    for (int x = 0; x < MB; ++x) {
+      const int m = mw + x * 16 + li;
+      const int mc = min(m, g.M - 1);
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int k = k0 + lg * 4 + q;
+        const double2 v = load_a<AT>(A + (int64_t)mc * g.lda + min(k, ke - 1));
+        const bool ok = m < g.M && k < ke;
+        a[x][q] = make_double2(ok ? v.x : 0.0, ok ? v.y : 0.0);
+      }
+    }
+#pragma unroll
+    for (int y = 0; y < NB; ++y) {
+      const int n = tl.n0 + y * 16 + li;
+      const int nc = min(n, g.N - 1);
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int k = k0 + lg * 4 + q;
+        const double2 v = gload(g.B + (int64_t)min(k, ke - 1) * g.ldb + nc);
+        const bool ok = n < g.N && k < ke;
+        b[y][q] = make_double2(ok ? v.x : 0.0, ok ? v.y : 0.0);
+      }
+    }
+  };
+  auto compute = [&](const double2 (&a)[MB][4], const double2 (&b)[NB][4]) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+#pragma unroll
+      for (int x = 0; x < MB; ++x) {
+        const real ar = (real)a[x][q].x, ai = (real)a[x][q].y;
+#pragma unroll
+        for (int y = 0; y < NB; ++y) {
+          const real br = (real)b[y][q].x, bi = (real)b[y][q].y;
+          if constexpr (F32) {
+            cre[x][y] = __builtin_amdgcn_mfma_f32_16x16x4f32(ar, br, cre[x][y], 0, 0, 0);
+            cre[x][y] = __builtin_amdgcn_mfma_f32_16x16x4f32(-ai, bi, cre[x][y], 0, 0, 0);
+            cim[x][y] = __builtin_amdgcn_mfma_f32_16x16x4f32(ar, bi, cim[x][y], 0, 0, 0);
+            cim[x][y] = __builtin_amdgcn_mfma_f32_16x16x4f32(ai, br, cim[x][y], 0, 0, 0);
+          } else {
+            cre[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar, br, cre[x][y], 0, 0, 0);
+            cre[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(-ai, bi, cre[x][y], 0, 0, 0);
+            cim[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar, bi, cim[x][y], 0, 0, 0);
+            cim[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai, br, cim[x][y], 0, 0, 0);
+          }
+        }
+      }
+    }
+  };
+  // two statically indexed register buffers: the next 16-wide k chunk is in
+  // flight while the current one feeds the MFMAs
+  load(a0, b0, kb);
+  for (int k0 = kb; k0 < ke; k0 += 32) {
+    if (k0 + 16 < ke) load(a1, b1, k0 + 16);
+    compute(a0, b0);
+    if (k0 + 16 >= ke) break;
+    if (k0 + 32 < ke) load(a0, b0, k0 + 32);
+    compute(a1, b1);
+  }
+  // result layouts of the 16x16x4 MFMAs: register r of lane l holds column l&15
+  // and row (l>>4)+4r for f64, row 4(l>>4)+r for f32
+#pragma unroll
+  for (int x = 0; x < MB; ++x)
+#pragma unroll
+    for (int y = 0; y < NB; ++y)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int row = F32 ? (lg * 4 + r) : (lg + 4 * r);
+        const int m = mw + x * 16 + row;
+        const int n = tl.n0 + y * 16 + li;
+        if (m < g.M && n < g.N && g.nsplit > 1) {
+          g.P[((int64_t)tl.ks * g.M + m) * g.N + n] = make_double2((double)cre[x][y][r], (double)cim[x][y][r]);
+        } else if (m < g.M && n < g.N) {
+          double2 v = make_double2(g.alpha * (double)cre[x][y][r], g.alpha * (double)cim[x][y][r]);
+          if (g.beta != 0.0) {
+            double2 c0 = g.C0[(int64_t)m * g.ldc0 + n];
+            v.x += g.beta * c0.x;
+            v.y += g.beta * c0.y;
+          }
+          g.C[(int64_t)m * g.ldc + n] = v;
+        }
+      }
+}
+
+// Fixed-order sum of the split-K partials: C = alpha sum_s P_s + beta C0.
+__global__ void k_splitk_reduce(const GemmGroup *__restrict__ groups, const int *__restrict__ gidx,
+                                const SolveDesc *__restrict__ descs, const int *__restrict__ gsolve) {
+  const int gi = gidx[blockIdx.y];
+  if (descs && !descs[gsolve[gi]].active) return;
+  const GemmGroup g = groups[gi];
+  const int64_t mn = (int64_t)g.M * g.N;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < mn; e += (int64_t)gridDim.x * blockDim.x) {
+    double2 acc = g.P[e];
+    for (int s = 1; s < g.nsplit; ++s) acc = cadd(acc, g.P[s * mn + e]);
+    const int m = (int)(e / g.N), n = (int)(e % g.N);
+    double2 v = cscale(g.alpha, acc);
+    if (g.beta != 0.0) v = cadd(v, cscale(g.beta, g.C0[(int64_t)m * g.ldc0 + n]));
+    g.C[(int64_t)m * g.ldc + n] = v;
+  }
+}
+
+void launch_splitk_reduce(const GemmGroup *groups, const int *gidx, int n, int64_t maxmn, const SolveDesc *dd,
+                          const int *gsolve, hipStream_t st) {
+  if (n <= 0) return;
+  unsigned gx = (unsigned)std::max<int64_t>(1, std::min<int64_t>((maxmn + 255) / 256, 64));
+  hipLaunchKernelGGL(k_splitk_reduce, dim3(gx, n), dim3(256), 0, st, groups, gidx, dd, gsolve);
 }
 
 // Down-conversion of the stored pppp blocks for the FP32 ladder.
@@ -772,10 +788,12 @@ void launch_energy(const SolveDesc *dd, int nsolve, int slot, hipStream_t st) {
 }
 
 void launch_i7(const SolveDesc *dd, int nsolve, int maxA, hipStream_t st) {
-  hipLaunchKernelGGL(k_i7, dim3((2 * maxA + 127) / 128, nsolve), dim3(128), 0, st, dd);
+  hipLaunchKernelGGL(k_i7, dim3((2 * maxA + 3) / 4, nsolve), dim3(256), 0, st, dd);
 }
-void launch_i6(const SolveDesc *dd, int nsolve, int maxnp, hipStream_t st) {
-  hipLaunchKernelGGL(k_i6, dim3((2 * maxnp + 3) / 4, nsolve), dim3(256), 0, st, dd);
+void launch_i6(const SolveDesc *dd, int nsolve, int maxnp, int64_t maxrows, hipStream_t st) {
+  unsigned gx = (unsigned)std::max<int64_t>(1, std::min<int64_t>((maxrows * 16 + 255) / 256, 4096));
+  hipLaunchKernelGGL(k_i6_rows, dim3(gx, nsolve), dim3(256), 0, st, dd);
+  hipLaunchKernelGGL(k_i6_reduce, dim3((2 * maxnp + 3) / 4, nsolve), dim3(256), 0, st, dd);
 }
 void launch_i9(const SolveDesc *dd, int nsolve, int64_t maxvhh, hipStream_t st) {
   hipLaunchKernelGGL(k_i9, egrid(maxvhh, nsolve), dim3(256), 0, st, dd);
@@ -793,8 +811,19 @@ void launch_update(const SolveDesc *dd, int nsolve, int64_t maxT, const int *slo
   hipLaunchKernelGGL(k_update, egrid(maxT, nsolve), dim3(256), 0, st, dd, slot);
 }
 void launch_gram(const SolveDesc *dd, int nsolve, const int *slots, int nstep, hipStream_t st) {
-  hipLaunchKernelGGL(k_gram_partial, dim3(kRedBlocks, nsolve), dim3(256), 0, st, dd, slots, nstep);
   int ne = nstep - 1;
+  dim3 gr(kRedBlocks, nsolve);
+  switch (ne) {
+    case 1: hipLaunchKernelGGL(k_gram_partial<1>, gr, dim3(256), 0, st, dd, slots); break;
+    case 2: hipLaunchKernelGGL(k_gram_partial<2>, gr, dim3(256), 0, st, dd, slots); break;
+    case 3: hipLaunchKernelGGL(k_gram_partial<3>, gr, dim3(256), 0, st, dd, slots); break;
+    case 4: hipLaunchKernelGGL(k_gram_partial<4>, gr, dim3(256), 0, st, dd, slots); break;
+    case 5: hipLaunchKernelGGL(k_gram_partial<5>, gr, dim3(256), 0, st, dd, slots); break;
+    case 6: hipLaunchKernelGGL(k_gram_partial<6>, gr, dim3(256), 0, st, dd, slots); break;
+    case 7: hipLaunchKernelGGL(k_gram_partial<7>, gr, dim3(256), 0, st, dd, slots); break;
+    case 8: hipLaunchKernelGGL(k_gram_partial<8>, gr, dim3(256), 0, st, dd, slots); break;
+    default: return;
+  }
   int nv = ne * (ne + 1) / 2;
   hipLaunchKernelGGL(k_reduce_final, dim3(1, nsolve), dim3(256), 0, st, dd, kRedBlocks, 36, nv, 4);
 }
@@ -807,18 +836,26 @@ void launch_gemm(const GemmGroup *groups, const GemmTile *tiles, int ntiles, con
                  const int *gsolve, int variant, hipStream_t st) {
   if (ntiles <= 0) return;
   switch (variant) {
-    case 0:
-      hipLaunchKernelGGL((k_zgemm_f64<64, 64, 16, double2>), dim3(ntiles), dim3(256), 0, st, groups, tiles, dd, gsolve);
-      break;
-    case 1:
-      hipLaunchKernelGGL((k_zgemm_f32<64, 64, 16, double2>), dim3(ntiles), dim3(256), 0, st, groups, tiles, dd, gsolve);
-      break;
-    case 2:
-      hipLaunchKernelGGL((k_zgemm_f32<64, 64, 16, float2>), dim3(ntiles), dim3(256), 0, st, groups, tiles, dd, gsolve);
-      break;
     case 3:
       hipLaunchKernelGGL(k_zgemm_valu, dim3(ntiles), dim3(256), 0, st, groups, tiles, dd, gsolve);
       break;
+    // direct-load MFMA kernels: 10+nb FP64, 20+nb FP32 (A double), 30+nb FP32 (A float);
+    // nb = 16-column blocks per tile (1,2 with two 16-row blocks per wave, 3-5 with one)
+    case 11: hipLaunchKernelGGL((k_zgemm_dl<2, 1, false, double2>), dim3(ntiles), dim3(256), 0, st, groups, tiles, dd, gsolve); break;
+    case 12: hipLaunchKernelGGL((k_zgemm_dl<2, 2, false, double2>), dim3(ntiles), dim3(256), 0, st, groups, tiles, dd, gsolve); break;
+    case 13: hipLaunchKernelGGL((k_zgemm_dl<1, 3, false, double2>), dim3(ntiles), dim3(256), 0, st, groups, tiles, dd, gsolve); break;
+    case 14: hipLaunchKernelGGL((k_zgemm_dl<1, 4, false, double2>), dim3(ntiles), dim3(256), 0, st, groups, tiles, dd, gsolve); break;
+    case 15: hipLaunchKernelGGL((k_zgemm_dl<1, 5, false, double2>), dim3(ntiles), dim3(256), 0, st, groups, tiles, dd, gsolve); break;
+    case 21: hipLaunchKernelGGL((k_zgemm_dl<2, 1, true, double2>), dim3(ntiles), dim3(256), 0, st, groups, tiles, dd, gsolve); break;
+    case 22: hipLaunchKernelGGL((k_zgemm_dl<2, 2, true, double2>), dim3(ntiles), dim3(256), 0, st, groups, tiles, dd, gsolve); break;
+    case 23: hipLaunchKernelGGL((k_zgemm_dl<1, 3, true, double2>), dim3(ntiles), dim3(256), 0, st, groups, tiles, dd, gsolve); break;
+    case 24: hipLaunchKernelGGL((k_zgemm_dl<1, 4, true, double2>), dim3(ntiles), dim3(256), 0, st, groups, tiles, dd, gsolve); break;
+    case 25: hipLaunchKernelGGL((k_zgemm_dl<1, 5, true, double2>), dim3(ntiles), dim3(256), 0, st, groups, tiles, dd, gsolve); break;
+    case 31: hipLaunchKernelGGL((k_zgemm_dl<2, 1, true, float2>), dim3(ntiles), dim3(256), 0, st, groups, tiles, dd, gsolve); break;
+    case 32: hipLaunchKernelGGL((k_zgemm_dl<2, 2, true, float2>), dim3(ntiles), dim3(256), 0, st, groups, tiles, dd, gsolve); break;
+    case 33: hipLaunchKernelGGL((k_zgemm_dl<1, 3, true, float2>), dim3(ntiles), dim3(256), 0, st, groups, tiles, dd, gsolve); break;
+    case 34: hipLaunchKernelGGL((k_zgemm_dl<1, 4, true, float2>), dim3(ntiles), dim3(256), 0, st, groups, tiles, dd, gsolve); break;
+    case 35: hipLaunchKernelGGL((k_zgemm_dl<1, 5, true, float2>), dim3(ntiles), dim3(256), 0, st, groups, tiles, dd, gsolve); break;
     default:
       break;
   }

@@ -196,6 +196,30 @@ void build_plan(const Orbitals &orb, Plan &P) {
     P.ob_colI[s] = vI;
   }
 
+  // I6 partial lists
+  {
+    size_t nr = P.pp_a.size();
+    P.row_ch.resize(nr);
+    for (int ch = 0; ch < P.nch; ++ch)
+      for (int64_t r = P.off_pp[ch]; r < P.off_pp[ch + 1]; ++r) P.row_ch[r] = ch;
+    std::vector<std::vector<int>> dl(np), ol(np);
+    for (size_t r = 0; r < nr; ++r) {
+      int x = P.pp_a[r], y = P.pp_b[r];
+      dl[y - A].push_back((int)(3 * r + 0));
+      dl[x - A].push_back((int)(3 * r + 0));
+      if (P.ob_rowB[r] >= 0) ol[y - A].push_back((int)(3 * r + 1));
+      if (P.ob_rowA[r] >= 0) ol[x - A].push_back((int)(3 * r + 2));
+    }
+    P.i6_dptr = {0};
+    P.i6_optr = {0};
+    for (int a = 0; a < np; ++a) {
+      for (int v : dl[a]) P.i6_didx.push_back(v);
+      for (int v : ol[a]) P.i6_oidx.push_back(v);
+      P.i6_dptr.push_back((int)P.i6_didx.size());
+      P.i6_optr.push_back((int)P.i6_oidx.size());
+    }
+  }
+
   // Pandya channels: ph pairs (a,i) with K' = n_a - n_i, hp pairs (j,b) with n_j - n_b
   auto keyd = [&](int x, int y) {
     return Key{orb.tz[x] - orb.tz[y], orb.nx[x] - orb.nx[y], orb.ny[x] - orb.ny[y], orb.nz[x] - orb.nz[y]};

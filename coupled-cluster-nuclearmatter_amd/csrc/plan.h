@@ -54,6 +54,11 @@ struct Plan {
   // rows of (a, partner(b)) and (b, partner(a)) with signs (0 = absent), and for
   // each unique hh pair (i<j) the columns of (i, partner(j)) and (j, partner(i)).
   std::vector<int> ob_rowB, ob_rowA, ob_colJ, ob_colI;   // packed: idx*4 + (sign+1)
+  // I6 row partials (3 per unique pp row (x<y): diag of x and y, I6(y,ybar),
+  // I6(x,xbar)) and, per particle a, the CSR lists of partials that sum into
+  // I6(a,a) and I6(a,abar), in increasing row order
+  std::vector<int> row_ch;
+  std::vector<int> i6_dptr, i6_didx, i6_optr, i6_oidx;
 
   // ---- Pandya channels K' = n_a - n_i ----
   int nphc = 0;

@@ -30,6 +30,7 @@ module ccgpu_iface
      integer(c_int) :: subspace, diis_step
      real(c_double) :: tol
      integer(c_int) :: maxit, precision, refine_every, use_graph, verbose
+     real(c_double) :: fp32_switch
   end type ccgpu_solve_opts
 
   type, bind(C) :: ccgpu_solve_result

@@ -92,6 +92,8 @@ typedef struct {
   int refine_every;  /* mixed precision: FP64 residual every k iterations */
   int use_graph;     /* capture one iteration in a hipGraph and replay it */
   int verbose;       /* print per-iteration lines like the reference */
+  double fp32_switch;/* mixed precision: FP32 contractions until max |dE| <= this
+                        (MeV), then DIIS restarts and FP64 finishes the solve */
 } ccgpu_solve_opts;
 
 typedef struct {
