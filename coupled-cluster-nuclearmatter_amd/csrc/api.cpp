@@ -13,6 +13,7 @@
 #include <map>
 #include <memory>
 #include <mutex>
+#include <tuple>
 #include <stdexcept>
 #include <string>
 #include <unordered_map>
@@ -45,6 +46,18 @@ void launch_gemm(const GemmGroup *groups, const GemmTile *tiles, int ntiles, con
                  const int *gsolve, int variant, hipStream_t st);
 void launch_splitk_reduce(const GemmGroup *groups, const int *gidx, int n, int64_t maxmn, const SolveDesc *dd,
                           const int *gsolve, hipStream_t st);
+struct TriplesDesc {
+  int k0, nk, tz0, ntz;
+  const int *chan;
+  const int *hptr;
+  const int *hijk;
+  const int *orbitof;
+  int nmax;
+  const int *pair_a, *pair_b;
+  int64_t npairs;
+};
+void launch_triples(const SolveDesc &d, const TriplesDesc &td, double volume, double2 *partial, int nblocks,
+                    double2 *out, hipStream_t st);
 }  // namespace ccgpu
 
 using namespace ccgpu;
@@ -209,6 +222,10 @@ struct Solve {
   SolveDesc *d_desc = nullptr;
   hipStream_t stream = nullptr;
   bool have_v = false, have_fock = false, derived = false, have_t = false;
+  bool native_v = false;   // V from the built-in generator (needed by (T))
+  bool have_t3 = false;
+  TriplesDesc t3{};
+  double2 *t3_partial = nullptr, *t3_out = nullptr;
   double2 e0{0, 0};
   // hhphp records of the drop-in path: key (i,b,j,a) -> <ib||ja>
   std::unordered_map<uint64_t, std::complex<double>> hphp;
@@ -941,6 +958,7 @@ int ccgpu_generate_interaction(int handle, int kind) {
     HIPCHK(hipMemcpyAsync(&S.e0, S.desc.e0, sizeof(double2), hipMemcpyDeviceToHost, S.stream));
     HIPCHK(hipStreamSynchronize(S.stream));
     S.have_v = S.have_fock = S.derived = true;
+    S.native_v = true;
   });
 }
 
@@ -1019,6 +1037,7 @@ int ccgpu_set_block(int handle, int kind, int nbra, const int *bra_p, const int 
       HIPCHK(hipMemcpy(dst, h.data(), n * sizeof(double2), hipMemcpyHostToDevice));
     }
     S.have_v = true;
+    S.native_v = false;
     S.derived = false;
   });
 }
@@ -1256,11 +1275,78 @@ int ccgpu_get_t2_labels(int handle, int *abij) {
   });
 }
 
+// (K3,Tz3) hole-triple channels, orbital lookup and particle-pair list of
+// setup_t3_channel_structures (ccm_t3full_channels.f90:4-284), built once.
+void build_triples(Solve &S) {
+  const Plan &P = S.plan;
+  const Orbitals &o = P.orb;
+  int nh = 0;
+  for (int i = 0; i < P.A; ++i) nh = std::max({nh, std::abs(o.nx[i]), std::abs(o.ny[i]), std::abs(o.nz[i])});
+  TriplesDesc &td = S.t3;
+  td.k0 = -3 * nh;
+  td.nk = 6 * nh + 1;
+  td.tz0 = -3;
+  td.ntz = 7;
+  td.nmax = o.nmax;
+  std::vector<int> chan((size_t)td.ntz * td.nk * td.nk * td.nk, -1);
+  std::map<std::tuple<int, int, int, int>, std::vector<int>> trip;
+  for (int i = 0; i < P.A; ++i)
+    for (int j = i + 1; j < P.A; ++j)
+      for (int k = j + 1; k < P.A; ++k) {
+        auto key = std::make_tuple(o.tz[i] + o.tz[j] + o.tz[k], o.nx[i] + o.nx[j] + o.nx[k],
+                                   o.ny[i] + o.ny[j] + o.ny[k], o.nz[i] + o.nz[j] + o.nz[k]);
+        auto &v = trip[key];
+        v.push_back(i); v.push_back(j); v.push_back(k);
+      }
+  std::vector<int> hptr = {0}, hijk;
+  int ch = 0;
+  for (auto &kv : trip) {
+    int tz, kx, ky, kz;
+    std::tie(tz, kx, ky, kz) = kv.first;
+    size_t idx = (((size_t)(tz - td.tz0) * td.nk + (kx - td.k0)) * td.nk + (ky - td.k0)) * td.nk + (kz - td.k0);
+    chan[idx] = ch++;
+    hijk.insert(hijk.end(), kv.second.begin(), kv.second.end());
+    hptr.push_back((int)(hijk.size() / 3));
+  }
+  const int n = 2 * o.nmax + 1;
+  std::vector<int> orbitof((size_t)n * n * n * 4, -1);
+  for (int x = 0; x < P.nsp; ++x) {
+    size_t idx = (((size_t)(o.nx[x] + o.nmax) * n + (o.ny[x] + o.nmax)) * n + (o.nz[x] + o.nmax)) * 4 +
+                 ((o.sz[x] + 1) / 2) * 2 + (o.tz[x] + 1) / 2;
+    orbitof[idx] = x;
+  }
+  std::vector<int> pa, pb;
+  for (int a = P.A; a < P.nsp; ++a)
+    for (int b = a + 1; b < P.nsp; ++b) { pa.push_back(a); pb.push_back(b); }
+  td.chan = dupload(chan, S.owned);
+  td.hptr = dupload(hptr, S.owned);
+  td.hijk = dupload(hijk, S.owned);
+  td.orbitof = dupload(orbitof, S.owned);
+  td.pair_a = dupload(pa, S.owned);
+  td.pair_b = dupload(pb, S.owned);
+  td.npairs = (int64_t)pa.size();
+  S.t3_partial = dalloc<double2>(8192, S.owned);
+  S.t3_out = dalloc<double2>(1, S.owned);
+  S.have_t3 = true;
+}
+
 int ccgpu_triples(int handle, double *et) {
   return guarded([&] {
-    (void)handle;
-    (void)et;
-    throw Error("ccgpu_triples: not available in this build");
+    Solve &S = get(handle);
+    ensure_ready(S);
+    if (!S.have_t) throw Error("no amplitudes: solve CCD first");
+    if (!S.native_v)
+      throw Error("ccgpu_triples evaluates <cb||kd> and <jk||lc> with the built-in generator; "
+                  "handles filled through ccgpu_set_block are not supported");
+    if (!S.have_t3) build_triples(S);
+    int nb = (int)std::min<int64_t>(8192, std::max<int64_t>(1, S.t3.npairs));
+    launch_triples(S.desc, S.t3, S.plan.orb.volume, S.t3_partial, nb, S.t3_out, S.stream);
+    check_launch();
+    double2 r;
+    HIPCHK(hipMemcpyAsync(&r, S.t3_out, sizeof(double2), hipMemcpyDeviceToHost, S.stream));
+    HIPCHK(hipStreamSynchronize(S.stream));
+    et[0] = r.x;
+    et[1] = r.y;
   });
 }
 

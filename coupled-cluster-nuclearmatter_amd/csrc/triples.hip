@@ -1,0 +1,192 @@
+// Perturbative triples (T) on the GPU: pert_triples2 (ccm_triples.f90:528-735)
+// with three_body_oneparticle_1 / three_body_onehole_1 (ccm_triples.f90:174-294)
+// and the (K3, Tz3) channels of setup_t3_channel_structures
+// (ccm_t3full_channels.f90:4-284).
+//
+// E(T) = sum over channels, a<b<c and i<j<k of conj(W) W / D with
+//   D = f_ii + f_jj + f_kk - f_aa - f_bb - f_cc            (diagonal Fock)
+//   W = sum of 9 signed F terms - sum of 9 signed H terms
+//   F(a,b,c,i,j,k) = sum_d conj(<cb||kd>) t(ad,ij),  n_d = n_i + n_j - n_a
+//   H(a,b,c,i,j,k) = sum_l conj(<jk||lc>) t(ab,il),  n_l = n_k + n_j - n_c
+// (d runs over the particle spin states of its momentum, l over the holes).
+// The interaction elements are evaluated in place by the same generator that
+// filled the V blocks.
+//
+// Work decomposition: one thread per particle triple (a<b<c): block x = the
+// particle pair (a,b), threads stride over c.  A thread whose (K3,Tz3) has hole
+// triples walks that channel's hole-triple list and accumulates its own sum in
+// fixed order; blocks reduce with a fixed tree and a final pass adds the block
+// partials in order, so the result is deterministic.
+#include "device.h"
+#include "interaction.h"
+
+#include <algorithm>
+
+namespace ccgpu {
+
+struct TriplesDesc {
+  // channel lookup over (Tz3, K3): index = ((tz - tz0) * nk + (kx - k0)) * nk^2 + ...
+  int k0, nk, tz0, ntz;
+  const int *chan;        // dense table -> channel or -1
+  const int *hptr;        // CSR over channels of hole triples
+  const int *hijk;        // 3 ints per hole triple
+  const int *orbitof;     // dense (n, sz, tz) -> orbital, -1 absent
+  int nmax;
+  const int *pair_a, *pair_b;  // every particle pair a<b (all momenta)
+  int64_t npairs;
+};
+
+__device__ __forceinline__ double minnesota_direct_t(const SolveDesc &d, double volume, int p, int q, int r, int s) {
+  Orb o;
+  o.kx = d.kx; o.ky = d.ky; o.kz = d.kz; o.sz = d.sz; o.tz = d.tz; o.volume = volume;
+  return minnesota_direct(o, p, q, r, s);
+}
+
+__device__ __forceinline__ double2 tcmul(double2 a, double2 b) {
+  return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+
+__device__ __forceinline__ int orb_lookup(const TriplesDesc &td, int nx, int ny, int nz, int sz, int tz) {
+  const int n = 2 * td.nmax + 1;
+  if (nx < -td.nmax || nx > td.nmax || ny < -td.nmax || ny > td.nmax || nz < -td.nmax || nz > td.nmax) return -1;
+  if (tz < -1 || tz > 1 || (tz & 1) == 0) return -1;
+  int idx = (((nx + td.nmax) * n + (ny + td.nmax)) * n + (nz + td.nmax)) * 4 + ((sz + 1) / 2) * 2 + (tz + 1) / 2;
+  return td.orbitof[idx];
+}
+
+// amplitude t(ab,ij) for any ordering, 0 when absent
+__device__ __forceinline__ double2 t_of(const SolveDesc &d, int a, int b, int i, int j) {
+  if (a == b || i == j) return make_double2(0.0, 0.0);
+  DPair p = d.ppref[(size_t)(a - d.A) * d.np + (b - d.A)];
+  DPair h = d.hhref[(size_t)i * d.A + j];
+  if (p.ch < 0 || p.ch != h.ch) return make_double2(0.0, 0.0);
+  double2 t = d.T[d.off_t[p.ch] + (int64_t)(p.v >> 1) * d.nhh[p.ch] + (h.v >> 1)];
+  double sg = (((p.v ^ h.v) & 1) ? -1.0 : 1.0);
+  return make_double2(sg * t.x, sg * t.y);
+}
+
+// antisymmetrized element, real for the built-in generator
+__device__ __forceinline__ double vas_t(const SolveDesc &d, double volume, int p, int q, int r, int s) {
+  return minnesota_direct_t(d, volume, p, q, r, s) - minnesota_direct_t(d, volume, p, q, s, r);
+}
+
+// F(a,b,c,i,j,k) = sum_d conj(<cb||kd>) t(ad,ij)  (three_body_oneparticle_1)
+__device__ double2 f_term(const SolveDesc &d, const TriplesDesc &td, double vol, int a, int b, int c, int i, int j,
+                          int k) {
+  double2 s = make_double2(0.0, 0.0);
+  const int nx = d.nx[i] + d.nx[j] - d.nx[a], ny = d.ny[i] + d.ny[j] - d.ny[a], nz = d.nz[i] + d.nz[j] - d.nz[a];
+  const int tz = d.tz[i] + d.tz[j] - d.tz[a];
+  for (int sz = -1; sz <= 1; sz += 2) {
+    int dd = orb_lookup(td, nx, ny, nz, sz, tz);
+    if (dd < d.A) continue;
+    double2 t = t_of(d, a, dd, i, j);
+    double v = vas_t(d, vol, c, b, k, dd);
+    s.x += v * t.x;
+    s.y += v * t.y;
+  }
+  return s;
+}
+
+// H(a,b,c,i,j,k) = sum_l conj(<jk||lc>) t(ab,il)  (three_body_onehole_1)
+__device__ double2 h_term(const SolveDesc &d, const TriplesDesc &td, double vol, int a, int b, int c, int i, int j,
+                          int k) {
+  double2 s = make_double2(0.0, 0.0);
+  const int nx = d.nx[k] + d.nx[j] - d.nx[c], ny = d.ny[k] + d.ny[j] - d.ny[c], nz = d.nz[k] + d.nz[j] - d.nz[c];
+  const int tz = d.tz[k] + d.tz[j] - d.tz[c];
+  for (int sz = -1; sz <= 1; sz += 2) {
+    int l = orb_lookup(td, nx, ny, nz, sz, tz);
+    if (l < 0 || l >= d.A) continue;
+    double2 t = t_of(d, a, b, i, l);
+    double v = vas_t(d, vol, j, k, l, c);
+    s.x += v * t.x;
+    s.y += v * t.y;
+  }
+  return s;
+}
+
+__global__ __launch_bounds__(256) void k_triples(SolveDesc d, TriplesDesc td, double volume, double2 *partial) {
+  __shared__ double sre[256], sim[256];
+  double re = 0.0, im = 0.0;
+  for (int64_t pr = blockIdx.x; pr < td.npairs; pr += gridDim.x) {
+    const int a = td.pair_a[pr], b = td.pair_b[pr];
+    for (int c = b + 1 + threadIdx.x; c < d.nsp; c += blockDim.x) {
+      const int kx = d.nx[a] + d.nx[b] + d.nx[c], ky = d.ny[a] + d.ny[b] + d.ny[c], kz = d.nz[a] + d.nz[b] + d.nz[c];
+      const int tz = d.tz[a] + d.tz[b] + d.tz[c];
+      const int ix = kx - td.k0, iy = ky - td.k0, iz = kz - td.k0, it = tz - td.tz0;
+      if (ix < 0 || iy < 0 || iz < 0 || ix >= td.nk || iy >= td.nk || iz >= td.nk || it < 0 || it >= td.ntz) continue;
+      const int ch = td.chan[((it * td.nk + ix) * td.nk + iy) * td.nk + iz];
+      if (ch < 0) continue;
+      const double fa = d.fdiag[a].x, fb = d.fdiag[b].x, fc = d.fdiag[c].x;
+      const double fai = d.fdiag[a].y, fbi = d.fdiag[b].y, fci = d.fdiag[c].y;
+      for (int h = td.hptr[ch]; h < td.hptr[ch + 1]; ++h) {
+        const int i = td.hijk[3 * h], j = td.hijk[3 * h + 1], k = td.hijk[3 * h + 2];
+        double2 w = make_double2(0.0, 0.0);
+        auto add = [&](double sg, double2 v) { w.x += sg * v.x; w.y += sg * v.y; };
+        add(1, f_term(d, td, volume, a, b, c, i, j, k));
+        add(-1, f_term(d, td, volume, b, a, c, i, j, k));
+        add(-1, f_term(d, td, volume, c, b, a, i, j, k));
+        add(-1, f_term(d, td, volume, a, b, c, k, j, i));
+        add(-1, f_term(d, td, volume, a, b, c, i, k, j));
+        add(1, f_term(d, td, volume, c, b, a, i, k, j));
+        add(1, f_term(d, td, volume, c, b, a, k, j, i));
+        add(1, f_term(d, td, volume, b, a, c, i, k, j));
+        add(1, f_term(d, td, volume, b, a, c, k, j, i));
+        add(-1, h_term(d, td, volume, a, b, c, i, j, k));
+        add(1, h_term(d, td, volume, a, c, b, i, j, k));
+        add(1, h_term(d, td, volume, c, b, a, i, j, k));
+        add(1, h_term(d, td, volume, a, b, c, k, j, i));
+        add(1, h_term(d, td, volume, a, b, c, j, i, k));
+        add(-1, h_term(d, td, volume, c, b, a, j, i, k));
+        add(-1, h_term(d, td, volume, c, b, a, k, j, i));
+        add(-1, h_term(d, td, volume, a, c, b, j, i, k));
+        add(-1, h_term(d, td, volume, a, c, b, k, j, i));
+        // conj(W) W / D with complex D
+        const double dre = d.fdiag[i].x + d.fdiag[j].x + d.fdiag[k].x - fa - fb - fc;
+        const double dim = d.fdiag[i].y + d.fdiag[j].y + d.fdiag[k].y - fai - fbi - fci;
+        const double ww = w.x * w.x + w.y * w.y;
+        const double den = dre * dre + dim * dim;
+        re += ww * dre / den;
+        im += -ww * dim / den;
+      }
+    }
+  }
+  sre[threadIdx.x] = re;
+  sim[threadIdx.x] = im;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) {
+      sre[threadIdx.x] += sre[threadIdx.x + s];
+      sim[threadIdx.x] += sim[threadIdx.x + s];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[blockIdx.x] = make_double2(sre[0], sim[0]);
+}
+
+__global__ void k_triples_final(const double2 *partial, int n, double2 *out) {
+  __shared__ double sre[256], sim[256];
+  double re = 0.0, im = 0.0;
+  for (int p = threadIdx.x; p < n; p += blockDim.x) {
+    re += partial[p].x;
+    im += partial[p].y;
+  }
+  sre[threadIdx.x] = re;
+  sim[threadIdx.x] = im;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) {
+      sre[threadIdx.x] += sre[threadIdx.x + s];
+      sim[threadIdx.x] += sim[threadIdx.x + s];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[0] = make_double2(sre[0], sim[0]);
+}
+
+void launch_triples(const SolveDesc &d, const TriplesDesc &td, double volume, double2 *partial, int nblocks,
+                    double2 *out, hipStream_t st) {
+  hipLaunchKernelGGL(k_triples, dim3(nblocks), dim3(256), 0, st, d, td, volume, partial);
+  hipLaunchKernelGGL(k_triples_final, dim3(1), dim3(256), 0, st, partial, nblocks, out);
+}
+
+}  // namespace ccgpu

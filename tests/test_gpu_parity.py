@@ -216,3 +216,78 @@ def test_fp32_mixed_precision_error(ccnm):
     d = abs(out[0]["eccd"] - out[1]["eccd"]) / 66
     assert out[1]["converged"]
     assert d < 1e-5, d
+
+
+def set_reference_t2(ccnm, s, g):
+    """Load the reference's converged T2 (its own orbital numbering) into the
+    engine's solver by matching orbitals through their labels (n, sz, tz)."""
+    mine = s.orbitals()
+    lab = {(mine["nx"][x], mine["ny"][x], mine["nz"][x], mine["sz"][x], mine["tz"][x]): x
+           for x in range(len(mine["e"]))}
+    oi = g["orb_int"]
+    ref2mine = np.array([lab[(oi[0][x], oi[1][x], oi[2][x], oi[3][x], oi[4][x])] for x in range(oi.shape[1])])
+    t = g["t2_idx"].astype(np.int64) - 1
+    s.set_t2(ref2mine[t[:, 0]], ref2mine[t[:, 1]], ref2mine[t[:, 2]], ref2mine[t[:, 3]], g["t2_val"])
+
+
+@pytest.mark.parametrize("name", ["pnm14_n1_minn", "snm4_n1_minn", "pnm14_n1_minn_twist"])
+def test_triples_against_reference(ccnm, name):
+    """(T) of the GPU kernel on the reference's converged T2 versus the
+    reference's pert_triples2 energy; and on the engine's own converged T2."""
+    g = golden(name)
+    A, nmax, sub, step, maxit = params(g)
+    matter = ccnm.PNM if str(g["matter"]) == "pnm" else ccnm.SNM
+    s = ccnm.Solver(A=A, matter=matter, rho=float(g["rho"]), nmax=nmax, twist=tuple(g["twist"]))
+    s.generate_interaction()
+    s.setup_t_amplitudes()
+    set_reference_t2(ccnm, s, g)
+    et = s.pert_triples2()
+    ref = g["eccdt"] - g["eccd"]
+    assert abs(et - ref) < 1e-12 * max(1.0, abs(ref)), (et, ref)
+    # own converged amplitudes: agreement at the stopping-tolerance scale
+    s.setup_t_amplitudes()
+    r = s.ccd_iter(subspace=sub, diis_step=step, tol=float(g["tol"]), maxit=maxit)
+    et2 = s.pert_triples2()
+    assert abs(et2 - ref) < 1e-6 * abs(ref) + float(g["tol"])
+
+
+def test_fortran_host_drop_in(ccnm, tmp_path):
+    """The Fortran driver ccm_gpu reads a reference ccm_in deck (plus the engine
+    option line) and prints the reference's result block; E/N values match the
+    reference run of the same physics (PNM A=14, nmax=1, Minnesota)."""
+    import subprocess
+    from conftest import ROOT
+    exe = os.path.join(ROOT, "coupled-cluster-nuclearmatter_amd", "host", "ccm_gpu")
+    if not os.path.exists(exe):
+        pytest.skip("Fortran host not built (flang absent)")
+    g = golden("pnm14_n1_minn")
+    deck = tmp_path / "ccm_in"
+    deck.write_text("""!Chiral order for Deltas(LO = 0,NLO=2,NNLO=3,N3LO=4) and cutoff
+3, 450
+! cE and cD 3nf parameters:
+ 0.017, 0.79
+! number of particles
+14
+! specify: pnm/snm, input type: density/kfermi
+pnm, density
+! specify boundary conditions (PBC/TABC/TABCsp)
+PBC
+! dens/kf, ntwist,  nmax, my_twists
+0.16,1,1,1
+! specify cluster approximation: CCD, CCD(T)
+CCD(T)
+! tnf switch (T/F) and specify 3nf approximation: 0=tnf0b, 1=tnf1b, 2=tnf2b
+F, 0
+! 3nf cutoff(MeV),non-local reg. exp
+450, 3
+! interaction, DIIS subspace, DIIS step, tolerance, max iterations, precision, refine
+minnesota, 8, 8, 1e-8, 100, fp64, 0
+""")
+    res = subprocess.run([exe, str(deck)], capture_output=True, text=True, timeout=120)
+    assert res.returncode == 0, res.stderr
+    line = [l for l in res.stdout.splitlines() if l.startswith("     0.01700")][-1]
+    vals = [float(x) for x in line.split()]
+    A = 14
+    ref = [0.017, 0.79, float(g["kf"]), 0.16, g["e0"].real / A, (g["e0"] + g["energies"][0]).real / A,
+           g["eccd"].real / A, g["eccdt"].real / A]
+    assert np.allclose(vals, ref, atol=2e-5), (vals, ref)
