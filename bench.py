@@ -29,6 +29,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 DENSITIES = [0.08, 0.12, 0.16, 0.20]
 A, NMAX = 66, 4
 TOL, DEPTH, MAXIT = 1e-8, 8, 100
+FP32_SWITCH = 1e-3   # |dE| below which the FP32 phase hands over to FP64
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFS = 78.6
 FP32_MFMA_PEAK_TFS = 157.3
@@ -56,7 +57,7 @@ def run_fast(ccnm, solvers):
     for s in solvers:
         s.setup_t_amplitudes()
     return ccnm.batch_solve(solvers, subspace=DEPTH, diis_step=DEPTH, tol=TOL, maxit=MAXIT,
-                            precision=ccnm.PREC_FP32, refine_every=4, use_graph=1)
+                            precision=ccnm.PREC_FP32, fp32_switch=FP32_SWITCH, use_graph=1)
 
 
 def ladder_roofline(ccnm, s, precision):

@@ -126,35 +126,55 @@ struct GemmList {
   static int cls_of(int N) { return N <= 16 ? 0 : N <= 32 ? 1 : N <= 48 ? 2 : N <= 64 ? 3 : N <= 80 ? 4 : 3; }
   static int rows_of(int c) { return c <= 1 ? 128 : 64; }
   static int cols_of(int c) { static const int w[NCLS] = {16, 32, 48, 64, 80}; return w[c]; }
+  bool finalized = false;
   void add(GemmGroup g, int solve) {
     if (g.M <= 0 || g.N <= 0 || g.K <= 0) return;
-    int gi = (int)groups.size();
-    // long-K, small-output products split K into 256-wide slices so that a
-    // channel spreads over many workgroups; partials are summed in fixed order
     g.nsplit = 1;
-    if (g.K >= 512 && g.K > 8 * std::max(g.M, g.N)) g.nsplit = std::min(32, (g.K + 255) / 256);
-    int ksz = (g.K + g.nsplit - 1) / g.nsplit;
-    ksz = (ksz + 15) / 16 * 16;
-    g.nsplit = (g.K + ksz - 1) / ksz;
-    poff.push_back(g.nsplit > 1 ? psize : -1);
-    if (g.nsplit > 1) {
-      psize += (int64_t)g.nsplit * g.M * g.N;
-      split.push_back(gi);
-      maxmn = std::max<int64_t>(maxmn, (int64_t)g.M * g.N);
-    }
     g.P = nullptr;
     groups.push_back(g);
     gsolve.push_back(solve);
-    int c = cls_of(g.N);
-    int bm = rows_of(c), bn = cols_of(c);
-    for (int ks = 0; ks < g.nsplit; ++ks)
-      for (int m = 0; m < g.M; m += bm)
-        for (int n = 0; n < g.N; n += bn)
-          tiles[c].push_back(GemmTile{gi, m, n, ks, ks * ksz, std::min(g.K, (ks + 1) * ksz), 0, 0});
-    for (int m = 0; m < g.M; m += 64)
-      for (int n = 0; n < g.N; n += 64) tiles64.push_back(GemmTile{gi, m, n, 0, 0, g.K, 0, 0});
     flops += 8.0 * g.M * (double)g.N * g.K;
   }
+  // Tiles per class.  A class whose tiles cannot fill the GPU (fewer than two
+  // per CU) splits K over several workgroups per tile; the partial products are
+  // summed in fixed order by k_splitk_reduce.
+  void finalize() {
+    if (finalized) return;
+    finalized = true;
+    const int target = 512;
+    std::vector<int64_t> cls_tiles(NCLS, 0);
+    std::vector<int> cls(groups.size());
+    for (size_t gi = 0; gi < groups.size(); ++gi) {
+      const GemmGroup &g = groups[gi];
+      cls[gi] = cls_of(g.N);
+      int bm = rows_of(cls[gi]), bn = cols_of(cls[gi]);
+      cls_tiles[cls[gi]] += (int64_t)((g.M + bm - 1) / bm) * ((g.N + bn - 1) / bn);
+    }
+    for (size_t gi = 0; gi < groups.size(); ++gi) {
+      GemmGroup &g = groups[gi];
+      const int c = cls[gi];
+      int split = 1;
+      if (cls_tiles[c] > 0 && cls_tiles[c] < target && g.K >= 256)
+        split = (int)std::min<int64_t>({(target + cls_tiles[c] - 1) / cls_tiles[c], (int64_t)(g.K / 128), 32});
+      int ksz = (g.K + split - 1) / split;
+      ksz = (ksz + 15) / 16 * 16;
+      g.nsplit = (g.K + ksz - 1) / ksz;
+      poff.push_back(g.nsplit > 1 ? psize : -1);
+      if (g.nsplit > 1) {
+        psize += (int64_t)g.nsplit * g.M * g.N;
+        split_list_push((int)gi);
+        maxmn = std::max<int64_t>(maxmn, (int64_t)g.M * g.N);
+      }
+      int bm = rows_of(c), bn = cols_of(c);
+      for (int ks = 0; ks < g.nsplit; ++ks)
+        for (int m = 0; m < g.M; m += bm)
+          for (int n = 0; n < g.N; n += bn)
+            tiles[c].push_back(GemmTile{(int)gi, m, n, ks, ks * ksz, std::min(g.K, (ks + 1) * ksz), 0, 0});
+      for (int m = 0; m < g.M; m += 64)
+        for (int n = 0; n < g.N; n += 64) tiles64.push_back(GemmTile{(int)gi, m, n, 0, 0, g.K, 0, 0});
+    }
+  }
+  void split_list_push(int gi) { split.push_back(gi); }
   int ntiles() const {
     size_t n = 0;
     for (int c = 0; c < NCLS; ++c) n += tiles[c].size();
@@ -387,6 +407,7 @@ void build_device(Solve &S) {
     // Rt = Tt I4 (ccm_t2_eqn.f90:995-1026)
     S.g_rt.add(mkgroup(d.Tt + P.off_rt[K], d.I4 + P.off_i4[K], d.Rt + P.off_rt[K], nullptr, nph, nhp, nhp, nhp, nhp, nhp, 0, 1.0, 0.0), 0);
   }
+  for (GemmList *L : {&S.g_x, &S.g_pp, &S.g_hh, &S.g_i4, &S.g_rt}) L->finalize();
   xcd_order(S.g_pp);
   for (GemmList *L : {&S.g_x, &S.g_pp, &S.g_hh, &S.g_i4, &S.g_rt}) L->upload(o);
   S.g_pp.bytes = 16.0 * (double)P.nVpp;
@@ -484,6 +505,7 @@ void merge(GemmList &dst, const GemmList &src, int solve) {
   dst.maxmn = std::max(dst.maxmn, src.maxmn);
   dst.flops += src.flops;
   dst.bytes += src.bytes;
+  dst.finalized = true;
 }
 
 std::unique_ptr<Batch> make_batch(const std::vector<Solve *> &ss, hipStream_t st) {
@@ -544,12 +566,13 @@ int term_mask() {
 }
 
 // One GEMM family: three launches (one per tile class), or the VALU check.
-void run_gemm(const GemmList &L, const Batch &B, int prec, bool a_f32, hipStream_t st) {
+void run_gemm(const GemmList &L, const Batch &B, int prec, bool a_f32, hipStream_t st, bool pp = false) {
   if (gemm_variant_override() == 3) {
     launch_gemm(L.d_groups, L.d_tiles64, (int)L.tiles64.size(), B.d_descs, L.d_gsolve, 3, st);
     return;
   }
-  const int base = prec == CCGPU_PREC_FP64 ? 10 : (a_f32 ? 30 : 20);
+  int base = prec == CCGPU_PREC_FP64 ? 10 : (a_f32 ? 30 : 20);
+  if (pp) base = prec == CCGPU_PREC_FP64 ? 40 : 60;
   const int nb[GemmList::NCLS] = {1, 2, 3, 4, 5};
   for (int c = GemmList::NCLS - 1; c >= 0; --c)
     launch_gemm(L.d_groups, L.d_tiles[c], (int)L.tiles[c].size(), B.d_descs, L.d_gsolve, base + nb[c], st);
@@ -576,8 +599,8 @@ void residual_launches(Batch &B, int prec, bool timing) {
   }
   if (terms & 2) {
     Timer t(timing, st, "gemm_pp_ladder");
-    if (prec == CCGPU_PREC_FP32) run_gemm(B.g_pp32, B, prec, true, st);
-    else run_gemm(B.g_pp, B, prec, false, st);
+    if (prec == CCGPU_PREC_FP32) run_gemm(B.g_pp32, B, prec, true, st, true);
+    else run_gemm(B.g_pp, B, prec, false, st, true);
   }
   if (terms & 4) {
     Timer t(timing, st, "gemm_hh_ladder");

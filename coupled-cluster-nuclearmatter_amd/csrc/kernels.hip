@@ -559,7 +559,9 @@ __global__ __launch_bounds__(256) void k_zgemm_valu(const GemmGroup *__restrict_
 // k0+4g..k0+4g+3 of a 16-wide k chunk (one 64-byte read of A per lane, a full
 // 128-byte line per row across four lanes) and the four MFMAs of the chunk run
 // over those four k values.
-template <int MB, int NB, bool F32, typename AT>
+// FAM only names the instantiation (1 = pp ladder) so profiles can tell the
+// pp-ladder launches from other families with the same tile grid.
+template <int MB, int NB, bool F32, typename AT, int FAM = 0>
 __global__ __launch_bounds__(256) void k_zgemm_dl(const GemmGroup *__restrict__ groups,
                                                   const GemmTile *__restrict__ tiles,
                                                   const SolveDesc *__restrict__ descs,
@@ -796,23 +798,34 @@ void launch_gemm(const GemmGroup *groups, const GemmTile *tiles, int ntiles, con
     case 3:
       hipLaunchKernelGGL(k_zgemm_valu, dim3(ntiles), dim3(256), 0, st, groups, tiles, dd, gsolve);
       break;
-    // direct-load MFMA kernels: 10+nb FP64, 20+nb FP32 (A double), 30+nb FP32 (A float);
+    // direct-load MFMA kernels: 10+nb FP64, 20+nb FP32 (A double), 30+nb FP32 (A float),
+    // 40+nb FP64 pp ladder, 60+nb FP32 pp ladder (A float);
     // nb = 16-column blocks per tile (1,2 with two 16-row blocks per wave, 3-5 with one)
-    case 11: hipLaunchKernelGGL((k_zgemm_dl<2, 1, false, double2>), dim3(ntiles), dim3(256), 0, st, groups, tiles, dd, gsolve); break;
-    case 12: hipLaunchKernelGGL((k_zgemm_dl<2, 2, false, double2>), dim3(ntiles), dim3(256), 0, st, groups, tiles, dd, gsolve); break;
-    case 13: hipLaunchKernelGGL((k_zgemm_dl<1, 3, false, double2>), dim3(ntiles), dim3(256), 0, st, groups, tiles, dd, gsolve); break;
-    case 14: hipLaunchKernelGGL((k_zgemm_dl<1, 4, false, double2>), dim3(ntiles), dim3(256), 0, st, groups, tiles, dd, gsolve); break;
-    case 15: hipLaunchKernelGGL((k_zgemm_dl<1, 5, false, double2>), dim3(ntiles), dim3(256), 0, st, groups, tiles, dd, gsolve); break;
-    case 21: hipLaunchKernelGGL((k_zgemm_dl<2, 1, true, double2>), dim3(ntiles), dim3(256), 0, st, groups, tiles, dd, gsolve); break;
-    case 22: hipLaunchKernelGGL((k_zgemm_dl<2, 2, true, double2>), dim3(ntiles), dim3(256), 0, st, groups, tiles, dd, gsolve); break;
-    case 23: hipLaunchKernelGGL((k_zgemm_dl<1, 3, true, double2>), dim3(ntiles), dim3(256), 0, st, groups, tiles, dd, gsolve); break;
-    case 24: hipLaunchKernelGGL((k_zgemm_dl<1, 4, true, double2>), dim3(ntiles), dim3(256), 0, st, groups, tiles, dd, gsolve); break;
-    case 25: hipLaunchKernelGGL((k_zgemm_dl<1, 5, true, double2>), dim3(ntiles), dim3(256), 0, st, groups, tiles, dd, gsolve); break;
-    case 31: hipLaunchKernelGGL((k_zgemm_dl<2, 1, true, float2>), dim3(ntiles), dim3(256), 0, st, groups, tiles, dd, gsolve); break;
-    case 32: hipLaunchKernelGGL((k_zgemm_dl<2, 2, true, float2>), dim3(ntiles), dim3(256), 0, st, groups, tiles, dd, gsolve); break;
-    case 33: hipLaunchKernelGGL((k_zgemm_dl<1, 3, true, float2>), dim3(ntiles), dim3(256), 0, st, groups, tiles, dd, gsolve); break;
-    case 34: hipLaunchKernelGGL((k_zgemm_dl<1, 4, true, float2>), dim3(ntiles), dim3(256), 0, st, groups, tiles, dd, gsolve); break;
-    case 35: hipLaunchKernelGGL((k_zgemm_dl<1, 5, true, float2>), dim3(ntiles), dim3(256), 0, st, groups, tiles, dd, gsolve); break;
+    case 11: hipLaunchKernelGGL((k_zgemm_dl<2, 1, false, double2, 0>), dim3(ntiles), dim3(256), 0, st, groups, tiles, dd, gsolve); break;
+    case 12: hipLaunchKernelGGL((k_zgemm_dl<2, 2, false, double2, 0>), dim3(ntiles), dim3(256), 0, st, groups, tiles, dd, gsolve); break;
+    case 13: hipLaunchKernelGGL((k_zgemm_dl<1, 3, false, double2, 0>), dim3(ntiles), dim3(256), 0, st, groups, tiles, dd, gsolve); break;
+    case 14: hipLaunchKernelGGL((k_zgemm_dl<1, 4, false, double2, 0>), dim3(ntiles), dim3(256), 0, st, groups, tiles, dd, gsolve); break;
+    case 15: hipLaunchKernelGGL((k_zgemm_dl<1, 5, false, double2, 0>), dim3(ntiles), dim3(256), 0, st, groups, tiles, dd, gsolve); break;
+    case 21: hipLaunchKernelGGL((k_zgemm_dl<2, 1, true, double2, 0>), dim3(ntiles), dim3(256), 0, st, groups, tiles, dd, gsolve); break;
+    case 22: hipLaunchKernelGGL((k_zgemm_dl<2, 2, true, double2, 0>), dim3(ntiles), dim3(256), 0, st, groups, tiles, dd, gsolve); break;
+    case 23: hipLaunchKernelGGL((k_zgemm_dl<1, 3, true, double2, 0>), dim3(ntiles), dim3(256), 0, st, groups, tiles, dd, gsolve); break;
+    case 24: hipLaunchKernelGGL((k_zgemm_dl<1, 4, true, double2, 0>), dim3(ntiles), dim3(256), 0, st, groups, tiles, dd, gsolve); break;
+    case 25: hipLaunchKernelGGL((k_zgemm_dl<1, 5, true, double2, 0>), dim3(ntiles), dim3(256), 0, st, groups, tiles, dd, gsolve); break;
+    case 31: hipLaunchKernelGGL((k_zgemm_dl<2, 1, true, float2, 0>), dim3(ntiles), dim3(256), 0, st, groups, tiles, dd, gsolve); break;
+    case 32: hipLaunchKernelGGL((k_zgemm_dl<2, 2, true, float2, 0>), dim3(ntiles), dim3(256), 0, st, groups, tiles, dd, gsolve); break;
+    case 33: hipLaunchKernelGGL((k_zgemm_dl<1, 3, true, float2, 0>), dim3(ntiles), dim3(256), 0, st, groups, tiles, dd, gsolve); break;
+    case 34: hipLaunchKernelGGL((k_zgemm_dl<1, 4, true, float2, 0>), dim3(ntiles), dim3(256), 0, st, groups, tiles, dd, gsolve); break;
+    case 35: hipLaunchKernelGGL((k_zgemm_dl<1, 5, true, float2, 0>), dim3(ntiles), dim3(256), 0, st, groups, tiles, dd, gsolve); break;
+    case 41: hipLaunchKernelGGL((k_zgemm_dl<2, 1, false, double2, 1>), dim3(ntiles), dim3(256), 0, st, groups, tiles, dd, gsolve); break;
+    case 42: hipLaunchKernelGGL((k_zgemm_dl<2, 2, false, double2, 1>), dim3(ntiles), dim3(256), 0, st, groups, tiles, dd, gsolve); break;
+    case 43: hipLaunchKernelGGL((k_zgemm_dl<1, 3, false, double2, 1>), dim3(ntiles), dim3(256), 0, st, groups, tiles, dd, gsolve); break;
+    case 44: hipLaunchKernelGGL((k_zgemm_dl<1, 4, false, double2, 1>), dim3(ntiles), dim3(256), 0, st, groups, tiles, dd, gsolve); break;
+    case 45: hipLaunchKernelGGL((k_zgemm_dl<1, 5, false, double2, 1>), dim3(ntiles), dim3(256), 0, st, groups, tiles, dd, gsolve); break;
+    case 61: hipLaunchKernelGGL((k_zgemm_dl<2, 1, true, float2, 1>), dim3(ntiles), dim3(256), 0, st, groups, tiles, dd, gsolve); break;
+    case 62: hipLaunchKernelGGL((k_zgemm_dl<2, 2, true, float2, 1>), dim3(ntiles), dim3(256), 0, st, groups, tiles, dd, gsolve); break;
+    case 63: hipLaunchKernelGGL((k_zgemm_dl<1, 3, true, float2, 1>), dim3(ntiles), dim3(256), 0, st, groups, tiles, dd, gsolve); break;
+    case 64: hipLaunchKernelGGL((k_zgemm_dl<1, 4, true, float2, 1>), dim3(ntiles), dim3(256), 0, st, groups, tiles, dd, gsolve); break;
+    case 65: hipLaunchKernelGGL((k_zgemm_dl<1, 5, true, float2, 1>), dim3(ntiles), dim3(256), 0, st, groups, tiles, dd, gsolve); break;
     default:
       break;
   }
