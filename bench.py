@@ -29,7 +29,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 DENSITIES = [0.08, 0.12, 0.16, 0.20]
 A, NMAX = 66, 4
 TOL, DEPTH, MAXIT = 1e-8, 8, 100
-FP32_SWITCH = 1e-3   # |dE| below which the FP32 phase hands over to FP64
+FP32_SWITCH = 1.0    # FP32 up to the first DIIS extrapolation after |dE| <= 1 MeV
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFS = 78.6
 FP32_MFMA_PEAK_TFS = 157.3

@@ -45,13 +45,15 @@ class SolveOpts(C.Structure):
 class SolveResult(C.Structure):
     _fields_ = [("iterations", C.c_int), ("converged", C.c_int), ("e0", C.c_double * 2),
                 ("embpt2", C.c_double * 2), ("eccd", C.c_double * 2),
-                ("ecorr_last", C.c_double * 2), ("seconds", C.c_double)]
+                ("ecorr_last", C.c_double * 2), ("seconds", C.c_double),
+                ("fp32_iterations", C.c_int), ("pad_", C.c_int)]
 
     def as_dict(self):
         z = lambda a: complex(a[0], a[1])
         return {"iterations": self.iterations, "converged": bool(self.converged),
                 "e0": z(self.e0), "embpt2": z(self.embpt2), "eccd": z(self.eccd),
-                "ecorr": z(self.ecorr_last), "seconds": self.seconds}
+                "ecorr": z(self.ecorr_last), "seconds": self.seconds,
+                "fp32_iterations": self.fp32_iterations}
 
 
 # every exported symbol with its argument types (kept in step with include/ccgpu.h)
@@ -129,7 +131,7 @@ def plan(A, matter, rho, nmax, twist=(0.0, 0.0, 0.0)):
 
 
 def default_opts(subspace=6, diis_step=6, tol=1e-6, maxit=100, precision=PREC_FP64,
-                 refine_every=0, use_graph=0, verbose=0, fp32_switch=1e-3):
+                 refine_every=0, use_graph=0, verbose=0, fp32_switch=1.0):
     return SolveOpts(subspace, diis_step, tol, maxit, precision, refine_every, use_graph, verbose,
                      fp32_switch)
 

@@ -465,6 +465,7 @@ struct Batch {
   SolveDesc *d_descs = nullptr;  // device array
   std::vector<SolveDesc> h_descs;
   GemmList g_x, g_pp, g_hh, g_i4, g_rt, g_pp32;
+  bool has_f32 = true;   // every solve has the FP32 copy of its pppp blocks
   int64_t maxT = 0, maxVhh = 0, maxRows = 0;
   int maxA = 0, maxNp = 0;
   int *d_slot = nullptr, *d_slots = nullptr;
@@ -539,6 +540,7 @@ std::unique_ptr<Batch> make_batch(const std::vector<Solve *> &ss, hipStream_t st
     const double2 *base = S.desc.Vpppp;
     const double2 *a = (const double2 *)B->g_pp32.groups[g].A;
     if (S.desc.Vpppp32) B->g_pp32.groups[g].A = S.desc.Vpppp32 + (a - base);
+    else B->has_f32 = false;
   }
   if (n > 1) {
     xcd_order(B->g_pp);
@@ -583,6 +585,7 @@ void residual_launches(Batch &B, int prec, bool timing) {
   const int n = (int)B.solves.size();
   hipStream_t st = B.stream;
   const int terms = term_mask();
+  if (prec == CCGPU_PREC_FP32 && !B.has_f32) throw Error("FP32 residual without the FP32 copy of Vpppp");
   {
     Timer t(timing, st, "gemm_x_vhhpp_t");
     run_gemm(B.g_x, B, prec, false, st);
@@ -734,7 +737,7 @@ void run_batch(Batch &B, const ccgpu_solve_opts &op, std::vector<ccgpu_solve_res
   auto e1 = energies(B, false);
   res.assign(n, ccgpu_solve_result{});
   std::vector<std::complex<double>> dener(n, 4.0), elast = e1;
-  std::vector<int> iters(n, 0), conv(n, 0);
+  std::vector<int> iters(n, 0), conv(n, 0), iters32(n, 0);
   for (int s = 0; s < n; ++s) {
     Solve &S = *B.solves[s];
     res[s].e0[0] = S.e0.x; res[s].e0[1] = S.e0.y;
@@ -745,10 +748,15 @@ void run_batch(Batch &B, const ccgpu_solve_opts &op, std::vector<ccgpu_solve_res
   auto t0 = std::chrono::steady_clock::now();
   bool graph_ok = op.use_graph != 0 && !timing && gemm_variant_override() < 0;
   int refine = op.refine_every > 0 ? op.refine_every : 0;
-  // mixed precision: FP32 contractions (with DIIS) until max |dE| <= fp32_switch,
-  // then the DIIS history restarts and FP64 iterations finish the solve, so the
-  // final fixed point and the stopping test are FP64
+  // Mixed precision: FP32 contractions (with the usual DIIS schedule) until a
+  // DIIS extrapolation that follows an iteration with max |dE| <= fp32_switch;
+  // from the next iteration on the contractions are FP64 and the DIIS history
+  // carries on, so every later error vector is an FP64 one and the final fixed
+  // point and stopping test are FP64.  Should the FP32 iteration itself come
+  // within 10 tol first (FP32 rounding then dominates |dE|), the switch happens
+  // at once and the DIIS history restarts.
   bool fp32_phase = op.precision == CCGPU_PREC_FP32;
+  double prev_maxd = 1e300;
   for (int it = 1; it <= op.maxit; ++it) {
     bool any = false;
     for (int s = 0; s < n; ++s) {
@@ -779,6 +787,7 @@ void run_batch(Batch &B, const ccgpu_solve_opts &op, std::vector<ccgpu_solve_res
     ++nstep;
     update_launches(B, count, nstep, op.subspace, op.diis_step, timing);
     check_launch();
+    const bool extrapolated = count == 0;
     auto e2 = energies(B, timing);
     double maxd = 0;
     for (int s = 0; s < n; ++s) {
@@ -786,19 +795,28 @@ void run_batch(Batch &B, const ccgpu_solve_opts &op, std::vector<ccgpu_solve_res
       dener[s] = e2[s] - elast[s];
       elast[s] = e2[s];
       iters[s] = it;
+      if (prec == CCGPU_PREC_FP32) ++iters32[s];
       maxd = std::max(maxd, std::abs(dener[s]));
-      if (op.verbose) printf("dener %5d %5d %16.8E  E %.12f\n", s, it, dener[s].real(), e2[s].real());
+      if (op.verbose) printf("dener %5d %5d %16.8E  E %.12f%s\n", s, it, dener[s].real(), e2[s].real(),
+                             prec == CCGPU_PREC_FP32 ? "  fp32" : "");
     }
-    if (fp32_phase && maxd <= std::max(op.fp32_switch, 10.0 * op.tol)) {
-      fp32_phase = false;
-      count = 0;
-      nstep = 0;
-      for (Solve *S : B.solves)
-        for (int k = 0; k < op.subspace; ++k) S->ring[k] = k;
-      // an FP32 |dE| is no FP64 convergence test: force at least one FP64 step
-      for (int s = 0; s < n; ++s)
-        if (B.h_descs[s].active) dener[s] = std::max(std::abs(dener[s]), 2.0 * op.tol);
+    if (fp32_phase) {
+      const bool at_boundary = extrapolated && prev_maxd <= op.fp32_switch;
+      const bool at_floor = maxd <= 10.0 * op.tol;
+      if (at_boundary || at_floor) {
+        fp32_phase = false;
+        if (!at_boundary) {
+          count = 0;
+          nstep = 0;
+          for (Solve *S : B.solves)
+            for (int k = 0; k < op.subspace; ++k) S->ring[k] = k;
+        }
+        // an FP32 |dE| is no FP64 convergence test: force at least one FP64 step
+        for (int s = 0; s < n; ++s)
+          if (B.h_descs[s].active) dener[s] = std::max(std::abs(dener[s]), 2.0 * op.tol);
+      }
     }
+    if (!extrapolated) prev_maxd = maxd;
   }
   HIPCHK(hipStreamSynchronize(B.stream));
   double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
@@ -812,6 +830,7 @@ void run_batch(Batch &B, const ccgpu_solve_opts &op, std::vector<ccgpu_solve_res
     res[s].eccd[0] = S.e0.x + elast[s].real();
     res[s].eccd[1] = S.e0.y + elast[s].imag();
     res[s].seconds = secs;
+    res[s].fp32_iterations = iters32[s];
     S.have_t = true;
   }
   // leave every solve active for later single calls
@@ -834,6 +853,18 @@ Batch &single(int h) {
   return *g_single[h];
 }
 void drop_single(int h) { g_single.erase(h); }
+
+// FP32 copy of the stored pppp blocks, made on the first FP32 request; the
+// cached single-solve batch is rebuilt so that its FP32 ladder list reads it.
+void ensure_f32(Solve &S, int handle) {
+  if (S.desc.Vpppp32) return;
+  S.desc.Vpppp32 = dalloc<float2>(S.plan.nVpp, S.owned);
+  launch_to_f32(S.desc.Vpppp, S.desc.Vpppp32, S.plan.nVpp, S.stream);
+  check_launch();
+  HIPCHK(hipStreamSynchronize(S.stream));
+  HIPCHK(hipMemcpy(S.d_desc, &S.desc, sizeof(SolveDesc), hipMemcpyHostToDevice));
+  drop_single(handle);
+}
 
 template <typename F>
 int guarded(F &&f) {
@@ -1153,6 +1184,8 @@ int ccgpu_residual(int handle, int precision) {
     Solve &S = get(handle);
     ensure_ready(S);
     if (!S.have_t) throw Error("no amplitudes: call ccgpu_mbpt2_start or ccgpu_set_t2 first");
+    if (precision != CCGPU_PREC_FP64 && precision != CCGPU_PREC_FP32) throw Error(fmt("unknown precision %d", precision));
+    if (precision == CCGPU_PREC_FP32) ensure_f32(S, handle);
     Batch &B = single(handle);
     residual_launches(B, precision, false);
     check_launch();
@@ -1199,14 +1232,7 @@ int ccgpu_solve(int handle, const ccgpu_solve_opts *opts, ccgpu_solve_result *re
     Solve &S = get(handle);
     ensure_ready(S);
     if (!S.have_t) throw Error("no amplitudes: call ccgpu_mbpt2_start or ccgpu_set_t2 first");
-    if (opts->precision == CCGPU_PREC_FP32 && !S.desc.Vpppp32) {
-      S.desc.Vpppp32 = dalloc<float2>(S.plan.nVpp, S.owned);
-      launch_to_f32(S.desc.Vpppp, S.desc.Vpppp32, S.plan.nVpp, S.stream);
-      check_launch();
-      HIPCHK(hipStreamSynchronize(S.stream));
-      HIPCHK(hipMemcpy(S.d_desc, &S.desc, sizeof(SolveDesc), hipMemcpyHostToDevice));
-      drop_single(handle);
-    }
+    if (opts->precision == CCGPU_PREC_FP32) ensure_f32(S, handle);
     Batch &B = single(handle);
     std::vector<ccgpu_solve_result> r;
     run_batch(B, *opts, r);
@@ -1222,13 +1248,7 @@ int ccgpu_batch_solve(int n, const int *handles, const ccgpu_solve_opts *opts, c
       Solve &S = get(handles[s]);
       ensure_ready(S);
       if (!S.have_t) throw Error(fmt("handle %d has no amplitudes", handles[s]));
-      if (opts->precision == CCGPU_PREC_FP32 && !S.desc.Vpppp32) {
-        S.desc.Vpppp32 = dalloc<float2>(S.plan.nVpp, S.owned);
-        launch_to_f32(S.desc.Vpppp, S.desc.Vpppp32, S.plan.nVpp, S.stream);
-        check_launch();
-        HIPCHK(hipStreamSynchronize(S.stream));
-        HIPCHK(hipMemcpy(S.d_desc, &S.desc, sizeof(SolveDesc), hipMemcpyHostToDevice));
-      }
+      if (opts->precision == CCGPU_PREC_FP32) ensure_f32(S, handles[s]);
       ss.push_back(&S);
     }
     auto B = make_batch(ss, ss[0]->stream);

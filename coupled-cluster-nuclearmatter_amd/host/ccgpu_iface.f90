@@ -36,6 +36,7 @@ module ccgpu_iface
   type, bind(C) :: ccgpu_solve_result
      integer(c_int) :: iterations, converged
      real(c_double) :: e0(2), embpt2(2), eccd(2), ecorr_last(2), seconds
+     integer(c_int) :: fp32_iterations, pad_
   end type ccgpu_solve_result
 
   interface

@@ -8,8 +8,11 @@
 ! (ccm_main.f90:322-348).
 !
 ! One optional line after the reference deck sets engine options:
-!   ! interaction, DIIS subspace, DIIS step, tolerance, max iterations, precision, refine
-!   minnesota, 6, 6, 1e-6, 100, fp64, 0
+!   ! interaction, DIIS subspace, DIIS step, tolerance, max iterations, precision, fp32 switch
+!   minnesota, 6, 6, 1e-6, 100, fp64, 1.0
+! With precision fp32 the contractions run in FP32 up to the first DIIS
+! extrapolation that follows an iteration with |dE| <= the switch (MeV), and in
+! FP64 afterwards (DIIS history kept), so the stopping test is always FP64.
 ! Without it the reference constants are used (DIIS 6/6, 1e-6, 100 iterations).
 program ccm_gpu
   use iso_c_binding
@@ -18,9 +21,10 @@ program ccm_gpu
   character(len=512) :: inputfile
   character(len=100) :: calc_type, input_type, boundary_conditions, cc_approx, interaction, precname
   integer :: order, below_ef, ntwist, nmax, my_twists, tnf_approx, nexp, ios
-  integer :: subspace, diis_step, maxit, refine, itimes, prec
+  integer :: subspace, diis_step, maxit, itimes, prec
   integer(c_int) :: h, count, nstep, st
-  real(8) :: lambda, ce, cd, input, cutoff3, dens, kf, tol4r
+  real(8) :: lambda, ce, cd, input, cutoff3, dens, kf, tol4r, fp32_switch, prev_dener
+  logical :: fp32_phase, extrapolated, at_boundary
   real(4) :: tol4
   real(8), parameter :: pi = 3.14159265358979d0
   logical :: tnf_switch
@@ -45,14 +49,14 @@ program ccm_gpu
   read(5,*); read(5,*) tnf_switch, tnf_approx
   read(5,*); read(5,*) cutoff3, nexp
   interaction = 'chiral'
-  subspace = 6; diis_step = 6; tol4 = 1.0e-6; maxit = 100; precname = 'fp64'; refine = 0
+  subspace = 6; diis_step = 6; tol4 = 1.0e-6; maxit = 100; precname = 'fp64'; fp32_switch = 1.d0
   read(5,*,iostat=ios)
-  if (ios == 0) read(5,*,iostat=ios) interaction, subspace, diis_step, tol4, maxit, precname, refine
+  if (ios == 0) read(5,*,iostat=ios) interaction, subspace, diis_step, tol4, maxit, precname, fp32_switch
   close(5)
 
   if (trim(interaction) /= 'minnesota') then
      write(0,'(a)') 'ccm_gpu: only the Minnesota interaction generator is built into this engine;'
-     write(0,'(a)') 'add the engine option line "minnesota, 6, 6, 1e-6, 100, fp64, 0" to the deck,'
+     write(0,'(a)') 'add the engine option line "minnesota, 6, 6, 1e-6, 100, fp64, 1.0" to the deck,'
      write(0,'(a)') 'or hand over chiral blocks through ccgpu_set_block.'
      error stop 3
   end if
@@ -124,24 +128,36 @@ program ccm_gpu
      nstep = 0
      ener2 = 0.d0
      dener = 4.0
+     fp32_phase = prec == CCGPU_PREC_FP32
+     prev_dener = huge(1.d0)
      do itimes = 1, maxit
         if (abs(dener) > tol4r) then
-           st = prec
-           if (prec == CCGPU_PREC_FP32) then
-              if (refine > 0) then
-                 if (mod(itimes, refine) == 0) st = CCGPU_PREC_FP64
-              end if
-              if (abs(dener) <= 100.d0 * tol4r) st = CCGPU_PREC_FP64
-           end if
+           st = CCGPU_PREC_FP64
+           if (fp32_phase) st = CCGPU_PREC_FP32
            call ccgpu_check(ccgpu_residual(h, st), 'ccgpu_residual')
            count = count + 1
            nstep = nstep + 1
            call ccgpu_check(ccgpu_update(h, count, nstep, subspace, diis_step), 'ccgpu_update')
+           extrapolated = count == 0
            call ccgpu_check(ccgpu_energy(h, ener2), 'ccgpu_energy')
            write(6,*) 'CCD energy', ener2 + e0
            dener = ener2 - ener1
            ener1 = ener2
            write(6,'(a,2i5,2e16.8)') 'dener', 0, itimes, real(dener)
+           if (fp32_phase) then
+              at_boundary = extrapolated .and. prev_dener <= fp32_switch
+              if (at_boundary .or. abs(dener) <= 10.d0 * tol4r) then
+                 fp32_phase = .false.
+                 if (.not. at_boundary) then
+                    ! FP32 rounding dominates |dE| here: restart the DIIS history
+                    count = 0
+                    nstep = 0
+                 end if
+                 ! an FP32 |dE| is no FP64 convergence test
+                 if (abs(dener) <= 2.d0 * tol4r) dener = 2.d0 * tol4r
+              end if
+           end if
+           if (.not. extrapolated) prev_dener = abs(dener)
         end if
      end do
      call cpu_time(t_end)

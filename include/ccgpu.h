@@ -92,8 +92,9 @@ typedef struct {
   int refine_every;  /* mixed precision: FP64 residual every k iterations */
   int use_graph;     /* capture one iteration in a hipGraph and replay it */
   int verbose;       /* print per-iteration lines like the reference */
-  double fp32_switch;/* mixed precision: FP32 contractions until max |dE| <= this
-                        (MeV), then DIIS restarts and FP64 finishes the solve */
+  double fp32_switch;/* mixed precision: FP32 contractions up to the first DIIS
+                        extrapolation that follows an iteration with max |dE| <=
+                        this (MeV); FP64 afterwards, DIIS history kept */
 } ccgpu_solve_opts;
 
 typedef struct {
@@ -104,6 +105,8 @@ typedef struct {
   double eccd[2];        /* E0 + CCD correlation energy */
   double ecorr_last[2];  /* last correlation energy */
   double seconds;        /* wall time of the iteration loop */
+  int fp32_iterations;   /* iterations whose contractions ran in FP32 */
+  int pad_;
 } ccgpu_solve_result;
 
 /* Host-only planning of a lattice: no GPU work.  Fills dims. */
