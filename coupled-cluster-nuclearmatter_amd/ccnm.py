@@ -58,7 +58,7 @@ class SolveResult(C.Structure):
 
 # every exported symbol with its argument types (kept in step with include/ccgpu.h)
 _I, _D, _P = C.c_int, C.c_double, C.c_void_p
-_IP, _DP = C.POINTER(C.c_int), C.POINTER(C.c_double)
+_IP, _DP, _FP = C.POINTER(C.c_int), C.POINTER(C.c_double), C.POINTER(C.c_float)
 SIGNATURES = {
     "ccgpu_plan": [C.POINTER(LatticeParams), C.POINTER(Dims)],
     "ccgpu_create": [C.POINTER(LatticeParams), _IP],
@@ -86,6 +86,16 @@ SIGNATURES = {
     "ccgpu_batch_solve": [_I, _IP, C.POINTER(SolveOpts), C.POINTER(SolveResult)],
     "ccgpu_kernel_stats": [C.c_char_p, _I, _DP, C.POINTER(C.c_longlong), _I],
     "ccgpu_debug_fetch": [_I, _I, _DP, C.c_longlong],
+    "ccgpu_mlp_create": [C.c_ulonglong, _IP],
+    "ccgpu_mlp_destroy": [_I],
+    "ccgpu_mlp_get_params": [_I, _FP, _I],
+    "ccgpu_mlp_set_params": [_I, _FP, _I],
+    "ccgpu_mlp_add_solve": [_I, _I, C.POINTER(C.c_longlong)],
+    "ccgpu_mlp_train": [_I, _I, _I, _D, C.c_ulonglong, _DP],
+    "ccgpu_mlp_warm_start": [_I, _I, _DP],
+    "ccgpu_mlp_forward": [_I, _I, _FP, _FP],
+    "ccgpu_mlp_loss_grad": [_I, _I, _FP, _FP, _DP, _FP],
+    "ccgpu_mlp_features": [_I, _I, _FP, _FP],
     "ccgpu_last_error": [C.c_char_p, _I],
 }
 
@@ -290,3 +300,72 @@ def kernel_stats():
     k = lib().ccgpu_kernel_stats(names, 4096, _dp(avg), cnt.ctypes.data_as(C.POINTER(C.c_longlong)), 64)
     nm = names.value.decode().split(";") if k else []
     return {nm[x]: (float(avg[x]), int(cnt[x])) for x in range(k)}
+
+
+MLP_NPARAM = 9473
+
+
+class WarmStart:
+    """The learned start (ccgpu_mlp_*): trained on converged solves, used in
+    place of setup_t_amplitudes."""
+
+    def __init__(self, seed=1):
+        h = C.c_int()
+        _check(lib().ccgpu_mlp_create(seed, C.byref(h)))
+        self.h = h.value
+
+    def __del__(self):
+        try:
+            if _lib is not None and getattr(self, "h", None):
+                _lib.ccgpu_mlp_destroy(self.h)
+        except Exception:  # noqa: BLE001
+            pass
+
+    @property
+    def params(self):
+        th = np.zeros(MLP_NPARAM, np.float32)
+        _check(lib().ccgpu_mlp_get_params(self.h, th.ctypes.data_as(_FP), MLP_NPARAM))
+        return th
+
+    @params.setter
+    def params(self, th):
+        th = np.ascontiguousarray(th, np.float32)
+        _check(lib().ccgpu_mlp_set_params(self.h, th.ctypes.data_as(_FP), len(th)))
+
+    def add_solve(self, solver):
+        n = C.c_longlong()
+        _check(lib().ccgpu_mlp_add_solve(self.h, solver.h, C.byref(n)))
+        return n.value
+
+    def train(self, steps, batch=16384, lr=1e-3, seed=7):
+        loss = C.c_double()
+        _check(lib().ccgpu_mlp_train(self.h, steps, batch, lr, seed, C.byref(loss)))
+        return loss.value
+
+    def warm_start(self, solver):
+        x = np.zeros(2)
+        _check(lib().ccgpu_mlp_warm_start(self.h, solver.h, _dp(x)))
+        return complex(x[0], x[1])
+
+    def forward(self, x):
+        x = np.ascontiguousarray(x, np.float32)
+        y = np.zeros(len(x), np.float32)
+        _check(lib().ccgpu_mlp_forward(self.h, len(x), x.ctypes.data_as(_FP), y.ctypes.data_as(_FP)))
+        return y
+
+    def loss_grad(self, x, t):
+        x = np.ascontiguousarray(x, np.float32)
+        t = np.ascontiguousarray(t, np.float32)
+        g = np.zeros(MLP_NPARAM, np.float32)
+        loss = C.c_double()
+        _check(lib().ccgpu_mlp_loss_grad(self.h, len(t), x.ctypes.data_as(_FP), t.ctypes.data_as(_FP),
+                                         C.byref(loss), g.ctypes.data_as(_FP)))
+        return loss.value, g
+
+    def features(self, solver, with_target=True):
+        n = solver.dims["t2_unique"]
+        f = np.zeros((n, 16), np.float32)
+        t = np.zeros(n, np.float32) if with_target else None
+        _check(lib().ccgpu_mlp_features(self.h, solver.h, f.ctypes.data_as(_FP),
+                                        t.ctypes.data_as(_FP) if with_target else None))
+        return (f, t) if with_target else f

@@ -13,6 +13,7 @@
 #include <map>
 #include <memory>
 #include <mutex>
+#include <random>
 #include <tuple>
 #include <stdexcept>
 #include <string>
@@ -58,6 +59,16 @@ struct TriplesDesc {
 };
 void launch_triples(const SolveDesc &d, const TriplesDesc &td, double volume, double2 *partial, int nblocks,
                     double2 *out, hipStream_t st);
+// mlp.hip
+void launch_mlp_forward(const float *params, const uint16_t *feat, int64_t n, float *y, hipStream_t st);
+void launch_mlp_loss_grad(const float *params, const uint16_t *feat, const float *target, const int64_t *idx,
+                          int64_t n, int64_t npad, const MlpTrainBufs &tb, float *part, int max_split, float *grad,
+                          double *loss_sum, hipStream_t st);
+void launch_mlp_adam(float *th, const float *g, float *m, float *v, float lr, int step, hipStream_t st);
+void launch_mlp_sample(int64_t *idx, int64_t B, int64_t ntot, uint64_t seed, hipStream_t st);
+void launch_mlp_features(const SolveDesc &d, float kf, float ef, const float tw[3], float scale, uint16_t *feat,
+                         float *target, hipStream_t st);
+void launch_mlp_apply(const SolveDesc &d, const float *y, float scale, float ef, hipStream_t st);
 }  // namespace ccgpu
 
 using namespace ccgpu;
@@ -911,6 +922,118 @@ int64_t t_index(const Plan &P, int a, int b, int i, int j, double &sign) {
   return P.off_t[p.ch] + (int64_t)p.idx * P.nhh[p.ch] + h.idx;
 }
 
+// ---------------------------------------------------------------------------
+// Warm-start network state: FP32 master parameters with Adam moments, the
+// device-resident training set (BF16 features and FP32 targets of every
+// amplitude of the solves added so far) and the buffers of one minibatch.
+// ---------------------------------------------------------------------------
+constexpr float kMlpScale = 0.01f;   // t = t_MBPT2 + kMlpScale * y * e_F / |D|
+
+uint16_t host_bf16(float f) {
+  uint32_t u;
+  std::memcpy(&u, &f, 4);
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+
+struct Mlp {
+  hipStream_t stream = nullptr;
+  std::vector<void *> owned;
+  float *params = nullptr, *m = nullptr, *v = nullptr, *grad = nullptr;
+  double *loss = nullptr;
+  int step = 0;
+  // training set
+  uint16_t *feat = nullptr;
+  float *target = nullptr;
+  int64_t n = 0, cap = 0;
+  // minibatch buffers
+  int64_t bcap = 0;
+  MlpTrainBufs tb{};
+  int64_t *idx = nullptr;
+  float *part = nullptr;
+  static constexpr int kMaxSplit = 256;
+  std::vector<void *> batch_owned;
+  // scratch for inference / host-provided data
+  int64_t scap = 0;
+  uint16_t *sfeat = nullptr;
+  float *sy = nullptr, *st = nullptr;
+  std::vector<void *> scratch_owned;
+  ~Mlp() {
+    for (void *p : owned) (void)hipFree(p);
+    for (void *p : batch_owned) (void)hipFree(p);
+    for (void *p : scratch_owned) (void)hipFree(p);
+    if (feat) (void)hipFree(feat);
+    if (target) (void)hipFree(target);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+  void ensure_batch(int64_t B) {
+    if (B <= bcap) return;
+    HIPCHK(hipStreamSynchronize(stream));
+    for (void *p : batch_owned) (void)hipFree(p);
+    batch_owned.clear();
+    tb.x1t = dalloc<uint16_t>((size_t)16 * B, batch_owned);
+    tb.h1t = dalloc<uint16_t>((size_t)64 * B, batch_owned);
+    tb.h2t = dalloc<uint16_t>((size_t)64 * B, batch_owned);
+    tb.h3t = dalloc<uint16_t>((size_t)64 * B, batch_owned);
+    tb.dz1t = dalloc<uint16_t>((size_t)64 * B, batch_owned);
+    tb.dz2t = dalloc<uint16_t>((size_t)64 * B, batch_owned);
+    tb.dz3t = dalloc<uint16_t>((size_t)64 * B, batch_owned);
+    tb.g = dalloc<float>((size_t)B, batch_owned);
+    tb.loss_part = dalloc<float>(2048, batch_owned);
+    idx = dalloc<int64_t>((size_t)B, batch_owned);
+    part = dalloc<float>((size_t)kMaxSplit * kMlpParamCount, batch_owned);
+    bcap = B;
+  }
+  void ensure_scratch(int64_t n_) {
+    if (n_ <= scap) return;
+    HIPCHK(hipStreamSynchronize(stream));
+    for (void *p : scratch_owned) (void)hipFree(p);
+    scratch_owned.clear();
+    sfeat = dalloc<uint16_t>((size_t)16 * n_, scratch_owned);
+    sy = dalloc<float>((size_t)n_, scratch_owned);
+    st = dalloc<float>((size_t)n_, scratch_owned);
+    scap = n_;
+  }
+  void grow_set(int64_t need) {
+    if (need <= cap) return;
+    int64_t nc = std::max<int64_t>(need, cap + cap / 2);
+    uint16_t *f2 = nullptr;
+    float *t2 = nullptr;
+    HIPCHK(hipMalloc((void **)&f2, (size_t)nc * 16 * sizeof(uint16_t)));
+    HIPCHK(hipMalloc((void **)&t2, (size_t)nc * sizeof(float)));
+    if (n > 0) {
+      HIPCHK(hipMemcpyAsync(f2, feat, (size_t)n * 16 * sizeof(uint16_t), hipMemcpyDeviceToDevice, stream));
+      HIPCHK(hipMemcpyAsync(t2, target, (size_t)n * sizeof(float), hipMemcpyDeviceToDevice, stream));
+    }
+    HIPCHK(hipStreamSynchronize(stream));
+    if (feat) (void)hipFree(feat);
+    if (target) (void)hipFree(target);
+    feat = f2;
+    target = t2;
+    cap = nc;
+  }
+};
+
+std::map<int, std::unique_ptr<Mlp>> g_mlps;
+int g_next_mlp = 1;
+
+Mlp &get_mlp(int h) {
+  auto it = g_mlps.find(h);
+  if (it == g_mlps.end()) throw Error(fmt("invalid network handle %d", h));
+  return *it->second;
+}
+
+// kF, e_F and twist/pi of a solve's lattice for the features
+void lattice_scalars(const Solve &S, float &kf, float &ef, float tw[3]) {
+  const Orbitals &o = S.plan.orb;
+  const double hbarc = 197.326968, pmass = (double)939.565f, pi = 3.14159265358979;
+  kf = (float)o.kf;
+  ef = (float)(hbarc * hbarc * o.kf * o.kf / (2.0 * pmass));
+  const double th[3] = {o.kx[0] * o.box - 2.0 * pi * o.nx[0], o.ky[0] * o.box - 2.0 * pi * o.ny[0],
+                        o.kz[0] * o.box - 2.0 * pi * o.nz[0]};
+  for (int k = 0; k < 3; ++k) tw[k] = (float)(th[k] / pi);
+}
+
 }  // namespace
 
 extern "C" {
@@ -1406,6 +1529,196 @@ int ccgpu_debug_fetch(int handle, int what, double *out, long long n) {
     if (n != a[what].n) throw Error(fmt("debug array %d has %lld elements, caller asked %lld", what, a[what].n, n));
     HIPCHK(hipDeviceSynchronize());
     HIPCHK(hipMemcpy(out, a[what].p, n * sizeof(double2), hipMemcpyDeviceToHost));
+  });
+}
+
+// ---------------------------------------------------------------------------
+// warm-start network
+// ---------------------------------------------------------------------------
+int ccgpu_mlp_create(unsigned long long seed, int *mlp) {
+  return guarded([&] {
+    auto M = std::make_unique<Mlp>();
+    HIPCHK(hipStreamCreateWithFlags(&M->stream, hipStreamNonBlocking));
+    M->params = dalloc<float>(kMlpParamCount, M->owned);
+    M->m = dalloc<float>(kMlpParamCount, M->owned);
+    M->v = dalloc<float>(kMlpParamCount, M->owned);
+    M->grad = dalloc<float>(kMlpParamCount, M->owned);
+    M->loss = dalloc<double>(1, M->owned);
+    // He-normal hidden weights, small output layer, zero biases
+    std::vector<float> th(kMlpParamCount, 0.f);
+    std::mt19937_64 rng(seed);
+    auto fill = [&](int off, int fan_in, int count, double gain) {
+      std::normal_distribution<double> nd(0.0, gain / std::sqrt((double)fan_in));
+      for (int k = 0; k < count; ++k) th[off + k] = (float)nd(rng);
+    };
+    fill(0, 16, 16 * 64, std::sqrt(2.0));
+    fill(1088, 64, 64 * 64, std::sqrt(2.0));
+    fill(5248, 64, 64 * 64, std::sqrt(2.0));
+    fill(9408, 64, 64, 0.1);
+    HIPCHK(hipMemcpy(M->params, th.data(), th.size() * sizeof(float), hipMemcpyHostToDevice));
+    std::vector<float> z(kMlpParamCount, 0.f);
+    HIPCHK(hipMemcpy(M->m, z.data(), z.size() * sizeof(float), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(M->v, z.data(), z.size() * sizeof(float), hipMemcpyHostToDevice));
+    int h = g_next_mlp++;
+    g_mlps[h] = std::move(M);
+    *mlp = h;
+  });
+}
+
+int ccgpu_mlp_destroy(int mlp) {
+  return guarded([&] {
+    get_mlp(mlp);
+    g_mlps.erase(mlp);
+  });
+}
+
+int ccgpu_mlp_get_params(int mlp, float *theta, int n) {
+  return guarded([&] {
+    Mlp &M = get_mlp(mlp);
+    if (n != kMlpParamCount) throw Error(fmt("the network has %d parameters, caller asked %d", kMlpParamCount, n));
+    HIPCHK(hipStreamSynchronize(M.stream));
+    HIPCHK(hipMemcpy(theta, M.params, n * sizeof(float), hipMemcpyDeviceToHost));
+  });
+}
+
+int ccgpu_mlp_set_params(int mlp, const float *theta, int n) {
+  return guarded([&] {
+    Mlp &M = get_mlp(mlp);
+    if (n != kMlpParamCount) throw Error(fmt("the network has %d parameters, caller gave %d", kMlpParamCount, n));
+    HIPCHK(hipStreamSynchronize(M.stream));
+    HIPCHK(hipMemcpy(M.params, theta, n * sizeof(float), hipMemcpyHostToDevice));
+    std::vector<float> z(kMlpParamCount, 0.f);
+    HIPCHK(hipMemcpy(M.m, z.data(), z.size() * sizeof(float), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(M.v, z.data(), z.size() * sizeof(float), hipMemcpyHostToDevice));
+    M.step = 0;
+  });
+}
+
+// host features (n x 16, rounded to BF16 here) into the scratch buffer
+static void upload_features(Mlp &M, int n, const float *x) {
+  if (n <= 0) throw Error("no samples");
+  M.ensure_scratch(n);
+  std::vector<uint16_t> xb((size_t)n * 16);
+  for (size_t k = 0; k < xb.size(); ++k) xb[k] = host_bf16(x[k]);
+  HIPCHK(hipMemcpyAsync(M.sfeat, xb.data(), xb.size() * sizeof(uint16_t), hipMemcpyHostToDevice, M.stream));
+  HIPCHK(hipStreamSynchronize(M.stream));
+}
+
+int ccgpu_mlp_forward(int mlp, int n, const float *x, float *y) {
+  return guarded([&] {
+    Mlp &M = get_mlp(mlp);
+    upload_features(M, n, x);
+    launch_mlp_forward(M.params, M.sfeat, n, M.sy, M.stream);
+    check_launch();
+    HIPCHK(hipMemcpyAsync(y, M.sy, (size_t)n * sizeof(float), hipMemcpyDeviceToHost, M.stream));
+    HIPCHK(hipStreamSynchronize(M.stream));
+  });
+}
+
+int ccgpu_mlp_loss_grad(int mlp, int n, const float *x, const float *t, double *loss, float *grad) {
+  return guarded([&] {
+    Mlp &M = get_mlp(mlp);
+    upload_features(M, n, x);
+    HIPCHK(hipMemcpyAsync(M.st, t, (size_t)n * sizeof(float), hipMemcpyHostToDevice, M.stream));
+    const int64_t npad = ((int64_t)n + 63) / 64 * 64;
+    M.ensure_batch(npad);
+    launch_mlp_loss_grad(M.params, M.sfeat, M.st, nullptr, n, npad, M.tb, M.part, Mlp::kMaxSplit, M.grad, M.loss,
+                         M.stream);
+    check_launch();
+    double l = 0;
+    HIPCHK(hipMemcpyAsync(&l, M.loss, sizeof(double), hipMemcpyDeviceToHost, M.stream));
+    HIPCHK(hipMemcpyAsync(grad, M.grad, kMlpParamCount * sizeof(float), hipMemcpyDeviceToHost, M.stream));
+    HIPCHK(hipStreamSynchronize(M.stream));
+    *loss = l / n;
+  });
+}
+
+int ccgpu_mlp_features(int mlp, int handle, float *feat, float *target) {
+  return guarded([&] {
+    Mlp &M = get_mlp(mlp);
+    Solve &S = get(handle);
+    ensure_ready(S);
+    if (target && !S.have_t) throw Error("no amplitudes for the targets");
+    const int64_t nT = S.plan.nT;
+    M.ensure_scratch(nT);
+    float kf, ef, tw[3];
+    lattice_scalars(S, kf, ef, tw);
+    HIPCHK(hipStreamSynchronize(S.stream));
+    launch_mlp_features(S.desc, kf, ef, tw, kMlpScale, M.sfeat, target ? M.st : nullptr, M.stream);
+    check_launch();
+    std::vector<uint16_t> fb((size_t)nT * 16);
+    HIPCHK(hipMemcpyAsync(fb.data(), M.sfeat, fb.size() * sizeof(uint16_t), hipMemcpyDeviceToHost, M.stream));
+    if (target) HIPCHK(hipMemcpyAsync(target, M.st, (size_t)nT * sizeof(float), hipMemcpyDeviceToHost, M.stream));
+    HIPCHK(hipStreamSynchronize(M.stream));
+    for (size_t k = 0; k < fb.size(); ++k) {
+      uint32_t u = (uint32_t)fb[k] << 16;
+      std::memcpy(&feat[k], &u, 4);
+    }
+  });
+}
+
+int ccgpu_mlp_add_solve(int mlp, int handle, long long *nsamples) {
+  return guarded([&] {
+    Mlp &M = get_mlp(mlp);
+    Solve &S = get(handle);
+    ensure_ready(S);
+    if (!S.have_t) throw Error("ccgpu_mlp_add_solve needs the solve's converged amplitudes");
+    const int64_t nT = S.plan.nT;
+    M.grow_set(M.n + nT);
+    float kf, ef, tw[3];
+    lattice_scalars(S, kf, ef, tw);
+    HIPCHK(hipStreamSynchronize(S.stream));
+    launch_mlp_features(S.desc, kf, ef, tw, kMlpScale, M.feat + M.n * 16, M.target + M.n, M.stream);
+    check_launch();
+    HIPCHK(hipStreamSynchronize(M.stream));
+    M.n += nT;
+    if (nsamples) *nsamples = M.n;
+  });
+}
+
+int ccgpu_mlp_train(int mlp, int steps, int batch, double lr, unsigned long long seed, double *loss_last) {
+  return guarded([&] {
+    Mlp &M = get_mlp(mlp);
+    if (M.n <= 0) throw Error("empty training set: call ccgpu_mlp_add_solve first");
+    if (batch <= 0 || batch % 64 != 0) throw Error(fmt("batch %d is not a positive multiple of 64", batch));
+    if (steps <= 0) throw Error("steps must be positive");
+    M.ensure_batch(batch);
+    for (int k = 0; k < steps; ++k) {
+      launch_mlp_sample(M.idx, batch, M.n, seed + 0x51ED27ull * (uint64_t)(M.step + 1), M.stream);
+      launch_mlp_loss_grad(M.params, M.feat, M.target, M.idx, batch, batch, M.tb, M.part, Mlp::kMaxSplit, M.grad,
+                           M.loss, M.stream);
+      ++M.step;
+      launch_mlp_adam(M.params, M.grad, M.m, M.v, (float)lr, M.step, M.stream);
+    }
+    check_launch();
+    double l = 0;
+    HIPCHK(hipMemcpyAsync(&l, M.loss, sizeof(double), hipMemcpyDeviceToHost, M.stream));
+    HIPCHK(hipStreamSynchronize(M.stream));
+    if (loss_last) *loss_last = l / batch;
+  });
+}
+
+int ccgpu_mlp_warm_start(int mlp, int handle, double *ecorr) {
+  return guarded([&] {
+    Mlp &M = get_mlp(mlp);
+    Solve &S = get(handle);
+    ensure_ready(S);
+    const int64_t nT = S.plan.nT;
+    M.ensure_scratch(nT);
+    float kf, ef, tw[3];
+    lattice_scalars(S, kf, ef, tw);
+    HIPCHK(hipStreamSynchronize(M.stream));
+    // on the solve's stream: features, network, t = t_MBPT2 + scale y
+    launch_mlp_features(S.desc, kf, ef, tw, kMlpScale, M.sfeat, nullptr, S.stream);
+    launch_mlp_forward(M.params, M.sfeat, nT, M.sy, S.stream);
+    launch_mlp_apply(S.desc, M.sy, kMlpScale, ef, S.stream);
+    check_launch();
+    Batch &B = single(handle);
+    auto e = energies(B, false);
+    S.have_t = true;
+    S.nstep = 0;
+    S.count = 0;
+    if (ecorr) { ecorr[0] = e[0].real(); ecorr[1] = e[0].imag(); }
   });
 }
 

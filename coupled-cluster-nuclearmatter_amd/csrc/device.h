@@ -71,4 +71,15 @@ struct GemmTile {
   int kb, ke, pad0, pad1;  // k range of this tile
 };
 
+// Warm-start network: number of parameters (flat FP32 vector, layout in
+// mlp.hip) and the buffers of one training step: feature-major activations and
+// pre-activation gradients of the minibatch (B columns, B a multiple of 64).
+constexpr int kMlpParamCount = 9473;
+struct MlpTrainBufs {
+  uint16_t *x1t, *h1t, *h2t, *h3t;   // [16][B], [64][B] x 3
+  uint16_t *dz1t, *dz2t, *dz3t;      // [64][B] x 3
+  float *g;                          // [B] dL/dy
+  float *loss_part;                  // [gridDim.x of the pass]
+};
+
 }  // namespace ccgpu

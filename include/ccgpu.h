@@ -33,6 +33,12 @@
  *   ccgpu_batch_solve                 no reference counterpart: the reference
  *                                     runs one twist per process
  *                                     (ccm_main.f90:163-279)
+ *   ccgpu_mlp_*                       no reference counterpart: a learned start
+ *                                     that replaces setup_t_amplitudes
+ *                                     (ccm_iter.f90:2329-2405) as the first
+ *                                     guess; the iteration that follows is
+ *                                     unchanged, so the converged energy does
+ *                                     not depend on the network
  */
 #ifndef CCGPU_H
 #define CCGPU_H
@@ -177,6 +183,32 @@ int ccgpu_debug_fetch(int handle, int what, double *out, long long n);
 /* Per-kernel timings (ms, averaged over the recorded launches) of the last
  * solve; names are written as a ';'-separated list. */
 int ccgpu_kernel_stats(char *names, int len, double *avg_ms, long long *launches, int max);
+
+/* Warm-start network (16 -> 64 -> 64 -> 64 -> 1, GELU, BF16 MFMA with FP32
+ * accumulation).  It predicts, per stored amplitude, the correction of the
+ * converged CCD amplitude over its MBPT2 value from momentum, spin, density and
+ * twist features:  t = t_MBPT2 + 0.01 * y * e_F / |D| with D = f_ii + f_jj -
+ * f_aa - f_bb (the network predicts the numerator correction).  Parameters: a flat FP32 vector of
+ * CCGPU_MLP_NPARAM values, W1[16][64] b1 W2[64][64] b2 W3[64][64] b3 w4[64] b4. */
+#define CCGPU_MLP_NPARAM 9473
+int ccgpu_mlp_create(unsigned long long seed, int *mlp);
+int ccgpu_mlp_destroy(int mlp);
+int ccgpu_mlp_get_params(int mlp, float *theta, int n);
+int ccgpu_mlp_set_params(int mlp, const float *theta, int n);   /* resets Adam */
+/* Append every amplitude of a converged solve to the device training set. */
+int ccgpu_mlp_add_solve(int mlp, int handle, long long *nsamples);
+/* Adam steps on minibatches of `batch` (multiple of 64) samples drawn with
+ * replacement; loss_last = mean squared error of the last minibatch. */
+int ccgpu_mlp_train(int mlp, int steps, int batch, double lr, unsigned long long seed, double *loss_last);
+/* Start amplitudes of a handle from the network (instead of ccgpu_mbpt2_start);
+ * returns the correlation energy of the start. */
+int ccgpu_mlp_warm_start(int mlp, int handle, double *ecorr);
+/* For checks: network output, mean-squared loss and its gradient on n host
+ * samples (n x 16 features, rounded to BF16 on entry), and the features and
+ * targets of a handle's amplitudes (nT x 16 and nT floats). */
+int ccgpu_mlp_forward(int mlp, int n, const float *x, float *y);
+int ccgpu_mlp_loss_grad(int mlp, int n, const float *x, const float *t, double *loss, float *grad);
+int ccgpu_mlp_features(int mlp, int handle, float *feat, float *target);
 
 int ccgpu_last_error(char *buf, int len);
 
