@@ -19,7 +19,7 @@ LIB_PATH = os.path.join(HERE, "libccgpu.so")
 PNM, SNM = 0, 1
 INT_MINNESOTA = 0
 BLOCK_HHHH, BLOCK_HHHP, BLOCK_HHPP, BLOCK_HPHP, BLOCK_HPPP, BLOCK_PPPP = 1, 2, 3, 4, 5, 6
-PREC_FP64, PREC_FP32 = 0, 1
+PREC_FP64, PREC_FP32, PREC_V32 = 0, 1, 2
 
 
 class LatticeParams(C.Structure):

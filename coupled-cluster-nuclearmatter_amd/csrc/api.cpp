@@ -486,10 +486,10 @@ struct Batch {
   std::vector<double2> h_coef, h_scal;
   std::vector<void *> owned;
   hipStream_t stream = nullptr;
-  hipGraph_t graph[2] = {nullptr, nullptr};
-  hipGraphExec_t gexec[2] = {nullptr, nullptr};
+  hipGraph_t graph[3] = {nullptr, nullptr, nullptr};        // per precision mode
+  hipGraphExec_t gexec[3] = {nullptr, nullptr, nullptr};
   ~Batch() {
-    for (int q = 0; q < 2; ++q) {
+    for (int q = 0; q < 3; ++q) {
       if (gexec[q]) (void)hipGraphExecDestroy(gexec[q]);
       if (graph[q]) (void)hipGraphDestroy(graph[q]);
     }
@@ -579,27 +579,40 @@ int term_mask() {
 }
 
 // One GEMM family: three launches (one per tile class), or the VALU check.
+// prec here is the arithmetic of this family: FP64, FP32, or V32 (pp ladder
+// only: FP32-stored A, FP64 arithmetic)
 void run_gemm(const GemmList &L, const Batch &B, int prec, bool a_f32, hipStream_t st, bool pp = false) {
   if (gemm_variant_override() == 3) {
     launch_gemm(L.d_groups, L.d_tiles64, (int)L.tiles64.size(), B.d_descs, L.d_gsolve, 3, st);
     return;
   }
   int base = prec == CCGPU_PREC_FP64 ? 10 : (a_f32 ? 30 : 20);
-  if (pp) base = prec == CCGPU_PREC_FP64 ? 40 : 60;
+  if (pp) base = prec == CCGPU_PREC_FP64 ? 40 : prec == CCGPU_PREC_V32 ? 70 : 60;
   const int nb[GemmList::NCLS] = {1, 2, 3, 4, 5};
   for (int c = GemmList::NCLS - 1; c >= 0; --c)
     launch_gemm(L.d_groups, L.d_tiles[c], (int)L.tiles[c].size(), B.d_descs, L.d_gsolve, base + nb[c], st);
   launch_splitk_reduce(L.d_groups, L.d_split, (int)L.split.size(), L.maxmn, B.d_descs, L.d_gsolve, st);
 }
 
+// Families in FP32 under CCGPU_PREC_FP32 (bit 0 X/I9 GEMM, 1 pp ladder, 2 hh
+// ladder, 3 ring GEMMs); all by default, CCGPU_FP32_FAMILIES narrows it down
+// for precision studies.
+int fp32_families() {
+  const char *e = getenv("CCGPU_FP32_FAMILIES");
+  return e ? atoi(e) : 15;
+}
+
 void residual_launches(Batch &B, int prec, bool timing) {
   const int n = (int)B.solves.size();
   hipStream_t st = B.stream;
   const int terms = term_mask();
-  if (prec == CCGPU_PREC_FP32 && !B.has_f32) throw Error("FP32 residual without the FP32 copy of Vpppp");
+  if (prec != CCGPU_PREC_FP64 && !B.has_f32) throw Error("FP32/V32 residual without the FP32 copy of Vpppp");
+  const int fam = prec == CCGPU_PREC_FP32 ? fp32_families() : 0;
+  auto pf = [&](int bit) { return (fam >> bit) & 1 ? CCGPU_PREC_FP32 : CCGPU_PREC_FP64; };
+  const int pp_prec = prec == CCGPU_PREC_V32 ? CCGPU_PREC_V32 : pf(1);
   {
     Timer t(timing, st, "gemm_x_vhhpp_t");
-    run_gemm(B.g_x, B, prec, false, st);
+    run_gemm(B.g_x, B, pf(0), false, st);
   }
   {
     Timer t(timing, st, "i7_i9_i6");
@@ -613,12 +626,12 @@ void residual_launches(Batch &B, int prec, bool timing) {
   }
   if (terms & 2) {
     Timer t(timing, st, "gemm_pp_ladder");
-    if (prec == CCGPU_PREC_FP32) run_gemm(B.g_pp32, B, prec, true, st, true);
-    else run_gemm(B.g_pp, B, prec, false, st, true);
+    if (pp_prec != CCGPU_PREC_FP64) run_gemm(B.g_pp32, B, pp_prec, true, st, true);
+    else run_gemm(B.g_pp, B, pp_prec, false, st, true);
   }
   if (terms & 4) {
     Timer t(timing, st, "gemm_hh_ladder");
-    run_gemm(B.g_hh, B, prec, false, st);
+    run_gemm(B.g_hh, B, pf(2), false, st);
   }
   if (!(terms & 8)) return;
   {
@@ -627,11 +640,11 @@ void residual_launches(Batch &B, int prec, bool timing) {
   }
   {
     Timer t(timing, st, "gemm_ring_i4");
-    run_gemm(B.g_i4, B, prec, false, st);
+    run_gemm(B.g_i4, B, pf(3), false, st);
   }
   {
     Timer t(timing, st, "gemm_ring_rt");
-    run_gemm(B.g_rt, B, prec, false, st);
+    run_gemm(B.g_rt, B, pf(3), false, st);
   }
   {
     Timer t(timing, st, "ring_scatter");
@@ -741,9 +754,11 @@ void run_batch(Batch &B, const ccgpu_solve_opts &op, std::vector<ccgpu_solve_res
     B.h_descs[s].active = 1;
   }
   HIPCHK(hipMemcpy(B.d_descs, B.h_descs.data(), n * sizeof(SolveDesc), hipMemcpyHostToDevice));
-  if (op.precision == CCGPU_PREC_FP32) {
+  if (op.precision != CCGPU_PREC_FP64 && op.precision != CCGPU_PREC_FP32 && op.precision != CCGPU_PREC_V32)
+    throw Error(fmt("unknown precision %d", op.precision));
+  if (op.precision != CCGPU_PREC_FP64) {
     for (Solve *S : B.solves)
-      if (!S->desc.Vpppp32) throw Error("FP32 ladder requested before the FP32 copy of Vpppp was made");
+      if (!S->desc.Vpppp32) throw Error("FP32/V32 ladder requested before the FP32 copy of Vpppp was made");
   }
   auto e1 = energies(B, false);
   res.assign(n, ccgpu_solve_result{});
@@ -759,30 +774,31 @@ void run_batch(Batch &B, const ccgpu_solve_opts &op, std::vector<ccgpu_solve_res
   auto t0 = std::chrono::steady_clock::now();
   bool graph_ok = op.use_graph != 0 && !timing && gemm_variant_override() < 0;
   int refine = op.refine_every > 0 ? op.refine_every : 0;
-  // Mixed precision: FP32 contractions (with the usual DIIS schedule) until a
-  // DIIS extrapolation that follows an iteration with max |dE| <= fp32_switch;
-  // from the next iteration on the contractions are FP64 and the DIIS history
-  // carries on, so every later error vector is an FP64 one and the final fixed
-  // point and stopping test are FP64.  Should the FP32 iteration itself come
-  // within 10 tol first (FP32 rounding then dominates |dE|), the switch happens
-  // at once and the DIIS history restarts.
-  bool fp32_phase = op.precision == CCGPU_PREC_FP32;
+  // Mixed precision: low-precision contractions (FP32 or V32, with the usual
+  // DIIS schedule) until
+  //  (a) a DIIS extrapolation that follows an iteration with max |dE| <=
+  //      fp32_switch (fp32_switch > 0), or
+  //  (b) every solve's low-precision |dE| <= tol: that stopping test is met.
+  // From the next iteration on the contractions are FP64 and the DIIS history
+  // carries on; at least one FP64 iteration follows, so the stopping test that
+  // ends a solve is always an FP64 one.  No solve ends inside the first phase.
+  bool fp32_phase = op.precision != CCGPU_PREC_FP64;
   double prev_maxd = 1e300;
   for (int it = 1; it <= op.maxit; ++it) {
     bool any = false;
     for (int s = 0; s < n; ++s) {
-      bool act = std::abs(dener[s]) > op.tol;
+      bool act = fp32_phase ? B.h_descs[s].active != 0 : std::abs(dener[s]) > op.tol;
       if (!act && B.h_descs[s].active) { set_active(B, s, 0); conv[s] = 1; }
       any = any || act;
     }
     if (!any) break;
     int prec = CCGPU_PREC_FP64;
     if (fp32_phase) {
-      prec = CCGPU_PREC_FP32;
+      prec = op.precision;
       if (refine && it % refine == 0) prec = CCGPU_PREC_FP64;
     }
     if (graph_ok) {
-      int q = prec == CCGPU_PREC_FP32 ? 1 : 0;
+      const int q = prec;
       if (!B.gexec[q]) {
         HIPCHK(hipStreamBeginCapture(B.stream, hipStreamCaptureModeThreadLocal));
         residual_launches(B, prec, false);
@@ -806,22 +822,16 @@ void run_batch(Batch &B, const ccgpu_solve_opts &op, std::vector<ccgpu_solve_res
       dener[s] = e2[s] - elast[s];
       elast[s] = e2[s];
       iters[s] = it;
-      if (prec == CCGPU_PREC_FP32) ++iters32[s];
+      if (prec != CCGPU_PREC_FP64) ++iters32[s];
       maxd = std::max(maxd, std::abs(dener[s]));
       if (op.verbose) printf("dener %5d %5d %16.8E  E %.12f%s\n", s, it, dener[s].real(), e2[s].real(),
-                             prec == CCGPU_PREC_FP32 ? "  fp32" : "");
+                             prec == CCGPU_PREC_FP32 ? "  fp32" : prec == CCGPU_PREC_V32 ? "  v32" : "");
     }
     if (fp32_phase) {
-      const bool at_boundary = extrapolated && prev_maxd <= op.fp32_switch;
-      const bool at_floor = maxd <= 10.0 * op.tol;
-      if (at_boundary || at_floor) {
+      const bool at_boundary = op.fp32_switch > 0.0 && extrapolated && prev_maxd <= op.fp32_switch;
+      const bool at_tol = maxd <= op.tol;
+      if (at_boundary || at_tol) {
         fp32_phase = false;
-        if (!at_boundary) {
-          count = 0;
-          nstep = 0;
-          for (Solve *S : B.solves)
-            for (int k = 0; k < op.subspace; ++k) S->ring[k] = k;
-        }
         // an FP32 |dE| is no FP64 convergence test: force at least one FP64 step
         for (int s = 0; s < n; ++s)
           if (B.h_descs[s].active) dener[s] = std::max(std::abs(dener[s]), 2.0 * op.tol);
@@ -1307,8 +1317,9 @@ int ccgpu_residual(int handle, int precision) {
     Solve &S = get(handle);
     ensure_ready(S);
     if (!S.have_t) throw Error("no amplitudes: call ccgpu_mbpt2_start or ccgpu_set_t2 first");
-    if (precision != CCGPU_PREC_FP64 && precision != CCGPU_PREC_FP32) throw Error(fmt("unknown precision %d", precision));
-    if (precision == CCGPU_PREC_FP32) ensure_f32(S, handle);
+    if (precision != CCGPU_PREC_FP64 && precision != CCGPU_PREC_FP32 && precision != CCGPU_PREC_V32)
+      throw Error(fmt("unknown precision %d", precision));
+    if (precision != CCGPU_PREC_FP64) ensure_f32(S, handle);
     Batch &B = single(handle);
     residual_launches(B, precision, false);
     check_launch();
@@ -1355,7 +1366,7 @@ int ccgpu_solve(int handle, const ccgpu_solve_opts *opts, ccgpu_solve_result *re
     Solve &S = get(handle);
     ensure_ready(S);
     if (!S.have_t) throw Error("no amplitudes: call ccgpu_mbpt2_start or ccgpu_set_t2 first");
-    if (opts->precision == CCGPU_PREC_FP32) ensure_f32(S, handle);
+    if (opts->precision != CCGPU_PREC_FP64) ensure_f32(S, handle);
     Batch &B = single(handle);
     std::vector<ccgpu_solve_result> r;
     run_batch(B, *opts, r);
@@ -1371,7 +1382,7 @@ int ccgpu_batch_solve(int n, const int *handles, const ccgpu_solve_opts *opts, c
       Solve &S = get(handles[s]);
       ensure_ready(S);
       if (!S.have_t) throw Error(fmt("handle %d has no amplitudes", handles[s]));
-      if (opts->precision == CCGPU_PREC_FP32) ensure_f32(S, handles[s]);
+      if (opts->precision != CCGPU_PREC_FP64) ensure_f32(S, handles[s]);
       ss.push_back(&S);
     }
     auto B = make_batch(ss, ss[0]->stream);

@@ -559,8 +559,9 @@ __global__ __launch_bounds__(256) void k_zgemm_valu(const GemmGroup *__restrict_
 // k0+4g..k0+4g+3 of a 16-wide k chunk (one 64-byte read of A per lane, a full
 // 128-byte line per row across four lanes) and the four MFMAs of the chunk run
 // over those four k values.
-// FAM only names the instantiation (1 = pp ladder) so profiles can tell the
-// pp-ladder launches from other families with the same tile grid.
+// FAM only names the instantiation (1 = pp ladder, 2 = pp ladder on the FP32
+// copy of V with FP64 arithmetic) so profiles can tell the pp-ladder launches
+// from other families with the same tile grid.
 template <int MB, int NB, bool F32, typename AT, int FAM = 0>
 __global__ __launch_bounds__(256) void k_zgemm_dl(const GemmGroup *__restrict__ groups,
                                                   const GemmTile *__restrict__ tiles,
@@ -821,6 +822,12 @@ void launch_gemm(const GemmGroup *groups, const GemmTile *tiles, int ntiles, con
     case 43: hipLaunchKernelGGL((k_zgemm_dl<1, 3, false, double2, 1>), dim3(ntiles), dim3(256), 0, st, groups, tiles, dd, gsolve); break;
     case 44: hipLaunchKernelGGL((k_zgemm_dl<1, 4, false, double2, 1>), dim3(ntiles), dim3(256), 0, st, groups, tiles, dd, gsolve); break;
     case 45: hipLaunchKernelGGL((k_zgemm_dl<1, 5, false, double2, 1>), dim3(ntiles), dim3(256), 0, st, groups, tiles, dd, gsolve); break;
+    // 70+nb: pp ladder reading the FP32 copy of V, FP64 MFMA and accumulation
+    case 71: hipLaunchKernelGGL((k_zgemm_dl<2, 1, false, float2, 2>), dim3(ntiles), dim3(256), 0, st, groups, tiles, dd, gsolve); break;
+    case 72: hipLaunchKernelGGL((k_zgemm_dl<2, 2, false, float2, 2>), dim3(ntiles), dim3(256), 0, st, groups, tiles, dd, gsolve); break;
+    case 73: hipLaunchKernelGGL((k_zgemm_dl<1, 3, false, float2, 2>), dim3(ntiles), dim3(256), 0, st, groups, tiles, dd, gsolve); break;
+    case 74: hipLaunchKernelGGL((k_zgemm_dl<1, 4, false, float2, 2>), dim3(ntiles), dim3(256), 0, st, groups, tiles, dd, gsolve); break;
+    case 75: hipLaunchKernelGGL((k_zgemm_dl<1, 5, false, float2, 2>), dim3(ntiles), dim3(256), 0, st, groups, tiles, dd, gsolve); break;
     case 61: hipLaunchKernelGGL((k_zgemm_dl<2, 1, true, float2, 1>), dim3(ntiles), dim3(256), 0, st, groups, tiles, dd, gsolve); break;
     case 62: hipLaunchKernelGGL((k_zgemm_dl<2, 2, true, float2, 1>), dim3(ntiles), dim3(256), 0, st, groups, tiles, dd, gsolve); break;
     case 63: hipLaunchKernelGGL((k_zgemm_dl<1, 3, true, float2, 1>), dim3(ntiles), dim3(256), 0, st, groups, tiles, dd, gsolve); break;

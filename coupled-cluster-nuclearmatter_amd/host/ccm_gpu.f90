@@ -10,9 +10,10 @@
 ! One optional line after the reference deck sets engine options:
 !   ! interaction, DIIS subspace, DIIS step, tolerance, max iterations, precision, fp32 switch
 !   minnesota, 6, 6, 1e-6, 100, fp64, 1.0
-! With precision fp32 the contractions run in FP32 up to the first DIIS
-! extrapolation that follows an iteration with |dE| <= the switch (MeV), and in
-! FP64 afterwards (DIIS history kept), so the stopping test is always FP64.
+! With precision fp32 (or v32: FP32-stored pppp elements, FP64 arithmetic) the contractions run in FP32 up to the first DIIS
+! extrapolation that follows an iteration with |dE| <= the switch (MeV; 0: never
+! there) or until the FP32 |dE| meets the tolerance, and in FP64 afterwards
+! (DIIS history kept), so the stopping test is always an FP64 one.
 ! Without it the reference constants are used (DIIS 6/6, 1e-6, 100 iterations).
 program ccm_gpu
   use iso_c_binding
@@ -122,18 +123,19 @@ program ccm_gpu
   if (trim(cc_approx) /= 'mbpt2') then
      prec = CCGPU_PREC_FP64
      if (trim(precname) == 'fp32') prec = CCGPU_PREC_FP32
+     if (trim(precname) == 'v32') prec = CCGPU_PREC_V32
      tol4r = real(tol4, 8)
      call cpu_time(t_start)
      count = 0
      nstep = 0
      ener2 = 0.d0
      dener = 4.0
-     fp32_phase = prec == CCGPU_PREC_FP32
+     fp32_phase = prec /= CCGPU_PREC_FP64
      prev_dener = huge(1.d0)
      do itimes = 1, maxit
         if (abs(dener) > tol4r) then
            st = CCGPU_PREC_FP64
-           if (fp32_phase) st = CCGPU_PREC_FP32
+           if (fp32_phase) st = prec
            call ccgpu_check(ccgpu_residual(h, st), 'ccgpu_residual')
            count = count + 1
            nstep = nstep + 1
@@ -145,14 +147,9 @@ program ccm_gpu
            ener1 = ener2
            write(6,'(a,2i5,2e16.8)') 'dener', 0, itimes, real(dener)
            if (fp32_phase) then
-              at_boundary = extrapolated .and. prev_dener <= fp32_switch
-              if (at_boundary .or. abs(dener) <= 10.d0 * tol4r) then
+              at_boundary = fp32_switch > 0.d0 .and. extrapolated .and. prev_dener <= fp32_switch
+              if (at_boundary .or. abs(dener) <= tol4r) then
                  fp32_phase = .false.
-                 if (.not. at_boundary) then
-                    ! FP32 rounding dominates |dE| here: restart the DIIS history
-                    count = 0
-                    nstep = 0
-                 end if
                  ! an FP32 |dE| is no FP64 convergence test
                  if (abs(dener) <= 2.d0 * tol4r) dener = 2.d0 * tol4r
               end if

@@ -62,9 +62,16 @@ extern "C" {
 #define CCGPU_BLOCK_HPPP 5
 #define CCGPU_BLOCK_PPPP 6
 
-/* precision modes of the pp/hh ladders and the ring contractions */
+/* precision modes of the pp/hh ladders and the ring contractions:
+ * FP64  everything FP64;
+ * FP32  every contraction on FP32 MFMA with FP32 accumulation (pp ladder on a
+ *       complex64 copy of V_pppp);
+ * V32   the pp ladder reads the complex64 copy of V_pppp (half the bytes) and
+ *       computes in FP64; the rounding of V is a fixed perturbation of the
+ *       problem, so the iteration and DIIS behave as in FP64 */
 #define CCGPU_PREC_FP64 0
 #define CCGPU_PREC_FP32 1
+#define CCGPU_PREC_V32 2
 
 typedef struct {
   int A;            /* number of nucleons (holes) */
@@ -98,9 +105,11 @@ typedef struct {
   int refine_every;  /* mixed precision: FP64 residual every k iterations */
   int use_graph;     /* capture one iteration in a hipGraph and replay it */
   int verbose;       /* print per-iteration lines like the reference */
-  double fp32_switch;/* mixed precision: FP32 contractions up to the first DIIS
+  double fp32_switch;/* mixed precision: the FP32 phase ends at the first DIIS
                         extrapolation that follows an iteration with max |dE| <=
-                        this (MeV); FP64 afterwards, DIIS history kept */
+                        fp32_switch (MeV; <= 0: never there), or once every
+                        FP32 |dE| <= tol; FP64 afterwards with the DIIS history
+                        kept, and at least one FP64 iteration before a stop */
 } ccgpu_solve_opts;
 
 typedef struct {
@@ -111,7 +120,7 @@ typedef struct {
   double eccd[2];        /* E0 + CCD correlation energy */
   double ecorr_last[2];  /* last correlation energy */
   double seconds;        /* wall time of the iteration loop */
-  int fp32_iterations;   /* iterations whose contractions ran in FP32 */
+  int fp32_iterations;   /* iterations whose contractions ran in FP32 or V32 */
   int pad_;
 } ccgpu_solve_result;
 

@@ -256,6 +256,24 @@ def test_triples_against_reference(ccnm, name):
     assert abs(et2 - ref) < 1e-6 * abs(ref) + float(g["tol"])
 
 
+def test_fp32_until_tolerance_with_fp64_confirmation(ccnm):
+    """fp32_switch <= 0: FP32 contractions until |dE| <= tol, then FP64
+    iterations until an FP64 |dE| <= tol; E/N within 1e-5 MeV of plain FP64."""
+    ref = None
+    for prec, tol in ((ccnm.PREC_FP64, 1e-8), (ccnm.PREC_FP32, 1e-5)):
+        s = ccnm.Solver(A=66, matter=ccnm.PNM, rho=0.16, nmax=2)
+        s.generate_interaction()
+        s.setup_t_amplitudes()
+        out = s.ccd_iter(subspace=8, diis_step=8, tol=tol, maxit=100, precision=prec, fp32_switch=0.0)
+        if ref is None:
+            ref = out
+            continue
+    assert out["converged"]
+    assert out["fp32_iterations"] >= 8
+    assert out["iterations"] > out["fp32_iterations"]
+    assert abs(out["eccd"] - ref["eccd"]) / 66 < 1e-5
+
+
 def test_step_api_fp32_residual(ccnm):
     """t2_intermediate in FP32 through the step API: the residual agrees with
     the FP64 one to FP32 rounding (relative 1e-5) and is not bit-identical."""
