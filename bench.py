@@ -208,6 +208,9 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--profile-run", action="store_true",
+                    help="short run for counter collection: 6 FP64 iterations of the rho=0.16 lattice, "
+                         "then 6 FP32 iterations of the four-density batch; no JSON line")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this engine runs on one GPU; --gpus above 1 is not supported "
@@ -223,6 +226,16 @@ def main():
         solvers.append(s)
     setup_s = time.perf_counter() - t0
     print(f"# setup (lattice, V, Fock) for 4 densities: {setup_s:.2f} s", file=sys.stderr, flush=True)
+    if args.profile_run:
+        solvers[2].setup_t_amplitudes()
+        r64 = solvers[2].ccd_iter(subspace=DEPTH, diis_step=DEPTH, tol=1e-30, maxit=6, use_graph=1)
+        for s in solvers:
+            s.setup_t_amplitudes()
+        r32 = ccnm.batch_solve(solvers, subspace=DEPTH, diis_step=DEPTH, tol=1e-30, maxit=6,
+                               precision=ccnm.PREC_FP32, fp32_switch=0.0, use_graph=1)
+        print(f"# profile run: FP64 {r64['seconds']*1e3:.1f} ms / 6 iterations, "
+              f"FP32 batch {r32[0]['seconds']*1e3:.1f} ms / 6 iterations", file=sys.stderr, flush=True)
+        return
 
     # plain FP64 arm (reference for the error check and the speedup)
     plain = run_plain(ccnm, solvers)   # warm
@@ -323,6 +336,7 @@ def main():
         "roofline": roof,
         "roofline_fp32_ladder": roof32,
         "kernel_ms_fp64": {k: round(v[0], 4) for k, v in st64.items()},
+        "tile_shapes": ccnm.autotune_report(),
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

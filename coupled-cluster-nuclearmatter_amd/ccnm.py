@@ -96,6 +96,7 @@ SIGNATURES = {
     "ccgpu_mlp_forward": [_I, _I, _FP, _FP],
     "ccgpu_mlp_loss_grad": [_I, _I, _FP, _FP, _DP, _FP],
     "ccgpu_mlp_features": [_I, _I, _FP, _FP],
+    "ccgpu_autotune_report": [C.c_char_p, _I],
     "ccgpu_last_error": [C.c_char_p, _I],
 }
 
@@ -300,6 +301,12 @@ def kernel_stats():
     k = lib().ccgpu_kernel_stats(names, 4096, _dp(avg), cnt.ctypes.data_as(C.POINTER(C.c_longlong)), 64)
     nm = names.value.decode().split(";") if k else []
     return {nm[x]: (float(avg[x]), int(cnt[x])) for x in range(k)}
+
+
+def autotune_report():
+    buf = C.create_string_buffer(65536)
+    lib().ccgpu_autotune_report(buf, 65536)
+    return buf.value.decode()
 
 
 MLP_NPARAM = 9473

@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <chrono>
 #include <cmath>
 #include <complex>
@@ -43,8 +44,10 @@ void launch_update(const SolveDesc *dd, int nsolve, int64_t maxT, const int *slo
 void launch_gram(const SolveDesc *dd, int nsolve, const int *slots, int nstep, hipStream_t st);
 void launch_diis_axpy(const SolveDesc *dd, int nsolve, int64_t maxT, const int *slots, const double2 *coef,
                       int nvec, hipStream_t st);
-void launch_gemm(const GemmGroup *groups, const GemmTile *tiles, int ntiles, const SolveDesc *dd,
-                 const int *gsolve, int variant, hipStream_t st);
+void launch_gemm_shape(const GemmGroup *groups, const GemmTile *tiles, int ntiles, const SolveDesc *dd,
+                       const int *gsolve, int kind, int mb, int nb, hipStream_t st);
+void launch_gemm_valu(const GemmGroup *groups, const GemmTile *tiles, int ntiles, const SolveDesc *dd,
+                      const int *gsolve, hipStream_t st);
 void launch_splitk_reduce(const GemmGroup *groups, const int *gidx, int n, int64_t maxmn, const SolveDesc *dd,
                           const int *gsolve, hipStream_t st);
 struct TriplesDesc {
@@ -114,110 +117,9 @@ T *dupload(const std::vector<T> &v, std::vector<void *> &owned) {
   return p;
 }
 
-// Work list of one GEMM family.  Tiles are classed by the column count of their
-// group: N <= 16 -> 128x16 tiles, N <= 32 -> 128x32, N <= 48 -> 64x48,
-// N <= 64 -> 64x64, N <= 80 -> 64x80, wider -> several 64x64 tiles, so narrow
-// amplitude blocks do not pay for padded MFMA columns.  tiles64 is a uniform
-// 64x64 tiling used by the VALU cross-check kernel.
-struct GemmList {
-  std::vector<GemmGroup> groups;
-  static constexpr int NCLS = 5;
-  std::vector<GemmTile> tiles[NCLS];
-  std::vector<GemmTile> tiles64;
-  std::vector<int> gsolve;
-  std::vector<int> split;        // groups reduced by k_splitk_reduce
-  std::vector<int64_t> poff;     // partial-buffer offset per group
-  int64_t psize = 0, maxmn = 0;
-  GemmGroup *d_groups = nullptr;
-  GemmTile *d_tiles[NCLS] = {nullptr, nullptr, nullptr, nullptr, nullptr};
-  GemmTile *d_tiles64 = nullptr;
-  int *d_gsolve = nullptr, *d_split = nullptr;
-  double flops = 0;       // algorithmic real flops per launch
-  double bytes = 0;       // algorithmic bytes of the A operand per launch
-  static int cls_of(int N) { return N <= 16 ? 0 : N <= 32 ? 1 : N <= 48 ? 2 : N <= 64 ? 3 : N <= 80 ? 4 : 3; }
-  static int rows_of(int c) { return c <= 1 ? 128 : 64; }
-  static int cols_of(int c) { static const int w[NCLS] = {16, 32, 48, 64, 80}; return w[c]; }
-  bool finalized = false;
-  void add(GemmGroup g, int solve) {
-    if (g.M <= 0 || g.N <= 0 || g.K <= 0) return;
-    g.nsplit = 1;
-    g.P = nullptr;
-    groups.push_back(g);
-    gsolve.push_back(solve);
-    flops += 8.0 * g.M * (double)g.N * g.K;
-  }
-  // Tiles per class.  A class whose tiles cannot fill the GPU (fewer than two
-  // per CU) splits K over several workgroups per tile; the partial products are
-  // summed in fixed order by k_splitk_reduce.
-  void finalize() {
-    if (finalized) return;
-    finalized = true;
-    const int target = 512;
-    std::vector<int64_t> cls_tiles(NCLS, 0);
-    std::vector<int> cls(groups.size());
-    for (size_t gi = 0; gi < groups.size(); ++gi) {
-      const GemmGroup &g = groups[gi];
-      cls[gi] = cls_of(g.N);
-      int bm = rows_of(cls[gi]), bn = cols_of(cls[gi]);
-      cls_tiles[cls[gi]] += (int64_t)((g.M + bm - 1) / bm) * ((g.N + bn - 1) / bn);
-    }
-    for (size_t gi = 0; gi < groups.size(); ++gi) {
-      GemmGroup &g = groups[gi];
-      const int c = cls[gi];
-      int split = 1;
-      if (cls_tiles[c] > 0 && cls_tiles[c] < target && g.K >= 256)
-        split = (int)std::min<int64_t>({(target + cls_tiles[c] - 1) / cls_tiles[c], (int64_t)(g.K / 128), 32});
-      int ksz = (g.K + split - 1) / split;
-      ksz = (ksz + 15) / 16 * 16;
-      g.nsplit = (g.K + ksz - 1) / ksz;
-      poff.push_back(g.nsplit > 1 ? psize : -1);
-      if (g.nsplit > 1) {
-        psize += (int64_t)g.nsplit * g.M * g.N;
-        split_list_push((int)gi);
-        maxmn = std::max<int64_t>(maxmn, (int64_t)g.M * g.N);
-      }
-      int bm = rows_of(c), bn = cols_of(c);
-      for (int ks = 0; ks < g.nsplit; ++ks)
-        for (int m = 0; m < g.M; m += bm)
-          for (int n = 0; n < g.N; n += bn)
-            tiles[c].push_back(GemmTile{(int)gi, m, n, ks, ks * ksz, std::min(g.K, (ks + 1) * ksz), 0, 0});
-      for (int m = 0; m < g.M; m += 64)
-        for (int n = 0; n < g.N; n += 64) tiles64.push_back(GemmTile{(int)gi, m, n, 0, 0, g.K, 0, 0});
-    }
-  }
-  void split_list_push(int gi) { split.push_back(gi); }
-  int ntiles() const {
-    size_t n = 0;
-    for (int c = 0; c < NCLS; ++c) n += tiles[c].size();
-    return (int)n;
-  }
-  // device partial buffer for the split groups, then upload
-  void upload(std::vector<void *> &owned, bool own_partials = true) {
-    if (own_partials && psize > 0) {
-      double2 *P = dalloc<double2>(psize, owned);
-      for (size_t g = 0; g < groups.size(); ++g)
-        if (poff[g] >= 0) groups[g].P = P + poff[g];
-    }
-    d_groups = dupload(groups, owned);
-    for (int c = 0; c < NCLS; ++c) d_tiles[c] = dupload(tiles[c], owned);
-    d_tiles64 = dupload(tiles64, owned);
-    d_gsolve = dupload(gsolve, owned);
-    d_split = dupload(split, owned);
-  }
-};
-
-GemmGroup mkgroup(const void *A, const double2 *B, double2 *C, const double2 *C0, int M, int N, int K, int lda,
-                  int ldb, int ldc, int ldc0, double alpha, double beta) {
-  GemmGroup g{};
-  g.A = A; g.B = B; g.C = C; g.C0 = C0; g.P = nullptr;
-  g.M = M; g.N = N; g.K = K; g.lda = lda; g.ldb = ldb; g.ldc = ldc; g.ldc0 = ldc0;
-  g.nsplit = 1; g.alpha = alpha; g.beta = beta;
-  return g;
-}
-
-// XCD-aware reorder: tiles of one group stay on one of the 8 XCDs (blocks are
-// dealt round-robin to XCDs), so the shared operand of a channel is read from
-// one L2.  Groups are packed onto XCD bins by largest-first.
+// Tiles reordered so that each channel's tiles go to one XCD (workgroup i runs
+// on XCD i mod 8, which keeps the channel's B operand in that XCD's L2) and the
+// XCDs get equal tile counts.
 void xcd_order_list(std::vector<GemmTile> &tiles, size_t ngroups) {
   const int nx = 8;
   std::vector<std::vector<GemmTile>> bins(nx);
@@ -242,9 +144,154 @@ void xcd_order_list(std::vector<GemmTile> &tiles, size_t ngroups) {
   tiles.swap(out);
 }
 
-void xcd_order(GemmList &L) {
-  for (int c = 0; c < GemmList::NCLS; ++c) xcd_order_list(L.tiles[c], L.groups.size());
+// Work list of one GEMM family (one group per channel and solve).  Groups are
+// classed by their column count N (<= 16, 32, 48, 64, 80; wider -> class 3),
+// and every class is one launch with its own tile shape: MB 16-row blocks per
+// wave (4 waves stacked along M, so 64*MB rows) by NB 16-column blocks; a group
+// wider than 16*NB gets several column tiles.  Narrow amplitude blocks thus do
+// not pay for padded MFMA columns.  The shapes come from a default table, the
+// autotuner, or CCGPU_SHAPES for studies.  tiles64 is a uniform 64x64 tiling
+// for the VALU cross-check kernel.
+struct GemmShape {
+  int mb, nb;
+};
+bool gemm_shape_ok(int mb, int nb) {
+  return (mb == 0 && nb == 4) || (mb == 1 && nb >= 1 && nb <= 5) || (mb == 2 && (nb == 1 || nb == 2)) ||
+         (mb == 4 && nb == 1);
 }
+// rows and columns of one tile of a shape; (0,4) is the thin VALU kernel
+int tile_rows(GemmShape s) { return s.mb == 0 ? 16 : 64 * s.mb; }
+int tile_cols(GemmShape s) { return s.mb == 0 ? 4 : 16 * s.nb; }
+
+struct GemmList {
+  static constexpr int NCLS = 6;
+  int fam = 0;   // 0 X = V_hhpp T, 1 pp ladder, 2 hh ladder, 3 ring I4, 4 ring R~
+  std::vector<GemmGroup> groups;
+  std::vector<int> gsolve;
+  GemmShape shape[NCLS] = {{2, 1}, {2, 2}, {1, 3}, {1, 4}, {1, 5}, {0, 4}};
+  std::vector<GemmTile> tiles[NCLS];
+  std::vector<GemmTile> tiles64;
+  std::vector<int> split;        // groups reduced by k_splitk_reduce
+  std::vector<int64_t> poff;     // partial-buffer offset per group
+  int64_t psize = 0, maxmn = 0;
+  bool xcd = false;              // order tiles channel-per-XCD
+  GemmGroup *d_groups = nullptr;
+  GemmTile *d_tiles[NCLS] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+  GemmTile *d_tiles64 = nullptr;
+  int *d_gsolve = nullptr, *d_split = nullptr;
+  double2 *d_part = nullptr;
+  double flops = 0;       // algorithmic real flops per launch
+  double bytes = 0;       // algorithmic bytes of the A operand per launch
+  // class 5: N <= 4; 0: N <= 16; 1: <= 32; 2: <= 48; 3: <= 64 or > 80; 4: <= 80
+  static int cls_of(int N) {
+    return N <= 4 ? 5 : N <= 16 ? 0 : N <= 32 ? 1 : N <= 48 ? 2 : N <= 64 ? 3 : N <= 80 ? 4 : 3;
+  }
+  void add(GemmGroup g, int solve) {
+    if (g.M <= 0 || g.N <= 0 || g.K <= 0) return;
+    g.nsplit = 1;
+    g.P = nullptr;
+    groups.push_back(g);
+    gsolve.push_back(solve);
+    flops += 8.0 * g.M * (double)g.N * g.K;
+  }
+  int ntiles() const {
+    size_t n = 0;
+    for (int c = 0; c < NCLS; ++c) n += tiles[c].size();
+    return (int)n;
+  }
+  bool has_class(int c) const {
+    for (const GemmGroup &g : groups)
+      if (cls_of(g.N) == c) return true;
+    return false;
+  }
+  void apply_env_shapes() {
+    // CCGPU_SHAPES="fam.class=mb.nb,..." (studies)
+    const char *e = getenv("CCGPU_SHAPES");
+    if (!e) return;
+    std::string spec(e);
+    size_t pos = 0;
+    while (pos < spec.size()) {
+      size_t end = spec.find(',', pos);
+      if (end == std::string::npos) end = spec.size();
+      int f, c, mb, nb;
+      if (sscanf(spec.substr(pos, end - pos).c_str(), "%d.%d=%d.%d", &f, &c, &mb, &nb) == 4 && f == fam && c >= 0 &&
+          c < NCLS - 1 && mb > 0 && gemm_shape_ok(mb, nb))
+        shape[c] = GemmShape{mb, nb};
+      pos = end + 1;
+    }
+  }
+  // Tiles of every class for the current shapes.  A group with few output
+  // tiles and a long k range splits k over several workgroups; the partial
+  // products are summed in fixed order by k_splitk_reduce.  The split depends
+  // on the group alone (not on the tile shape or on the other groups of a
+  // batch), and the MFMA kernels sum k in the same order for every shape, so
+  // results do not depend on the autotuner's choice or on batching.
+  static int split_of(const GemmGroup &g) {
+    const int64_t ref_tiles = (int64_t)((g.M + 63) / 64) * ((g.N + 15) / 16);
+    if (g.K < 512 || ref_tiles >= 32) return 1;
+    return (int)std::min<int64_t>({(32 + ref_tiles - 1) / ref_tiles, (int64_t)(g.K / 256), 8});
+  }
+  void build_tiles() {
+    for (int c = 0; c < NCLS; ++c) tiles[c].clear();
+    tiles64.clear();
+    split.clear();
+    poff.clear();
+    psize = 0;
+    maxmn = 0;
+    for (size_t gi = 0; gi < groups.size(); ++gi) {
+      GemmGroup &g = groups[gi];
+      const int c = cls_of(g.N);
+      const int nsp = split_of(g);
+      int ksz = (g.K + nsp - 1) / nsp;
+      ksz = (ksz + 15) / 16 * 16;
+      g.nsplit = (g.K + ksz - 1) / ksz;
+      g.P = nullptr;
+      poff.push_back(g.nsplit > 1 ? psize : -1);
+      if (g.nsplit > 1) {
+        psize += (int64_t)g.nsplit * g.M * g.N;
+        split.push_back((int)gi);
+        maxmn = std::max<int64_t>(maxmn, (int64_t)g.M * g.N);
+      }
+      const int bm = tile_rows(shape[c]), bn = tile_cols(shape[c]);
+      for (int ks = 0; ks < g.nsplit; ++ks)
+        for (int m = 0; m < g.M; m += bm)
+          for (int n = 0; n < g.N; n += bn)
+            tiles[c].push_back(GemmTile{(int)gi, m, n, ks, ks * ksz, std::min(g.K, (ks + 1) * ksz), 0, 0});
+      for (int m = 0; m < g.M; m += 64)
+        for (int n = 0; n < g.N; n += 64) tiles64.push_back(GemmTile{(int)gi, m, n, 0, 0, g.K, 0, 0});
+    }
+    if (xcd)
+      for (int c = 0; c < NCLS; ++c) xcd_order_list(tiles[c], groups.size());
+  }
+  // partial buffer for the split groups, then upload everything
+  void upload(std::vector<void *> &owned) {
+    d_part = psize > 0 ? dalloc<double2>(psize, owned) : nullptr;
+    for (size_t g = 0; g < groups.size(); ++g) groups[g].P = poff[g] >= 0 ? d_part + poff[g] : nullptr;
+    d_groups = dupload(groups, owned);
+    for (int c = 0; c < NCLS; ++c) d_tiles[c] = dupload(tiles[c], owned);
+    d_tiles64 = dupload(tiles64, owned);
+    d_gsolve = dupload(gsolve, owned);
+    d_split = dupload(split, owned);
+  }
+  void prepare(std::vector<void *> &owned) {
+    apply_env_shapes();
+    build_tiles();
+    upload(owned);
+  }
+};
+
+GemmGroup mkgroup(const void *A, const double2 *B, double2 *C, const double2 *C0, int M, int N, int K, int lda,
+                  int ldb, int ldc, int ldc0, double alpha, double beta) {
+  GemmGroup g{};
+  g.A = A; g.B = B; g.C = C; g.C0 = C0; g.P = nullptr;
+  g.M = M; g.N = N; g.K = K; g.lda = lda; g.ldb = ldb; g.ldc = ldc; g.ldc0 = ldc0;
+  g.nsplit = 1; g.alpha = alpha; g.beta = beta;
+  return g;
+}
+
+// XCD-aware reorder: tiles of one group stay on one of the 8 XCDs (blocks are
+// dealt round-robin to XCDs), so the shared operand of a channel is read from
+// one L2.  Groups are packed onto XCD bins by largest-first.
 
 struct Solve {
   Plan plan;
@@ -418,9 +465,7 @@ void build_device(Solve &S) {
     // Rt = Tt I4 (ccm_t2_eqn.f90:995-1026)
     S.g_rt.add(mkgroup(d.Tt + P.off_rt[K], d.I4 + P.off_i4[K], d.Rt + P.off_rt[K], nullptr, nph, nhp, nhp, nhp, nhp, nhp, 0, 1.0, 0.0), 0);
   }
-  for (GemmList *L : {&S.g_x, &S.g_pp, &S.g_hh, &S.g_i4, &S.g_rt}) L->finalize();
-  xcd_order(S.g_pp);
-  for (GemmList *L : {&S.g_x, &S.g_pp, &S.g_hh, &S.g_i4, &S.g_rt}) L->upload(o);
+  // tiles are made per batch (make_batch), for the batch's own class sizes
   S.g_pp.bytes = 16.0 * (double)P.nVpp;
 }
 
@@ -486,6 +531,10 @@ struct Batch {
   std::vector<double2> h_coef, h_scal;
   std::vector<void *> owned;
   hipStream_t stream = nullptr;
+  // second stream (default priority) for the ring branch of the residual, and
+  // the fork/join events
+  hipStream_t stream2 = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   hipGraph_t graph[3] = {nullptr, nullptr, nullptr};        // per precision mode
   hipGraphExec_t gexec[3] = {nullptr, nullptr, nullptr};
   ~Batch() {
@@ -493,31 +542,109 @@ struct Batch {
       if (gexec[q]) (void)hipGraphExecDestroy(gexec[q]);
       if (graph[q]) (void)hipGraphDestroy(graph[q]);
     }
+    if (ev_fork) (void)hipEventDestroy(ev_fork);
+    if (ev_join) (void)hipEventDestroy(ev_join);
+    if (stream2) (void)hipStreamDestroy(stream2);
     for (void *p : owned) (void)hipFree(p);
   }
 };
 
 void merge(GemmList &dst, const GemmList &src, int solve) {
-  int base = (int)dst.groups.size();
   for (size_t i = 0; i < src.groups.size(); ++i) {
     dst.groups.push_back(src.groups[i]);
     dst.gsolve.push_back(solve);
   }
-  for (int c = 0; c < GemmList::NCLS; ++c)
-    for (auto t : src.tiles[c]) {
-      t.g += base;
-      dst.tiles[c].push_back(t);
-    }
-  for (auto t : src.tiles64) {
-    t.g += base;
-    dst.tiles64.push_back(t);
-  }
-  for (int g : src.split) dst.split.push_back(g + base);
-  for (size_t g = 0; g < src.groups.size(); ++g) dst.poff.push_back(-1);  // partial buffers stay the solve's
-  dst.maxmn = std::max(dst.maxmn, src.maxmn);
   dst.flops += src.flops;
   dst.bytes += src.bytes;
-  dst.finalized = true;
+}
+
+// ---------------------------------------------------------------------------
+// Tile-shape autotuner.  For every work list and tile class the candidate
+// shapes (MB, NB) are timed with HIP events on the list's own groups (the
+// launches overwrite only outputs that every residual recomputes), and the
+// fastest is kept.  Results are cached by a signature of the list (family,
+// kernel kind, group dimensions), so repeated batches of the same lattices
+// reuse them.  CCGPU_AUTOTUNE=0 keeps the default table.
+// ---------------------------------------------------------------------------
+std::map<std::string, std::array<GemmShape, GemmList::NCLS>> g_tuned;
+std::string g_tune_report;
+
+std::string list_signature(const GemmList &L, int kind) {
+  uint64_t h = 1469598103934665603ull;
+  auto mix = [&](uint64_t v) { h = (h ^ v) * 1099511628211ull; };
+  for (const GemmGroup &g : L.groups) {
+    mix((uint64_t)g.M);
+    mix((uint64_t)g.N);
+    mix((uint64_t)g.K);
+  }
+  return fmt("%d.%d.%zu.%016llx", L.fam, kind, L.groups.size(), (unsigned long long)h);
+}
+
+std::vector<GemmShape> shape_candidates(int c) {
+  std::vector<GemmShape> v;
+  if (c == 5) return {{0, 4}};   // N <= 4: always the thin kernel (fixed summation order)
+  for (int nb = 1; nb <= std::min(5, c + 1); ++nb) v.push_back({1, nb});
+  v.push_back({2, 1});
+  if (c >= 1) v.push_back({2, 2});
+  v.push_back({4, 1});
+  return v;
+}
+
+bool autotune_enabled() {
+  const char *e = getenv("CCGPU_AUTOTUNE");
+  return !(e && atoi(e) == 0) && !getenv("CCGPU_SHAPES");
+}
+
+void autotune(GemmList &L, int kind, const SolveDesc *d_descs, hipStream_t st) {
+  if (!autotune_enabled() || L.groups.empty()) return;
+  const std::string sig = list_signature(L, kind);
+  auto it = g_tuned.find(sig);
+  if (it != g_tuned.end()) {
+    for (int c = 0; c < GemmList::NCLS; ++c) L.shape[c] = it->second[c];
+    return;
+  }
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  std::string rep = fmt("fam%d/kind%d:", L.fam, kind);
+  for (int c = 0; c < GemmList::NCLS; ++c) {
+    if (!L.has_class(c) || shape_candidates(c).size() < 2) continue;
+    float best = 1e30f;
+    GemmShape best_shape = L.shape[c];
+    for (const GemmShape cand : shape_candidates(c)) {
+      GemmList T = L;   // copy of the groups; tiles for this candidate
+      T.shape[c] = cand;
+      std::vector<void *> tmp;
+      T.build_tiles();
+      T.upload(tmp);
+      float ms = 0.f;
+      for (int rep_ = 0; rep_ < 4; ++rep_) {
+        HIPCHK(hipEventRecord(e0, st));
+        launch_gemm_shape(T.d_groups, T.d_tiles[c], (int)T.tiles[c].size(), d_descs, T.d_gsolve, kind, cand.mb,
+                          cand.nb, st);
+        HIPCHK(hipEventRecord(e1, st));
+        HIPCHK(hipEventSynchronize(e1));
+        float t = 0.f;
+        HIPCHK(hipEventElapsedTime(&t, e0, e1));
+        if (rep_ > 0) ms += t;   // the first launch warms caches and code
+      }
+      HIPCHK(hipGetLastError());
+      for (void *p : tmp) (void)hipFree(p);
+      if (ms < best) {
+        best = ms;
+        best_shape = cand;
+      }
+    }
+    L.shape[c] = best_shape;
+    rep += fmt(" c%d=%dx%d(%.1fus)", c, best_shape.mb, best_shape.nb, 1e3f * best / 3.f);
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  std::array<GemmShape, GemmList::NCLS> arr;
+  for (int c = 0; c < GemmList::NCLS; ++c) arr[c] = L.shape[c];
+  g_tuned[sig] = arr;
+  if (!g_tune_report.empty()) g_tune_report += "; ";
+  g_tune_report += rep;
 }
 
 std::unique_ptr<Batch> make_batch(const std::vector<Solve *> &ss, hipStream_t st) {
@@ -553,12 +680,21 @@ std::unique_ptr<Batch> make_batch(const std::vector<Solve *> &ss, hipStream_t st
     if (S.desc.Vpppp32) B->g_pp32.groups[g].A = S.desc.Vpppp32 + (a - base);
     else B->has_f32 = false;
   }
-  if (n > 1) {
-    xcd_order(B->g_pp);
-    xcd_order(B->g_pp32);
-  }
-  for (GemmList *L : {&B->g_x, &B->g_pp, &B->g_hh, &B->g_i4, &B->g_rt, &B->g_pp32}) L->upload(B->owned, false);
+  const int fams[6] = {0, 1, 2, 3, 4, 1};
+  const int kinds[6] = {0, 3, 0, 0, 0, 4};   // the kernel kind each list is tuned for
+  GemmList *lists[6] = {&B->g_x, &B->g_pp, &B->g_hh, &B->g_i4, &B->g_rt, &B->g_pp32};
   B->d_descs = dupload(B->h_descs, B->owned);
+  for (int k = 0; k < 6; ++k) {
+    lists[k]->fam = fams[k];
+    lists[k]->xcd = fams[k] == 1;
+    lists[k]->apply_env_shapes();
+    if (k < 5 || B->has_f32) autotune(*lists[k], kinds[k], B->d_descs, st);
+    lists[k]->build_tiles();
+    lists[k]->upload(B->owned);
+  }
+  HIPCHK(hipStreamCreateWithFlags(&B->stream2, hipStreamNonBlocking));
+  HIPCHK(hipEventCreateWithFlags(&B->ev_fork, hipEventDisableTiming));
+  HIPCHK(hipEventCreateWithFlags(&B->ev_join, hipEventDisableTiming));
   B->d_slot = dalloc<int>(n, B->owned);
   B->d_slots = dalloc<int>((size_t)n * 16, B->owned);
   B->d_coef = dalloc<double2>((size_t)n * 16, B->owned);
@@ -581,16 +717,27 @@ int term_mask() {
 // One GEMM family: three launches (one per tile class), or the VALU check.
 // prec here is the arithmetic of this family: FP64, FP32, or V32 (pp ladder
 // only: FP32-stored A, FP64 arithmetic)
+int gemm_kind(int prec, bool a_f32, bool pp) {
+  if (pp) return prec == CCGPU_PREC_FP64 ? 3 : prec == CCGPU_PREC_V32 ? 5 : 4;
+  return prec == CCGPU_PREC_FP64 ? 0 : (a_f32 ? 2 : 1);
+}
+
 void run_gemm(const GemmList &L, const Batch &B, int prec, bool a_f32, hipStream_t st, bool pp = false) {
   if (gemm_variant_override() == 3) {
-    launch_gemm(L.d_groups, L.d_tiles64, (int)L.tiles64.size(), B.d_descs, L.d_gsolve, 3, st);
-    return;
+    launch_gemm_valu(L.d_groups, L.d_tiles64, (int)L.tiles64.size(), B.d_descs, L.d_gsolve, st);
+  } else {
+    const int kind = gemm_kind(prec, a_f32, pp);
+    // CCGPU_TIMING=2 also times every tile class of every family
+    static const bool detail = getenv("CCGPU_TIMING") && atoi(getenv("CCGPU_TIMING")) == 2;
+    for (int c = GemmList::NCLS - 1; c >= 0; --c) {
+      if (L.tiles[c].empty()) continue;
+      char nm[48];
+      snprintf(nm, sizeof nm, "fam%d.k%d.c%d.%dx%d", L.fam, kind, c, L.shape[c].mb, L.shape[c].nb);
+      Timer t(detail, st, nm);
+      launch_gemm_shape(L.d_groups, L.d_tiles[c], (int)L.tiles[c].size(), B.d_descs, L.d_gsolve, kind, L.shape[c].mb,
+                        L.shape[c].nb, st);
+    }
   }
-  int base = prec == CCGPU_PREC_FP64 ? 10 : (a_f32 ? 30 : 20);
-  if (pp) base = prec == CCGPU_PREC_FP64 ? 40 : prec == CCGPU_PREC_V32 ? 70 : 60;
-  const int nb[GemmList::NCLS] = {1, 2, 3, 4, 5};
-  for (int c = GemmList::NCLS - 1; c >= 0; --c)
-    launch_gemm(L.d_groups, L.d_tiles[c], (int)L.tiles[c].size(), B.d_descs, L.d_gsolve, base + nb[c], st);
   launch_splitk_reduce(L.d_groups, L.d_split, (int)L.split.size(), L.maxmn, B.d_descs, L.d_gsolve, st);
 }
 
@@ -600,6 +747,11 @@ void run_gemm(const GemmList &L, const Batch &B, int prec, bool a_f32, hipStream
 int fp32_families() {
   const char *e = getenv("CCGPU_FP32_FAMILIES");
   return e ? atoi(e) : 15;
+}
+
+bool fork_enabled() {
+  const char *e = getenv("CCGPU_FORK");
+  return !(e && atoi(e) == 0);
 }
 
 void residual_launches(Batch &B, int prec, bool timing) {
@@ -624,6 +776,31 @@ void residual_launches(Batch &B, int prec, bool timing) {
     Timer t(timing, st, "resid_init_onebody");
     launch_resid_init(B.d_descs, n, B.maxT, (terms & 1) != 0, st);
   }
+  // The particle-particle and hole-hole ladders update R in place; the ring
+  // chain (T -> T~, I4, R~) touches R only in the final scatter.  Outside the
+  // timing mode the ring chain runs on a second stream beside the ladders
+  // (fork/join through events, also inside a captured graph): the ladder
+  // streams V_pppp from HBM while the ring GEMMs keep the matrix cores busy.
+  const bool fork = !timing && (terms & 8) && fork_enabled();
+  hipStream_t sr = fork ? B.stream2 : st;
+  if (fork) {
+    HIPCHK(hipEventRecord(B.ev_fork, st));
+    HIPCHK(hipStreamWaitEvent(sr, B.ev_fork, 0));
+  }
+  if (terms & 8) {
+    {
+      Timer t(timing, sr, "ring_gather");
+      launch_ring_gather(B.d_descs, n, B.maxT, sr);
+    }
+    {
+      Timer t(timing, sr, "gemm_ring_i4");
+      run_gemm(B.g_i4, B, pf(3), false, sr);
+    }
+    {
+      Timer t(timing, sr, "gemm_ring_rt");
+      run_gemm(B.g_rt, B, pf(3), false, sr);
+    }
+  }
   if (terms & 2) {
     Timer t(timing, st, "gemm_pp_ladder");
     if (pp_prec != CCGPU_PREC_FP64) run_gemm(B.g_pp32, B, pp_prec, true, st, true);
@@ -633,20 +810,11 @@ void residual_launches(Batch &B, int prec, bool timing) {
     Timer t(timing, st, "gemm_hh_ladder");
     run_gemm(B.g_hh, B, pf(2), false, st);
   }
-  if (!(terms & 8)) return;
-  {
-    Timer t(timing, st, "ring_gather");
-    launch_ring_gather(B.d_descs, n, B.maxT, st);
+  if (fork) {
+    HIPCHK(hipEventRecord(B.ev_join, sr));
+    HIPCHK(hipStreamWaitEvent(st, B.ev_join, 0));
   }
-  {
-    Timer t(timing, st, "gemm_ring_i4");
-    run_gemm(B.g_i4, B, pf(3), false, st);
-  }
-  {
-    Timer t(timing, st, "gemm_ring_rt");
-    run_gemm(B.g_rt, B, pf(3), false, st);
-  }
-  {
+  if (terms & 8) {
     Timer t(timing, st, "ring_scatter");
     launch_ring_scatter(B.d_descs, n, B.maxT, st);
   }
@@ -1731,6 +1899,15 @@ int ccgpu_mlp_warm_start(int mlp, int handle, double *ecorr) {
     S.count = 0;
     if (ecorr) { ecorr[0] = e[0].real(); ecorr[1] = e[0].imag(); }
   });
+}
+
+int ccgpu_autotune_report(char *buf, int len) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  if (buf && len > 0) {
+    strncpy(buf, g_tune_report.c_str(), len - 1);
+    buf[len - 1] = 0;
+  }
+  return (int)g_tune_report.size();
 }
 
 int ccgpu_kernel_stats(char *names, int len, double *avg_ms, long long *launches, int max) {

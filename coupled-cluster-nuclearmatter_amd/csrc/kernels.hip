@@ -485,8 +485,9 @@ __global__ void k_diis_axpy(const SolveDesc *descs, const int *slots, const doub
 
 // ---------------------------------------------------------------------------
 // Grouped complex GEMMs: C = alpha A B + beta C0, every group a different
-// channel (and solve).  A complex multiply-accumulate is four real MFMAs:
-// re += ar br, re += (-ai) bi, im += ar bi, im += ai br.
+// channel (and solve).  A complex multiply-accumulate is four real MFMAs in
+// FP32 (re += ar br, re += (-ai) bi, im += ar bi, im += ai br) and three in
+// FP64 (Gauss: ar br, ai bi, (ar+ai)(br+bi)).
 // ---------------------------------------------------------------------------
 // Loads through global-address-space pointers: the compiler then emits
 // global_load (not flat_load) and can count outstanding loads precisely.
@@ -576,13 +577,17 @@ __global__ __launch_bounds__(256) void k_zgemm_dl(const GemmGroup *__restrict__ 
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int li = lane & 15, lg = lane >> 4;
   const int mw = tl.m0 + w * (MB * 16);
-  acc4 cre[MB][NB], cim[MB][NB];
+  // FP32: cre, cim with four real MFMAs per complex MAC.  FP64: three products
+  // (Gauss): cre = sum ar br, cim = sum ai bi, cs = sum (ar+ai)(br+bi); then
+  // re = cre - cim and im = cs - cre - cim, one MFMA in four saved.
+  acc4 cre[MB][NB], cim[MB][NB], cs[MB][NB];
 #pragma unroll
   for (int x = 0; x < MB; ++x)
 #pragma unroll
     for (int y = 0; y < NB; ++y) {
       cre[x][y] = (acc4){0, 0, 0, 0};
       cim[x][y] = (acc4){0, 0, 0, 0};
+      cs[x][y] = (acc4){0, 0, 0, 0};
     }
   double2 a0[MB][4], b0[NB][4], a1[MB][4], b1[NB][4];
   const int kb = tl.kb, ke = min(tl.ke, g.K);
@@ -630,9 +635,8 @@ __global__ __launch_bounds__(256) void k_zgemm_dl(const GemmGroup *__restrict__ 
             cim[x][y] = __builtin_amdgcn_mfma_f32_16x16x4f32(ai, br, cim[x][y], 0, 0, 0);
           } else {
             cre[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar, br, cre[x][y], 0, 0, 0);
-            cre[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(-ai, bi, cre[x][y], 0, 0, 0);
-            cim[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar, bi, cim[x][y], 0, 0, 0);
-            cim[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai, br, cim[x][y], 0, 0, 0);
+            cim[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai, bi, cim[x][y], 0, 0, 0);
+            cs[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar + ai, br + bi, cs[x][y], 0, 0, 0);
           }
         }
       }
@@ -659,10 +663,18 @@ __global__ __launch_bounds__(256) void k_zgemm_dl(const GemmGroup *__restrict__ 
         const int row = F32 ? (lg * 4 + r) : (lg + 4 * r);
         const int m = mw + x * 16 + row;
         const int n = tl.n0 + y * 16 + li;
+        double vr, vi;
+        if constexpr (F32) {
+          vr = (double)cre[x][y][r];
+          vi = (double)cim[x][y][r];
+        } else {
+          vr = cre[x][y][r] - cim[x][y][r];
+          vi = cs[x][y][r] - cre[x][y][r] - cim[x][y][r];
+        }
         if (m < g.M && n < g.N && g.nsplit > 1) {
-          g.P[((int64_t)tl.ks * g.M + m) * g.N + n] = make_double2((double)cre[x][y][r], (double)cim[x][y][r]);
+          g.P[((int64_t)tl.ks * g.M + m) * g.N + n] = make_double2(vr, vi);
         } else if (m < g.M && n < g.N) {
-          double2 v = make_double2(g.alpha * (double)cre[x][y][r], g.alpha * (double)cim[x][y][r]);
+          double2 v = make_double2(g.alpha * vr, g.alpha * vi);
           if (g.beta != 0.0) {
             double2 c0 = g.C0[(int64_t)m * g.ldc0 + n];
             v.x += g.beta * c0.x;
@@ -671,6 +683,89 @@ __global__ __launch_bounds__(256) void k_zgemm_dl(const GemmGroup *__restrict__ 
           g.C[(int64_t)m * g.ldc + n] = v;
         }
       }
+}
+
+// Thin GEMM for groups with N <= 4 columns (most pp-ladder channels with few
+// hole pairs): FP64 VALU instead of 16-column MFMA tiles, which would be 75% or
+// more padding.  A tile is 16 rows: each wave owns 4 rows, its 64 lanes walk
+// the k range 64 consecutive elements at a time (one coalesced 1 KB read of a
+// row per instruction), each lane reuses its B row (4 columns) for the 4 rows,
+// and the 64 partial sums of every output are combined with a fixed xor tree.
+template <typename AT, int FAM>
+__global__ __launch_bounds__(256) void k_zgemm_thin(const GemmGroup *__restrict__ groups,
+                                                    const GemmTile *__restrict__ tiles,
+                                                    const SolveDesc *__restrict__ descs,
+                                                    const int *__restrict__ gsolve) {
+  const GemmTile tl = tiles[blockIdx.x];
+  if (descs && !descs[gsolve[tl.g]].active) return;
+  const GemmGroup g = groups[tl.g];
+  const AT *A = (const AT *)g.A;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int m0 = tl.m0 + 4 * w;
+  const int kb = tl.kb, ke = min(tl.ke, g.K);
+  double2 acc[4][4];
+#pragma unroll
+  for (int r = 0; r < 4; ++r)
+#pragma unroll
+    for (int n = 0; n < 4; ++n) acc[r][n] = make_double2(0.0, 0.0);
+  for (int k0 = kb; k0 < ke; k0 += 64) {
+    const int k = k0 + lane;
+    const bool okk = k < ke;
+    const int kc = min(k, ke - 1);
+    double2 b[4], a[4];
+#pragma unroll
+    for (int n = 0; n < 4; ++n) {
+      const double2 v = gload(g.B + (int64_t)kc * g.ldb + min(n, g.N - 1));
+      const bool ok = okk && n < g.N;
+      b[n] = make_double2(ok ? v.x : 0.0, ok ? v.y : 0.0);
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int m = m0 + r;
+      const double2 v = load_a<AT>(A + (int64_t)min(m, g.M - 1) * g.lda + kc);
+      const bool ok = okk && m < g.M;
+      a[r] = make_double2(ok ? v.x : 0.0, ok ? v.y : 0.0);
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+#pragma unroll
+      for (int n = 0; n < 4; ++n) {
+        acc[r][n].x = fma(a[r].x, b[n].x, fma(-a[r].y, b[n].y, acc[r][n].x));
+        acc[r][n].y = fma(a[r].x, b[n].y, fma(a[r].y, b[n].x, acc[r][n].y));
+      }
+  }
+#pragma unroll
+  for (int r = 0; r < 4; ++r)
+#pragma unroll
+    for (int n = 0; n < 4; ++n)
+#pragma unroll
+      for (int s = 32; s > 0; s >>= 1) {
+        acc[r][n].x += __shfl_xor(acc[r][n].x, s);
+        acc[r][n].y += __shfl_xor(acc[r][n].y, s);
+      }
+  if (lane < 16) {
+    const int r = lane >> 2, n = lane & 3;
+    const int m = m0 + r;
+    double2 v = make_double2(0.0, 0.0);
+#pragma unroll
+    for (int rr = 0; rr < 4; ++rr)
+#pragma unroll
+      for (int nn = 0; nn < 4; ++nn)
+        if (rr == r && nn == n) v = acc[rr][nn];
+    if (m < g.M && n < g.N) {
+      if (g.nsplit > 1) {
+        g.P[((int64_t)tl.ks * g.M + m) * g.N + n] = v;
+      } else {
+        double2 o = make_double2(g.alpha * v.x, g.alpha * v.y);
+        if (g.beta != 0.0) {
+          const double2 c0 = g.C0[(int64_t)m * g.ldc0 + n];
+          o.x += g.beta * c0.x;
+          o.y += g.beta * c0.y;
+        }
+        g.C[(int64_t)m * g.ldc + n] = o;
+      }
+    }
+  }
 }
 
 // Fixed-order sum of the split-K partials: C = alpha sum_s P_s + beta C0.
@@ -792,50 +887,56 @@ void launch_diis_axpy(const SolveDesc *dd, int nsolve, int64_t maxT, const int *
   hipLaunchKernelGGL(k_diis_axpy, egrid(maxT, nsolve), dim3(256), 0, st, dd, slots, coef, nvec);
 }
 
-void launch_gemm(const GemmGroup *groups, const GemmTile *tiles, int ntiles, const SolveDesc *dd,
-                 const int *gsolve, int variant, hipStream_t st) {
-  if (ntiles <= 0) return;
-  switch (variant) {
-    case 3:
-      hipLaunchKernelGGL(k_zgemm_valu, dim3(ntiles), dim3(256), 0, st, groups, tiles, dd, gsolve);
-      break;
-    // direct-load MFMA kernels: 10+nb FP64, 20+nb FP32 (A double), 30+nb FP32 (A float),
-    // 40+nb FP64 pp ladder, 60+nb FP32 pp ladder (A float);
-    // nb = 16-column blocks per tile (1,2 with two 16-row blocks per wave, 3-5 with one)
-    case 11: hipLaunchKernelGGL((k_zgemm_dl<2, 1, false, double2, 0>), dim3(ntiles), dim3(256), 0, st, groups, tiles, dd, gsolve); break;
-    case 12: hipLaunchKernelGGL((k_zgemm_dl<2, 2, false, double2, 0>), dim3(ntiles), dim3(256), 0, st, groups, tiles, dd, gsolve); break;
-    case 13: hipLaunchKernelGGL((k_zgemm_dl<1, 3, false, double2, 0>), dim3(ntiles), dim3(256), 0, st, groups, tiles, dd, gsolve); break;
-    case 14: hipLaunchKernelGGL((k_zgemm_dl<1, 4, false, double2, 0>), dim3(ntiles), dim3(256), 0, st, groups, tiles, dd, gsolve); break;
-    case 15: hipLaunchKernelGGL((k_zgemm_dl<1, 5, false, double2, 0>), dim3(ntiles), dim3(256), 0, st, groups, tiles, dd, gsolve); break;
-    case 21: hipLaunchKernelGGL((k_zgemm_dl<2, 1, true, double2, 0>), dim3(ntiles), dim3(256), 0, st, groups, tiles, dd, gsolve); break;
-    case 22: hipLaunchKernelGGL((k_zgemm_dl<2, 2, true, double2, 0>), dim3(ntiles), dim3(256), 0, st, groups, tiles, dd, gsolve); break;
-    case 23: hipLaunchKernelGGL((k_zgemm_dl<1, 3, true, double2, 0>), dim3(ntiles), dim3(256), 0, st, groups, tiles, dd, gsolve); break;
-    case 24: hipLaunchKernelGGL((k_zgemm_dl<1, 4, true, double2, 0>), dim3(ntiles), dim3(256), 0, st, groups, tiles, dd, gsolve); break;
-    case 25: hipLaunchKernelGGL((k_zgemm_dl<1, 5, true, double2, 0>), dim3(ntiles), dim3(256), 0, st, groups, tiles, dd, gsolve); break;
-    case 31: hipLaunchKernelGGL((k_zgemm_dl<2, 1, true, float2, 0>), dim3(ntiles), dim3(256), 0, st, groups, tiles, dd, gsolve); break;
-    case 32: hipLaunchKernelGGL((k_zgemm_dl<2, 2, true, float2, 0>), dim3(ntiles), dim3(256), 0, st, groups, tiles, dd, gsolve); break;
-    case 33: hipLaunchKernelGGL((k_zgemm_dl<1, 3, true, float2, 0>), dim3(ntiles), dim3(256), 0, st, groups, tiles, dd, gsolve); break;
-    case 34: hipLaunchKernelGGL((k_zgemm_dl<1, 4, true, float2, 0>), dim3(ntiles), dim3(256), 0, st, groups, tiles, dd, gsolve); break;
-    case 35: hipLaunchKernelGGL((k_zgemm_dl<1, 5, true, float2, 0>), dim3(ntiles), dim3(256), 0, st, groups, tiles, dd, gsolve); break;
-    case 41: hipLaunchKernelGGL((k_zgemm_dl<2, 1, false, double2, 1>), dim3(ntiles), dim3(256), 0, st, groups, tiles, dd, gsolve); break;
-    case 42: hipLaunchKernelGGL((k_zgemm_dl<2, 2, false, double2, 1>), dim3(ntiles), dim3(256), 0, st, groups, tiles, dd, gsolve); break;
-    case 43: hipLaunchKernelGGL((k_zgemm_dl<1, 3, false, double2, 1>), dim3(ntiles), dim3(256), 0, st, groups, tiles, dd, gsolve); break;
-    case 44: hipLaunchKernelGGL((k_zgemm_dl<1, 4, false, double2, 1>), dim3(ntiles), dim3(256), 0, st, groups, tiles, dd, gsolve); break;
-    case 45: hipLaunchKernelGGL((k_zgemm_dl<1, 5, false, double2, 1>), dim3(ntiles), dim3(256), 0, st, groups, tiles, dd, gsolve); break;
-    // 70+nb: pp ladder reading the FP32 copy of V, FP64 MFMA and accumulation
-    case 71: hipLaunchKernelGGL((k_zgemm_dl<2, 1, false, float2, 2>), dim3(ntiles), dim3(256), 0, st, groups, tiles, dd, gsolve); break;
-    case 72: hipLaunchKernelGGL((k_zgemm_dl<2, 2, false, float2, 2>), dim3(ntiles), dim3(256), 0, st, groups, tiles, dd, gsolve); break;
-    case 73: hipLaunchKernelGGL((k_zgemm_dl<1, 3, false, float2, 2>), dim3(ntiles), dim3(256), 0, st, groups, tiles, dd, gsolve); break;
-    case 74: hipLaunchKernelGGL((k_zgemm_dl<1, 4, false, float2, 2>), dim3(ntiles), dim3(256), 0, st, groups, tiles, dd, gsolve); break;
-    case 75: hipLaunchKernelGGL((k_zgemm_dl<1, 5, false, float2, 2>), dim3(ntiles), dim3(256), 0, st, groups, tiles, dd, gsolve); break;
-    case 61: hipLaunchKernelGGL((k_zgemm_dl<2, 1, true, float2, 1>), dim3(ntiles), dim3(256), 0, st, groups, tiles, dd, gsolve); break;
-    case 62: hipLaunchKernelGGL((k_zgemm_dl<2, 2, true, float2, 1>), dim3(ntiles), dim3(256), 0, st, groups, tiles, dd, gsolve); break;
-    case 63: hipLaunchKernelGGL((k_zgemm_dl<1, 3, true, float2, 1>), dim3(ntiles), dim3(256), 0, st, groups, tiles, dd, gsolve); break;
-    case 64: hipLaunchKernelGGL((k_zgemm_dl<1, 4, true, float2, 1>), dim3(ntiles), dim3(256), 0, st, groups, tiles, dd, gsolve); break;
-    case 65: hipLaunchKernelGGL((k_zgemm_dl<1, 5, true, float2, 1>), dim3(ntiles), dim3(256), 0, st, groups, tiles, dd, gsolve); break;
-    default:
-      break;
+// Kernel kinds: 0 FP64, 1 FP32 (A double), 2 FP32 (A float), 3 FP64 pp ladder,
+// 4 FP32 pp ladder (A float), 5 pp ladder with A float and FP64 arithmetic.
+// Tile shapes (MB, NB): MB 16-row blocks per wave (4 waves stacked along M),
+// NB 16-column blocks: (1,1) (1,2) (1,3) (1,4) (1,5) (2,1) (2,2) (4,1); and
+// (0,4): the thin VALU kernel (16-row tiles, N <= 4).
+template <int MB, int NB>
+static void launch_shape(const GemmGroup *groups, const GemmTile *tiles, int ntiles, const SolveDesc *dd,
+                         const int *gsolve, int kind, hipStream_t st) {
+  switch (kind) {
+    case 0: hipLaunchKernelGGL((k_zgemm_dl<MB, NB, false, double2, 0>), dim3(ntiles), dim3(256), 0, st, groups, tiles, dd, gsolve); break;
+    case 1: hipLaunchKernelGGL((k_zgemm_dl<MB, NB, true, double2, 0>), dim3(ntiles), dim3(256), 0, st, groups, tiles, dd, gsolve); break;
+    case 2: hipLaunchKernelGGL((k_zgemm_dl<MB, NB, true, float2, 0>), dim3(ntiles), dim3(256), 0, st, groups, tiles, dd, gsolve); break;
+    case 3: hipLaunchKernelGGL((k_zgemm_dl<MB, NB, false, double2, 1>), dim3(ntiles), dim3(256), 0, st, groups, tiles, dd, gsolve); break;
+    case 4: hipLaunchKernelGGL((k_zgemm_dl<MB, NB, true, float2, 1>), dim3(ntiles), dim3(256), 0, st, groups, tiles, dd, gsolve); break;
+    case 5: hipLaunchKernelGGL((k_zgemm_dl<MB, NB, false, float2, 2>), dim3(ntiles), dim3(256), 0, st, groups, tiles, dd, gsolve); break;
+    default: break;
   }
+}
+
+void launch_gemm_shape(const GemmGroup *groups, const GemmTile *tiles, int ntiles, const SolveDesc *dd,
+                       const int *gsolve, int kind, int mb, int nb, hipStream_t st) {
+  if (ntiles <= 0) return;
+  if (mb == 0) {   // thin VALU kernel, N <= 4, FP64 arithmetic
+    const bool af = kind == 2 || kind == 4 || kind == 5;
+    const bool pp = kind >= 3;
+    if (af && pp) hipLaunchKernelGGL((k_zgemm_thin<float2, 1>), dim3(ntiles), dim3(256), 0, st, groups, tiles, dd, gsolve);
+    else if (af) hipLaunchKernelGGL((k_zgemm_thin<float2, 0>), dim3(ntiles), dim3(256), 0, st, groups, tiles, dd, gsolve);
+    else if (pp) hipLaunchKernelGGL((k_zgemm_thin<double2, 1>), dim3(ntiles), dim3(256), 0, st, groups, tiles, dd, gsolve);
+    else hipLaunchKernelGGL((k_zgemm_thin<double2, 0>), dim3(ntiles), dim3(256), 0, st, groups, tiles, dd, gsolve);
+    return;
+  }
+  const int code = mb * 10 + nb;
+  switch (code) {
+    case 11: launch_shape<1, 1>(groups, tiles, ntiles, dd, gsolve, kind, st); break;
+    case 12: launch_shape<1, 2>(groups, tiles, ntiles, dd, gsolve, kind, st); break;
+    case 13: launch_shape<1, 3>(groups, tiles, ntiles, dd, gsolve, kind, st); break;
+    case 14: launch_shape<1, 4>(groups, tiles, ntiles, dd, gsolve, kind, st); break;
+    case 15: launch_shape<1, 5>(groups, tiles, ntiles, dd, gsolve, kind, st); break;
+    case 21: launch_shape<2, 1>(groups, tiles, ntiles, dd, gsolve, kind, st); break;
+    case 22: launch_shape<2, 2>(groups, tiles, ntiles, dd, gsolve, kind, st); break;
+    case 41: launch_shape<4, 1>(groups, tiles, ntiles, dd, gsolve, kind, st); break;
+    default: break;
+  }
+}
+
+// Plain FP64 VALU cross-check on uniform 64x64 tiles.
+void launch_gemm_valu(const GemmGroup *groups, const GemmTile *tiles, int ntiles, const SolveDesc *dd,
+                      const int *gsolve, hipStream_t st) {
+  if (ntiles <= 0) return;
+  hipLaunchKernelGGL(k_zgemm_valu, dim3(ntiles), dim3(256), 0, st, groups, tiles, dd, gsolve);
 }
 
 }  // namespace ccgpu

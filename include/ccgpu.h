@@ -193,6 +193,14 @@ int ccgpu_debug_fetch(int handle, int what, double *out, long long n);
  * solve; names are written as a ';'-separated list. */
 int ccgpu_kernel_stats(char *names, int len, double *avg_ms, long long *launches, int max);
 
+/* Tile shapes chosen by the autotuner so far, one entry per work list:
+ * "famF/kindK: cC=MBxNB(time) ..."; returns the report length.  The tuner
+ * times every candidate shape of every tile class of a batch's GEMM work lists
+ * on first use and caches the choice by list signature (CCGPU_AUTOTUNE=0: the
+ * default table).  No counterpart in the reference (its work partition,
+ * ccm_mapping.f90:2-934, splits loops over MPI ranks). */
+int ccgpu_autotune_report(char *buf, int len);
+
 /* Warm-start network (16 -> 64 -> 64 -> 64 -> 1, GELU, BF16 MFMA with FP32
  * accumulation).  It predicts, per stored amplitude, the correction of the
  * converged CCD amplitude over its MBPT2 value from momentum, spin, density and
