@@ -50,16 +50,6 @@ void launch_gemm_valu(const GemmGroup *groups, const GemmTile *tiles, int ntiles
                       const int *gsolve, hipStream_t st);
 void launch_splitk_reduce(const GemmGroup *groups, const int *gidx, int n, int64_t maxmn, const SolveDesc *dd,
                           const int *gsolve, hipStream_t st);
-struct TriplesDesc {
-  int k0, nk, tz0, ntz;
-  const int *chan;
-  const int *hptr;
-  const int *hijk;
-  const int *orbitof;
-  int nmax;
-  const int *pair_a, *pair_b;
-  int64_t npairs;
-};
 void launch_triples(const SolveDesc &d, const TriplesDesc &td, double volume, double2 *partial, int nblocks,
                     double2 *out, hipStream_t st);
 // mlp.hip
@@ -307,6 +297,8 @@ struct Solve {
   double2 e0{0, 0};
   // hhphp records of the drop-in path: key (i,b,j,a) -> <ib||ja>
   std::unordered_map<uint64_t, std::complex<double>> hphp;
+  // <cb||kd> and <jk||lc> records of the drop-in path for (T): key (p,q,r,s)
+  std::unordered_map<uint64_t, std::complex<double>> t3v;
   // DIIS state
   int depth = 0;
   std::vector<int> ring;  // logical column -> physical slot
@@ -1041,7 +1033,15 @@ Batch &single(int h) {
   g_single[h] = single_batch(S);
   return *g_single[h];
 }
-void drop_single(int h) { g_single.erase(h); }
+// cached multi-solve batches, keyed by their handle lists
+std::map<std::vector<int>, std::unique_ptr<Batch>> g_batches;
+void drop_single(int h) {
+  g_single.erase(h);
+  for (auto it = g_batches.begin(); it != g_batches.end();) {
+    if (std::find(it->first.begin(), it->first.end(), h) != it->first.end()) it = g_batches.erase(it);
+    else ++it;
+  }
+}
 
 // FP32 copy of the stored pppp blocks, made on the first FP32 request; the
 // cached single-solve batch is rebuilt so that its FP32 ladder list reads it.
@@ -1380,6 +1380,35 @@ int ccgpu_set_block(int handle, int kind, int nbra, const int *bra_p, const int 
         }
         break;
       }
+      case CCGPU_BLOCK_HHHP:
+      case CCGPU_BLOCK_HPPP: {
+        // HHHP: rows hole pairs (j,k), columns (hole l, particle c), element
+        // <jk||lc> (vnn_hhhp, ccm_iter.f90:903-920).  HPPP: rows (hole k,
+        // particle d), columns particle pairs (c,b), element <cb||kd>
+        // (vnn_hppp, ccm_iter.f90:1621-1711).  Stored for (T) keyed by the
+        // element's own index order, with both orders of the like pair.
+        const uint64_t n = (uint64_t)P.nsp;
+        auto key = [&](int p, int q, int r, int t) { return (((uint64_t)p * n + q) * n + r) * n + t; };
+        for (int r = 0; r < nbra; ++r) {
+          int p = chk(bra_p[r]), q = chk(bra_q[r]);
+          for (int c = 0; c < nket; ++c) {
+            int x = chk(ket_r[c]), y = chk(ket_s[c]);
+            const double2 v = V(r, c);
+            const std::complex<double> z(v.x, v.y);
+            if (kind == CCGPU_BLOCK_HHHP) {
+              if (p >= P.A || q >= P.A || x >= P.A || y < P.A) throw Error("hhhp block with wrong index classes");
+              S.t3v[key(p, q, x, y)] = z;
+              S.t3v.emplace(key(q, p, x, y), -z);
+            } else {
+              if (p >= P.A || q < P.A || x < P.A || y < P.A) throw Error("hppp block with wrong index classes");
+              S.t3v[key(x, y, p, q)] = z;
+              S.t3v.emplace(key(y, x, p, q), -z);
+            }
+          }
+        }
+        S.have_t3 = false;
+        break;
+      }
       default:
         throw Error(fmt("block kind %d not accepted by ccgpu_set_block", kind));
     }
@@ -1553,9 +1582,16 @@ int ccgpu_batch_solve(int n, const int *handles, const ccgpu_solve_opts *opts, c
       if (opts->precision != CCGPU_PREC_FP64) ensure_f32(S, handles[s]);
       ss.push_back(&S);
     }
-    auto B = make_batch(ss, ss[0]->stream);
+    // the work lists, tiles and captured graphs of a handle set are kept for
+    // the next call on the same handles (dropped when one of them changes)
+    std::vector<int> key(handles, handles + n);
+    auto it = g_batches.find(key);
+    if (it == g_batches.end()) {
+      if (g_batches.size() >= 8) g_batches.clear();
+      it = g_batches.emplace(key, make_batch(ss, ss[0]->stream)).first;
+    }
     std::vector<ccgpu_solve_result> r;
-    run_batch(*B, *opts, r);
+    run_batch(*it->second, *opts, r);
     for (int s = 0; s < n; ++s) results[s] = r[s];
   });
 }
@@ -1632,7 +1668,9 @@ void build_triples(Solve &S) {
   td.nk = 6 * nh + 1;
   td.tz0 = -3;
   td.ntz = 7;
-  td.nmax = o.nmax;
+  int nm = 0;   // lattice extent from the orbitals themselves (drop-in tables too)
+  for (int x = 0; x < P.nsp; ++x) nm = std::max({nm, std::abs(o.nx[x]), std::abs(o.ny[x]), std::abs(o.nz[x])});
+  td.nmax = nm;
   std::vector<int> chan((size_t)td.ntz * td.nk * td.nk * td.nk, -1);
   std::map<std::tuple<int, int, int, int>, std::vector<int>> trip;
   for (int i = 0; i < P.A; ++i)
@@ -1653,10 +1691,10 @@ void build_triples(Solve &S) {
     hijk.insert(hijk.end(), kv.second.begin(), kv.second.end());
     hptr.push_back((int)(hijk.size() / 3));
   }
-  const int n = 2 * o.nmax + 1;
+  const int n = 2 * nm + 1;
   std::vector<int> orbitof((size_t)n * n * n * 4, -1);
   for (int x = 0; x < P.nsp; ++x) {
-    size_t idx = (((size_t)(o.nx[x] + o.nmax) * n + (o.ny[x] + o.nmax)) * n + (o.nz[x] + o.nmax)) * 4 +
+    size_t idx = (((size_t)(o.nx[x] + nm) * n + (o.ny[x] + nm)) * n + (o.nz[x] + nm)) * 4 +
                  ((o.sz[x] + 1) / 2) * 2 + (o.tz[x] + 1) / 2;
     orbitof[idx] = x;
   }
@@ -1670,6 +1708,31 @@ void build_triples(Solve &S) {
   td.pair_a = dupload(pa, S.owned);
   td.pair_b = dupload(pb, S.owned);
   td.npairs = (int64_t)pa.size();
+  td.vkeys = nullptr;
+  td.vvals = nullptr;
+  td.vmask = 0;
+  if (!S.native_v) {
+    if (S.t3v.empty())
+      throw Error("(T) of a handed-over interaction needs its HHHP and HPPP blocks (ccgpu_set_block)");
+    uint64_t cap = 1;
+    while (cap < 2 * S.t3v.size()) cap <<= 1;
+    std::vector<unsigned long long> keys(cap, ~0ull);
+    std::vector<double2> vals(cap, double2{0.0, 0.0});
+    for (auto &kv : S.t3v) {
+      // the probe start of the device lookup (murmur3 finalizer, low bits)
+      uint64_t h = kv.first;
+      h ^= h >> 33;
+      h *= 0xff51afd7ed558ccdull;
+      h ^= h >> 33;
+      h &= cap - 1;
+      while (keys[h] != ~0ull) h = (h + 1) & (cap - 1);
+      keys[h] = kv.first;
+      vals[h] = double2{kv.second.real(), kv.second.imag()};
+    }
+    td.vkeys = dupload(keys, S.owned);
+    td.vvals = dupload(vals, S.owned);
+    td.vmask = cap - 1;
+  }
   S.t3_partial = dalloc<double2>(8192, S.owned);
   S.t3_out = dalloc<double2>(1, S.owned);
   S.have_t3 = true;
@@ -1680,9 +1743,6 @@ int ccgpu_triples(int handle, double *et) {
     Solve &S = get(handle);
     ensure_ready(S);
     if (!S.have_t) throw Error("no amplitudes: solve CCD first");
-    if (!S.native_v)
-      throw Error("ccgpu_triples evaluates <cb||kd> and <jk||lc> with the built-in generator; "
-                  "handles filled through ccgpu_set_block are not supported");
     if (!S.have_t3) build_triples(S);
     int nb = (int)std::min<int64_t>(8192, std::max<int64_t>(1, S.t3.npairs));
     launch_triples(S.desc, S.t3, S.plan.orb.volume, S.t3_partial, nb, S.t3_out, S.stream);

@@ -71,6 +71,24 @@ struct GemmTile {
   int kb, ke, pad0, pad1;  // k range of this tile
 };
 
+// (T) channel tables and, for handed-over interactions, the open-addressing
+// table of the <cb||kd> and <jk||lc> elements (key ((p*nsp+q)*nsp+r)*nsp+s,
+// empty slots ~0, capacity a power of two at least twice the entries).
+struct TriplesDesc {
+  // channel lookup over (Tz3, K3): index = ((tz - tz0) * nk + (kx - k0)) * nk^2 + ...
+  int k0, nk, tz0, ntz;
+  const int *chan;        // dense table -> channel or -1
+  const int *hptr;        // CSR over channels of hole triples
+  const int *hijk;        // 3 ints per hole triple
+  const int *orbitof;     // dense (n, sz, tz) -> orbital, -1 absent
+  int nmax;
+  const int *pair_a, *pair_b;  // every particle pair a<b (all momenta)
+  int64_t npairs;
+  const unsigned long long *vkeys;  // null: elements from the built-in generator
+  const double2 *vvals;
+  unsigned long long vmask;
+};
+
 // Warm-start network: number of parameters (flat FP32 vector, layout in
 // mlp.hip) and the buffers of one training step: feature-major activations and
 // pre-activation gradients of the minibatch (B columns, B a multiple of 64).

@@ -10,7 +10,8 @@
 //   H(a,b,c,i,j,k) = sum_l conj(<jk||lc>) t(ab,il),  n_l = n_k + n_j - n_c
 // (d runs over the particle spin states of its momentum, l over the holes).
 // The interaction elements are evaluated in place by the same generator that
-// filled the V blocks.
+// filled the V blocks, or, for handed-over interactions, read from a device
+// hash table of the <cb||kd> (vnn_hppp) and <jk||lc> (vnn_hhhp) blocks.
 //
 // Work decomposition: one thread per particle triple (a<b<c): block x = the
 // particle pair (a,b), threads stride over c.  A thread whose (K3,Tz3) has hole
@@ -24,17 +25,6 @@
 
 namespace ccgpu {
 
-struct TriplesDesc {
-  // channel lookup over (Tz3, K3): index = ((tz - tz0) * nk + (kx - k0)) * nk^2 + ...
-  int k0, nk, tz0, ntz;
-  const int *chan;        // dense table -> channel or -1
-  const int *hptr;        // CSR over channels of hole triples
-  const int *hijk;        // 3 ints per hole triple
-  const int *orbitof;     // dense (n, sz, tz) -> orbital, -1 absent
-  int nmax;
-  const int *pair_a, *pair_b;  // every particle pair a<b (all momenta)
-  int64_t npairs;
-};
 
 __device__ __forceinline__ double minnesota_direct_t(const SolveDesc &d, double volume, int p, int q, int r, int s) {
   Orb o;
@@ -65,9 +55,27 @@ __device__ __forceinline__ double2 t_of(const SolveDesc &d, int a, int b, int i,
   return make_double2(sg * t.x, sg * t.y);
 }
 
-// antisymmetrized element, real for the built-in generator
-__device__ __forceinline__ double vas_t(const SolveDesc &d, double volume, int p, int q, int r, int s) {
-  return minnesota_direct_t(d, volume, p, q, r, s) - minnesota_direct_t(d, volume, p, q, s, r);
+// antisymmetrized element <pq||rs>: from the table of handed-over elements,
+// else from the built-in generator (real)
+__device__ __forceinline__ double2 vas_t(const SolveDesc &d, const TriplesDesc &td, double volume, int p, int q,
+                                         int r, int s) {
+  if (td.vkeys) {
+    const unsigned long long n = (unsigned long long)d.nsp;
+    const unsigned long long key = (((unsigned long long)p * n + q) * n + r) * n + s;
+    unsigned long long h = key;
+    h ^= h >> 33;
+    h *= 0xff51afd7ed558ccdull;
+    h ^= h >> 33;
+    h &= td.vmask;
+    // the table is at most half full, so an empty slot ends every probe
+    for (;;) {
+      const unsigned long long k = td.vkeys[h];
+      if (k == key) return td.vvals[h];
+      if (k == ~0ull) return make_double2(0.0, 0.0);
+      h = (h + 1) & td.vmask;
+    }
+  }
+  return make_double2(minnesota_direct_t(d, volume, p, q, r, s) - minnesota_direct_t(d, volume, p, q, s, r), 0.0);
 }
 
 // F(a,b,c,i,j,k) = sum_d conj(<cb||kd>) t(ad,ij)  (three_body_oneparticle_1)
@@ -79,10 +87,10 @@ __device__ double2 f_term(const SolveDesc &d, const TriplesDesc &td, double vol,
   for (int sz = -1; sz <= 1; sz += 2) {
     int dd = orb_lookup(td, nx, ny, nz, sz, tz);
     if (dd < d.A) continue;
-    double2 t = t_of(d, a, dd, i, j);
-    double v = vas_t(d, vol, c, b, k, dd);
-    s.x += v * t.x;
-    s.y += v * t.y;
+    const double2 t = t_of(d, a, dd, i, j);
+    const double2 v = vas_t(d, td, vol, c, b, k, dd);   // conj(v) t
+    s.x += v.x * t.x + v.y * t.y;
+    s.y += v.x * t.y - v.y * t.x;
   }
   return s;
 }
@@ -96,10 +104,10 @@ __device__ double2 h_term(const SolveDesc &d, const TriplesDesc &td, double vol,
   for (int sz = -1; sz <= 1; sz += 2) {
     int l = orb_lookup(td, nx, ny, nz, sz, tz);
     if (l < 0 || l >= d.A) continue;
-    double2 t = t_of(d, a, b, i, l);
-    double v = vas_t(d, vol, j, k, l, c);
-    s.x += v * t.x;
-    s.y += v * t.y;
+    const double2 t = t_of(d, a, b, i, l);
+    const double2 v = vas_t(d, td, vol, j, k, l, c);    // conj(v) t
+    s.x += v.x * t.x + v.y * t.y;
+    s.y += v.x * t.y - v.y * t.x;
   }
   return s;
 }

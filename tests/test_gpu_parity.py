@@ -73,6 +73,32 @@ def test_native_minnesota_against_reference(ccnm, name):
     check_trajectory(en, g["energies"], step, float(g["tol"]))
 
 
+def hand_over(s, kind, rows, vals):
+    """One reference-layout block from records rows[:, 0:4] (1-based orbitals of
+    the bra pair and the ket pair) and their values."""
+    bra = np.unique(rows[:, 0:2], axis=0)
+    ket = np.unique(rows[:, 2:4], axis=0)
+    bpos = {tuple(x): n for n, x in enumerate(bra)}
+    kpos = {tuple(x): n for n, x in enumerate(ket)}
+    val = np.zeros((len(bra), len(ket)), np.complex128)
+    for row, v in zip(rows, vals):
+        val[bpos[(row[0], row[1])], kpos[(row[2], row[3])]] = v
+    s.set_block(kind, bra[:, 0], bra[:, 1], ket[:, 0], ket[:, 1], val)
+
+
+def hand_over_blocks(ccnm, s, g, triples=False):
+    idx = g["v_idx"].astype(np.int64)
+    kinds = {1: ccnm.BLOCK_HHHH, 3: ccnm.BLOCK_HHPP, 4: ccnm.BLOCK_HPHP, 6: ccnm.BLOCK_PPPP}
+    if triples:
+        kinds[2] = ccnm.BLOCK_HHHP
+    for k, kind in kinds.items():
+        sel = idx[:, 0] == k
+        hand_over(s, kind, idx[sel][:, 1:5], g["v_val"][sel])
+    if triples:
+        # vnn_hppp records (k, d, c, b) hold <cb||kd>: rows (k,d), columns (c,b)
+        hand_over(s, ccnm.BLOCK_HPPP, g["hppp_idx"].astype(np.int64), g["hppp_val"])
+
+
 @pytest.mark.parametrize("name", ["pnm14_n1_nnlo", "snm4_n1_nnlo", "pnm14_n1_minn"])
 def test_dropin_reference_blocks(ccnm, name):
     """Drop-in path: the reference's own orbitals, V blocks (chiral NNLO here) and
@@ -84,18 +110,7 @@ def test_dropin_reference_blocks(ccnm, name):
             "kx": od[0], "ky": od[1], "kz": od[2], "e": od[3], "A": A,
             "box": float(g["box"]), "volume": float(g["volume"])}
     s = ccnm.Solver(orbitals=orbs)
-    idx = g["v_idx"].astype(np.int64)
-    kinds = {1: ccnm.BLOCK_HHHH, 3: ccnm.BLOCK_HHPP, 4: ccnm.BLOCK_HPHP, 6: ccnm.BLOCK_PPPP}
-    for k, kind in kinds.items():
-        sel = idx[:, 0] == k
-        bra = np.unique(idx[sel][:, 1:3], axis=0)
-        ket = np.unique(idx[sel][:, 3:5], axis=0)
-        bpos = {tuple(x): n for n, x in enumerate(bra)}
-        kpos = {tuple(x): n for n, x in enumerate(ket)}
-        val = np.zeros((len(bra), len(ket)), np.complex128)
-        for row, v in zip(idx[sel], g["v_val"][sel]):
-            val[bpos[(row[1], row[2])], kpos[(row[3], row[4])]] = v
-        s.set_block(kind, bra[:, 0], bra[:, 1], ket[:, 0], ket[:, 1], val)
+    hand_over_blocks(ccnm, s, g)
     s.set_fock(g["fock"], g["e0"])
     s.finalize_setup()
     emp2 = s.setup_t_amplitudes()
@@ -254,6 +269,28 @@ def test_triples_against_reference(ccnm, name):
     r = s.ccd_iter(subspace=sub, diis_step=step, tol=float(g["tol"]), maxit=maxit)
     et2 = s.pert_triples2()
     assert abs(et2 - ref) < 1e-6 * abs(ref) + float(g["tol"])
+
+
+@pytest.mark.parametrize("name", ["snm4_n1_nnlo", "snm4_n1_minn", "pnm14_n1_minn_twist"])
+def test_triples_dropin_blocks(ccnm, name):
+    """(T) from handed-over blocks (the reference's chiral NNLO or Minnesota
+    hhhp and hppp elements) on the reference's converged T2, against the
+    reference's pert_triples2 energy."""
+    g = golden(name)
+    A = int(g["params"][0])
+    oi, od = g["orb_int"], g["orb_dbl"]
+    orbs = {"nx": oi[0], "ny": oi[1], "nz": oi[2], "sz": oi[3], "tz": oi[4],
+            "kx": od[0], "ky": od[1], "kz": od[2], "e": od[3], "A": A,
+            "box": float(g["box"]), "volume": float(g["volume"])}
+    s = ccnm.Solver(orbitals=orbs)
+    hand_over_blocks(ccnm, s, g, triples=True)
+    s.set_fock(g["fock"], g["e0"])
+    s.finalize_setup()
+    s.setup_t_amplitudes()
+    set_reference_t2(ccnm, s, g)
+    et = s.pert_triples2()
+    ref = g["eccdt"] - g["eccd"]
+    assert abs(et - ref) < 1e-11 * max(1.0, abs(ref)), (et, ref)
 
 
 def test_fp32_until_tolerance_with_fp64_confirmation(ccnm):
