@@ -45,7 +45,7 @@ void launch_gram(const SolveDesc *dd, int nsolve, const int *slots, int nstep, h
 void launch_diis_axpy(const SolveDesc *dd, int nsolve, int64_t maxT, const int *slots, const double2 *coef,
                       int nvec, hipStream_t st);
 void launch_gemm_shape(const GemmGroup *groups, const GemmTile *tiles, int ntiles, const SolveDesc *dd,
-                       const int *gsolve, int kind, int mb, int nb, hipStream_t st);
+                       const int *gsolve, int kind, int mb, int nb, int lb, hipStream_t st);
 void launch_gemm_valu(const GemmGroup *groups, const GemmTile *tiles, int ntiles, const SolveDesc *dd,
                       const int *gsolve, hipStream_t st);
 void launch_splitk_reduce(const GemmGroup *groups, const int *gidx, int n, int64_t maxmn, const SolveDesc *dd,
@@ -144,6 +144,7 @@ void xcd_order_list(std::vector<GemmTile> &tiles, size_t ngroups) {
 // for the VALU cross-check kernel.
 struct GemmShape {
   int mb, nb;
+  int lb = 0;   // 1: B operand staged in LDS (same tiles, same results)
 };
 bool gemm_shape_ok(int mb, int nb) {
   return (mb == 0 && nb == 4) || (mb == 1 && nb >= 1 && nb <= 5) || (mb == 2 && (nb == 1 || nb == 2)) ||
@@ -217,9 +218,10 @@ struct GemmList {
   // batch), and the MFMA kernels sum k in the same order for every shape, so
   // results do not depend on the autotuner's choice or on batching.
   static int split_of(const GemmGroup &g) {
+    static const int64_t lim = getenv("CCGPU_SPLIT_TILES") ? atoi(getenv("CCGPU_SPLIT_TILES")) : 128;
     const int64_t ref_tiles = (int64_t)((g.M + 63) / 64) * ((g.N + 15) / 16);
-    if (g.K < 512 || ref_tiles >= 32) return 1;
-    return (int)std::min<int64_t>({(32 + ref_tiles - 1) / ref_tiles, (int64_t)(g.K / 256), 8});
+    if (g.K < 512 || ref_tiles >= lim) return 1;
+    return (int)std::min<int64_t>({(2 * lim + ref_tiles - 1) / ref_tiles, (int64_t)(g.K / 256), 8});
   }
   void build_tiles() {
     for (int c = 0; c < NCLS; ++c) tiles[c].clear();
@@ -579,6 +581,8 @@ std::vector<GemmShape> shape_candidates(int c) {
   v.push_back({2, 1});
   if (c >= 1) v.push_back({2, 2});
   v.push_back({4, 1});
+  const size_t n = v.size();
+  for (size_t i = 0; i < n; ++i) v.push_back({v[i].mb, v[i].nb, 1});   // LDS-staged B variants
   return v;
 }
 
@@ -613,7 +617,7 @@ void autotune(GemmList &L, int kind, const SolveDesc *d_descs, hipStream_t st) {
       for (int rep_ = 0; rep_ < 4; ++rep_) {
         HIPCHK(hipEventRecord(e0, st));
         launch_gemm_shape(T.d_groups, T.d_tiles[c], (int)T.tiles[c].size(), d_descs, T.d_gsolve, kind, cand.mb,
-                          cand.nb, st);
+                          cand.nb, cand.lb, st);
         HIPCHK(hipEventRecord(e1, st));
         HIPCHK(hipEventSynchronize(e1));
         float t = 0.f;
@@ -628,7 +632,7 @@ void autotune(GemmList &L, int kind, const SolveDesc *d_descs, hipStream_t st) {
       }
     }
     L.shape[c] = best_shape;
-    rep += fmt(" c%d=%dx%d(%.1fus)", c, best_shape.mb, best_shape.nb, 1e3f * best / 3.f);
+    rep += fmt(" c%d=%dx%d%s(%.1fus)", c, best_shape.mb, best_shape.nb, best_shape.lb ? "L" : "", 1e3f * best / 3.f);
   }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
@@ -724,10 +728,11 @@ void run_gemm(const GemmList &L, const Batch &B, int prec, bool a_f32, hipStream
     for (int c = GemmList::NCLS - 1; c >= 0; --c) {
       if (L.tiles[c].empty()) continue;
       char nm[48];
-      snprintf(nm, sizeof nm, "fam%d.k%d.c%d.%dx%d", L.fam, kind, c, L.shape[c].mb, L.shape[c].nb);
+      snprintf(nm, sizeof nm, "fam%d.k%d.c%d.%dx%d%s", L.fam, kind, c, L.shape[c].mb, L.shape[c].nb,
+               L.shape[c].lb ? "L" : "");
       Timer t(detail, st, nm);
       launch_gemm_shape(L.d_groups, L.d_tiles[c], (int)L.tiles[c].size(), B.d_descs, L.d_gsolve, kind, L.shape[c].mb,
-                        L.shape[c].nb, st);
+                        L.shape[c].nb, L.shape[c].lb, st);
     }
   }
   launch_splitk_reduce(L.d_groups, L.d_split, (int)L.split.size(), L.maxmn, B.d_descs, L.d_gsolve, st);

@@ -685,6 +685,157 @@ __global__ __launch_bounds__(256) void k_zgemm_dl(const GemmGroup *__restrict__ 
       }
 }
 
+// The same grouped GEMM with the B operand staged once per workgroup in LDS
+// (double-buffered, one barrier per 16-wide k chunk) instead of being loaded
+// by each of the four waves: a quarter of the B traffic and no B registers.
+// The k order of every sum is the one of k_zgemm_dl, so results are identical.
+template <int MB, int NB, bool F32, typename AT, int FAM = 0>
+__global__ __launch_bounds__(256) void k_zgemm_lb(const GemmGroup *__restrict__ groups,
+                                                  const GemmTile *__restrict__ tiles,
+                                                  const SolveDesc *__restrict__ descs,
+                                                  const int *__restrict__ gsolve) {
+  typedef typename std::conditional<F32, float, double>::type real;
+  typedef typename std::conditional<F32, f4, d4>::type acc4;
+  constexpr int BC = 16 * NB;                 // tile columns
+  constexpr int BE = 16 * BC;                 // B elements per chunk
+  constexpr int BPT = (BE + 255) / 256;       // B elements per thread
+  __shared__ double2 Bs[2][16][BC + 1];
+  const GemmTile tl = tiles[blockIdx.x];
+  if (descs && !descs[gsolve[tl.g]].active) return;
+  const GemmGroup g = groups[tl.g];
+  const AT *A = (const AT *)g.A;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int li = lane & 15, lg = lane >> 4;
+  const int mw = tl.m0 + w * (MB * 16);
+  acc4 cre[MB][NB], cim[MB][NB], cs[MB][NB];
+#pragma unroll
+  for (int x = 0; x < MB; ++x)
+#pragma unroll
+    for (int y = 0; y < NB; ++y) {
+      cre[x][y] = (acc4){0, 0, 0, 0};
+      cim[x][y] = (acc4){0, 0, 0, 0};
+      cs[x][y] = (acc4){0, 0, 0, 0};
+    }
+  double2 a0[MB][4], a1[MB][4], bn[BPT];
+  const int kb = tl.kb, ke = min(tl.ke, g.K);
+  auto load_a = [&](double2 (&a)[MB][4], int k0) {
+#pragma unroll
+    for (int x = 0; x < MB; ++x) {
+      const int m = mw + x * 16 + li;
+      const int mc = min(m, g.M - 1);
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int k = k0 + lg * 4 + q;
+        const double2 v = ccgpu::load_a<AT>(A + (int64_t)mc * g.lda + min(k, ke - 1));
+        const bool ok = m < g.M && k < ke;
+        a[x][q] = make_double2(ok ? v.x : 0.0, ok ? v.y : 0.0);
+      }
+    }
+  };
+  // cooperative B chunk load: element e = threadIdx.x + 256 j -> (k row, column)
+  auto load_b = [&](int k0) {
+#pragma unroll
+    for (int j = 0; j < BPT; ++j) {
+      const int e = threadIdx.x + 256 * j;
+      const int kr = e / BC, c = e % BC;
+      const int k = k0 + kr, n = tl.n0 + c;
+      const double2 v = gload(g.B + (int64_t)min(k, ke - 1) * g.ldb + min(n, g.N - 1));
+      const bool ok = e < BE && k < ke && n < g.N;
+      bn[j] = make_double2(ok ? v.x : 0.0, ok ? v.y : 0.0);
+    }
+  };
+  auto store_b = [&](int buf) {
+#pragma unroll
+    for (int j = 0; j < BPT; ++j) {
+      const int e = threadIdx.x + 256 * j;
+      if (e < BE) Bs[buf][e / BC][e % BC] = bn[j];
+    }
+  };
+  auto compute = [&](const double2 (&a)[MB][4], int buf) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      double2 b[NB];
+#pragma unroll
+      for (int y = 0; y < NB; ++y) b[y] = Bs[buf][lg * 4 + q][y * 16 + li];
+#pragma unroll
+      for (int x = 0; x < MB; ++x) {
+        const real ar = (real)a[x][q].x, ai = (real)a[x][q].y;
+#pragma unroll
+        for (int y = 0; y < NB; ++y) {
+          const real br = (real)b[y].x, bi = (real)b[y].y;
+          if constexpr (F32) {
+            cre[x][y] = __builtin_amdgcn_mfma_f32_16x16x4f32(ar, br, cre[x][y], 0, 0, 0);
+            cre[x][y] = __builtin_amdgcn_mfma_f32_16x16x4f32(-ai, bi, cre[x][y], 0, 0, 0);
+            cim[x][y] = __builtin_amdgcn_mfma_f32_16x16x4f32(ar, bi, cim[x][y], 0, 0, 0);
+            cim[x][y] = __builtin_amdgcn_mfma_f32_16x16x4f32(ai, br, cim[x][y], 0, 0, 0);
+          } else {
+            cre[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar, br, cre[x][y], 0, 0, 0);
+            cim[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai, bi, cim[x][y], 0, 0, 0);
+            cs[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar + ai, br + bi, cs[x][y], 0, 0, 0);
+          }
+        }
+      }
+    }
+  };
+  // prologue: chunk kb in LDS buffer 0 and registers a0
+  load_b(kb);
+  load_a(a0, kb);
+  store_b(0);
+  __syncthreads();
+  int buf = 0;
+  for (int k0 = kb; k0 < ke; k0 += 32) {
+    const bool more1 = k0 + 16 < ke;
+    if (more1) {
+      load_b(k0 + 16);
+      load_a(a1, k0 + 16);
+    }
+    compute(a0, buf);
+    if (!more1) break;
+    store_b(buf ^ 1);
+    __syncthreads();
+    buf ^= 1;
+    const bool more2 = k0 + 32 < ke;
+    if (more2) {
+      load_b(k0 + 32);
+      load_a(a0, k0 + 32);
+    }
+    compute(a1, buf);
+    if (!more2) break;
+    store_b(buf ^ 1);
+    __syncthreads();
+    buf ^= 1;
+  }
+#pragma unroll
+  for (int x = 0; x < MB; ++x)
+#pragma unroll
+    for (int y = 0; y < NB; ++y)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int row = F32 ? (lg * 4 + r) : (lg + 4 * r);
+        const int m = mw + x * 16 + row;
+        const int n = tl.n0 + y * 16 + li;
+        double vr, vi;
+        if constexpr (F32) {
+          vr = (double)cre[x][y][r];
+          vi = (double)cim[x][y][r];
+        } else {
+          vr = cre[x][y][r] - cim[x][y][r];
+          vi = cs[x][y][r] - cre[x][y][r] - cim[x][y][r];
+        }
+        if (m < g.M && n < g.N && g.nsplit > 1) {
+          g.P[((int64_t)tl.ks * g.M + m) * g.N + n] = make_double2(vr, vi);
+        } else if (m < g.M && n < g.N) {
+          double2 v = make_double2(g.alpha * vr, g.alpha * vi);
+          if (g.beta != 0.0) {
+            double2 c0 = g.C0[(int64_t)m * g.ldc0 + n];
+            v.x += g.beta * c0.x;
+            v.y += g.beta * c0.y;
+          }
+          g.C[(int64_t)m * g.ldc + n] = v;
+        }
+      }
+}
+
 // Thin GEMM for groups with N <= 4 columns (most pp-ladder channels with few
 // hole pairs): FP64 VALU instead of 16-column MFMA tiles, which would be 75% or
 // more padding.  A tile is 16 rows: each wave owns 4 rows, its 64 lanes walk
@@ -891,7 +1042,22 @@ void launch_diis_axpy(const SolveDesc *dd, int nsolve, int64_t maxT, const int *
 // 4 FP32 pp ladder (A float), 5 pp ladder with A float and FP64 arithmetic.
 // Tile shapes (MB, NB): MB 16-row blocks per wave (4 waves stacked along M),
 // NB 16-column blocks: (1,1) (1,2) (1,3) (1,4) (1,5) (2,1) (2,2) (4,1); and
-// (0,4): the thin VALU kernel (16-row tiles, N <= 4).
+// (0,4): the thin VALU kernel (16-row tiles, N <= 4).  lb = 1 selects the
+// variant with B staged in LDS (k_zgemm_lb).
+template <int MB, int NB>
+static void launch_shape_lb(const GemmGroup *groups, const GemmTile *tiles, int ntiles, const SolveDesc *dd,
+                            const int *gsolve, int kind, hipStream_t st) {
+  switch (kind) {
+    case 0: hipLaunchKernelGGL((k_zgemm_lb<MB, NB, false, double2, 0>), dim3(ntiles), dim3(256), 0, st, groups, tiles, dd, gsolve); break;
+    case 1: hipLaunchKernelGGL((k_zgemm_lb<MB, NB, true, double2, 0>), dim3(ntiles), dim3(256), 0, st, groups, tiles, dd, gsolve); break;
+    case 2: hipLaunchKernelGGL((k_zgemm_lb<MB, NB, true, float2, 0>), dim3(ntiles), dim3(256), 0, st, groups, tiles, dd, gsolve); break;
+    case 3: hipLaunchKernelGGL((k_zgemm_lb<MB, NB, false, double2, 1>), dim3(ntiles), dim3(256), 0, st, groups, tiles, dd, gsolve); break;
+    case 4: hipLaunchKernelGGL((k_zgemm_lb<MB, NB, true, float2, 1>), dim3(ntiles), dim3(256), 0, st, groups, tiles, dd, gsolve); break;
+    case 5: hipLaunchKernelGGL((k_zgemm_lb<MB, NB, false, float2, 2>), dim3(ntiles), dim3(256), 0, st, groups, tiles, dd, gsolve); break;
+    default: break;
+  }
+}
+
 template <int MB, int NB>
 static void launch_shape(const GemmGroup *groups, const GemmTile *tiles, int ntiles, const SolveDesc *dd,
                          const int *gsolve, int kind, hipStream_t st) {
@@ -907,8 +1073,22 @@ static void launch_shape(const GemmGroup *groups, const GemmTile *tiles, int nti
 }
 
 void launch_gemm_shape(const GemmGroup *groups, const GemmTile *tiles, int ntiles, const SolveDesc *dd,
-                       const int *gsolve, int kind, int mb, int nb, hipStream_t st) {
+                       const int *gsolve, int kind, int mb, int nb, int lb, hipStream_t st) {
   if (ntiles <= 0) return;
+  if (lb && mb > 0) {
+    switch (mb * 10 + nb) {
+      case 11: launch_shape_lb<1, 1>(groups, tiles, ntiles, dd, gsolve, kind, st); break;
+      case 12: launch_shape_lb<1, 2>(groups, tiles, ntiles, dd, gsolve, kind, st); break;
+      case 13: launch_shape_lb<1, 3>(groups, tiles, ntiles, dd, gsolve, kind, st); break;
+      case 14: launch_shape_lb<1, 4>(groups, tiles, ntiles, dd, gsolve, kind, st); break;
+      case 15: launch_shape_lb<1, 5>(groups, tiles, ntiles, dd, gsolve, kind, st); break;
+      case 21: launch_shape_lb<2, 1>(groups, tiles, ntiles, dd, gsolve, kind, st); break;
+      case 22: launch_shape_lb<2, 2>(groups, tiles, ntiles, dd, gsolve, kind, st); break;
+      case 41: launch_shape_lb<4, 1>(groups, tiles, ntiles, dd, gsolve, kind, st); break;
+      default: break;
+    }
+    return;
+  }
   if (mb == 0) {   // thin VALU kernel, N <= 4, FP64 arithmetic
     const bool af = kind == 2 || kind == 4 || kind == 5;
     const bool pp = kind >= 3;
