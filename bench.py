@@ -194,6 +194,20 @@ def ladder_roofline(ccnm, s, precision):
     return ms, n, alg_bytes, flops, st
 
 
+def profiled_traffic():
+    """HBM bytes per pp-ladder pass from the committed rocprofv3 FETCH_SIZE and
+    WRITE_SIZE summary (profiles/pp_ladder_pnm66_nmax4_fp64.md: the line
+    "HBM traffic per pass = ... = **X GB**"), or None."""
+    import re
+    path = os.path.join(ROOT, "profiles", "pp_ladder_pnm66_nmax4_fp64.md")
+    try:
+        text = open(path).read()
+    except OSError:
+        return None
+    m = re.search(r"HBM traffic per pass[^*]*= \*\*([0-9.]+) GB\*\*", text, re.S)
+    return float(m.group(1)) * 1e9 if m else None
+
+
 def cpu_baseline(plain_iters):
     """The compiled reference (oracle/_ref/ref_driver, the reference's own
     t2_intermediate/diis_t1t2/ccd_energy_save) on this host's cores: a bounded
@@ -327,12 +341,13 @@ def main():
     tfs64 = flops64 / (ms64 * 1e-3) / 1e12
     ai64 = flops64 / bytes64
     bound = "hbm" if ai64 < FP64_PEAK_TFS * 1e12 / (HBM_PEAK_GBS * 1e9) else "mfma"
+    traffic = profiled_traffic()
     if bound == "hbm":
         roof = {"bound": "hbm", "achieved": round(gbs64, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(gbs64 / HBM_PEAK_GBS, 4), "traffic": None}
+                "frac": round(gbs64 / HBM_PEAK_GBS, 4), "traffic": traffic}
     else:
         roof = {"bound": "mfma", "achieved": round(tfs64, 2), "peak": FP64_PEAK_TFS, "unit": "TFLOP/s",
-                "frac": round(tfs64 / FP64_PEAK_TFS, 4), "traffic": None}
+                "frac": round(tfs64 / FP64_PEAK_TFS, 4), "traffic": traffic}
     roof.update({"kernel": "k_zgemm_f64 (pp ladder, FP64)", "avg_ms": round(ms64, 4), "launches": n64,
                  "alg_bytes_per_launch": bytes64, "alg_flops_per_launch": flops64,
                  "tflops": round(tfs64, 2), "arith_intensity": round(ai64, 2)})
