@@ -1716,6 +1716,25 @@ void build_triples(Solve &S) {
   td.vkeys = nullptr;
   td.vvals = nullptr;
   td.vmask = 0;
+  td.mtab = nullptr;
+  td.mmax = 0;
+  if (S.native_v) {
+    // radial Minnesota factors over the integer momentum transfer (the
+    // constants of the generator, float32 literals as in the reference)
+    const double pi = 3.14159265358979;
+    const double v0[3] = {200.0, -178.0, -(double)91.85f};
+    const double kap[3] = {(double)1.487f, (double)0.639f, (double)0.465f};
+    const int dmax = 4 * nm;
+    td.mmax = 3 * dmax * dmax;
+    const double f = pi / o.box;
+    std::vector<double> tab((size_t)3 * (td.mmax + 1));
+    for (int m = 0; m <= td.mmax; ++m) {
+      const double q2 = f * f * (double)m;
+      for (int g = 0; g < 3; ++g)
+        tab[3 * m + g] = v0[g] * std::pow(pi, 1.5) * std::exp(-q2 / (4.0 * kap[g])) / std::pow(kap[g], 1.5);
+    }
+    td.mtab = dupload(tab, S.owned);
+  }
   if (!S.native_v) {
     if (S.t3v.empty())
       throw Error("(T) of a handed-over interaction needs its HHHP and HPPP blocks (ccgpu_set_block)");

@@ -87,6 +87,10 @@ struct TriplesDesc {
   const unsigned long long *vkeys;  // null: elements from the built-in generator
   const double2 *vvals;
   unsigned long long vmask;
+  // built-in generator: radial factors of the three Minnesota Gaussians for
+  // q^2 = (pi/L)^2 m, m = |n_p - n_q - n_r + n_s|^2 (twists cancel), 3 per m
+  const double *mtab;
+  int mmax;
 };
 
 // Warm-start network: number of parameters (flat FP32 vector, layout in

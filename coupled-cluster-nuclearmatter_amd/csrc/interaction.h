@@ -49,4 +49,21 @@ static __device__ __forceinline__ double minnesota_direct(const Orb &o, int p, i
   return (vs + vr + vt) / o.volume;
 }
 
+// The same element with the radial factors read from a table over the integer
+// momentum transfer m = |n_p - n_q - n_r + n_s|^2 (q = (pi/L) dn for every
+// twist): rad[0..2] = v0 pi^1.5 exp(-q^2/(4 kappa)) / kappa^1.5 for the R, T
+// and S Gaussians (the last two carry their minus sign).
+static __device__ __forceinline__ double minnesota_direct_tab(const double *rad, int m1, int m2, int m3, int m4,
+                                                              int t1, int t2, int t3, int t4, double volume) {
+  if (t1 + t2 != t3 + t4) return 0.0;
+  const double ds = (m1 == m3 && m2 == m4) ? 1.0 : 0.0;
+  const double dt = (t1 == t3 && t2 == t4) ? 1.0 : 0.0;
+  const double ss = sig_dot_sig(m1, m2, m3, m4);
+  const double tt = sig_dot_sig(t1, t2, t3, t4);
+  const double vr = rad[0] * (3.0 * ds * dt - tt * ds - ss * dt - tt * ss) / 8.0;
+  const double vs = rad[2] * (3.0 * ds * dt + tt * ds - 3.0 * ss * dt - tt * ss) / 16.0;
+  const double vt = rad[1] * (3.0 * ds * dt - 3.0 * tt * ds + ss * dt - tt * ss) / 16.0;
+  return (vs + vr + vt) / volume;
+}
+
 }  // namespace ccgpu

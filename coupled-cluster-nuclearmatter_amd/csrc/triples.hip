@@ -75,6 +75,17 @@ __device__ __forceinline__ double2 vas_t(const SolveDesc &d, const TriplesDesc &
       h = (h + 1) & td.vmask;
     }
   }
+  if (td.mtab) {
+    const int ax = d.nx[p] - d.nx[q], ay = d.ny[p] - d.ny[q], az = d.nz[p] - d.nz[q];
+    const int dx1 = ax - d.nx[r] + d.nx[s], dy1 = ay - d.ny[r] + d.ny[s], dz1 = az - d.nz[r] + d.nz[s];
+    const int dx2 = ax - d.nx[s] + d.nx[r], dy2 = ay - d.ny[s] + d.ny[r], dz2 = az - d.nz[s] + d.nz[r];
+    const int m1 = dx1 * dx1 + dy1 * dy1 + dz1 * dz1, m2 = dx2 * dx2 + dy2 * dy2 + dz2 * dz2;
+    const int sp = d.sz[p], sq = d.sz[q], sr = d.sz[r], ss = d.sz[s];
+    const int tp = d.tz[p], tq = d.tz[q], tr = d.tz[r], ts = d.tz[s];
+    const double v = minnesota_direct_tab(td.mtab + 3 * m1, sp, sq, sr, ss, tp, tq, tr, ts, volume) -
+                     minnesota_direct_tab(td.mtab + 3 * m2, sp, sq, ss, sr, tp, tq, ts, tr, volume);
+    return make_double2(v, 0.0);
+  }
   return make_double2(minnesota_direct_t(d, volume, p, q, r, s) - minnesota_direct_t(d, volume, p, q, s, r), 0.0);
 }
 
