@@ -78,6 +78,7 @@ SIGNATURES = {
     "ccgpu_energy": [_I, _DP],
     "ccgpu_solve": [_I, C.POINTER(SolveOpts), C.POINTER(SolveResult)],
     "ccgpu_triples": [_I, _DP],
+    "ccgpu_triples_stats": [_I, C.POINTER(C.c_longlong), _DP],
     "ccgpu_get_t2": [_I, _I, _IP, _IP, _IP, _IP, _DP],
     "ccgpu_set_t2": [_I, _I, _IP, _IP, _IP, _IP, _DP],
     "ccgpu_get_t2_unique": [_I, _DP],
@@ -249,6 +250,12 @@ class Solver:
         x = np.zeros(2)
         _check(lib().ccgpu_triples(self.h, _dp(x)))
         return complex(x[0], x[1])
+
+    def triples_stats(self):
+        """(elements, seconds) of the last pert_triples2 call."""
+        n, t = C.c_longlong(), C.c_double()
+        _check(lib().ccgpu_triples_stats(self.h, C.byref(n), C.byref(t)))
+        return n.value, t.value
 
     # amplitudes
     def get_t2(self, a, b, i, j):

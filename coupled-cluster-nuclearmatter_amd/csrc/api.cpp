@@ -50,8 +50,9 @@ void launch_gemm_valu(const GemmGroup *groups, const GemmTile *tiles, int ntiles
                       const int *gsolve, hipStream_t st);
 void launch_splitk_reduce(const GemmGroup *groups, const int *gidx, int n, int64_t maxmn, const SolveDesc *dd,
                           const int *gsolve, hipStream_t st);
-void launch_triples(const SolveDesc &d, const TriplesDesc &td, double volume, double2 *partial, int nblocks,
-                    double2 *out, hipStream_t st);
+void launch_triples(const SolveDesc &d, const TriplesDesc &td, double volume, double2 *partial,
+                    unsigned long long *count, int nblocks, double2 *out, unsigned long long *count_out,
+                    hipStream_t st);
 // mlp.hip
 void launch_mlp_forward(const float *params, const uint16_t *feat, int64_t n, float *y, hipStream_t st);
 void launch_mlp_loss_grad(const float *params, const uint16_t *feat, const float *target, const int64_t *idx,
@@ -296,6 +297,9 @@ struct Solve {
   bool have_t3 = false;
   TriplesDesc t3{};
   double2 *t3_partial = nullptr, *t3_out = nullptr;
+  unsigned long long *t3_count = nullptr;   // per block, then the total at [8192]
+  long long t3_elements = 0;
+  double t3_seconds = 0.0;
   double2 e0{0, 0};
   // hhphp records of the drop-in path: key (i,b,j,a) -> <ib||ja>
   std::unordered_map<uint64_t, std::complex<double>> hphp;
@@ -1759,6 +1763,7 @@ void build_triples(Solve &S) {
   }
   S.t3_partial = dalloc<double2>(8192, S.owned);
   S.t3_out = dalloc<double2>(1, S.owned);
+  S.t3_count = dalloc<unsigned long long>(8193, S.owned);
   S.have_t3 = true;
 }
 
@@ -1769,13 +1774,27 @@ int ccgpu_triples(int handle, double *et) {
     if (!S.have_t) throw Error("no amplitudes: solve CCD first");
     if (!S.have_t3) build_triples(S);
     int nb = (int)std::min<int64_t>(8192, std::max<int64_t>(1, S.t3.npairs));
-    launch_triples(S.desc, S.t3, S.plan.orb.volume, S.t3_partial, nb, S.t3_out, S.stream);
+    auto t0 = std::chrono::steady_clock::now();
+    launch_triples(S.desc, S.t3, S.plan.orb.volume, S.t3_partial, S.t3_count, nb, S.t3_out, S.t3_count + 8192,
+                   S.stream);
     check_launch();
     double2 r;
+    unsigned long long cnt = 0;
     HIPCHK(hipMemcpyAsync(&r, S.t3_out, sizeof(double2), hipMemcpyDeviceToHost, S.stream));
+    HIPCHK(hipMemcpyAsync(&cnt, S.t3_count + 8192, sizeof(cnt), hipMemcpyDeviceToHost, S.stream));
     HIPCHK(hipStreamSynchronize(S.stream));
+    S.t3_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    S.t3_elements = (long long)cnt;
     et[0] = r.x;
     et[1] = r.y;
+  });
+}
+
+int ccgpu_triples_stats(int handle, long long *elements, double *seconds) {
+  return guarded([&] {
+    Solve &S = get(handle);
+    if (elements) *elements = S.t3_elements;
+    if (seconds) *seconds = S.t3_seconds;
   });
 }
 

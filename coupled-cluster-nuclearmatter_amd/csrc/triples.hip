@@ -123,9 +123,12 @@ __device__ double2 h_term(const SolveDesc &d, const TriplesDesc &td, double vol,
   return s;
 }
 
-__global__ __launch_bounds__(256) void k_triples(SolveDesc d, TriplesDesc td, double volume, double2 *partial) {
+__global__ __launch_bounds__(256) void k_triples(SolveDesc d, TriplesDesc td, double volume, double2 *partial,
+                                                 unsigned long long *count) {
   __shared__ double sre[256], sim[256];
+  __shared__ unsigned long long scnt[256];
   double re = 0.0, im = 0.0;
+  unsigned long long nel = 0;
   for (int64_t pr = blockIdx.x; pr < td.npairs; pr += gridDim.x) {
     const int a = td.pair_a[pr], b = td.pair_b[pr];
     for (int c = b + 1 + threadIdx.x; c < d.nsp; c += blockDim.x) {
@@ -137,6 +140,7 @@ __global__ __launch_bounds__(256) void k_triples(SolveDesc d, TriplesDesc td, do
       if (ch < 0) continue;
       const double fa = d.fdiag[a].x, fb = d.fdiag[b].x, fc = d.fdiag[c].x;
       const double fai = d.fdiag[a].y, fbi = d.fdiag[b].y, fci = d.fdiag[c].y;
+      nel += (unsigned long long)(td.hptr[ch + 1] - td.hptr[ch]);
       for (int h = td.hptr[ch]; h < td.hptr[ch + 1]; ++h) {
         const int i = td.hijk[3 * h], j = td.hijk[3 * h + 1], k = td.hijk[3 * h + 2];
         double2 w = make_double2(0.0, 0.0);
@@ -171,15 +175,27 @@ __global__ __launch_bounds__(256) void k_triples(SolveDesc d, TriplesDesc td, do
   }
   sre[threadIdx.x] = re;
   sim[threadIdx.x] = im;
+  scnt[threadIdx.x] = nel;
   __syncthreads();
   for (int s = 128; s > 0; s >>= 1) {
     if ((int)threadIdx.x < s) {
       sre[threadIdx.x] += sre[threadIdx.x + s];
       sim[threadIdx.x] += sim[threadIdx.x + s];
+      scnt[threadIdx.x] += scnt[threadIdx.x + s];
     }
     __syncthreads();
   }
-  if (threadIdx.x == 0) partial[blockIdx.x] = make_double2(sre[0], sim[0]);
+  if (threadIdx.x == 0) {
+    partial[blockIdx.x] = make_double2(sre[0], sim[0]);
+    count[blockIdx.x] = scnt[0];
+  }
+}
+
+__global__ void k_triples_count(const unsigned long long *count, int n, unsigned long long *out) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  unsigned long long s = 0;
+  for (int k = 0; k < n; ++k) s += count[k];
+  out[0] = s;
 }
 
 __global__ void k_triples_final(const double2 *partial, int n, double2 *out) {
@@ -202,10 +218,12 @@ __global__ void k_triples_final(const double2 *partial, int n, double2 *out) {
   if (threadIdx.x == 0) out[0] = make_double2(sre[0], sim[0]);
 }
 
-void launch_triples(const SolveDesc &d, const TriplesDesc &td, double volume, double2 *partial, int nblocks,
-                    double2 *out, hipStream_t st) {
-  hipLaunchKernelGGL(k_triples, dim3(nblocks), dim3(256), 0, st, d, td, volume, partial);
+void launch_triples(const SolveDesc &d, const TriplesDesc &td, double volume, double2 *partial,
+                    unsigned long long *count, int nblocks, double2 *out, unsigned long long *count_out,
+                    hipStream_t st) {
+  hipLaunchKernelGGL(k_triples, dim3(nblocks), dim3(256), 0, st, d, td, volume, partial, count);
   hipLaunchKernelGGL(k_triples_final, dim3(1), dim3(256), 0, st, partial, nblocks, out);
+  hipLaunchKernelGGL(k_triples_count, dim3(1), dim3(64), 0, st, count, nblocks, count_out);
 }
 
 }  // namespace ccgpu

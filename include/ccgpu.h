@@ -168,6 +168,10 @@ int ccgpu_update(int handle, int *count, int *nstep, int subspace, int diis_step
 int ccgpu_energy(int handle, double *ecorr);                     /* ecorr[2] */
 int ccgpu_solve(int handle, const ccgpu_solve_opts *opts, ccgpu_solve_result *res);
 int ccgpu_triples(int handle, double *et);                       /* et[2] */
+/* Of the last ccgpu_triples call: the number of (abc, ijk) elements summed
+ * (a<b<c particles, i<j<k holes, same total momentum and isospin) and its
+ * wall time in seconds (kernels and the result copy). */
+int ccgpu_triples_stats(int handle, long long *elements, double *seconds);
 
 /* Amplitudes t(ab,ij) for n orbital quadruples (0-based), any ordering. */
 int ccgpu_get_t2(int handle, int n, const int *a, const int *b, const int *i,

@@ -264,6 +264,21 @@ def test_triples_against_reference(ccnm, name):
     et = s.pert_triples2()
     ref = g["eccdt"] - g["eccd"]
     assert abs(et - ref) < 1e-12 * max(1.0, abs(ref)), (et, ref)
+    # element count: brute force over (a<b<c, i<j<k) with equal total momentum and isospin
+    o = s.orbitals()
+    A_, nsp = s.dims["nholes"], s.dims["nsp"]
+    key = {}
+    for x in range(nsp):
+        key[x] = (o["nx"][x], o["ny"][x], o["nz"][x], o["tz"][x])
+    import itertools
+    hk = {}
+    for i, j, k in itertools.combinations(range(A_), 3):
+        kk = tuple(key[i][q] + key[j][q] + key[k][q] for q in range(4))
+        hk[kk] = hk.get(kk, 0) + 1
+    want = 0
+    for a, b, c in itertools.combinations(range(A_, nsp), 3):
+        want += hk.get(tuple(key[a][q] + key[b][q] + key[c][q] for q in range(4)), 0)
+    assert s.triples_stats()[0] == want
     # own converged amplitudes: agreement at the stopping-tolerance scale
     s.setup_t_amplitudes()
     r = s.ccd_iter(subspace=sub, diis_step=step, tol=float(g["tol"]), maxit=maxit)
