@@ -18,6 +18,8 @@ LIB_PATH = os.path.join(HERE, "libccgpu.so")
 
 PNM, SNM = 0, 1
 INT_MINNESOTA = 0
+INT_CHIRAL = 1
+LO, NLO, NNLO = 0, 2, 3
 BLOCK_HHHH, BLOCK_HHHP, BLOCK_HHPP, BLOCK_HPHP, BLOCK_HPPP, BLOCK_PPPP = 1, 2, 3, 4, 5, 6
 PREC_FP64, PREC_FP32, PREC_V32 = 0, 1, 2
 
@@ -67,6 +69,8 @@ SIGNATURES = {
     "ccgpu_get_dims": [_I, C.POINTER(Dims)],
     "ccgpu_get_orbitals": [_I, _IP, _IP, _IP, _IP, _IP, _DP],
     "ccgpu_generate_interaction": [_I, _I],
+    "ccgpu_set_chiral": [_I, _I, _D, _DP],
+    "ccgpu_vnn_elements": [_I, _I, _IP, _IP, _IP, _IP, _DP],
     "ccgpu_set_block": [_I, _I, _I, _IP, _IP, _I, _IP, _IP, _DP, _I],
     "ccgpu_set_fock": [_I, _DP, _D, _D],
     "ccgpu_finalize_setup": [_I],
@@ -192,8 +196,26 @@ class Solver:
         return out
 
     # setup
+    def set_chiral(self, order=NNLO, cutoff=450.0, ci=None):
+        """Chiral order and cutoff of INT_CHIRAL (ccm_in's first line); ci
+        replaces the pi-N LECs c1..c4 (GeV^-1)."""
+        arr = None if ci is None else np.ascontiguousarray(ci, np.float64)
+        if arr is not None and arr.shape != (4,):
+            raise ValueError("ci must hold c1, c2, c3, c4")
+        _check(lib().ccgpu_set_chiral(self.h, int(order), float(cutoff),
+                                      None if arr is None else arr.ctypes.data_as(_DP)))
+
     def generate_interaction(self, kind=INT_MINNESOTA):
         _check(lib().ccgpu_generate_interaction(self.h, kind))
+
+    def vnn_elements(self, p, q, r, s):
+        """<pq||rs> of the generated interaction (0-based orbital indices)."""
+        arrs = [np.ascontiguousarray(np.ravel(x), np.int32) for x in (p, q, r, s)]
+        n = len(arrs[0])
+        out = np.zeros(n, np.complex128)
+        _check(lib().ccgpu_vnn_elements(self.h, n, *[a.ctypes.data_as(_IP) for a in arrs],
+                                        out.ctypes.data_as(_DP)))
+        return out
 
     def set_block(self, kind, bra_p, bra_q, ket_r, ket_s, val):
         """Reference-layout block: 1-based pair lists, val[nbra, nket] complex."""

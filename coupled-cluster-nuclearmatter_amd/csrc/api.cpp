@@ -22,6 +22,7 @@
 #include <vector>
 
 #include "../../include/ccgpu.h"
+#include "chiral.h"
 #include "device.h"
 #include "plan.h"
 
@@ -29,6 +30,10 @@ namespace ccgpu {
 // kernels.hip
 void launch_fill_blocks(const SolveDesc &d, double volume, int nch, int nphc, int max_npp, int max_nhh,
                         int max_nhp, hipStream_t st);
+void launch_fill_blocks_chiral(const SolveDesc &d, const ChiralDev &c, int nch, int nphc, int max_npp,
+                               int max_nhh, int max_nhp, hipStream_t st);
+void launch_elements(const SolveDesc &d, double volume, const ChiralDev *c, const int4 *pqrs, int n, double2 *out,
+                     hipStream_t st);
 void launch_derive(const SolveDesc &d, int nch, int nphc, int max_npp, int max_nhh, int max_nph, int max_nhp,
                    hipStream_t st);
 void launch_to_f32(const double2 *src, float2 *dst, int64_t n, hipStream_t st);
@@ -294,6 +299,9 @@ struct Solve {
   hipStream_t stream = nullptr;
   bool have_v = false, have_fock = false, derived = false, have_t = false;
   bool native_v = false;   // V from the built-in generator (needed by (T))
+  int vkind = -1;          // CCGPU_INT_* of the generated V
+  ChiralModel chiral = chiral_model(3, 450.0, nullptr);   // constants of CCGPU_INT_CHIRAL
+  ChiralDev chiral_dev{};  // its device tables (set by the generator)
   bool have_t3 = false;
   TriplesDesc t3{};
   double2 *t3_partial = nullptr, *t3_out = nullptr;
@@ -1308,14 +1316,50 @@ int ccgpu_get_orbitals(int handle, int *nx, int *ny, int *nz, int *sz, int *tz, 
   });
 }
 
+// Device tables of the chiral NN for this lattice: radial functions over
+// m = |dn|^2 <= 3 (4 nmax)^2 and the per-side factors over j <= 3 (2 nmax)^2.
+void make_chiral_tables(Solve &S) {
+  const Orbitals &o = S.plan.orb;
+  int nm = 0;
+  for (int x = 0; x < o.nsp; ++x) nm = std::max({nm, std::abs(o.nx[x]), std::abs(o.ny[x]), std::abs(o.nz[x])});
+  const double hbarc = 197.326968, pi = 3.14159265358979;
+  const int mmax = 3 * (4 * nm) * (4 * nm), jmax = 3 * (2 * nm) * (2 * nm);
+  const double qunit = pi * hbarc / o.box;
+  const ChiralModel &c = S.chiral;
+  std::vector<double> rad = chiral_radial_table(c, qunit, mmax);
+  std::vector<double> side = chiral_side_table(c, qunit, jmax);
+  ChiralDev &d = S.chiral_dev;
+  d.rad = dupload(rad, S.owned);
+  d.side = dupload(side, S.owned);
+  d.jmax = jmax;
+  d.order = c.order;
+  d.qunit = qunit;
+  d.scale = std::pow(hbarc, 3) / o.volume;
+  for (int i = 0; i < 3; ++i) { d.cs[i] = c.cs[i]; d.ct[i] = c.ct[i]; }
+  for (int i = 0; i < 7; ++i) d.cnlo[i] = c.cnlo[i];
+  for (int i = 0; i < 2; ++i) { d.cg0[i] = c.cg0[i]; d.cg1[i] = c.cg1[i]; }
+}
+
+int ccgpu_set_chiral(int handle, int order, double cutoff, const double *ci) {
+  return guarded([&] {
+    Solve &S = get(handle);
+    S.chiral = chiral_model(order, cutoff, ci);
+  });
+}
+
 int ccgpu_generate_interaction(int handle, int kind) {
   return guarded([&] {
     Solve &S = get(handle);
-    if (kind != CCGPU_INT_MINNESOTA) throw Error(fmt("unknown interaction kind %d", kind));
+    if (kind != CCGPU_INT_MINNESOTA && kind != CCGPU_INT_CHIRAL) throw Error(fmt("unknown interaction kind %d", kind));
     const Plan &P = S.plan;
     int max_nph = 0, max_nhp = 0;
     for (int K = 0; K < P.nphc; ++K) { max_nph = std::max(max_nph, P.nph[K]); max_nhp = std::max(max_nhp, P.nhp[K]); }
-    launch_fill_blocks(S.desc, P.orb.volume, P.nch, P.nphc, P.max_npp, P.max_nhh, max_nhp, S.stream);
+    if (kind == CCGPU_INT_CHIRAL) {
+      make_chiral_tables(S);
+      launch_fill_blocks_chiral(S.desc, S.chiral_dev, P.nch, P.nphc, P.max_npp, P.max_nhh, max_nhp, S.stream);
+    } else {
+      launch_fill_blocks(S.desc, P.orb.volume, P.nch, P.nphc, P.max_npp, P.max_nhh, max_nhp, S.stream);
+    }
     check_launch();
     launch_derive(S.desc, P.nch, P.nphc, P.max_npp, P.max_nhh, max_nph, max_nhp, S.stream);
     check_launch();
@@ -1323,6 +1367,45 @@ int ccgpu_generate_interaction(int handle, int kind) {
     HIPCHK(hipStreamSynchronize(S.stream));
     S.have_v = S.have_fock = S.derived = true;
     S.native_v = true;
+    S.vkind = kind;
+    S.have_t3 = false;
+    // V changed: an FP32 copy and cached work lists of this handle are stale
+    if (S.desc.Vpppp32) {
+      launch_to_f32(S.desc.Vpppp, S.desc.Vpppp32, P.nVpp, S.stream);
+      check_launch();
+      HIPCHK(hipStreamSynchronize(S.stream));
+    }
+  });
+}
+
+int ccgpu_vnn_elements(int handle, int n, const int *p, const int *q, const int *r, const int *s, double *v) {
+  return guarded([&] {
+    Solve &S = get(handle);
+    if (S.vkind < 0) throw Error("no generated interaction: call ccgpu_generate_interaction first");
+    if (n < 0) throw Error("negative element count");
+    if (n == 0) return;
+    const int nsp = S.plan.nsp;
+    std::vector<int4> h(n);
+    for (int x = 0; x < n; ++x) {
+      for (int y : {p[x], q[x], r[x], s[x]})
+        if (y < 0 || y >= nsp) throw Error(fmt("orbital index %d out of range", y));
+      h[x] = int4{p[x], q[x], r[x], s[x]};
+    }
+    int4 *d_in = nullptr;
+    double2 *d_out = nullptr;
+    HIPCHK(hipMalloc(&d_in, n * sizeof(int4)));
+    HIPCHK(hipMalloc(&d_out, n * sizeof(double2)));
+    HIPCHK(hipMemcpy(d_in, h.data(), n * sizeof(int4), hipMemcpyHostToDevice));
+    launch_elements(S.desc, S.plan.orb.volume, S.vkind == CCGPU_INT_CHIRAL ? &S.chiral_dev : nullptr, d_in, n,
+                    d_out, S.stream);
+    const hipError_t e = hipGetLastError();
+    if (e == hipSuccess) {
+      (void)hipStreamSynchronize(S.stream);
+      (void)hipMemcpy(v, d_out, n * sizeof(double2), hipMemcpyDeviceToHost);
+    }
+    (void)hipFree(d_in);
+    (void)hipFree(d_out);
+    if (e != hipSuccess) throw Error(std::string("kernel launch: ") + hipGetErrorString(e));
   });
 }
 
@@ -1431,6 +1514,7 @@ int ccgpu_set_block(int handle, int kind, int nbra, const int *bra_p, const int 
     }
     S.have_v = true;
     S.native_v = false;
+    S.vkind = -1;
     S.derived = false;
   });
 }
@@ -1722,7 +1806,11 @@ void build_triples(Solve &S) {
   td.vmask = 0;
   td.mtab = nullptr;
   td.mmax = 0;
-  if (S.native_v) {
+  td.chiral = 0;
+  if (S.native_v && S.vkind == CCGPU_INT_CHIRAL) {
+    td.chiral = 1;
+    td.ch = S.chiral_dev;
+  } else if (S.native_v) {
     // radial Minnesota factors over the integer momentum transfer (the
     // constants of the generator, float32 literals as in the reference)
     const double pi = 3.14159265358979;

@@ -1,5 +1,6 @@
 // Device-side descriptors shared by the kernels and the host runtime.
 #pragma once
+#include "interaction.h"
 #include <hip/hip_runtime.h>
 #include <cstdint>
 
@@ -91,6 +92,9 @@ struct TriplesDesc {
   // q^2 = (pi/L)^2 m, m = |n_p - n_q - n_r + n_s|^2 (twists cancel), 3 per m
   const double *mtab;
   int mmax;
+  // built-in chiral NN (used when chiral != 0)
+  int chiral;
+  ChiralDev ch;
 };
 
 // Warm-start network: number of parameters (flat FP32 vector, layout in

@@ -30,95 +30,103 @@ __device__ __forceinline__ double2 cdiv(double2 a, double2 b) {
   return make_double2((a.x * b.x + a.y * b.y) / d, (a.y * b.x - a.x * b.y) / d);
 }
 
-// antisymmetrized <pq||rs> = v(p,q,r,s) - v(p,q,s,r); momentum conservation holds
-// for every element the callers request.
-__device__ __forceinline__ double2 vas(const Orb &o, int p, int q, int r, int s) {
-  return make_double2(minnesota_direct(o, p, q, r, s) - minnesota_direct(o, p, q, s, r), 0.0);
-}
-
-__device__ __forceinline__ Orb orb_of(const SolveDesc &d, double volume) {
+// Interaction generators: antisymmetrized <pq||rs> = v(p,q,r,s) - v(p,q,s,r)
+// (the combination of every vnn_* block, ccm_iter.f90:854, 917, 996, 1126,
+// 1254); momentum conservation holds for every element the callers request.
+struct MinnesotaV {
   Orb o;
-  o.kx = d.kx; o.ky = d.ky; o.kz = d.kz; o.sz = d.sz; o.tz = d.tz; o.volume = volume;
-  return o;
-}
+  __device__ __forceinline__ double2 operator()(int p, int q, int r, int s) const {
+    return make_double2(minnesota_direct(o, p, q, r, s) - minnesota_direct(o, p, q, s, r), 0.0);
+  }
+};
+
+struct ChiralV {
+  ChiralDev c;
+  const int *nx, *ny, *nz, *sz, *tz;
+  __device__ __forceinline__ double2 operator()(int p, int q, int r, int s) const {
+    const double2 a = chiral_direct(c, nx, ny, nz, sz, tz, p, q, r, s);
+    const double2 b = chiral_direct(c, nx, ny, nz, sz, tz, p, q, s, r);
+    return make_double2(a.x - b.x, a.y - b.y);
+  }
+};
 
 // V blocks of setup_channel_structures (ccm_iter.f90:819-1311), unique pairs.
-__global__ void k_fill_vpppp(SolveDesc d, double volume) {
+template <class V>
+__global__ void k_fill_vpppp(SolveDesc d, V vas) {
   const int c = blockIdx.y;
-  const Orb o = orb_of(d, volume);
   const int n = d.npp[c];
   const int64_t ne = (int64_t)n * n;
   const int *pa = d.pp_a + d.off_pp[c], *pb = d.pp_b + d.off_pp[c];
   for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < ne; e += (int64_t)gridDim.x * blockDim.x) {
     int r = (int)(e / n), s = (int)(e % n);
-    d.Vpppp[d.off_vpp[c] + e] = vas(o, pa[r], pb[r], pa[s], pb[s]);
+    d.Vpppp[d.off_vpp[c] + e] = vas(pa[r], pb[r], pa[s], pb[s]);
   }
 }
 
-__global__ void k_fill_vhhpp(SolveDesc d, double volume) {
+template <class V>
+__global__ void k_fill_vhhpp(SolveDesc d, V vas) {
   const int c = blockIdx.y;
-  const Orb o = orb_of(d, volume);
   const int np_ = d.npp[c], nh = d.nhh[c];
   const int64_t ne = (int64_t)np_ * nh;
   const int *pa = d.pp_a + d.off_pp[c], *pb = d.pp_b + d.off_pp[c];
   const int *hi = d.hh_i + d.off_hh[c], *hj = d.hh_j + d.off_hh[c];
   for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < ne; e += (int64_t)gridDim.x * blockDim.x) {
     int s = (int)(e / np_), r = (int)(e % np_);
-    d.Vhhpp[d.off_t[c] + e] = vas(o, hi[s], hj[s], pa[r], pb[r]);
+    d.Vhhpp[d.off_t[c] + e] = vas(hi[s], hj[s], pa[r], pb[r]);
   }
 }
 
-__global__ void k_fill_vhhhh(SolveDesc d, double volume) {
+template <class V>
+__global__ void k_fill_vhhhh(SolveDesc d, V vas) {
   const int c = blockIdx.y;
-  const Orb o = orb_of(d, volume);
   const int nh = d.nhh[c];
   const int64_t ne = (int64_t)nh * nh;
   const int *hi = d.hh_i + d.off_hh[c], *hj = d.hh_j + d.off_hh[c];
   for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < ne; e += (int64_t)gridDim.x * blockDim.x) {
     int s = (int)(e / nh), t = (int)(e % nh);
-    d.Vhhhh[d.off_vhh[c] + e] = vas(o, hi[s], hj[s], hi[t], hj[t]);
+    d.Vhhhh[d.off_vhh[c] + e] = vas(hi[s], hj[s], hi[t], hj[t]);
   }
 }
 
 // W0((i,a),(j,b)) = -<ib||ja> (the recoupled hphp term of I4, ccm_t2_eqn.f90:740-780).
-__global__ void k_fill_w0(SolveDesc d, double volume) {
+template <class V>
+__global__ void k_fill_w0(SolveDesc d, V vas) {
   const int K = blockIdx.y;
-  const Orb o = orb_of(d, volume);
   const int n = d.nhp[K];
   const int64_t ne = (int64_t)n * n;
   const int *hj = d.hp_j + d.off_hp[K], *hb = d.hp_b + d.off_hp[K];
   for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < ne; e += (int64_t)gridDim.x * blockDim.x) {
     int r = (int)(e / n), s = (int)(e % n);
     int i = hj[r], a = hb[r], j = hj[s], b = hb[s];
-    double2 v = vas(o, i, b, j, a);
+    double2 v = vas(i, b, j, a);
     d.W0[d.off_i4[K] + e] = make_double2(-v.x, -v.y);
   }
 }
 
 // Fock matrix f_pq = t_p delta_pq + sum_h <ph||qh> for q in {p, partner(p)}
 // (normal_ordered_hamiltonian, ccm_main.f90:1586-1657).
-__global__ void k_fock(SolveDesc d, double volume) {
-  const Orb o = orb_of(d, volume);
+template <class V>
+__global__ void k_fock(SolveDesc d, V vas) {
   int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= d.nsp) return;
   double2 fd = make_double2(d.e[p], 0.0), fp = make_double2(0.0, 0.0);
   int pp = d.partner[p];
   for (int h = 0; h < d.A; ++h) {
-    fd = cadd(fd, vas(o, p, h, p, h));
-    if (pp >= 0) fp = cadd(fp, vas(o, p, h, pp, h));
+    fd = cadd(fd, vas(p, h, p, h));
+    if (pp >= 0) fp = cadd(fp, vas(p, h, pp, h));
   }
   d.fdiag[p] = fd;
   d.fpart[p] = fp;
 }
 
 // E0 = sum_i t_ii + 1/2 sum_ij <ij||ij> (ccm_main.f90:1661-1684).  One block.
-__global__ void k_e0(SolveDesc d, double volume) {
-  const Orb o = orb_of(d, volume);
+template <class V>
+__global__ void k_e0(SolveDesc d, V vas) {
   __shared__ double sre[256], sim[256];
   double re = 0.0, im = 0.0;
   for (int64_t e = threadIdx.x; e < (int64_t)d.A * d.A; e += blockDim.x) {
     int i = (int)(e / d.A), j = (int)(e % d.A);
-    double2 v = vas(o, i, j, i, j);
+    double2 v = vas(i, j, i, j);
     re += 0.5 * v.x; im += 0.5 * v.y;
   }
   for (int i = threadIdx.x; i < d.A; i += blockDim.x) re += d.e[i];
@@ -129,6 +137,18 @@ __global__ void k_e0(SolveDesc d, double volume) {
     __syncthreads();
   }
   if (threadIdx.x == 0) d.e0[0] = make_double2(sre[0], sim[0]);
+}
+
+// <pq||rs> for a list of orbital quadruples; zero where momentum or isospin
+// is not conserved.
+template <class V>
+__global__ void k_elements(SolveDesc d, V vas, const int4 *pqrs, int n, double2 *out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int4 x = pqrs[i];
+  const bool ok = d.nx[x.x] + d.nx[x.y] == d.nx[x.z] + d.nx[x.w] && d.ny[x.x] + d.ny[x.y] == d.ny[x.z] + d.ny[x.w] &&
+                  d.nz[x.x] + d.nz[x.y] == d.nz[x.z] + d.nz[x.w] && d.tz[x.x] + d.tz[x.y] == d.tz[x.z] + d.tz[x.w];
+  out[i] = ok ? vas(x.x, x.y, x.z, x.w) : make_double2(0.0, 0.0);
 }
 
 // R0(ab,ij) = conj(<ij||ab>): the V term of the residual (ccm_t2_eqn.f90:107-138),
@@ -954,15 +974,46 @@ __global__ void k_to_f32(const double2 *src, float2 *dst, int64_t n) {
 // ---------------------------------------------------------------------------
 // launch wrappers (host side)
 // ---------------------------------------------------------------------------
+template <class V>
+static void fill_blocks(const SolveDesc &d, const V &vas, int nch, int nphc, int max_npp, int max_nhh, int max_nhp,
+                        hipStream_t st) {
+  auto gx = [](int64_t n) { return (unsigned)std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, 8192)); };
+  hipLaunchKernelGGL(k_fill_vpppp<V>, dim3(gx((int64_t)max_npp * max_npp), nch), dim3(256), 0, st, d, vas);
+  hipLaunchKernelGGL(k_fill_vhhpp<V>, dim3(gx((int64_t)max_npp * max_nhh), nch), dim3(256), 0, st, d, vas);
+  hipLaunchKernelGGL(k_fill_vhhhh<V>, dim3(gx((int64_t)max_nhh * max_nhh), nch), dim3(256), 0, st, d, vas);
+  hipLaunchKernelGGL(k_fill_w0<V>, dim3(gx((int64_t)max_nhp * max_nhp), nphc), dim3(256), 0, st, d, vas);
+  hipLaunchKernelGGL(k_fock<V>, dim3((d.nsp + 255) / 256), dim3(256), 0, st, d, vas);
+  hipLaunchKernelGGL(k_e0<V>, dim3(1), dim3(256), 0, st, d, vas);
+}
+
 void launch_fill_blocks(const SolveDesc &d, double volume, int nch, int nphc, int max_npp,
                         int max_nhh, int max_nhp, hipStream_t st) {
-  auto gx = [](int64_t n) { return (unsigned)std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, 8192)); };
-  hipLaunchKernelGGL(k_fill_vpppp, dim3(gx((int64_t)max_npp * max_npp), nch), dim3(256), 0, st, d, volume);
-  hipLaunchKernelGGL(k_fill_vhhpp, dim3(gx((int64_t)max_npp * max_nhh), nch), dim3(256), 0, st, d, volume);
-  hipLaunchKernelGGL(k_fill_vhhhh, dim3(gx((int64_t)max_nhh * max_nhh), nch), dim3(256), 0, st, d, volume);
-  hipLaunchKernelGGL(k_fill_w0, dim3(gx((int64_t)max_nhp * max_nhp), nphc), dim3(256), 0, st, d, volume);
-  hipLaunchKernelGGL(k_fock, dim3((d.nsp + 255) / 256), dim3(256), 0, st, d, volume);
-  hipLaunchKernelGGL(k_e0, dim3(1), dim3(256), 0, st, d, volume);
+  MinnesotaV v;
+  v.o.kx = d.kx; v.o.ky = d.ky; v.o.kz = d.kz; v.o.sz = d.sz; v.o.tz = d.tz; v.o.volume = volume;
+  fill_blocks(d, v, nch, nphc, max_npp, max_nhh, max_nhp, st);
+}
+
+void launch_fill_blocks_chiral(const SolveDesc &d, const ChiralDev &c, int nch, int nphc, int max_npp,
+                               int max_nhh, int max_nhp, hipStream_t st) {
+  ChiralV v;
+  v.c = c;
+  v.nx = d.nx; v.ny = d.ny; v.nz = d.nz; v.sz = d.sz; v.tz = d.tz;
+  fill_blocks(d, v, nch, nphc, max_npp, max_nhh, max_nhp, st);
+}
+
+void launch_elements(const SolveDesc &d, double volume, const ChiralDev *c, const int4 *pqrs, int n, double2 *out,
+                     hipStream_t st) {
+  const dim3 g((unsigned)std::max(1, (n + 255) / 256));
+  if (c) {
+    ChiralV v;
+    v.c = *c;
+    v.nx = d.nx; v.ny = d.ny; v.nz = d.nz; v.sz = d.sz; v.tz = d.tz;
+    hipLaunchKernelGGL(k_elements<ChiralV>, g, dim3(256), 0, st, d, v, pqrs, n, out);
+  } else {
+    MinnesotaV v;
+    v.o.kx = d.kx; v.o.ky = d.ky; v.o.kz = d.kz; v.o.sz = d.sz; v.o.tz = d.tz; v.o.volume = volume;
+    hipLaunchKernelGGL(k_elements<MinnesotaV>, g, dim3(256), 0, st, d, v, pqrs, n, out);
+  }
 }
 
 void launch_derive(const SolveDesc &d, int nch, int nphc, int max_npp, int max_nhh, int max_nph,

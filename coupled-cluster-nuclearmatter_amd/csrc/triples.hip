@@ -56,7 +56,7 @@ __device__ __forceinline__ double2 t_of(const SolveDesc &d, int a, int b, int i,
 }
 
 // antisymmetrized element <pq||rs>: from the table of handed-over elements,
-// else from the built-in generator (real)
+// else from the built-in generator (chiral, or Minnesota which is real)
 __device__ __forceinline__ double2 vas_t(const SolveDesc &d, const TriplesDesc &td, double volume, int p, int q,
                                          int r, int s) {
   if (td.vkeys) {
@@ -74,6 +74,11 @@ __device__ __forceinline__ double2 vas_t(const SolveDesc &d, const TriplesDesc &
       if (k == ~0ull) return make_double2(0.0, 0.0);
       h = (h + 1) & td.vmask;
     }
+  }
+  if (td.chiral) {
+    const double2 a = chiral_direct(td.ch, d.nx, d.ny, d.nz, d.sz, d.tz, p, q, r, s);
+    const double2 b = chiral_direct(td.ch, d.nx, d.ny, d.nz, d.sz, d.tz, p, q, s, r);
+    return make_double2(a.x - b.x, a.y - b.y);
   }
   if (td.mtab) {
     const int ax = d.nx[p] - d.nx[q], ay = d.ny[p] - d.ny[q], az = d.nz[p] - d.nz[q];

@@ -6,7 +6,7 @@ module ccgpu_iface
   implicit none
 
   integer(c_int), parameter :: CCGPU_PNM = 0, CCGPU_SNM = 1
-  integer(c_int), parameter :: CCGPU_INT_MINNESOTA = 0
+  integer(c_int), parameter :: CCGPU_INT_MINNESOTA = 0, CCGPU_INT_CHIRAL = 1
   integer(c_int), parameter :: CCGPU_PREC_FP64 = 0, CCGPU_PREC_FP32 = 1, CCGPU_PREC_V32 = 2
   integer(c_int), parameter :: CCGPU_BLOCK_HHHH = 1, CCGPU_BLOCK_HHHP = 2, CCGPU_BLOCK_HHPP = 3, &
        CCGPU_BLOCK_HPHP = 4, CCGPU_BLOCK_HPPP = 5, CCGPU_BLOCK_PPPP = 6
@@ -63,6 +63,12 @@ module ccgpu_iface
        import :: c_int
        integer(c_int), value :: h, kind
      end function ccgpu_generate_interaction
+     integer(c_int) function ccgpu_set_chiral(h, order, cutoff, ci) bind(C, name='ccgpu_set_chiral')
+       import :: c_int, c_double, c_ptr
+       integer(c_int), value :: h, order
+       real(c_double), value :: cutoff
+       type(c_ptr), value :: ci
+     end function ccgpu_set_chiral
      integer(c_int) function ccgpu_get_e0(h, e0) bind(C, name='ccgpu_get_e0')
        import :: c_int, c_double_complex
        integer(c_int), value :: h

@@ -9,7 +9,9 @@
 !
 ! One optional line after the reference deck sets engine options:
 !   ! interaction, DIIS subspace, DIIS step, tolerance, max iterations, precision, fp32 switch
-!   minnesota, 6, 6, 1e-6, 100, fp64, 1.0
+!   chiral, 6, 6, 1e-6, 100, fp64, 1.0
+! The interaction is chiral (the Delta-full chiral NN of the deck's first line,
+! chiral order and cutoff, as the reference) or minnesota.
 ! With precision fp32 (or v32: FP32-stored pppp elements, FP64 arithmetic) the contractions run in FP32 up to the first DIIS
 ! extrapolation that follows an iteration with |dE| <= the switch (MeV; 0: never
 ! there) or until the FP32 |dE| meets the tolerance, and in FP64 afterwards
@@ -55,10 +57,8 @@ program ccm_gpu
   if (ios == 0) read(5,*,iostat=ios) interaction, subspace, diis_step, tol4, maxit, precname, fp32_switch
   close(5)
 
-  if (trim(interaction) /= 'minnesota') then
-     write(0,'(a)') 'ccm_gpu: only the Minnesota interaction generator is built into this engine;'
-     write(0,'(a)') 'add the engine option line "minnesota, 6, 6, 1e-6, 100, fp64, 1.0" to the deck,'
-     write(0,'(a)') 'or hand over chiral blocks through ccgpu_set_block.'
+  if (trim(interaction) /= 'minnesota' .and. trim(interaction) /= 'chiral') then
+     write(0,'(2a)') 'ccm_gpu: unknown interaction ', trim(interaction)
      error stop 3
   end if
   if (tnf_switch) then
@@ -109,7 +109,13 @@ program ccm_gpu
   write(6,*) 'Total number of states', dims%nsp
   write(6,*) 'channels', dims%nchannels, 'ph channels', dims%nph_channels, 't2 unique', dims%t2_unique
 
-  call ccgpu_check(ccgpu_generate_interaction(h, CCGPU_INT_MINNESOTA), 'ccgpu_generate_interaction')
+  if (trim(interaction) == 'chiral') then
+     ! delta_chiral_order and lambda_delta of the deck (ccm_main.f90:31, init_chp_constants)
+     call ccgpu_check(ccgpu_set_chiral(h, int(order, c_int), real(lambda, c_double), c_null_ptr), 'ccgpu_set_chiral')
+     call ccgpu_check(ccgpu_generate_interaction(h, CCGPU_INT_CHIRAL), 'ccgpu_generate_interaction')
+  else
+     call ccgpu_check(ccgpu_generate_interaction(h, CCGPU_INT_MINNESOTA), 'ccgpu_generate_interaction')
+  end if
   call ccgpu_check(ccgpu_get_e0(h, e0), 'ccgpu_get_e0')
   write(6,*) 'Vacuum expectation value', e0
 

@@ -20,7 +20,20 @@
  *   ccgpu_generate_interaction        the V-block loops of setup_channel_structures
  *                                     (ccm_iter.f90:819-1311) plus
  *                                     normal_ordered_hamiltonian Fock/E0
- *                                     (ccm_main.f90:1586-1684)
+ *                                     (ccm_main.f90:1586-1684), with the
+ *                                     elements of chiral_NN_with_delta or
+ *                                     vmom_minnesota
+ *                                     (chiral_module_andreas_with_delta.f90:
+ *                                     6304-6668, 8980-9111)
+ *   ccgpu_vnn_elements                the antisymmetrized element calls
+ *                                     chiral_NN_with_delta(p,q,r,s) -
+ *                                     chiral_NN_with_delta(p,q,s,r)
+ *                                     (ccm_iter.f90:854, 2564)
+ *   ccgpu_set_chiral                  delta_chiral_order / lambda_delta of
+ *                                     ccm_in (ccm_main.f90:31) and
+ *                                     init_chp_constants
+ *                                     (chiral_module_andreas_with_delta.f90:
+ *                                     122-808)
  *   ccgpu_set_block / ccgpu_set_fock  hand-over of the host's vnn_* blocks and
  *                                     fock_mtx/e0 (ccm_modules.f90:1167-1171, 528)
  *   ccgpu_mbpt2_start                 setup_t_amplitudes (ccm_iter.f90:2329-2405)
@@ -53,6 +66,7 @@ extern "C" {
 
 /* interaction generators for ccgpu_generate_interaction */
 #define CCGPU_INT_MINNESOTA 0
+#define CCGPU_INT_CHIRAL 1     /* Delta-full chiral NN, order/cutoff of ccgpu_set_chiral */
 
 /* block kinds for ccgpu_set_block (reference block names) */
 #define CCGPU_BLOCK_HHHH 1
@@ -145,6 +159,16 @@ int ccgpu_get_orbitals(int handle, int *nx, int *ny, int *nz, int *sz, int *tz, 
 
 /* Fill every V block, the Fock matrix and E0 on the GPU. */
 int ccgpu_generate_interaction(int handle, int kind);
+
+/* Chiral order (0 LO, 2 NLO, 3 NNLO) and regulator cutoff (394, 450 or 500
+ * MeV) of CCGPU_INT_CHIRAL, as the first line of ccm_in; ci = null keeps the
+ * pi-N LECs of that order, else 4 values c1..c4 in GeV^-1 replace them.  The
+ * default of a new handle is NNLO at 450 MeV. */
+int ccgpu_set_chiral(int handle, int order, double cutoff, const double *ci);
+
+/* <pq||rs> of the generated interaction for n orbital quadruples (0-based),
+ * complex into v[2n]; zero where momentum or isospin is not conserved. */
+int ccgpu_vnn_elements(int handle, int n, const int *p, const int *q, const int *r, const int *s, double *v);
 
 /* Hand over one reference-layout block: rows are pairs (bra_p[r], bra_q[r]),
  * columns pairs (ket_r[c], ket_s[c]), 1-based orbital indices as in the

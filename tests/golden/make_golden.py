@@ -22,7 +22,7 @@ ROOT = os.path.dirname(os.path.dirname(HERE))
 DRIVER = os.path.join(ROOT, "oracle", "_ref", "ref_driver")
 
 # name: (A, matter, rho, nmax, order, interaction, subspace, step, tol, maxit,
-#        twist, dump, triples, ranks)
+#        twist, dump, triples, ranks[, cutoff MeV, default 450])
 CASES = {
     "pnm14_n1_nnlo": (14, "pnm", 0.16, 1, 3, "chiral", 6, 6, 1e-6, 100, (0, 0, 0), 1, 1, 1),
     "pnm14_n1_minn": (14, "pnm", 0.16, 1, 3, "minnesota", 8, 8, 1e-8, 100, (0, 0, 0), 1, 1, 1),
@@ -36,10 +36,16 @@ CASES = {
                             (0.7, 1.9, 2.6), 0, 0, 1),
     "pnm66_n2_nnlo": (66, "pnm", 0.16, 2, 3, "chiral", 8, 8, 1e-8, 100, (0, 0, 0), 0, 0, 8),
     "snm28_n2_minn": (28, "snm", 0.16, 2, 3, "minnesota", 8, 8, 1e-11, 300, (0, 0, 0), 0, 0, 1),
+    # chiral orders and cutoffs of the Delta-full family (V records only)
+    "snm4_n1_lo450": (4, "snm", 0.16, 1, 0, "chiral", 6, 6, 1e-6, 20, (0, 0, 0), 1, 0, 1),
+    "snm4_n1_nlo450": (4, "snm", 0.16, 1, 2, "chiral", 6, 6, 1e-6, 20, (0, 0, 0), 1, 0, 1),
+    "snm4_n1_nnlo394": (4, "snm", 0.16, 1, 3, "chiral", 6, 6, 1e-6, 20, (0, 0, 0), 1, 0, 1, 394),
+    "snm4_n1_nnlo500": (4, "snm", 0.16, 1, 3, "chiral", 6, 6, 1e-6, 20, (0, 0, 0), 1, 0, 1, 500),
+    "pnm14_n1_nnlo_twist": (14, "pnm", 0.12, 1, 3, "chiral", 6, 6, 1e-6, 100, (0.4, 1.3, 2.2), 1, 1, 1),
 }
 
 CCM_IN = """!Chiral order for Deltas(LO = 0,NLO=2,NNLO=3,N3LO=4) and cutoff
-{order}, 450
+{order}, {cutoff}
 ! cE and cD 3nf parameters:
  0.017, 0.79
 ! number of particles
@@ -60,11 +66,12 @@ F, 0
 
 
 def run_case(name):
-    (A, matter, rho, nmax, order, inter, sub, step, tol, maxit, tw, dump, trip, ranks) = CASES[name]
+    (A, matter, rho, nmax, order, inter, sub, step, tol, maxit, tw, dump, trip, ranks) = CASES[name][:14]
+    cutoff = CASES[name][14] if len(CASES[name]) > 14 else 450
     with tempfile.TemporaryDirectory() as tmp:
         inp = os.path.join(tmp, "ccm_in")
         with open(inp, "w") as fh:
-            fh.write(CCM_IN.format(order=order, A=A, matter=matter, rho=rho, nmax=nmax))
+            fh.write(CCM_IN.format(order=order, cutoff=cutoff, A=A, matter=matter, rho=rho, nmax=nmax))
         cmd = [DRIVER, inp, tmp, inter, str(sub), str(step), repr(tol), str(maxit),
                repr(tw[0]), repr(tw[1]), repr(tw[2]), str(dump), str(trip)]
         if ranks > 1:
@@ -74,6 +81,7 @@ def run_case(name):
         if res.returncode != 0:
             raise RuntimeError(f"{name}: reference driver failed\n{res.stdout[-2000:]}\n{res.stderr[-2000:]}")
         out = {"params": np.array([A, nmax, sub, step, maxit, order], np.int64),
+               "cutoff": np.float64(cutoff),
                "matter": np.array(matter), "interaction": np.array(inter),
                "rho": np.float64(rho), "tol": np.float64(np.float32(tol)),
                "twist": np.array(tw, np.float64)}
