@@ -172,6 +172,7 @@ struct GemmList {
   std::vector<int64_t> poff;     // partial-buffer offset per group
   int64_t psize = 0, maxmn = 0;
   bool xcd = false;              // order tiles channel-per-XCD
+  bool ready = false;            // shapes chosen, tiles built and uploaded
   GemmGroup *d_groups = nullptr;
   GemmTile *d_tiles[NCLS] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
   GemmTile *d_tiles64 = nullptr;
@@ -689,16 +690,12 @@ std::unique_ptr<Batch> make_batch(const std::vector<Solve *> &ss, hipStream_t st
     else B->has_f32 = false;
   }
   const int fams[6] = {0, 1, 2, 3, 4, 1};
-  const int kinds[6] = {0, 3, 0, 0, 0, 4};   // the kernel kind each list is tuned for
   GemmList *lists[6] = {&B->g_x, &B->g_pp, &B->g_hh, &B->g_i4, &B->g_rt, &B->g_pp32};
   B->d_descs = dupload(B->h_descs, B->owned);
   for (int k = 0; k < 6; ++k) {
     lists[k]->fam = fams[k];
     lists[k]->xcd = fams[k] == 1;
     lists[k]->apply_env_shapes();
-    if (k < 5 || B->has_f32) autotune(*lists[k], kinds[k], B->d_descs, st);
-    lists[k]->build_tiles();
-    lists[k]->upload(B->owned);
   }
   HIPCHK(hipStreamCreateWithFlags(&B->stream2, hipStreamNonBlocking));
   HIPCHK(hipEventCreateWithFlags(&B->ev_fork, hipEventDisableTiming));
@@ -707,6 +704,24 @@ std::unique_ptr<Batch> make_batch(const std::vector<Solve *> &ss, hipStream_t st
   B->d_slots = dalloc<int>((size_t)n * 16, B->owned);
   B->d_coef = dalloc<double2>((size_t)n * 16, B->owned);
   return B;
+}
+
+// Shapes, tiles and device copies of the work lists a residual in precision
+// prec needs, made on first use (outside any graph capture): lists a batch
+// never contracts (a single-solve batch used only for its MBPT2 start or
+// energy, the FP32 ladder of an FP64 solve) are never tuned.
+void prepare_lists(Batch &B, int prec) {
+  const int kinds[6] = {0, 3, 0, 0, 0, 4};   // the kernel kind each list is tuned for
+  GemmList *lists[6] = {&B.g_x, &B.g_pp, &B.g_hh, &B.g_i4, &B.g_rt, &B.g_pp32};
+  for (int k = 0; k < 6; ++k) {
+    GemmList &L = *lists[k];
+    if (L.ready) continue;
+    if (k == 5 && (prec == CCGPU_PREC_FP64 || !B.has_f32)) continue;
+    autotune(L, kinds[k], B.d_descs, B.stream);
+    L.build_tiles();
+    L.upload(B.owned);
+    L.ready = true;
+  }
 }
 
 void set_active(Batch &B, int s, int active) {
@@ -937,6 +952,7 @@ void run_batch(Batch &B, const ccgpu_solve_opts &op, std::vector<ccgpu_solve_res
     for (Solve *S : B.solves)
       if (!S->desc.Vpppp32) throw Error("FP32/V32 ladder requested before the FP32 copy of Vpppp was made");
   }
+  prepare_lists(B, op.precision);
   auto e1 = energies(B, false);
   res.assign(n, ccgpu_solve_result{});
   std::vector<std::complex<double>> dener(n, 4.0), elast = e1;
@@ -1611,6 +1627,7 @@ int ccgpu_residual(int handle, int precision) {
       throw Error(fmt("unknown precision %d", precision));
     if (precision != CCGPU_PREC_FP64) ensure_f32(S, handle);
     Batch &B = single(handle);
+    prepare_lists(B, precision);
     residual_launches(B, precision, false);
     check_launch();
     HIPCHK(hipStreamSynchronize(B.stream));
