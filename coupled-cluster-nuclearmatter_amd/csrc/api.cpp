@@ -55,6 +55,8 @@ void launch_gemm_valu(const GemmGroup *groups, const GemmTile *tiles, int ntiles
                       const int *gsolve, hipStream_t st);
 void launch_splitk_reduce(const GemmGroup *groups, const int *gidx, int n, int64_t maxmn, const SolveDesc *dd,
                           const int *gsolve, hipStream_t st);
+void launch_t3_tables(const SolveDesc &d, const TriplesDesc &td, double volume, double2 *hppp, double2 *hhhp,
+                      hipStream_t st);
 void launch_triples(const SolveDesc &d, const TriplesDesc &td, double volume, double2 *partial,
                     unsigned long long *count, int nblocks, double2 *out, unsigned long long *count_out,
                     hipStream_t st);
@@ -1865,6 +1867,23 @@ void build_triples(Solve &S) {
     td.vkeys = dupload(keys, S.owned);
     td.vvals = dupload(vals, S.owned);
     td.vmask = cap - 1;
+  }
+  // dense <cb||kd> and <jk||lc> tables, evaluated once per interaction, when
+  // they fit in CCGPU_T3_TABLE_GB (default 16 GB); otherwise the kernel
+  // evaluates the elements in place
+  td.vhppp = nullptr;
+  td.vhhhp = nullptr;
+  {
+    const double lim = (getenv("CCGPU_T3_TABLE_GB") ? atof(getenv("CCGPU_T3_TABLE_GB")) : 16.0) * 1e9;
+    const int64_t nh = (int64_t)P.np * P.np * P.A * 2, nl = (int64_t)P.A * P.A * P.np * 2;
+    if ((double)(nh + nl) * sizeof(double2) <= lim && nh > 0 && nl > 0) {
+      double2 *hppp = dalloc<double2>(nh, S.owned), *hhhp = dalloc<double2>(nl, S.owned);
+      launch_t3_tables(S.desc, td, P.orb.volume, hppp, hhhp, S.stream);
+      check_launch();
+      HIPCHK(hipStreamSynchronize(S.stream));
+      td.vhppp = hppp;
+      td.vhhhp = hhhp;
+    }
   }
   S.t3_partial = dalloc<double2>(8192, S.owned);
   S.t3_out = dalloc<double2>(1, S.owned);

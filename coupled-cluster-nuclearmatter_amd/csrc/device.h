@@ -95,6 +95,11 @@ struct TriplesDesc {
   // built-in chiral NN (used when chiral != 0)
   int chiral;
   ChiralDev ch;
+  // dense element tables (null: elements evaluated in place): vhppp holds
+  // <cb||kd> at ((c-A) np + (b-A)) A + k, times 2 for the spin of d (d's
+  // momentum and isospin follow from conservation); vhhhp holds <jk||lc> at
+  // (j A + k) np + (c-A), times 2 for the spin of l
+  const double2 *vhppp, *vhhhp;
 };
 
 // Warm-start network: number of parameters (flat FP32 vector, layout in

@@ -94,17 +94,22 @@ __device__ __forceinline__ double2 vas_t(const SolveDesc &d, const TriplesDesc &
   return make_double2(minnesota_direct_t(d, volume, p, q, r, s) - minnesota_direct_t(d, volume, p, q, s, r), 0.0);
 }
 
-// F(a,b,c,i,j,k) = sum_d conj(<cb||kd>) t(ad,ij)  (three_body_oneparticle_1)
+// F(a,b,c,i,j,k) = sum_d conj(<cb||kd>) t(ad,ij)  (three_body_oneparticle_1);
+// TAB: elements from the dense tables, else evaluated in place
+template <bool TAB>
 __device__ double2 f_term(const SolveDesc &d, const TriplesDesc &td, double vol, int a, int b, int c, int i, int j,
                           int k) {
   double2 s = make_double2(0.0, 0.0);
   const int nx = d.nx[i] + d.nx[j] - d.nx[a], ny = d.ny[i] + d.ny[j] - d.ny[a], nz = d.nz[i] + d.nz[j] - d.nz[a];
   const int tz = d.tz[i] + d.tz[j] - d.tz[a];
+  const int64_t row = TAB ? ((int64_t)(c - d.A) * d.np + (b - d.A)) * d.A + k : 0;
   for (int sz = -1; sz <= 1; sz += 2) {
     int dd = orb_lookup(td, nx, ny, nz, sz, tz);
     if (dd < d.A) continue;
     const double2 t = t_of(d, a, dd, i, j);
-    const double2 v = vas_t(d, td, vol, c, b, k, dd);   // conj(v) t
+    double2 v;   // conj(v) t
+    if constexpr (TAB) v = td.vhppp[2 * row + (sz + 1) / 2];
+    else v = vas_t(d, td, vol, c, b, k, dd);
     s.x += v.x * t.x + v.y * t.y;
     s.y += v.x * t.y - v.y * t.x;
   }
@@ -112,22 +117,73 @@ __device__ double2 f_term(const SolveDesc &d, const TriplesDesc &td, double vol,
 }
 
 // H(a,b,c,i,j,k) = sum_l conj(<jk||lc>) t(ab,il)  (three_body_onehole_1)
+template <bool TAB>
 __device__ double2 h_term(const SolveDesc &d, const TriplesDesc &td, double vol, int a, int b, int c, int i, int j,
                           int k) {
   double2 s = make_double2(0.0, 0.0);
   const int nx = d.nx[k] + d.nx[j] - d.nx[c], ny = d.ny[k] + d.ny[j] - d.ny[c], nz = d.nz[k] + d.nz[j] - d.nz[c];
   const int tz = d.tz[k] + d.tz[j] - d.tz[c];
+  const int64_t row = TAB ? ((int64_t)j * d.A + k) * d.np + (c - d.A) : 0;
   for (int sz = -1; sz <= 1; sz += 2) {
     int l = orb_lookup(td, nx, ny, nz, sz, tz);
     if (l < 0 || l >= d.A) continue;
     const double2 t = t_of(d, a, b, i, l);
-    const double2 v = vas_t(d, td, vol, j, k, l, c);    // conj(v) t
+    double2 v;   // conj(v) t
+    if constexpr (TAB) v = td.vhhhp[2 * row + (sz + 1) / 2];
+    else v = vas_t(d, td, vol, j, k, l, c);
     s.x += v.x * t.x + v.y * t.y;
     s.y += v.x * t.y - v.y * t.x;
   }
   return s;
 }
 
+// Dense element tables for (T): every <cb||kd> (particles c, b, hole k, the
+// particle d of either spin that conserves momentum and isospin) and every
+// <jk||lc>, evaluated once by the same element source as the in-place path.
+__global__ void k_t3_fill_hppp(SolveDesc d, TriplesDesc td, double vol, double2 *out) {
+  const int64_t n = (int64_t)d.np * d.np * d.A * 2;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x) {
+    const int sd = (int)(e & 1) * 2 - 1;
+    const int64_t r = e >> 1;
+    const int k = (int)(r % d.A);
+    const int64_t cb = r / d.A;
+    const int b = (int)(cb % d.np) + d.A, c = (int)(cb / d.np) + d.A;
+    const int dd = orb_lookup(td, d.nx[c] + d.nx[b] - d.nx[k], d.ny[c] + d.ny[b] - d.ny[k],
+                              d.nz[c] + d.nz[b] - d.nz[k], sd, d.tz[c] + d.tz[b] - d.tz[k]);
+    out[e] = dd >= d.A ? vas_t(d, td, vol, c, b, k, dd) : make_double2(0.0, 0.0);
+  }
+}
+
+__global__ void k_t3_fill_hhhp(SolveDesc d, TriplesDesc td, double vol, double2 *out) {
+  const int64_t n = (int64_t)d.A * d.A * d.np * 2;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x) {
+    const int sl = (int)(e & 1) * 2 - 1;
+    const int64_t r = e >> 1;
+    const int c = (int)(r % d.np) + d.A;
+    const int64_t jk = r / d.np;
+    const int k = (int)(jk % d.A), j = (int)(jk / d.A);
+    const int l = orb_lookup(td, d.nx[j] + d.nx[k] - d.nx[c], d.ny[j] + d.ny[k] - d.ny[c],
+                             d.nz[j] + d.nz[k] - d.nz[c], sl, d.tz[j] + d.tz[k] - d.tz[c]);
+    out[e] = (l >= 0 && l < d.A) ? vas_t(d, td, vol, j, k, l, c) : make_double2(0.0, 0.0);
+  }
+}
+
+// The 18 signed terms of W (pert_triples2, ccm_triples.f90:611-700): kind
+// (0 F, 1 H), sign, the particle order as indices into (a, b, c) and the hole
+// order as indices into (i, j, k).
+struct TermCode {
+  signed char kind, sign, p0, p1, p2, h0, h1, h2;
+};
+__constant__ TermCode kTerms[18] = {
+    {0, 1, 0, 1, 2, 0, 1, 2},  {0, -1, 1, 0, 2, 0, 1, 2}, {0, -1, 2, 1, 0, 0, 1, 2}, {0, -1, 0, 1, 2, 2, 1, 0},
+    {0, -1, 0, 1, 2, 0, 2, 1}, {0, 1, 2, 1, 0, 0, 2, 1},  {0, 1, 2, 1, 0, 2, 1, 0},  {0, 1, 1, 0, 2, 0, 2, 1},
+    {0, 1, 1, 0, 2, 2, 1, 0},  {1, -1, 0, 1, 2, 0, 1, 2}, {1, 1, 0, 2, 1, 0, 1, 2},  {1, 1, 2, 1, 0, 0, 1, 2},
+    {1, 1, 0, 1, 2, 2, 1, 0},  {1, 1, 0, 1, 2, 1, 0, 2},  {1, -1, 2, 1, 0, 1, 0, 2}, {1, -1, 2, 1, 0, 2, 1, 0},
+    {1, -1, 0, 2, 1, 1, 0, 2}, {1, -1, 0, 2, 1, 2, 1, 0}};
+
+__device__ __forceinline__ int sel3(int s, int x0, int x1, int x2) { return s == 0 ? x0 : (s == 1 ? x1 : x2); }
+
+template <bool TAB>
 __global__ __launch_bounds__(256) void k_triples(SolveDesc d, TriplesDesc td, double volume, double2 *partial,
                                                  unsigned long long *count) {
   __shared__ double sre[256], sim[256];
@@ -150,24 +206,37 @@ __global__ __launch_bounds__(256) void k_triples(SolveDesc d, TriplesDesc td, do
         const int i = td.hijk[3 * h], j = td.hijk[3 * h + 1], k = td.hijk[3 * h + 2];
         double2 w = make_double2(0.0, 0.0);
         auto add = [&](double sg, double2 v) { w.x += sg * v.x; w.y += sg * v.y; };
-        add(1, f_term(d, td, volume, a, b, c, i, j, k));
-        add(-1, f_term(d, td, volume, b, a, c, i, j, k));
-        add(-1, f_term(d, td, volume, c, b, a, i, j, k));
-        add(-1, f_term(d, td, volume, a, b, c, k, j, i));
-        add(-1, f_term(d, td, volume, a, b, c, i, k, j));
-        add(1, f_term(d, td, volume, c, b, a, i, k, j));
-        add(1, f_term(d, td, volume, c, b, a, k, j, i));
-        add(1, f_term(d, td, volume, b, a, c, i, k, j));
-        add(1, f_term(d, td, volume, b, a, c, k, j, i));
-        add(-1, h_term(d, td, volume, a, b, c, i, j, k));
-        add(1, h_term(d, td, volume, a, c, b, i, j, k));
-        add(1, h_term(d, td, volume, c, b, a, i, j, k));
-        add(1, h_term(d, td, volume, a, b, c, k, j, i));
-        add(1, h_term(d, td, volume, a, b, c, j, i, k));
-        add(-1, h_term(d, td, volume, c, b, a, j, i, k));
-        add(-1, h_term(d, td, volume, c, b, a, k, j, i));
-        add(-1, h_term(d, td, volume, a, c, b, j, i, k));
-        add(-1, h_term(d, td, volume, a, c, b, k, j, i));
+        if constexpr (TAB) {
+          // the same 18 terms in the same order, one at a time (fewer live
+          // registers than the unrolled form)
+#pragma unroll 1
+          for (int t = 0; t < 18; ++t) {
+            const TermCode tc = kTerms[t];
+            const int x = sel3(tc.p0, a, b, c), y = sel3(tc.p1, a, b, c), z = sel3(tc.p2, a, b, c);
+            const int u = sel3(tc.h0, i, j, k), v = sel3(tc.h1, i, j, k), q = sel3(tc.h2, i, j, k);
+            add((double)tc.sign, tc.kind == 0 ? f_term<true>(d, td, volume, x, y, z, u, v, q)
+                                              : h_term<true>(d, td, volume, x, y, z, u, v, q));
+          }
+        } else {
+        add(1, f_term<TAB>(d, td, volume, a, b, c, i, j, k));
+        add(-1, f_term<TAB>(d, td, volume, b, a, c, i, j, k));
+        add(-1, f_term<TAB>(d, td, volume, c, b, a, i, j, k));
+        add(-1, f_term<TAB>(d, td, volume, a, b, c, k, j, i));
+        add(-1, f_term<TAB>(d, td, volume, a, b, c, i, k, j));
+        add(1, f_term<TAB>(d, td, volume, c, b, a, i, k, j));
+        add(1, f_term<TAB>(d, td, volume, c, b, a, k, j, i));
+        add(1, f_term<TAB>(d, td, volume, b, a, c, i, k, j));
+        add(1, f_term<TAB>(d, td, volume, b, a, c, k, j, i));
+        add(-1, h_term<TAB>(d, td, volume, a, b, c, i, j, k));
+        add(1, h_term<TAB>(d, td, volume, a, c, b, i, j, k));
+        add(1, h_term<TAB>(d, td, volume, c, b, a, i, j, k));
+        add(1, h_term<TAB>(d, td, volume, a, b, c, k, j, i));
+        add(1, h_term<TAB>(d, td, volume, a, b, c, j, i, k));
+        add(-1, h_term<TAB>(d, td, volume, c, b, a, j, i, k));
+        add(-1, h_term<TAB>(d, td, volume, c, b, a, k, j, i));
+        add(-1, h_term<TAB>(d, td, volume, a, c, b, j, i, k));
+        add(-1, h_term<TAB>(d, td, volume, a, c, b, k, j, i));
+        }
         // conj(W) W / D with complex D
         const double dre = d.fdiag[i].x + d.fdiag[j].x + d.fdiag[k].x - fa - fb - fc;
         const double dim = d.fdiag[i].y + d.fdiag[j].y + d.fdiag[k].y - fai - fbi - fci;
@@ -223,10 +292,19 @@ __global__ void k_triples_final(const double2 *partial, int n, double2 *out) {
   if (threadIdx.x == 0) out[0] = make_double2(sre[0], sim[0]);
 }
 
+void launch_t3_tables(const SolveDesc &d, const TriplesDesc &td, double volume, double2 *hppp, double2 *hhhp,
+                      hipStream_t st) {
+  hipLaunchKernelGGL(k_t3_fill_hppp, dim3(8192), dim3(256), 0, st, d, td, volume, hppp);
+  hipLaunchKernelGGL(k_t3_fill_hhhp, dim3(1024), dim3(256), 0, st, d, td, volume, hhhp);
+}
+
 void launch_triples(const SolveDesc &d, const TriplesDesc &td, double volume, double2 *partial,
                     unsigned long long *count, int nblocks, double2 *out, unsigned long long *count_out,
                     hipStream_t st) {
-  hipLaunchKernelGGL(k_triples, dim3(nblocks), dim3(256), 0, st, d, td, volume, partial, count);
+  if (td.vhppp && td.vhhhp)
+    hipLaunchKernelGGL(k_triples<true>, dim3(nblocks), dim3(256), 0, st, d, td, volume, partial, count);
+  else
+    hipLaunchKernelGGL(k_triples<false>, dim3(nblocks), dim3(256), 0, st, d, td, volume, partial, count);
   hipLaunchKernelGGL(k_triples_final, dim3(1), dim3(256), 0, st, partial, nblocks, out);
   hipLaunchKernelGGL(k_triples_count, dim3(1), dim3(64), 0, st, count, nblocks, count_out);
 }
