@@ -71,6 +71,7 @@ SIGNATURES = {
     "ccgpu_generate_interaction": [_I, _I],
     "ccgpu_set_chiral": [_I, _I, _D, _DP],
     "ccgpu_vnn_elements": [_I, _I, _IP, _IP, _IP, _IP, _DP],
+    "ccgpu_set_3nf": [_I, _I, _D, _D, _D, _I],
     "ccgpu_set_block": [_I, _I, _I, _IP, _IP, _I, _IP, _IP, _DP, _I],
     "ccgpu_set_fock": [_I, _DP, _D, _D],
     "ccgpu_finalize_setup": [_I],
@@ -204,6 +205,11 @@ class Solver:
             raise ValueError("ci must hold c1, c2, c3, c4")
         _check(lib().ccgpu_set_chiral(self.h, int(order), float(cutoff),
                                       None if arr is None else arr.ctypes.data_as(_DP)))
+
+    def set_3nf(self, approx=2, cE=0.017, cD=0.79, cutoff=450.0, nexp=3):
+        """NNLO three-nucleon force of INT_CHIRAL, normal-ordered to zero-,
+        one- or two-body (approx 0/1/2; -1 off), as tnf_approx of ccm_in."""
+        _check(lib().ccgpu_set_3nf(self.h, int(approx), float(cE), float(cD), float(cutoff), int(nexp)))
 
     def generate_interaction(self, kind=INT_MINNESOTA):
         _check(lib().ccgpu_generate_interaction(self.h, kind))

@@ -34,6 +34,8 @@ void launch_fill_blocks_chiral(const SolveDesc &d, const ChiralDev &c, int nch, 
                                int max_nhh, int max_nhp, hipStream_t st);
 void launch_elements(const SolveDesc &d, double volume, const ChiralDev *c, const int4 *pqrs, int n, double2 *out,
                      hipStream_t st);
+void launch_3nf(const SolveDesc &d, const Chiral3Dev &c, int approx, int nch, int nphc, int max_npp, int max_nhh,
+                int max_nhp, hipStream_t st);
 void launch_derive(const SolveDesc &d, int nch, int nphc, int max_npp, int max_nhh, int max_nph, int max_nhp,
                    hipStream_t st);
 void launch_to_f32(const double2 *src, float2 *dst, int64_t n, hipStream_t st);
@@ -305,6 +307,12 @@ struct Solve {
   int vkind = -1;          // CCGPU_INT_* of the generated V
   ChiralModel chiral = chiral_model(3, 450.0, nullptr);   // constants of CCGPU_INT_CHIRAL
   ChiralDev chiral_dev{};  // its device tables (set by the generator)
+  // three-nucleon force of CCGPU_INT_CHIRAL: -1 off, else the normal-ordering
+  // level (0 E0, 1 and Fock, 2 and the two-body blocks) and its constants
+  int tnf = -1;
+  double tnf_ce = 0.017, tnf_cd = 0.79, tnf_cutoff = 450.0;
+  int tnf_nexp = 3;
+  Chiral3Dev chiral3{};
   bool have_t3 = false;
   TriplesDesc t3{};
   double2 *t3_partial = nullptr, *t3_out = nullptr;
@@ -1365,6 +1373,19 @@ int ccgpu_set_chiral(int handle, int order, double cutoff, const double *ci) {
   });
 }
 
+int ccgpu_set_3nf(int handle, int approx, double cE, double cD, double cutoff, int nexp) {
+  return guarded([&] {
+    Solve &S = get(handle);
+    if (approx < -1 || approx > 2) throw Error(fmt("3NF approximation %d: -1 off, 0, 1 or 2", approx));
+    if (approx >= 0 && (cutoff <= 0.0 || nexp < 1)) throw Error("3NF regulator needs cutoff > 0 and nexp >= 1");
+    S.tnf = approx;
+    S.tnf_ce = cE;
+    S.tnf_cd = cD;
+    S.tnf_cutoff = cutoff;
+    S.tnf_nexp = nexp;
+  });
+}
+
 int ccgpu_generate_interaction(int handle, int kind) {
   return guarded([&] {
     Solve &S = get(handle);
@@ -1372,9 +1393,20 @@ int ccgpu_generate_interaction(int handle, int kind) {
     const Plan &P = S.plan;
     int max_nph = 0, max_nhp = 0;
     for (int K = 0; K < P.nphc; ++K) { max_nph = std::max(max_nph, P.nph[K]); max_nhp = std::max(max_nhp, P.nhp[K]); }
+    if (kind != CCGPU_INT_CHIRAL && S.tnf >= 0)
+      throw Error("the three-nucleon force belongs to the chiral interaction (CCGPU_INT_CHIRAL)");
     if (kind == CCGPU_INT_CHIRAL) {
       make_chiral_tables(S);
       launch_fill_blocks_chiral(S.desc, S.chiral_dev, P.nch, P.nphc, P.max_npp, P.max_nhh, max_nhp, S.stream);
+      if (S.tnf >= 0) {
+        Chiral3Dev &c = S.chiral3;
+        c = chiral3_constants(S.chiral, S.tnf_ce, S.tnf_cd, S.tnf_cutoff, S.tnf_nexp);
+        c.kx = S.desc.kx; c.ky = S.desc.ky; c.kz = S.desc.kz;
+        c.nx = S.desc.nx; c.ny = S.desc.ny; c.nz = S.desc.nz; c.sz = S.desc.sz; c.tz = S.desc.tz;
+        c.scale = std::pow(197.326968, 6) / (P.orb.volume * P.orb.volume);
+        check_launch();
+        launch_3nf(S.desc, c, S.tnf, P.nch, P.nphc, P.max_npp, P.max_nhh, max_nhp, S.stream);
+      }
     } else {
       launch_fill_blocks(S.desc, P.orb.volume, P.nch, P.nphc, P.max_npp, P.max_nhh, max_nhp, S.stream);
     }
@@ -1826,9 +1858,12 @@ void build_triples(Solve &S) {
   td.mtab = nullptr;
   td.mmax = 0;
   td.chiral = 0;
+  td.tnf = 0;
   if (S.native_v && S.vkind == CCGPU_INT_CHIRAL) {
     td.chiral = 1;
     td.ch = S.chiral_dev;
+    td.tnf = S.tnf >= 2 ? 1 : 0;
+    td.c3 = S.chiral3;
   } else if (S.native_v) {
     // radial Minnesota factors over the integer momentum transfer (the
     // constants of the generator, float32 literals as in the reference)
@@ -1884,6 +1919,8 @@ void build_triples(Solve &S) {
       td.vhppp = hppp;
       td.vhhhp = hhhp;
     }
+    if (td.tnf && !td.vhppp)
+      throw Error("(T) with the three-nucleon force needs its element tables: raise CCGPU_T3_TABLE_GB");
   }
   S.t3_partial = dalloc<double2>(8192, S.owned);
   S.t3_out = dalloc<double2>(1, S.owned);

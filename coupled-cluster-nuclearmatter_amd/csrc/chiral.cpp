@@ -303,4 +303,29 @@ std::vector<double> chiral_side_table(const ChiralModel &c, double qunit, int jm
   return tab;
 }
 
+Chiral3Dev chiral3_constants(const ChiralModel &c, double cE, double cD, double cutoff3, int nexp3) {
+  const double fpi2 = kFpi * kFpi, fpi4 = fpi2 * fpi2, lam_chi = 700.0;
+  const double ga2 = kGA * kGA, ha2 = kHA * kHA;
+  const double mpi_avg = (kMpiCharged + kMpiNeutral + kMpiCharged) / 3.0;
+  const double de = kDeltaMass - kMnuc[1];
+  const bool nnlo = c.order >= 3;
+  // init_cEcD (:1057-1066); the 3NF c_i follow the NN ones at NNLO (:270-290)
+  const double ce = nnlo ? cE : 0.0, cd = nnlo ? cD : 0.0;
+  const double c1 = nnlo ? c.c1 : 0.0, c3 = nnlo ? c.c3 : 0.0, c4 = nnlo ? c.c4 : 0.0;
+  const double k2 = ga2 / (4.0 * fpi2);
+  Chiral3Dev t{};
+  t.econst = ce / (fpi4 * lam_chi);
+  t.dconst = cd / (fpi2 * lam_chi);
+  t.k21 = kGA / (8.0 * fpi2);
+  const double c1_bar = c1, c3_bar = c3 - 4.0 * ha2 / (9.0 * de), c4_bar = c4 + 2.0 * ha2 / (9.0 * de);
+  t.c1t = -c1_bar * k2 * (4.0 * (mpi_avg * mpi_avg) / fpi2);
+  t.c3f = c3_bar * k2 * (2.0 / fpi2);
+  t.c4f = c4_bar * k2 * (1.0 / fpi2) * 0.5;
+  t.mpi2 = mpi_avg * mpi_avg;
+  t.llambda = 4.0 * cutoff3 * cutoff3;
+  t.two_thirds = (double)(2.0f / 3.0f);
+  t.nexp = nexp3;
+  return t;
+}
+
 }  // namespace ccgpu

@@ -3,6 +3,8 @@
 #pragma once
 #include <vector>
 
+#include "interaction.h"
+
 namespace ccgpu {
 
 struct ChiralModel {
@@ -27,5 +29,12 @@ std::vector<double> chiral_radial_table(const ChiralModel &c, double qunit, int 
 // Regulator x relativity half-factor per pair charge (pp, np, nn) and
 // j = 0..jmax with p^2 = (qunit^2) j: 3 x (jmax + 1) values.
 std::vector<double> chiral_side_table(const ChiralModel &c, double qunit, int jmax);
+
+// Constants of the NNLO three-nucleon force for the chiral model c: cE, cD
+// (zero below NNLO, as ccm_main.f90:33-36), its c1, c3, c4 (those of the NN
+// at NNLO, zero below; init_chp_constants) shifted by the Delta, and the
+// regulator cutoff (MeV) and power of freg_3nfnonlocal.  Momenta pointers and
+// the volume scale are set by the caller.
+Chiral3Dev chiral3_constants(const ChiralModel &c, double cE, double cD, double cutoff3, int nexp3);
 
 }  // namespace ccgpu

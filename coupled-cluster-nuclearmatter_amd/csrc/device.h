@@ -100,6 +100,10 @@ struct TriplesDesc {
   // momentum and isospin follow from conservation); vhhhp holds <jk||lc> at
   // (j A + k) np + (c-A), times 2 for the spin of l
   const double2 *vhppp, *vhhhp;
+  // normal-ordered 3NF in <cb||kd> (the reference adds it to vnn_hppp, not to
+  // vnn_hhhp: ccm_iter.f90:1641-1690)
+  int tnf;
+  Chiral3Dev c3;
 };
 
 // Warm-start network: number of parameters (flat FP32 vector, layout in

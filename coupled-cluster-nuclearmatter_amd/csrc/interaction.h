@@ -183,4 +183,169 @@ static __device__ __forceinline__ double2 chiral_direct(const ChiralDev &c, cons
   return make_double2(g * v.x, g * v.y);
 }
 
+// ---------------------------------------------------------------------------
+// The NNLO three-nucleon force, chiral_3nf_with_delta
+// (chiral_module_andreas_with_delta.f90:6670-6902): cD and cE contacts,
+// two-pion exchange with c1, c3, c4 shifted by the Delta (c3 - 4hA^2/9Delta,
+// c4 + 2hA^2/9Delta), summed over the three choices of the middle nucleon, with
+// the nonlocal regulator freg_3nfnonlocal (:3400-3423) on both sides.  The
+// reference's default-real (2./3.) in the regulator is kept (float32 value).
+// ---------------------------------------------------------------------------
+struct Chiral3Dev {
+  double econst, dconst;     // cE / (fpi^4 Lambda_chi), cD / (fpi^2 Lambda_chi)
+  double k21;                // gA / (8 fpi^2)
+  double c1t, c3f, c4f;      // -c1 gA^2/(4fpi^2) 4mpi^2/fpi^2, c3 gA^2/(4fpi^2) 2/fpi^2, c4 gA^2/(4fpi^2) 1/(2fpi^2)
+  double mpi2;               // average pion mass squared
+  double llambda;            // 4 cutoff^2
+  double two_thirds;         // float32(2/3)
+  double scale;              // hbarc^6 / volume^2
+  int nexp;                  // regulator power
+  const double *kx, *ky, *kz;   // single-particle momenta, fm^-1
+  const int *nx, *ny, *nz, *sz, *tz;
+};
+
+static __device__ __forceinline__ double freg3(const Chiral3Dev &c, const double *k1, const double *k2,
+                                               const double *k3) {
+  double q2 = 0.0;
+  double pp = 0.0, qq = 0.0;
+  for (int x = 0; x < 3; ++x) {
+    const double p = 0.5 * (k1[x] - k2[x]);
+    const double q = c.two_thirds * (k3[x] - 0.5 * (k1[x] + k2[x]));
+    pp += p * p;
+    qq += q * q;
+  }
+  q2 = 4.0 * pp + 3.0 * qq;
+  const double r = q2 / c.llambda;
+  double e = 1.0;
+  for (int i = 0; i < c.nexp; ++i) e *= r;
+  return exp(-e);
+}
+
+// <t1|tau|t4> . (<t2|tau|t5> x <t3|tau|t6>)  (chp_tau1_dot_tauXtau, :3544-3602)
+static __device__ __forceinline__ double2 tau_triple(int t1, int t2, int t3, int t4, int t5, int t6) {
+  // Cartesian components of <a|tau|b>: x, y (complex), z
+  auto comp = [](int a, int b, int axis) -> double2 {
+    if (axis == 0) return make_double2(a != b ? 1.0 : 0.0, 0.0);
+    if (axis == 1) return a != b ? make_double2(0.0, a > 0 ? -1.0 : 1.0) : make_double2(0.0, 0.0);
+    return make_double2(a == b ? (double)a : 0.0, 0.0);
+  };
+  double2 s2[3], s3[3], s1[3];
+  for (int a = 0; a < 3; ++a) {
+    s2[a] = comp(t2, t5, a);
+    s3[a] = comp(t3, t6, a);
+    s1[a] = comp(t1, t4, a);
+  }
+  auto sub = [](double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); };
+  const double2 fx = sub(cmul_i(s2[1], s3[2]), cmul_i(s2[2], s3[1]));
+  const double2 fy = sub(cmul_i(s2[2], s3[0]), cmul_i(s2[0], s3[2]));
+  const double2 fz = sub(cmul_i(s2[0], s3[1]), cmul_i(s2[1], s3[0]));
+  const double2 a = cmul_i(fx, s1[0]), b = cmul_i(fy, s1[1]), e = cmul_i(fz, s1[2]);
+  return make_double2(a.x + b.x + e.x, a.y + b.y + e.y);
+}
+
+// v3(p,q,r,s,t,u) in MeV; the bra regulator is passed in (it is the same for
+// every ket permutation of the antisymmetrized element)
+static __device__ __forceinline__ double2 chiral3_direct(const Chiral3Dev &c, double freg_bra, int p, int q, int r,
+                                                         int s, int t, int u) {
+  const int id[6] = {p, q, r, s, t, u};
+  if (c.tz[p] + c.tz[q] + c.tz[r] != c.tz[s] + c.tz[t] + c.tz[u] ||
+      c.nx[p] + c.nx[q] + c.nx[r] != c.nx[s] + c.nx[t] + c.nx[u] ||
+      c.ny[p] + c.ny[q] + c.ny[r] != c.ny[s] + c.ny[t] + c.ny[u] ||
+      c.nz[p] + c.nz[q] + c.nz[r] != c.nz[s] + c.nz[t] + c.nz[u])
+    return make_double2(0.0, 0.0);
+  const double hbarc = 197.326968;
+  double k[6][3];
+  for (int a = 0; a < 6; ++a) {
+    k[a][0] = c.kx[id[a]] * hbarc;
+    k[a][1] = c.ky[id[a]] * hbarc;
+    k[a][2] = c.kz[id[a]] * hbarc;
+  }
+  double q1[3], q2[3], q3[3];
+  for (int x = 0; x < 3; ++x) {
+    q1[x] = k[3][x] - k[0][x];
+    q2[x] = k[4][x] - k[1][x];
+    q3[x] = k[5][x] - k[2][x];
+  }
+  auto cross = [](const double *a, const double *b, double *o) {
+    o[0] = a[1] * b[2] - a[2] * b[1];
+    o[1] = a[2] * b[0] - a[0] * b[2];
+    o[2] = a[0] * b[1] - a[1] * b[0];
+  };
+  double q1xq2[3], q1xq3[3], q2xq3[3];
+  cross(q1, q2, q1xq2);
+  cross(q1, q3, q1xq3);
+  cross(q2, q3, q2xq3);
+  const double Q1 = q1[0] * q1[0] + q1[1] * q1[1] + q1[2] * q1[2];
+  const double Q2 = q2[0] * q2[0] + q2[1] * q2[1] + q2[2] * q2[2];
+  const double Q3 = q3[0] * q3[0] + q3[1] * q3[1] + q3[2] * q3[2];
+  const int m1 = c.sz[p], m2 = c.sz[q], m3 = c.sz[r], m4 = c.sz[s], m5 = c.sz[t], m6 = c.sz[u];
+  const int t1 = c.tz[p], t2 = c.tz[q], t3 = c.tz[r], t4 = c.tz[s], t5 = c.tz[t], t6 = c.tz[u];
+  auto sdq = [](int a, int b, const double *v) { return pauli_dot(a, b, v[0], v[1], v[2]); };
+  const double2 q252 = sdq(m2, m5, q2), q142 = sdq(m1, m4, q2), q362 = sdq(m3, m6, q2);
+  const double2 q251 = sdq(m2, m5, q1), q141 = sdq(m1, m4, q1), q361 = sdq(m3, m6, q1);
+  const double2 q363 = sdq(m3, m6, q3), q143 = sdq(m1, m4, q3), q253 = sdq(m2, m5, q3);
+  const double w1 = 1.0 / (Q1 + c.mpi2), w2 = 1.0 / (Q2 + c.mpi2), w3 = 1.0 / (Q3 + c.mpi2);
+  auto scl = [](double a, double2 b) { return make_double2(a * b.x, a * b.y); };
+  auto add = [](double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); };
+  auto dl = [](int a, int b) { return a == b ? 1.0 : 0.0; };
+  const double2 ope141 = scl(w1, q141), ope252 = scl(w2, q252), ope363 = scl(w3, q363);
+  // one "middle nucleon" term; (x,y) the two pion-coupled nucleons
+  auto term = [&](double ttf, double dspec, double e_d, double2 qa_b, double2 qc_d, double wab, double2 qe_f,
+                  double2 qg_h, double wef, double2 opex, double2 opey, double qdot, double2 sig_qxq, double2 tau_a,
+                  double2 tau_b) {
+    double2 in = make_double2(c.econst * e_d, 0.0);
+    const double2 dterm = add(scl(wab, cmul_i(qa_b, qc_d)), scl(wef, cmul_i(qe_f, qg_h)));
+    in = make_double2(in.x - c.dconst * c.k21 * dterm.x, in.y - c.dconst * c.k21 * dterm.y);
+    const double2 oo = cmul_i(opex, opey);
+    in = add(in, scl(c.c1t + c.c3f * qdot, oo));
+    double2 r = scl(ttf * dspec, in);
+    const double2 tdiff = make_double2(tau_a.x - tau_b.x, tau_a.y - tau_b.y);
+    r = add(r, scl(c.c4f, cmul_i(oo, cmul_i(sig_qxq, tdiff))));
+    return r;
+  };
+  const double2 s1 = term(sig_dot_sig(t1, t2, t4, t5), dl(m3, m6) * dl(t3, t6), dl(m1, m4) * dl(m2, m5), q252, q142,
+                          w2, q141, q251, w1, ope141, ope252, q1[0] * q2[0] + q1[1] * q2[1] + q1[2] * q2[2],
+                          sdq(m3, m6, q1xq2), tau_triple(t3, t1, t2, t6, t4, t5), tau_triple(t3, t2, t1, t6, t5, t4));
+  const double2 s2 = term(sig_dot_sig(t1, t3, t4, t6), dl(m2, m5) * dl(t2, t5), dl(m1, m4) * dl(m3, m6), q363, q143,
+                          w3, q141, q361, w1, ope141, ope363, q1[0] * q3[0] + q1[1] * q3[1] + q1[2] * q3[2],
+                          sdq(m2, m5, q1xq3), tau_triple(t2, t1, t3, t5, t4, t6), tau_triple(t2, t3, t1, t5, t6, t4));
+  const double2 s3 = term(sig_dot_sig(t2, t3, t5, t6), dl(m1, m4) * dl(t1, t4), dl(m2, m5) * dl(m3, m6), q363, q253,
+                          w3, q252, q362, w2, ope252, ope363, q2[0] * q3[0] + q2[1] * q3[1] + q2[2] * q3[2],
+                          sdq(m1, m4, q2xq3), tau_triple(t1, t2, t3, t4, t5, t6), tau_triple(t1, t3, t2, t4, t6, t5));
+  const double2 v = add(add(s1, s2), s3);
+  const double f = freg_bra * freg3(c, k[3], k[4], k[5]) * c.scale;
+  return make_double2(f * v.x, f * v.y);
+}
+
+static __device__ __forceinline__ double freg3_of(const Chiral3Dev &c, int p, int q, int r) {
+  const double hbarc = 197.326968;
+  const double k1[3] = {c.kx[p] * hbarc, c.ky[p] * hbarc, c.kz[p] * hbarc};
+  const double k2[3] = {c.kx[q] * hbarc, c.ky[q] * hbarc, c.kz[q] * hbarc};
+  const double k3[3] = {c.kx[r] * hbarc, c.ky[r] * hbarc, c.kz[r] * hbarc};
+  return freg3(c, k1, k2, k3);
+}
+
+// chiral_3nf_with_delta_asym(i,j,p,k,l,q) (:6904-6917): the ket antisymmetrized
+static __device__ double2 chiral3_asym(const Chiral3Dev &c, int i, int j, int p, int k, int l, int q) {
+  const double fb = freg3_of(c, i, j, p);
+  double2 r = chiral3_direct(c, fb, i, j, p, k, l, q);
+  const double2 a = chiral3_direct(c, fb, i, j, p, l, k, q), b = chiral3_direct(c, fb, i, j, p, q, l, k);
+  const double2 d = chiral3_direct(c, fb, i, j, p, k, q, l), e = chiral3_direct(c, fb, i, j, p, l, q, k);
+  const double2 f = chiral3_direct(c, fb, i, j, p, q, k, l);
+  r.x = r.x - a.x - b.x - d.x + e.x + f.x;
+  r.y = r.y - a.y - b.y - d.y + e.y + f.y;
+  return r;
+}
+
+// sum over the holes h of <pqh||rsh> (the normal-ordered two-body part)
+static __device__ double2 no2b_sum(const Chiral3Dev &c, int A, int p, int q, int r, int s) {
+  double2 acc = make_double2(0.0, 0.0);
+  for (int h = 0; h < A; ++h) {
+    const double2 v = chiral3_asym(c, p, q, h, r, s, h);
+    acc.x += v.x;
+    acc.y += v.y;
+  }
+  return acc;
+}
+
 }  // namespace ccgpu

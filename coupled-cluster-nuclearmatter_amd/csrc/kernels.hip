@@ -139,6 +139,133 @@ __global__ void k_e0(SolveDesc d, V vas) {
   if (threadIdx.x == 0) d.e0[0] = make_double2(sre[0], sim[0]);
 }
 
+// ---------------------------------------------------------------------------
+// Normal-ordered NNLO three-nucleon force added to the NN blocks
+// (normal_ordered_hamiltonian and setup_3nf_2bNO, ccm_main.f90:1687-2568).
+// N(p,q,r,s) = sum_h <pqh||rsh>.  The reference's block conventions are kept:
+//   hhhh, pppp: computed for the descending pair orientation and, between the
+//     two pairs, for the bra that comes first (lexicographic orbital order),
+//     the other triangle as the complex conjugate (ccm_main.f90:2046-2140,
+//     2420-2540);
+//   hphp: the same Hermitian fill on the (hole, particle) pairs (2290-2410);
+//   hhpp: <ij||ab> receives N(a,b,i,j), the sum with the particle pair in the
+//     bra (2160-2260).
+// The float32 2/3 of the 3NF regulator makes the element symmetric under
+// pair permutations only to ~1e-7, which is why the orientation matters.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ bool pair_le(int a, int b, int c, int d) { return a < c || (a == c && b <= d); }
+
+__device__ __forceinline__ double2 herm_no2b(const Chiral3Dev &c, int A, int p, int q, int r, int s) {
+  if (pair_le(p, q, r, s)) return no2b_sum(c, A, p, q, r, s);
+  const double2 v = no2b_sum(c, A, r, s, p, q);
+  return make_double2(v.x, -v.y);
+}
+
+__global__ __launch_bounds__(256) void k_no2b_vpppp(SolveDesc d, Chiral3Dev c) {
+  const int ch = blockIdx.y;
+  const int n = d.npp[ch];
+  const int64_t ne = (int64_t)n * n;
+  const int *pa = d.pp_a + d.off_pp[ch], *pb = d.pp_b + d.off_pp[ch];
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < ne; e += (int64_t)gridDim.x * blockDim.x) {
+    const int r = (int)(e / n), s = (int)(e % n);
+    const double2 v = herm_no2b(c, d.A, pb[r], pa[r], pb[s], pa[s]);   // descending orientation
+    double2 &o = d.Vpppp[d.off_vpp[ch] + e];
+    o.x += v.x;
+    o.y += v.y;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_no2b_vhhhh(SolveDesc d, Chiral3Dev c) {
+  const int ch = blockIdx.y;
+  const int nh = d.nhh[ch];
+  const int64_t ne = (int64_t)nh * nh;
+  const int *hi = d.hh_i + d.off_hh[ch], *hj = d.hh_j + d.off_hh[ch];
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < ne; e += (int64_t)gridDim.x * blockDim.x) {
+    const int s = (int)(e / nh), t = (int)(e % nh);
+    const double2 v = herm_no2b(c, d.A, hj[s], hi[s], hj[t], hi[t]);
+    double2 &o = d.Vhhhh[d.off_vhh[ch] + e];
+    o.x += v.x;
+    o.y += v.y;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_no2b_vhhpp(SolveDesc d, Chiral3Dev c) {
+  const int ch = blockIdx.y;
+  const int np_ = d.npp[ch], nh = d.nhh[ch];
+  const int64_t ne = (int64_t)np_ * nh;
+  const int *pa = d.pp_a + d.off_pp[ch], *pb = d.pp_b + d.off_pp[ch];
+  const int *hi = d.hh_i + d.off_hh[ch], *hj = d.hh_j + d.off_hh[ch];
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < ne; e += (int64_t)gridDim.x * blockDim.x) {
+    const int s = (int)(e / np_), r = (int)(e % np_);
+    const double2 v = no2b_sum(c, d.A, pa[r], pb[r], hi[s], hj[s]);
+    double2 &o = d.Vhhpp[d.off_t[ch] + e];
+    o.x += v.x;
+    o.y += v.y;
+  }
+}
+
+// W0((i,a),(j,b)) = -<ib||ja>
+__global__ __launch_bounds__(256) void k_no2b_w0(SolveDesc d, Chiral3Dev c) {
+  const int K = blockIdx.y;
+  const int n = d.nhp[K];
+  const int64_t ne = (int64_t)n * n;
+  const int *hj = d.hp_j + d.off_hp[K], *hb = d.hp_b + d.off_hp[K];
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < ne; e += (int64_t)gridDim.x * blockDim.x) {
+    const int r = (int)(e / n), s = (int)(e % n);
+    const int i = hj[r], a = hb[r], j = hj[s], b = hb[s];
+    const double2 v = herm_no2b(c, d.A, i, b, j, a);
+    double2 &o = d.W0[d.off_i4[K] + e];
+    o.x -= v.x;
+    o.y -= v.y;
+  }
+}
+
+// f_pq += sum_{i<=j} <ijp||ijq> for q in {p, partner(p)} (ccm_main.f90:1826-1995)
+__global__ __launch_bounds__(256) void k_3nf_fock(SolveDesc d, Chiral3Dev c) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= d.nsp) return;
+  const int pp = d.partner[p];
+  double2 fd = make_double2(0.0, 0.0), fp = make_double2(0.0, 0.0);
+  for (int i = 0; i < d.A; ++i)
+    for (int j = i; j < d.A; ++j) {
+      const double2 v = chiral3_asym(c, i, j, p, i, j, p);
+      fd.x += v.x;
+      fd.y += v.y;
+      if (pp >= 0) {
+        const double2 w = chiral3_asym(c, i, j, p, i, j, pp);
+        fp.x += w.x;
+        fp.y += w.y;
+      }
+    }
+  d.fdiag[p] = make_double2(d.fdiag[p].x + fd.x, d.fdiag[p].y + fd.y);
+  d.fpart[p] = make_double2(d.fpart[p].x + fp.x, d.fpart[p].y + fp.y);
+}
+
+// E0 += sum_{i<j<k} <ijk||ijk> (ccm_main.f90:1687-1800).  One block.
+__global__ __launch_bounds__(256) void k_3nf_e0(SolveDesc d, Chiral3Dev c) {
+  __shared__ double sre[256], sim[256];
+  double re = 0.0, im = 0.0;
+  const int64_t A = d.A, n = A * A * A;
+  for (int64_t e = threadIdx.x; e < n; e += blockDim.x) {
+    const int i = (int)(e / (A * A)), j = (int)((e / A) % A), k = (int)(e % A);
+    if (!(i < j && j < k)) continue;
+    const double2 v = chiral3_asym(c, i, j, k, i, j, k);
+    re += v.x;
+    im += v.y;
+  }
+  sre[threadIdx.x] = re;
+  sim[threadIdx.x] = im;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+      sre[threadIdx.x] += sre[threadIdx.x + w];
+      sim[threadIdx.x] += sim[threadIdx.x + w];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) d.e0[0] = make_double2(d.e0[0].x + sre[0], d.e0[0].y + sim[0]);
+}
+
 // <pq||rs> for a list of orbital quadruples; zero where momentum or isospin
 // is not conserved.
 template <class V>
@@ -1014,6 +1141,20 @@ void launch_elements(const SolveDesc &d, double volume, const ChiralDev *c, cons
     v.o.kx = d.kx; v.o.ky = d.ky; v.o.kz = d.kz; v.o.sz = d.sz; v.o.tz = d.tz; v.o.volume = volume;
     hipLaunchKernelGGL(k_elements<MinnesotaV>, g, dim3(256), 0, st, d, v, pqrs, n, out);
   }
+}
+
+// approx 0: E0 only; 1: and the Fock matrix; 2: and the two-body blocks
+void launch_3nf(const SolveDesc &d, const Chiral3Dev &c, int approx, int nch, int nphc, int max_npp, int max_nhh,
+                int max_nhp, hipStream_t st) {
+  auto gx = [](int64_t n) { return (unsigned)std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, 8192)); };
+  hipLaunchKernelGGL(k_3nf_e0, dim3(1), dim3(256), 0, st, d, c);
+  if (approx < 1) return;
+  hipLaunchKernelGGL(k_3nf_fock, dim3((d.nsp + 255) / 256), dim3(256), 0, st, d, c);
+  if (approx < 2) return;
+  hipLaunchKernelGGL(k_no2b_vpppp, dim3(gx((int64_t)max_npp * max_npp), nch), dim3(256), 0, st, d, c);
+  hipLaunchKernelGGL(k_no2b_vhhpp, dim3(gx((int64_t)max_npp * max_nhh), nch), dim3(256), 0, st, d, c);
+  hipLaunchKernelGGL(k_no2b_vhhhh, dim3(gx((int64_t)max_nhh * max_nhh), nch), dim3(256), 0, st, d, c);
+  hipLaunchKernelGGL(k_no2b_w0, dim3(gx((int64_t)max_nhp * max_nhp), nphc), dim3(256), 0, st, d, c);
 }
 
 void launch_derive(const SolveDesc &d, int nch, int nphc, int max_npp, int max_nhh, int max_nph,

@@ -150,7 +150,16 @@ __global__ void k_t3_fill_hppp(SolveDesc d, TriplesDesc td, double vol, double2 
     const int b = (int)(cb % d.np) + d.A, c = (int)(cb / d.np) + d.A;
     const int dd = orb_lookup(td, d.nx[c] + d.nx[b] - d.nx[k], d.ny[c] + d.ny[b] - d.ny[k],
                               d.nz[c] + d.nz[b] - d.nz[k], sd, d.tz[c] + d.tz[b] - d.tz[k]);
-    out[e] = dd >= d.A ? vas_t(d, td, vol, c, b, k, dd) : make_double2(0.0, 0.0);
+    double2 v = make_double2(0.0, 0.0);
+    if (dd >= d.A) {
+      v = vas_t(d, td, vol, c, b, k, dd);
+      if (td.tnf) {
+        const double2 w = no2b_sum(td.c3, d.A, c, b, k, dd);
+        v.x += w.x;
+        v.y += w.y;
+      }
+    }
+    out[e] = v;
   }
 }
 

@@ -69,6 +69,11 @@ module ccgpu_iface
        real(c_double), value :: cutoff
        type(c_ptr), value :: ci
      end function ccgpu_set_chiral
+     integer(c_int) function ccgpu_set_3nf(h, approx, ce, cd, cutoff, nexp) bind(C, name='ccgpu_set_3nf')
+       import :: c_int, c_double
+       integer(c_int), value :: h, approx, nexp
+       real(c_double), value :: ce, cd, cutoff
+     end function ccgpu_set_3nf
      integer(c_int) function ccgpu_get_e0(h, e0) bind(C, name='ccgpu_get_e0')
        import :: c_int, c_double_complex
        integer(c_int), value :: h

@@ -61,9 +61,14 @@ program ccm_gpu
      write(0,'(2a)') 'ccm_gpu: unknown interaction ', trim(interaction)
      error stop 3
   end if
-  if (tnf_switch) then
-     write(0,'(a)') 'ccm_gpu: three-nucleon forces are not part of this engine (tnf switch must be F)'
+  if (tnf_switch .and. trim(interaction) /= 'chiral') then
+     write(0,'(a)') 'ccm_gpu: the three-nucleon force belongs to the chiral interaction'
      error stop 3
+  end if
+  ! cE and cD vanish below NNLO (ccm_main.f90:33-36)
+  if (order < 3) then
+     ce = 0.d0
+     cd = 0.d0
   end if
   if (trim(cc_approx) /= 'CCD' .and. trim(cc_approx) /= 'CCD(T)' .and. trim(cc_approx) /= 'mbpt2') then
      write(0,'(2a)') 'ccm_gpu: unsupported cluster approximation ', trim(cc_approx)
@@ -112,6 +117,10 @@ program ccm_gpu
   if (trim(interaction) == 'chiral') then
      ! delta_chiral_order and lambda_delta of the deck (ccm_main.f90:31, init_chp_constants)
      call ccgpu_check(ccgpu_set_chiral(h, int(order, c_int), real(lambda, c_double), c_null_ptr), 'ccgpu_set_chiral')
+     ! tnf switch and approximation, 3NF regulator (ccm_main.f90:899-900): the
+     ! normal ordering of normal_ordered_hamiltonian / setup_3nf_2bNO
+     if (tnf_switch) call ccgpu_check(ccgpu_set_3nf(h, int(tnf_approx, c_int), real(ce, c_double), &
+          real(cd, c_double), real(cutoff3, c_double), int(nexp, c_int)), 'ccgpu_set_3nf')
      call ccgpu_check(ccgpu_generate_interaction(h, CCGPU_INT_CHIRAL), 'ccgpu_generate_interaction')
   else
      call ccgpu_check(ccgpu_generate_interaction(h, CCGPU_INT_MINNESOTA), 'ccgpu_generate_interaction')

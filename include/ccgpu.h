@@ -25,6 +25,13 @@
  *                                     vmom_minnesota
  *                                     (chiral_module_andreas_with_delta.f90:
  *                                     6304-6668, 8980-9111)
+ *   ccgpu_set_3nf                     tnf_switch / tnf_approx and the 3NF
+ *                                     regulator line of ccm_in
+ *                                     (ccm_main.f90:893-900) with cE, cD;
+ *                                     the normal ordering of
+ *                                     normal_ordered_hamiltonian and
+ *                                     setup_3nf_2bNO (ccm_main.f90:1687-2568)
+ *                                     runs inside ccgpu_generate_interaction
  *   ccgpu_vnn_elements                the antisymmetrized element calls
  *                                     chiral_NN_with_delta(p,q,r,s) -
  *                                     chiral_NN_with_delta(p,q,s,r)
@@ -166,7 +173,16 @@ int ccgpu_generate_interaction(int handle, int kind);
  * default of a new handle is NNLO at 450 MeV. */
 int ccgpu_set_chiral(int handle, int order, double cutoff, const double *ci);
 
-/* <pq||rs> of the generated interaction for n orbital quadruples (0-based),
+/* NNLO three-nucleon force of CCGPU_INT_CHIRAL, normal-ordered with respect to
+ * the holes: approx -1 off (default), 0 E0, 1 E0 and Fock, 2 E0, Fock and the
+ * two-body blocks (and the <cb||kd> elements of (T)), as tnf_approx of ccm_in.
+ * cE, cD as the deck's second line (zero below NNLO); cutoff (MeV) and nexp
+ * of its regulator as the deck's last line.  Call before
+ * ccgpu_generate_interaction. */
+int ccgpu_set_3nf(int handle, int approx, double cE, double cD, double cutoff, int nexp);
+
+/* <pq||rs> of the generated NN interaction (without the normal-ordered 3NF)
+ * for n orbital quadruples (0-based),
  * complex into v[2n]; zero where momentum or isospin is not conserved. */
 int ccgpu_vnn_elements(int handle, int n, const int *p, const int *q, const int *r, const int *s, double *v);
 

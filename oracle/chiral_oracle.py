@@ -404,3 +404,186 @@ class ChiralInteraction:
     def __call__(self, p, q, r, s):
         p, q, r, s = (np.asarray(x, np.int64) for x in (p, q, r, s))
         return chiral_direct(self.lat, self.cc, p, q, r, s) - chiral_direct(self.lat, self.cc, p, q, s, r)
+
+
+# --------------------------------------------------------------------------- #
+# The NNLO three-nucleon force and its normal ordering
+# --------------------------------------------------------------------------- #
+
+F32_TWO_THIRDS = float(np.float32(2.0) / np.float32(3.0))   # the default-real (2./3.) of freg_3nfnonlocal
+
+
+class ThreeBodyConstants:
+    """The 3NF constants of init_chp_constants / init_cEcD
+    (chiral_module_andreas_with_delta.f90:122-808, 1057-1066): cE, cD (zero
+    below NNLO, ccm_main.f90:33-36), the 3NF c1, c3, c4 (those of the NN at
+    NNLO, zero below) and the regulator cutoff and power of the deck's last
+    line (freg_3nfnonlocal, :3400-3423)."""
+
+    def __init__(self, cc: ChiralConstants, cE=0.017, cD=0.79, cutoff3=450.0, nexp3=3):
+        if cc.order < NNLO:
+            cE = cD = 0.0
+            c1 = c3 = c4 = 0.0
+        else:
+            c1, c3, c4 = cc.c1, cc.c3, cc.c4
+        fpi2 = FPI * FPI
+        fpi4 = fpi2 * fpi2
+        lam_chi = 700.0
+        self.econst = cE / (fpi4 * lam_chi)
+        self.dconst = cD / (fpi2 * lam_chi)
+        self.k2 = GA * GA / (4.0 * fpi2)
+        self.k21 = GA / (8.0 * fpi2)
+        self.fpi2 = fpi2
+        ha2 = HA * HA
+        self.c1_bar = c1
+        self.c3_bar = c3 - 4.0 * ha2 / (9.0 * cc.delta)
+        self.c4_bar = c4 + 2.0 * ha2 / (9.0 * cc.delta)
+        self.mpi2 = cc.mpi2[2]
+        self.llambda = 4.0 * float(cutoff3) ** 2
+        self.nexp = int(nexp3)
+
+
+def _tau1_dot_tau_x_tau(t1, t2, t3, t4, t5, t6):
+    """<t1|tau|t4> . (<t2|tau|t5> x <t3|tau|t6>) (chp_tau1_dot_tauXtau, :3544-3602)."""
+    def comp(a, b, axis):
+        up_up = (a == 1) & (b == 1)
+        dn_dn = (a == -1) & (b == -1)
+        up_dn = (a == 1) & (b == -1)
+        dn_up = (a == -1) & (b == 1)
+        if axis == 0:
+            return np.where(up_dn | dn_up, 1.0 + 0j, 0j)
+        if axis == 1:
+            return np.where(up_dn, -1j, 0j) + np.where(dn_up, 1j, 0j)
+        return np.where(up_up, 1.0 + 0j, 0j) + np.where(dn_dn, -1.0 + 0j, 0j)
+    s2 = [comp(t2, t5, a) for a in range(3)]
+    s3 = [comp(t3, t6, a) for a in range(3)]
+    fx = s2[1] * s3[2] - s2[2] * s3[1]
+    fy = s2[2] * s3[0] - s2[0] * s3[2]
+    fz = s2[0] * s3[1] - s2[1] * s3[0]
+    return fx * comp(t1, t4, 0) + fy * comp(t1, t4, 1) + fz * comp(t1, t4, 2)
+
+
+def _freg3(c3: ThreeBodyConstants, k1, k2, k3):
+    """freg_3nfnonlocal (:3400-3423), k in MeV."""
+    p = 0.5 * (k1 - k2)
+    q = F32_TWO_THIRDS * (k3 - 0.5 * (k1 + k2))
+    q2 = 4.0 * np.sum(p * p, axis=-1) + 3.0 * np.sum(q * q, axis=-1)
+    return np.exp(-((q2 / c3.llambda) ** c3.nexp))
+
+
+def chiral3_direct(lat, c3: ThreeBodyConstants, p, q, r, s, t, u):
+    """chiral_3nf_with_delta(p,q,r,s,t,u) (:6670-6902), complex, in MeV."""
+    idx = [np.asarray(x, np.int64) for x in (p, q, r, s, t, u)]
+    k = [lat.k[x] * HBARC for x in idx]
+    m = [lat.sz[x] for x in idx]
+    t_ = [lat.tz[x] for x in idx]
+    ok = (t_[0] + t_[1] + t_[2] == t_[3] + t_[4] + t_[5]) & np.all(
+        lat.n[idx[0]] + lat.n[idx[1]] + lat.n[idx[2]] == lat.n[idx[3]] + lat.n[idx[4]] + lat.n[idx[5]], axis=-1)
+    q1, q2, q3 = k[3] - k[0], k[4] - k[1], k[5] - k[2]
+    q1xq2, q1xq3, q2xq3 = np.cross(q1, q2), np.cross(q1, q3), np.cross(q2, q3)
+    Q1, Q2, Q3 = (np.sum(x * x, axis=-1) for x in (q1, q2, q3))
+    m1, m2, m3, m4, m5, m6 = m
+    t1, t2, t3, t4, t5, t6 = t_
+    sdq = lambda a, b, v: _pauli(a, b, v)
+    q252, q142, q362 = sdq(m2, m5, q2), sdq(m1, m4, q2), sdq(m3, m6, q2)
+    q251, q141, q361 = sdq(m2, m5, q1), sdq(m1, m4, q1), sdq(m3, m6, q1)
+    q363, q143, q253 = sdq(m3, m6, q3), sdq(m1, m4, q3), sdq(m2, m5, q3)
+    mp2 = c3.mpi2
+    ope252, ope141, ope363 = q252 / (Q2 + mp2), q141 / (Q1 + mp2), q363 / (Q3 + mp2)
+    s3_q1xq2, s2_q1xq3, s1_q2xq3 = sdq(m3, m6, q1xq2), sdq(m2, m5, q1xq3), sdq(m1, m4, q2xq3)
+    d = lambda a, b: (a == b).astype(np.float64)
+    k2, k21, fpi2 = c3.k2, c3.k21, c3.fpi2
+    E, D = c3.econst, c3.dconst
+    c1t = -c3.c1_bar * k2 * (4.0 * mp2 / fpi2)
+    c3f = c3.c3_bar * k2 * (2.0 / fpi2)
+    c4f = c3.c4_bar * k2 * (1.0 / fpi2) * 0.5
+    sum1 = _pair_dot(t1, t2, t4, t5) * d(m3, m6) * d(t3, t6) * (
+        E * d(m1, m4) * d(m2, m5)
+        - D * k21 * (q252 * q142 / (Q2 + mp2) + q141 * q251 / (Q1 + mp2))
+        + ope141 * ope252 * (c1t + c3f * np.sum(q1 * q2, axis=-1)))
+    sum1 = sum1 + c4f * ope141 * ope252 * (s3_q1xq2 * (_tau1_dot_tau_x_tau(t3, t1, t2, t6, t4, t5)
+                                                       - _tau1_dot_tau_x_tau(t3, t2, t1, t6, t5, t4)))
+    sum2 = _pair_dot(t1, t3, t4, t6) * d(m2, m5) * d(t2, t5) * (
+        E * d(m1, m4) * d(m3, m6)
+        - D * k21 * (q363 * q143 / (Q3 + mp2) + q141 * q361 / (Q1 + mp2))
+        + ope141 * ope363 * (c1t + c3f * np.sum(q1 * q3, axis=-1)))
+    sum2 = sum2 + c4f * ope141 * ope363 * (s2_q1xq3 * (_tau1_dot_tau_x_tau(t2, t1, t3, t5, t4, t6)
+                                                       - _tau1_dot_tau_x_tau(t2, t3, t1, t5, t6, t4)))
+    sum3 = _pair_dot(t2, t3, t5, t6) * d(m1, m4) * d(t1, t4) * (
+        E * d(m2, m5) * d(m3, m6)
+        - D * k21 * (q363 * q253 / (Q3 + mp2) + q252 * q362 / (Q2 + mp2))
+        + ope252 * ope363 * (c1t + c3f * np.sum(q2 * q3, axis=-1)))
+    sum3 = sum3 + c4f * ope252 * ope363 * (s1_q2xq3 * (_tau1_dot_tau_x_tau(t1, t2, t3, t4, t5, t6)
+                                                       - _tau1_dot_tau_x_tau(t1, t3, t2, t4, t6, t5)))
+    v = sum1 + sum2 + sum3
+    res = _freg3(c3, k[0], k[1], k[2]) * _freg3(c3, k[3], k[4], k[5]) * HBARC ** 6 * v / lat.volume ** 2
+    return np.where(ok, res, 0.0)
+
+
+def chiral3_asym(lat, c3, i, j, p, k, l, q):
+    """chiral_3nf_with_delta_asym (:6904-6917): the ket antisymmetrized."""
+    f = lambda a, b, c: chiral3_direct(lat, c3, i, j, p, a, b, c)
+    return f(k, l, q) - f(l, k, q) - f(q, l, k) - f(k, q, l) + f(l, q, k) + f(q, k, l)
+
+
+class NormalOrdered3NF:
+    """The 3NF normal-ordered with respect to the hole states
+    (normal_ordered_hamiltonian, ccm_main.f90:1687-2024; setup_3nf_2bNO,
+    2025-2568): zero-body sum_{i<j<k} <ijk||ijk>, one-body
+    f_pq += sum_{i<=j} <ijp||ijq>, two-body V_pqrs += sum_h <pqh||rsh>."""
+
+    def __init__(self, lat, cc: ChiralConstants, cE=0.017, cD=0.79, cutoff3=450.0, nexp3=3):
+        self.lat = lat
+        self.c3 = ThreeBodyConstants(cc, cE, cD, cutoff3, nexp3)
+
+    def e0(self):
+        A = self.lat.A
+        ijk = np.array([(i, j, k) for i in range(A) for j in range(i + 1, A) for k in range(j + 1, A)], np.int64)
+        if len(ijk) == 0:
+            return 0j
+        i, j, k = ijk.T
+        return np.sum(chiral3_asym(self.lat, self.c3, i, j, k, i, j, k))
+
+    def fock(self, p, q):
+        p, q = np.asarray(p, np.int64), np.asarray(q, np.int64)
+        out = np.zeros(p.shape, np.complex128)
+        A = self.lat.A
+        for i in range(A):
+            for j in range(i, A):
+                ii, jj = np.full(p.shape, i), np.full(p.shape, j)
+                out = out + chiral3_asym(self.lat, self.c3, ii, jj, p, ii, jj, q)
+        return out
+
+    def two_body(self, p, q, r, s):
+        p, q, r, s = (np.asarray(x, np.int64) for x in (p, q, r, s))
+        out = np.zeros(p.shape, np.complex128)
+        for h in range(self.lat.A):
+            hh = np.full(p.shape, h)
+            out = out + chiral3_asym(self.lat, self.c3, p, q, hh, r, s, hh)
+        return out
+
+    def herm(self, p, q, r, s):
+        """N(p,q,r,s) where the pair (p,q) comes first in lexicographic orbital
+        order, else conj N(r,s,p,q): the Hermitian fill of the reference's
+        hhhh, hphp and pppp blocks (ccm_main.f90:2046-2140, 2290-2540)."""
+        p, q, r, s = (np.asarray(x, np.int64) for x in (p, q, r, s))
+        first = (p < r) | ((p == r) & (q <= s))
+        return np.where(first, self.two_body(p, q, r, s), np.conj(self.two_body(r, s, p, q)))
+
+    def block(self, kind, p, q, r, s):
+        """The normal-ordered 3NF part of a canonical record (p<q, r<s) of the
+        reference's blocks, with its conventions: hhhh (1) and pppp (6) from
+        the descending pair orientation with the Hermitian fill, hphp (4) with
+        the Hermitian fill, hhpp (3) with the particle pair in the bra
+        (ccm_main.f90:2160-2260), hppp <cb||kd> directly (ccm_iter.f90:
+        1641-1690); hhhp (2) carries none."""
+        p, q, r, s = (np.asarray(x, np.int64) for x in (p, q, r, s))
+        if kind in (1, 6):
+            return self.herm(q, p, s, r)
+        if kind == 4:
+            return self.herm(p, q, r, s)
+        if kind == 3:
+            return self.two_body(r, s, p, q)
+        if kind == 5:
+            return self.two_body(p, q, r, s)
+        return np.zeros(p.shape, np.complex128)

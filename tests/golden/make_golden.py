@@ -22,7 +22,8 @@ ROOT = os.path.dirname(os.path.dirname(HERE))
 DRIVER = os.path.join(ROOT, "oracle", "_ref", "ref_driver")
 
 # name: (A, matter, rho, nmax, order, interaction, subspace, step, tol, maxit,
-#        twist, dump, triples, ranks[, cutoff MeV, default 450])
+#        twist, dump, triples, ranks[, cutoff MeV, default 450[, 3NF
+#        approximation 0/1/2, default off]])
 CASES = {
     "pnm14_n1_nnlo": (14, "pnm", 0.16, 1, 3, "chiral", 6, 6, 1e-6, 100, (0, 0, 0), 1, 1, 1),
     "pnm14_n1_minn": (14, "pnm", 0.16, 1, 3, "minnesota", 8, 8, 1e-8, 100, (0, 0, 0), 1, 1, 1),
@@ -42,6 +43,12 @@ CASES = {
     "snm4_n1_nnlo394": (4, "snm", 0.16, 1, 3, "chiral", 6, 6, 1e-6, 20, (0, 0, 0), 1, 0, 1, 394),
     "snm4_n1_nnlo500": (4, "snm", 0.16, 1, 3, "chiral", 6, 6, 1e-6, 20, (0, 0, 0), 1, 0, 1, 500),
     "pnm14_n1_nnlo_twist": (14, "pnm", 0.12, 1, 3, "chiral", 6, 6, 1e-6, 100, (0.4, 1.3, 2.2), 1, 1, 1),
+    # the NNLO three-nucleon force normal-ordered to two-body (and only to
+    # zero- and one-body), NNLO and NLO (Delta terms only)
+    "pnm14_n1_nnlo_3nf": (14, "pnm", 0.16, 1, 3, "chiral", 6, 6, 1e-6, 100, (0, 0, 0), 1, 1, 1, 450, 2),
+    "snm4_n1_nnlo_3nf": (4, "snm", 0.16, 1, 3, "chiral", 6, 6, 1e-6, 100, (0, 0, 0), 1, 1, 1, 450, 2),
+    "snm4_n1_nlo450_3nf": (4, "snm", 0.16, 1, 2, "chiral", 6, 6, 1e-6, 20, (0, 0, 0), 1, 0, 1, 450, 2),
+    "pnm14_n1_nnlo_3nf1": (14, "pnm", 0.16, 1, 3, "chiral", 6, 6, 1e-6, 100, (0.3, 2.1, 1.2), 1, 0, 1, 450, 1),
 }
 
 CCM_IN = """!Chiral order for Deltas(LO = 0,NLO=2,NNLO=3,N3LO=4) and cutoff
@@ -59,7 +66,7 @@ PBC
 ! specify cluster approximation: CCD, CCD(T)
 CCD
 ! tnf switch (T/F) and specify 3nf approximation: 0=tnf0b, 1=tnf1b, 2=tnf2b
-F, 0
+{tnf}, {tnf_approx}
 ! 3nf cutoff(MeV),non-local reg. exp
 450, 3
 """
@@ -68,10 +75,12 @@ F, 0
 def run_case(name):
     (A, matter, rho, nmax, order, inter, sub, step, tol, maxit, tw, dump, trip, ranks) = CASES[name][:14]
     cutoff = CASES[name][14] if len(CASES[name]) > 14 else 450
+    tnf_approx = CASES[name][15] if len(CASES[name]) > 15 else -1
     with tempfile.TemporaryDirectory() as tmp:
         inp = os.path.join(tmp, "ccm_in")
         with open(inp, "w") as fh:
-            fh.write(CCM_IN.format(order=order, cutoff=cutoff, A=A, matter=matter, rho=rho, nmax=nmax))
+            fh.write(CCM_IN.format(order=order, cutoff=cutoff, A=A, matter=matter, rho=rho, nmax=nmax,
+                                  tnf="T" if tnf_approx >= 0 else "F", tnf_approx=max(tnf_approx, 0)))
         cmd = [DRIVER, inp, tmp, inter, str(sub), str(step), repr(tol), str(maxit),
                repr(tw[0]), repr(tw[1]), repr(tw[2]), str(dump), str(trip)]
         if ranks > 1:
@@ -82,6 +91,7 @@ def run_case(name):
             raise RuntimeError(f"{name}: reference driver failed\n{res.stdout[-2000:]}\n{res.stderr[-2000:]}")
         out = {"params": np.array([A, nmax, sub, step, maxit, order], np.int64),
                "cutoff": np.float64(cutoff),
+               "tnf_approx": np.int64(tnf_approx),
                "matter": np.array(matter), "interaction": np.array(inter),
                "rho": np.float64(rho), "tol": np.float64(np.float32(tol)),
                "twist": np.array(tw, np.float64)}

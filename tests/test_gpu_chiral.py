@@ -101,6 +101,93 @@ def test_chiral_elements_against_oracle(ccnm, A, matter, rho, nmax, twist, ci):
     assert np.max(np.abs(v - want)) < 1e-12 * np.max(np.abs(want))
 
 
+# (fixture, chiral order, 3NF approximation)
+TNF_CASES = [("pnm14_n1_nnlo_3nf", 3, 2), ("snm4_n1_nnlo_3nf", 3, 2), ("snm4_n1_nlo450_3nf", 2, 2),
+             ("pnm14_n1_nnlo_3nf1", 3, 1)]
+
+
+def chiral_3nf_solver(ccnm, g, order, approx, reference_orbitals=True):
+    """The chiral NN plus the normal-ordered 3NF generated on the GPU, on the
+    reference's own orbital table (its numbering) or on the engine's lattice."""
+    A, nmax, sub, step, maxit = params(g)
+    if reference_orbitals:
+        oi, od = g["orb_int"], g["orb_dbl"]
+        s = ccnm.Solver(orbitals={"nx": oi[0], "ny": oi[1], "nz": oi[2], "sz": oi[3], "tz": oi[4],
+                                  "kx": od[0], "ky": od[1], "kz": od[2], "e": od[3], "A": A,
+                                  "box": float(g["box"]), "volume": float(g["volume"])})
+    else:
+        matter = ccnm.PNM if str(g["matter"]) == "pnm" else ccnm.SNM
+        s = ccnm.Solver(A=A, matter=matter, rho=float(g["rho"]), nmax=nmax, twist=tuple(g["twist"]))
+    s.set_chiral(order, 450.0)
+    s.set_3nf(approx, 0.017, 0.79, 450.0, 3)
+    s.generate_interaction(ccnm.INT_CHIRAL)
+    return s
+
+
+@pytest.mark.parametrize("name,order,approx", TNF_CASES)
+def test_native_chiral_3nf_against_reference(ccnm, name, order, approx):
+    """Chiral NN plus the NNLO three-nucleon force normal-ordered on the GPU
+    (E0; and Fock; and the two-body blocks), on the reference's orbital table,
+    against the reference run: E0 and the Fock matrix to rounding.  The
+    reference fills the non-canonical orderings of its hhpp block with their
+    own 3NF sums, which the float32 2/3 of the 3NF regulator makes differ from
+    the antisymmetric partner by ~1e-7 of the 3NF part, so MBPT2 and the
+    iterations agree to 1e-7 relative."""
+    g = golden(name)
+    A, nmax, sub, step, maxit = params(g)
+    s = chiral_3nf_solver(ccnm, g, order, approx)
+    assert abs(s.e0() - g["e0"]) < 1e-11 * abs(g["e0"])
+    fd = s.fock_diag()
+    assert np.max(np.abs(fd - np.diag(g["fock"]))) < 1e-11 * np.max(np.abs(fd))
+    emp2 = s.setup_t_amplitudes()
+    scale = np.max(np.abs(g["energies"]))
+    assert abs(emp2 - g["energies"][0]) < 1e-7 * scale
+    en = step_loop(s, sub, step, float(g["tol"]), maxit)
+    k = min(step, len(en), len(g["energies"]))
+    assert np.max(np.abs(en[:k] - g["energies"][:k])) < 1e-7 * scale
+    assert abs(en[-1] - g["energies"][-1]) < 1e-7 * scale + 20 * float(g["tol"])
+
+
+@pytest.mark.parametrize("name,order,approx", TNF_CASES[:2])
+def test_native_lattice_3nf_against_oracle(ccnm, name, order, approx):
+    """On the engine's own lattice (its tie order of degenerate momenta differs
+    from the reference's quicksort) the normal-ordered E0 and Fock matrix equal
+    the oracle's on the same numbering; against the reference they differ by
+    the ordering dependence the float32 2/3 brings (3e-10 of E0 measured)."""
+    import ccd_oracle as co
+    import chiral_oracle as ch
+    g = golden(name)
+    A, nmax, sub, step, maxit = params(g)
+    s = chiral_3nf_solver(ccnm, g, order, approx, reference_orbitals=False)
+    lat = co.Lattice.build(A, str(g["matter"]), float(g["rho"]), nmax, twist=tuple(g["twist"]))
+    mine = s.orbitals()
+    assert np.array_equal(np.asarray(mine["nx"]), lat.n[:, 0]) and np.array_equal(np.asarray(mine["sz"]), lat.sz)
+    nn = ch.ChiralInteraction(lat, order, 450.0)
+    no = ch.NormalOrdered3NF(lat, nn.cc)
+    i, j = (x.ravel() for x in np.meshgrid(np.arange(A), np.arange(A), indexing="ij"))
+    e0 = np.sum(lat.e[:A]) + 0.5 * np.sum(nn(i, j, i, j)) + no.e0()
+    assert abs(s.e0() - e0) < 1e-12 * abs(e0)
+    assert abs(s.e0() - g["e0"]) < 1e-8 * abs(g["e0"])
+    pd = np.arange(lat.nsp)
+    f = lat.e.astype(np.complex128) + no.fock(pd, pd)
+    for h in range(A):
+        f = f + nn(pd, np.full(lat.nsp, h), pd, np.full(lat.nsp, h))
+    assert np.max(np.abs(s.fock_diag() - f)) < 1e-12 * np.max(np.abs(f))
+
+
+@pytest.mark.parametrize("name", ["pnm14_n1_nnlo_3nf", "snm4_n1_nnlo_3nf"])
+def test_triples_3nf_against_reference(ccnm, name):
+    """(T) with <cb||kd> carrying the normal-ordered 3NF (and <jk||lc> the NN
+    alone, as the reference's vnn_hhhp), on the reference's converged T2."""
+    g = golden(name)
+    s = chiral_3nf_solver(ccnm, g, int(g["params"][5]), int(g["tnf_approx"]))
+    s.setup_t_amplitudes()
+    set_reference_t2(ccnm, s, g)
+    et = s.pert_triples2()
+    ref = g["eccdt"] - g["eccd"]
+    assert abs(et - ref) < 1e-11 * max(1.0, abs(ref)), (et, ref)
+
+
 @pytest.mark.parametrize("name", ["pnm14_n1_nnlo", "snm4_n1_nnlo", "pnm14_n1_nnlo_twist"])
 def test_triples_native_chiral_against_reference(ccnm, name):
     """(T) with the <cb||kd> and <jk||lc> elements from the device chiral
@@ -114,17 +201,19 @@ def test_triples_native_chiral_against_reference(ccnm, name):
     assert abs(et - ref) < 1e-11 * max(1.0, abs(ref)), (et, ref)
 
 
-def test_fortran_host_reference_deck_chiral(ccnm, tmp_path):
+@pytest.mark.parametrize("name,tnf", [("pnm14_n1_nnlo", "F, 0"), ("pnm14_n1_nnlo_3nf", "T, 2")])
+def test_fortran_host_reference_deck_chiral(ccnm, tmp_path, name, tnf):
     """The unmodified reference deck (no engine line: chiral NNLO at 450 MeV,
-    DIIS 6/6, tolerance 1e-6, CCD(T)) through the Fortran driver ccm_gpu; its
-    result block matches the reference run of the same deck."""
+    DIIS 6/6, tolerance 1e-6, CCD(T); without and with the normal-ordered 3NF)
+    through the Fortran driver ccm_gpu; its result block matches the
+    reference run of the same deck."""
     import os
     import subprocess
     from conftest import ROOT
     exe = os.path.join(ROOT, "coupled-cluster-nuclearmatter_amd", "host", "ccm_gpu")
     if not os.path.exists(exe):
         pytest.skip("Fortran host not built (flang absent)")
-    g = golden("pnm14_n1_nnlo")
+    g = golden(name)
     deck = tmp_path / "ccm_in"
     deck.write_text("""!Chiral order for Deltas(LO = 0,NLO=2,NNLO=3,N3LO=4) and cutoff
 3, 450
@@ -141,10 +230,10 @@ PBC
 ! specify cluster approximation: CCD, CCD(T)
 CCD(T)
 ! tnf switch (T/F) and specify 3nf approximation: 0=tnf0b, 1=tnf1b, 2=tnf2b
-F, 0
+TNF
 ! 3nf cutoff(MeV),non-local reg. exp
 450, 3
-""")
+""".replace("TNF", tnf))
     res = subprocess.run([exe, str(deck)], capture_output=True, text=True, timeout=120)
     assert res.returncode == 0, res.stderr
     line = [l for l in res.stdout.splitlines() if l.startswith("     0.01700")][-1]

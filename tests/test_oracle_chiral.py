@@ -133,3 +133,72 @@ def test_lo_one_pion_exchange_closed_form():
     reg = np.exp(-(q2 ** 3) / 450.0 ** 6)
     want = (q2 * ope + cc.CS[0] + cc.CT[0]) * rel * reg * ch.HBARC ** 3 / lat.volume
     assert abs(v - want) < 1e-13 * abs(want)
+
+
+TNF_CASES = ["pnm14_n1_nnlo_3nf", "snm4_n1_nnlo_3nf", "snm4_n1_nlo450_3nf", "pnm14_n1_nnlo_3nf1"]
+
+
+@pytest.mark.parametrize("name", TNF_CASES)
+def test_oracle_3nf_normal_ordering_matches_reference(name):
+    """The NNLO three-nucleon force normal-ordered with respect to the holes
+    (normal_ordered_hamiltonian, setup_3nf_2bNO): E0 and the Fock matrix
+    against the reference (zero- and one-body parts), and at the two-body
+    level every stored block record (hhhh, hhpp, hphp, pppp with the NO2B
+    term, hhhp without it), NNLO and NLO (Delta terms), PNM and SNM."""
+    g = golden(name)
+    lat = fixture_lattice(g)
+    order, approx = int(g["params"][5]), int(g["tnf_approx"])
+    nn = ch.ChiralInteraction(lat, order, 450.0)
+    no = ch.NormalOrdered3NF(lat, nn.cc)
+    A = lat.A
+    i, j = (x.ravel() for x in np.meshgrid(np.arange(A), np.arange(A), indexing="ij"))
+    e0 = np.sum(lat.e[:A]) + 0.5 * np.sum(nn(i, j, i, j)) + no.e0()
+    assert abs(e0 - g["e0"]) < 1e-12 * abs(g["e0"])
+    pd = np.arange(lat.nsp)
+    f = lat.e.astype(np.complex128)
+    for h in range(A):
+        f = f + nn(pd, np.full(lat.nsp, h), pd, np.full(lat.nsp, h))
+    if approx >= 1:
+        f = f + no.fock(pd, pd)
+    assert np.max(np.abs(f - np.diag(g["fock"]))) < 1e-12 * np.max(np.abs(f))
+    if approx >= 2:
+        idx = g["v_idx"].astype(np.int64)
+        ref = g["v_val"]
+        for kind in np.unique(idx[:, 0]):
+            m = idx[:, 0] == kind
+            p, q, r, s = (idx[m, c] - 1 for c in range(1, 5))
+            v = nn(p, q, r, s) + no.block(int(kind), p, q, r, s)
+            assert np.max(np.abs(v - ref[m])) < 1e-13 * max(1.0, np.max(np.abs(ref[m]))), kind
+        # (T)'s <cb||kd> records (vnn_hppp) carry the NO2B term as well
+        if "hppp_idx" in g:
+            h = g["hppp_idx"].astype(np.int64) - 1
+            k_, d_, c_, b_ = h.T
+            v = nn(c_, b_, k_, d_) + no.block(5, c_, b_, k_, d_)
+            assert np.max(np.abs(v - g["hppp_val"])) < 1e-13 * np.max(np.abs(g["hppp_val"]))
+
+
+def test_3nf_pair_symmetry_broken_only_by_the_float32_two_thirds():
+    """The reference's 3NF regulator uses the default-real (2./3.): the
+    normal-ordered element is antisymmetric and Hermitian only to ~1e-7 (the
+    reason the block conventions above matter), and exactly so with 2/3 in
+    double precision."""
+    lat = co.Lattice.build(14, "pnm", 0.16, 1)
+    nn = ch.ChiralInteraction(lat)
+    no = ch.NormalOrdered3NF(lat, nn.cc)
+    p, q, r, s = _random_quadruples(lat, 300, 5)
+    keep = (p != q) & (r != s)
+    p, q, r, s = p[keep], q[keep], r[keep], s[keep]
+    a = no.two_body(p, q, r, s)
+    scale = np.max(np.abs(a))
+    assert scale > 0.1
+    asym = np.max(np.abs(a + no.two_body(q, p, r, s)))
+    herm = np.max(np.abs(a - np.conj(no.two_body(r, s, p, q))))
+    assert 1e-12 * scale < max(asym, herm) < 1e-5 * scale
+    saved = ch.F32_TWO_THIRDS
+    try:
+        ch.F32_TWO_THIRDS = 2.0 / 3.0
+        a = no.two_body(p, q, r, s)
+        assert np.max(np.abs(a + no.two_body(q, p, r, s))) < 1e-13 * scale
+        assert np.max(np.abs(a - np.conj(no.two_body(r, s, p, q)))) < 1e-13 * scale
+    finally:
+        ch.F32_TWO_THIRDS = saved
