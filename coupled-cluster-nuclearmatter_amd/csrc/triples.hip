@@ -22,6 +22,7 @@
 #include "interaction.h"
 
 #include <algorithm>
+#include <cstdlib>
 
 namespace ccgpu {
 
@@ -140,7 +141,7 @@ __device__ double2 h_term(const SolveDesc &d, const TriplesDesc &td, double vol,
 // Dense element tables for (T): every <cb||kd> (particles c, b, hole k, the
 // particle d of either spin that conserves momentum and isospin) and every
 // <jk||lc>, evaluated once by the same element source as the in-place path.
-__global__ void k_t3_fill_hppp(SolveDesc d, TriplesDesc td, double vol, double2 *out) {
+__global__ __launch_bounds__(256) void k_t3_fill_hppp(SolveDesc d, TriplesDesc td, double vol, double2 *out) {
   const int64_t n = (int64_t)d.np * d.np * d.A * 2;
   for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x) {
     const int sd = (int)(e & 1) * 2 - 1;
@@ -163,7 +164,7 @@ __global__ void k_t3_fill_hppp(SolveDesc d, TriplesDesc td, double vol, double2 
   }
 }
 
-__global__ void k_t3_fill_hhhp(SolveDesc d, TriplesDesc td, double vol, double2 *out) {
+__global__ __launch_bounds__(256) void k_t3_fill_hhhp(SolveDesc d, TriplesDesc td, double vol, double2 *out) {
   const int64_t n = (int64_t)d.A * d.A * d.np * 2;
   for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x) {
     const int sl = (int)(e & 1) * 2 - 1;
@@ -274,6 +275,71 @@ __global__ __launch_bounds__(256) void k_triples(SolveDesc d, TriplesDesc td, do
   }
 }
 
+// Table mode, one wave per particle triple: a block takes particle pairs
+// (a,b), its four waves take c = b+1+wave, b+5+wave, ..., and the 64 lanes of
+// a wave stride over the hole triples of that (K3,Tz3) channel.  Every lane of
+// a wave reads the same <c.||..> and <..||.c> table rows (cache-friendly), and
+// no lane waits for a longer hole-triple list than its wave's.
+__global__ __launch_bounds__(256) void k_triples_w(SolveDesc d, TriplesDesc td, double volume, double2 *partial,
+                                                   unsigned long long *count) {
+  __shared__ double sre[256], sim[256];
+  __shared__ unsigned long long scnt[256];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  double re = 0.0, im = 0.0;
+  unsigned long long nel = 0;
+  for (int64_t pr = blockIdx.x; pr < td.npairs; pr += gridDim.x) {
+    const int a = td.pair_a[pr], b = td.pair_b[pr];
+    for (int c = b + 1 + wave; c < d.nsp; c += 4) {
+      const int kx = d.nx[a] + d.nx[b] + d.nx[c], ky = d.ny[a] + d.ny[b] + d.ny[c], kz = d.nz[a] + d.nz[b] + d.nz[c];
+      const int tz = d.tz[a] + d.tz[b] + d.tz[c];
+      const int ix = kx - td.k0, iy = ky - td.k0, iz = kz - td.k0, it = tz - td.tz0;
+      if (ix < 0 || iy < 0 || iz < 0 || ix >= td.nk || iy >= td.nk || iz >= td.nk || it < 0 || it >= td.ntz) continue;
+      const int ch = td.chan[((it * td.nk + ix) * td.nk + iy) * td.nk + iz];
+      if (ch < 0) continue;
+      const double fa = d.fdiag[a].x, fb = d.fdiag[b].x, fc = d.fdiag[c].x;
+      const double fai = d.fdiag[a].y, fbi = d.fdiag[b].y, fci = d.fdiag[c].y;
+      const int h0 = td.hptr[ch], h1 = td.hptr[ch + 1];
+      if (lane == 0) nel += (unsigned long long)(h1 - h0);
+      for (int h = h0 + lane; h < h1; h += 64) {
+        const int i = td.hijk[3 * h], j = td.hijk[3 * h + 1], k = td.hijk[3 * h + 2];
+        double2 w = make_double2(0.0, 0.0);
+#pragma unroll 1
+        for (int t = 0; t < 18; ++t) {
+          const TermCode tc = kTerms[t];
+          const int x = sel3(tc.p0, a, b, c), y = sel3(tc.p1, a, b, c), z = sel3(tc.p2, a, b, c);
+          const int u = sel3(tc.h0, i, j, k), v = sel3(tc.h1, i, j, k), q = sel3(tc.h2, i, j, k);
+          const double2 f = tc.kind == 0 ? f_term<true>(d, td, volume, x, y, z, u, v, q)
+                                         : h_term<true>(d, td, volume, x, y, z, u, v, q);
+          w.x += (double)tc.sign * f.x;
+          w.y += (double)tc.sign * f.y;
+        }
+        const double dre = d.fdiag[i].x + d.fdiag[j].x + d.fdiag[k].x - fa - fb - fc;
+        const double dim = d.fdiag[i].y + d.fdiag[j].y + d.fdiag[k].y - fai - fbi - fci;
+        const double ww = w.x * w.x + w.y * w.y;
+        const double den = dre * dre + dim * dim;
+        re += ww * dre / den;
+        im += -ww * dim / den;
+      }
+    }
+  }
+  sre[threadIdx.x] = re;
+  sim[threadIdx.x] = im;
+  scnt[threadIdx.x] = nel;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) {
+      sre[threadIdx.x] += sre[threadIdx.x + s];
+      sim[threadIdx.x] += sim[threadIdx.x + s];
+      scnt[threadIdx.x] += scnt[threadIdx.x + s];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    partial[blockIdx.x] = make_double2(sre[0], sim[0]);
+    count[blockIdx.x] = scnt[0];
+  }
+}
+
 __global__ void k_triples_count(const unsigned long long *count, int n, unsigned long long *out) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
   unsigned long long s = 0;
@@ -310,7 +376,10 @@ void launch_t3_tables(const SolveDesc &d, const TriplesDesc &td, double volume, 
 void launch_triples(const SolveDesc &d, const TriplesDesc &td, double volume, double2 *partial,
                     unsigned long long *count, int nblocks, double2 *out, unsigned long long *count_out,
                     hipStream_t st) {
-  if (td.vhppp && td.vhhhp)
+  static const int variant = getenv("CCGPU_T3_KERNEL") ? atoi(getenv("CCGPU_T3_KERNEL")) : 1;
+  if (td.vhppp && td.vhhhp && variant == 1)
+    hipLaunchKernelGGL(k_triples_w, dim3(nblocks), dim3(256), 0, st, d, td, volume, partial, count);
+  else if (td.vhppp && td.vhhhp)
     hipLaunchKernelGGL(k_triples<true>, dim3(nblocks), dim3(256), 0, st, d, td, volume, partial, count);
   else
     hipLaunchKernelGGL(k_triples<false>, dim3(nblocks), dim3(256), 0, st, d, td, volume, partial, count);
