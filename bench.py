@@ -325,6 +325,9 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--profile-triples", action="store_true",
+                    help="short run for counter collection: the (T) workload (SNM A=132 |n_i|<=3, chiral NNLO + "
+                         "normal-ordered 3NF), its tables and one (T) kernel; no JSON line")
     ap.add_argument("--profile-run", action="store_true",
                     help="short run for counter collection: 6 FP64 iterations of the rho=0.16 lattice, "
                          "then 6 FP32 iterations of the four-density batch; no JSON line")
@@ -335,6 +338,10 @@ def main():
         sys.exit(2)
 
     ccnm = load_ccnm()
+    if args.profile_triples:
+        t3 = triples_rate(ccnm)
+        print(f"# profile (T): {t3['elements']:.3e} elements in {t3['seconds']:.3f} s", file=sys.stderr, flush=True)
+        return
     t0 = time.perf_counter()
     solvers = []
     for rho in DENSITIES:

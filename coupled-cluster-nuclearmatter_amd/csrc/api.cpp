@@ -609,6 +609,44 @@ std::vector<GemmShape> shape_candidates(int c) {
   return v;
 }
 
+// CCGPU_TUNE_CACHE=path: the tuned shapes persist across processes in a text
+// file, one line per list signature ("sig mb nb lb" for each of the NCLS
+// classes), read once and appended after each new tuning; runs that share the
+// file (profiling passes, production restarts) use identical tiles.
+void load_tune_cache() {
+  static bool loaded = false;
+  if (loaded) return;
+  loaded = true;
+  const char *path = getenv("CCGPU_TUNE_CACHE");
+  if (!path) return;
+  FILE *f = fopen(path, "r");
+  if (!f) return;
+  char sig[128];
+  while (fscanf(f, "%127s", sig) == 1) {
+    std::array<GemmShape, GemmList::NCLS> arr;
+    bool ok = true;
+    for (int c = 0; c < GemmList::NCLS && ok; ++c) {
+      int mb, nb, lb;
+      ok = fscanf(f, "%d %d %d", &mb, &nb, &lb) == 3 && (mb == 0 || gemm_shape_ok(mb, nb));
+      arr[c] = GemmShape{mb, nb, lb};
+    }
+    if (!ok) break;
+    g_tuned[sig] = arr;
+  }
+  fclose(f);
+}
+
+void save_tune_cache(const std::string &sig, const std::array<GemmShape, GemmList::NCLS> &arr) {
+  const char *path = getenv("CCGPU_TUNE_CACHE");
+  if (!path) return;
+  FILE *f = fopen(path, "a");
+  if (!f) return;
+  fprintf(f, "%s", sig.c_str());
+  for (int c = 0; c < GemmList::NCLS; ++c) fprintf(f, " %d %d %d", arr[c].mb, arr[c].nb, arr[c].lb);
+  fprintf(f, "\n");
+  fclose(f);
+}
+
 bool autotune_enabled() {
   const char *e = getenv("CCGPU_AUTOTUNE");
   return !(e && atoi(e) == 0) && !getenv("CCGPU_SHAPES");
@@ -617,6 +655,7 @@ bool autotune_enabled() {
 void autotune(GemmList &L, int kind, const SolveDesc *d_descs, hipStream_t st) {
   if (!autotune_enabled() || L.groups.empty()) return;
   const std::string sig = list_signature(L, kind);
+  load_tune_cache();
   auto it = g_tuned.find(sig);
   if (it != g_tuned.end()) {
     for (int c = 0; c < GemmList::NCLS; ++c) L.shape[c] = it->second[c];
@@ -662,6 +701,7 @@ void autotune(GemmList &L, int kind, const SolveDesc *d_descs, hipStream_t st) {
   std::array<GemmShape, GemmList::NCLS> arr;
   for (int c = 0; c < GemmList::NCLS; ++c) arr[c] = L.shape[c];
   g_tuned[sig] = arr;
+  save_tune_cache(sig, arr);
   if (!g_tune_report.empty()) g_tune_report += "; ";
   g_tune_report += rep;
 }
