@@ -24,6 +24,60 @@ const double kPMass = (double)939.565f;
 const double kPi = 3.14159265358979;
 }  // namespace
 
+// Ascending order of arr as the reference's `index` leaves it
+// (ccm_main.f90:2654-2732): quicksort with the median of three as pivot,
+// partitions below 7 elements finished by insertion, the larger part deferred
+// on a stack.  The order of equal keys is that routine's, which decides which
+// of several degenerate momenta become holes.  Returns 0-based positions.
+std::vector<int> reference_index_order(const std::vector<double> &arr) {
+  const int n = (int)arr.size();
+  std::vector<int> ix(n + 1);  // 1-based positions, ix[r] = element at rank r
+  for (int r = 0; r <= n; ++r) ix[r] = r;
+  auto key = [&](int r) { return arr[ix[r] - 1]; };
+  std::vector<int> stack;
+  int lo = 1, hi = n;
+  for (;;) {
+    if (hi - lo < 7) {
+      for (int j = lo + 1; j <= hi; ++j) {
+        const int moving = ix[j];
+        const double v = arr[moving - 1];
+        int i = j - 1;
+        for (; i >= 1 && key(i) > v; --i) ix[i + 1] = ix[i];
+        ix[i + 1] = moving;
+      }
+      if (stack.empty()) break;
+      hi = stack.back(); stack.pop_back();
+      lo = stack.back(); stack.pop_back();
+      continue;
+    }
+    std::swap(ix[(lo + hi) / 2], ix[lo + 1]);
+    if (key(lo + 1) > key(hi)) std::swap(ix[lo + 1], ix[hi]);
+    if (key(lo) > key(hi)) std::swap(ix[lo], ix[hi]);
+    if (key(lo + 1) > key(lo)) std::swap(ix[lo + 1], ix[lo]);
+    const int pivot = ix[lo];
+    const double v = arr[pivot - 1];
+    int i = lo + 1, j = hi;
+    for (;;) {
+      do ++i; while (key(i) < v);
+      do --j; while (key(j) > v);
+      if (j < i) break;
+      std::swap(ix[i], ix[j]);
+    }
+    ix[lo] = ix[j];
+    ix[j] = pivot;
+    if (hi - i + 1 >= j - lo) {
+      stack.push_back(i); stack.push_back(hi);
+      hi = j - 1;
+    } else {
+      stack.push_back(lo); stack.push_back(j - 1);
+      lo = i;
+    }
+  }
+  std::vector<int> out(n);
+  for (int r = 0; r < n; ++r) out[r] = ix[r + 1] - 1;
+  return out;
+}
+
 Orbitals build_orbitals(int A, int matter, double rho, int nmax, const double twist[3]) {
   if (A <= 0) throw std::runtime_error("A must be positive");
   if (nmax < 0 || nmax > 12) throw std::runtime_error("nmax out of range [0,12]");
@@ -57,8 +111,13 @@ Orbitals build_orbitals(int A, int matter, double rho, int nmax, const double tw
         m.order = ord++;
         mesh.push_back(m);
       }
-  std::stable_sort(mesh.begin(), mesh.end(), [](const M &a, const M &b) { return a.k2 < b.k2; });
-  for (const M &m : mesh)
+  // mesh points by k^2 in the reference's sort order (setup_sp_data,
+  // ccm_main.f90:1100-1110)
+  std::vector<double> k2(mesh.size());
+  for (size_t q = 0; q < mesh.size(); ++q) k2[q] = mesh[q].k2;
+  std::vector<M> sorted;
+  for (int q : reference_index_order(k2)) sorted.push_back(mesh[q]);
+  for (const M &m : sorted)
     for (int t : tzs)
       for (int s : {-1, 1}) {
         o.nx.push_back(m.n[0]); o.ny.push_back(m.n[1]); o.nz.push_back(m.n[2]);

@@ -76,7 +76,8 @@ struct Plan {
   int64_t t2_ordered = 0;
 };
 
-// Builds the orbital table exactly as setup_sp_data does (our tie order).
+// Builds the orbital table exactly as setup_sp_data does, degenerate momenta in
+// the order of the reference's quicksort.
 Orbitals build_orbitals(int A, int matter, double rho, int nmax, const double twist[3]);
 // Builds all channel structures and maps from an orbital table.
 void build_plan(const Orbitals &orb, Plan &plan);

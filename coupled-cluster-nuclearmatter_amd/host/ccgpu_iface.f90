@@ -98,6 +98,13 @@ module ccgpu_iface
        integer(c_int), value :: h
        complex(c_double_complex), intent(out) :: e
      end function ccgpu_energy
+     integer(c_int) function ccgpu_batch_solve(n, hs, o, r) bind(C, name='ccgpu_batch_solve')
+       import :: c_int, ccgpu_solve_opts, ccgpu_solve_result
+       integer(c_int), value :: n
+       integer(c_int), intent(in) :: hs(*)
+       type(ccgpu_solve_opts), intent(in) :: o
+       type(ccgpu_solve_result), intent(out) :: r(*)
+     end function ccgpu_batch_solve
      integer(c_int) function ccgpu_solve(h, o, r) bind(C, name='ccgpu_solve')
        import :: c_int, ccgpu_solve_opts, ccgpu_solve_result
        integer(c_int), value :: h

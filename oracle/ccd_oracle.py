@@ -35,15 +35,73 @@ P_MASS = float(np.float32(939.565))
 PI = 3.14159265358979
 
 
+def reference_index_order(arr):
+    """Ascending order of arr as the reference's `index` leaves it
+    (ccm_main.f90:2654-2732): quicksort, median of three as pivot, partitions
+    below 7 elements finished by insertion, the larger part deferred on a
+    stack.  Equal keys come out in that routine's order, which decides which of
+    several degenerate momenta become holes.  Returns 0-based positions."""
+    n = len(arr)
+    ix = list(range(n + 1))                 # 1-based: ix[r] = element at rank r
+    key = lambda r: arr[ix[r] - 1]
+    stack = []
+    lo, hi = 1, n
+    while True:
+        if hi - lo < 7:
+            for j in range(lo + 1, hi + 1):
+                moving = ix[j]
+                v = arr[moving - 1]
+                i = j - 1
+                while i >= 1 and key(i) > v:
+                    ix[i + 1] = ix[i]
+                    i -= 1
+                ix[i + 1] = moving
+            if not stack:
+                return [x - 1 for x in ix[1:]]
+            hi = stack.pop()
+            lo = stack.pop()
+            continue
+        mid = (lo + hi) // 2
+        ix[mid], ix[lo + 1] = ix[lo + 1], ix[mid]
+        if key(lo + 1) > key(hi):
+            ix[lo + 1], ix[hi] = ix[hi], ix[lo + 1]
+        if key(lo) > key(hi):
+            ix[lo], ix[hi] = ix[hi], ix[lo]
+        if key(lo + 1) > key(lo):
+            ix[lo + 1], ix[lo] = ix[lo], ix[lo + 1]
+        pivot = ix[lo]
+        v = arr[pivot - 1]
+        i, j = lo + 1, hi
+        while True:
+            i += 1
+            while key(i) < v:
+                i += 1
+            j -= 1
+            while key(j) > v:
+                j -= 1
+            if j < i:
+                break
+            ix[i], ix[j] = ix[j], ix[i]
+        ix[lo] = ix[j]
+        ix[j] = pivot
+        if hi - i + 1 >= j - lo:
+            stack += [i, hi]
+            hi = j - 1
+        else:
+            stack += [lo, j - 1]
+            lo = i
+
+
 class Lattice:
     """Single-particle plane-wave basis on the periodic cubic lattice.
 
     Orbital arrays follow setup_sp_data (ccm_main.f90:1003-1197): momenta
     k = (2 pi n + theta)/L per axis, e = hbar^2 k^2 / 2m, states ordered by k^2
     and, within one momentum, sz fastest then tz; the first A states are holes.
-    Ties in k^2 are ordered by the reference mesh loop order (our order; the
-    reference's quicksort leaves ties in an arbitrary order, so comparisons are
-    made by labels, never by orbital index).
+    Mesh points are sorted by k^2 with the reference's own quicksort
+    (reference_index_order), so degenerate momenta come out in its order and
+    orbital indices, the hole set of an open shell included, equal the
+    reference's.
     """
 
     def __init__(self, nx, ny, nz, sz, tz, kx, ky, kz, e, nholes, box, volume,
@@ -86,7 +144,7 @@ class Lattice:
                     ky = (2.0 * nyv * PI + th[1]) / L
                     kz = (2.0 * nzv * PI + th[2]) / L
                     mesh.append((kx * kx + ky * ky + kz * kz, nxv, nyv, nzv, kx, ky, kz))
-        order = sorted(range(len(mesh)), key=lambda m: (mesh[m][0], m))
+        order = reference_index_order([m[0] for m in mesh])
         cols = {k: [] for k in ("nx", "ny", "nz", "sz", "tz", "kx", "ky", "kz", "e")}
         for m in order:
             k2, nxv, nyv, nzv, kx, ky, kz = mesh[m]

@@ -7,14 +7,17 @@
 !     tolerance and iteration cap exposed as arguments (the reference hard-codes
 !     6/6, 1e-6 and 100 at ccm_iter.f90:44-45, 71-74),
 !   * an optional twist vector (the reference reads twist_angle(:,1) in
-!     setup_sp_data, ccm_main.f90:1036-1044),
+!     setup_sp_data, ccm_main.f90:1036-1044), or the twist of one TABC
+!     combination: the reference's gauss_legendre nodes on [0, pi]
+!     (ccm_main.f90:909-921) and its (nx <= ny <= nz) enumeration
+!     (ccm_main.f90:169-204), with the weight factor w w w / pi^3 (236-242),
 !   * an optional swap of every NN block for the reference's own Minnesota
 !     function vmom_minnesota (chiral_module_andreas_with_delta.f90:8980-9111),
 !   * the (T) energy through pert_triples2 (ccm_triples.f90:528-735),
 !   * binary dumps of orbitals, Fock matrix, canonical V elements and converged T2.
 !
 ! Usage: ref_driver ccm_in outdir interaction subspace diis_step tol maxit
-!                   twx twy twz dump triples
+!                   twx twy twz dump triples [ntwist combination]
 subroutine ref_driver_main() bind(C, name='ref_driver_main')
   use parallel
   use constants
@@ -25,12 +28,15 @@ subroutine ref_driver_main() bind(C, name='ref_driver_main')
   use chiral_constants
   use chiral_potentials
   use ang_mom_functions
+  use wave_functions, only: gauss_legendre
   implicit none
   character(len=512) :: inputfile, outdir, arg, interaction
   integer :: nsub, nstepdiis, maxit, dodump, dotriples, count, itimes, u
   real*4 :: tol4
   real*8 :: tstart, tloop
-  real*8 :: tol, tw(3)
+  real*8 :: tol, tw(3), tabc_w
+  integer :: ntw, combo, ia, ib, ic, kc
+  real*8, allocatable :: xg(:), wg(:)
   complex*16 :: ener1, ener2, dener
 
   call mpi_init(ierror)
@@ -51,6 +57,33 @@ subroutine ref_driver_main() bind(C, name='ref_driver_main')
   call get_command_argument(10, arg); read(arg,*) tw(3)
   call get_command_argument(11, arg); read(arg,*) dodump
   call get_command_argument(12, arg); read(arg,*) dotriples
+  ntw = 0
+  combo = 0
+  tabc_w = 1.d0
+  if (command_argument_count() >= 14) then
+     call get_command_argument(13, arg); read(arg,*) ntw
+     call get_command_argument(14, arg); read(arg,*) combo
+  end if
+  if (ntw > 0) then
+     allocate(xg(ntw), wg(ntw))
+     xg = 0.d0; wg = 0.d0
+     call gauss_legendre(0.d0, pi, xg, wg, ntw)
+     kc = 0
+     do ia = 1, ntw
+        do ib = ia, ntw
+           do ic = ib, ntw
+              kc = kc + 1
+              if (kc == combo) then
+                 tw = (/ xg(ia), xg(ib), xg(ic) /)
+                 tabc_w = 6.d0
+                 if (ia == ib .or. ib == ic) tabc_w = 3.d0
+                 if (ia == ib .and. ib == ic) tabc_w = 1.d0
+                 tabc_w = tabc_w * wg(ia) * wg(ib) * wg(ic) / (1.d0 * pi)**3
+              end if
+           end do
+        end do
+     end do
+  end if
   ! the reference compares |dE| against a default-real literal (ccm_iter.f90:72)
   tol = real(tol4, 8)
 
@@ -85,6 +118,7 @@ subroutine ref_driver_main() bind(C, name='ref_driver_main')
      open(newunit=u, file=trim(outdir)//'/energies.txt', status='replace')
   end if
   write(u,'(a,2es26.17)') 'e0 ', real(e0), aimag(e0)
+  write(u,'(a,4es26.17)') 'twist ', tw(1), tw(2), tw(3), tabc_w
   call ccd_energy_save(ener1)
   mbpt2 = e0 + ener1
   write(u,'(a,i6,2es26.17)') 'it ', 0, real(ener1), aimag(ener1)

@@ -20,10 +20,14 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(os.path.dirname(HERE))
 DRIVER = os.path.join(ROOT, "oracle", "_ref", "ref_driver")
+PROGRAM = os.path.join(ROOT, "oracle", "_ref", "ccm.exe")
 
 # name: (A, matter, rho, nmax, order, interaction, subspace, step, tol, maxit,
 #        twist, dump, triples, ranks[, cutoff MeV, default 450[, 3NF
 #        approximation 0/1/2, default off]])
+# twist is a vector, or ("tabc", ntwist, k): the k-th combination of the
+# reference's own Gauss-Legendre twists, also run through the reference program
+# ccm.exe with my_twists = k for its printed twist-weighted E0.
 CASES = {
     "pnm14_n1_nnlo": (14, "pnm", 0.16, 1, 3, "chiral", 6, 6, 1e-6, 100, (0, 0, 0), 1, 1, 1),
     "pnm14_n1_minn": (14, "pnm", 0.16, 1, 3, "minnesota", 8, 8, 1e-8, 100, (0, 0, 0), 1, 1, 1),
@@ -51,6 +55,16 @@ CASES = {
     "pnm14_n1_nnlo_3nf1": (14, "pnm", 0.16, 1, 3, "chiral", 6, 6, 1e-6, 100, (0.3, 2.1, 1.2), 1, 0, 1, 450, 1),
 }
 
+
+
+# every twist combination of the ntwist=2 TABC deck: PNM A=14, nmax=1, NNLO
+# (the second also with its T2 and blocks dumped)
+for _k in range(1, 5):
+    CASES[f"pnm14_n1_nnlo_tabc2_{_k}"] = (14, "pnm", 0.16, 1, 3, "chiral", 6, 6, 1e-6, 100, ("tabc", 2, _k),
+                                          int(_k == 2), 1, 1)
+# one combination of the ntwist=3 deck (E0 and twist only)
+CASES["pnm14_n1_nnlo_tabc3_5"] = (14, "pnm", 0.16, 1, 3, "chiral", 6, 6, 1e-6, 100, ("tabc", 3, 5), 0, 0, 1)
+
 CCM_IN = """!Chiral order for Deltas(LO = 0,NLO=2,NNLO=3,N3LO=4) and cutoff
 {order}, {cutoff}
 ! cE and cD 3nf parameters:
@@ -60,11 +74,11 @@ CCM_IN = """!Chiral order for Deltas(LO = 0,NLO=2,NNLO=3,N3LO=4) and cutoff
 ! specify: pnm/snm, input type: density/kfermi
 {matter}, density
 ! specify boundary conditions (PBC/TABC/TABCsp)
-PBC
+{bc}
 ! dens/kf, ntwist,  nmax, my_twists
-{rho},1,{nmax},1
+{rho},{ntwist},{nmax},{my_twists}
 ! specify cluster approximation: CCD, CCD(T)
-CCD
+{cc}
 ! tnf switch (T/F) and specify 3nf approximation: 0=tnf0b, 1=tnf1b, 2=tnf2b
 {tnf}, {tnf_approx}
 ! 3nf cutoff(MeV),non-local reg. exp
@@ -80,9 +94,12 @@ def run_case(name):
         inp = os.path.join(tmp, "ccm_in")
         with open(inp, "w") as fh:
             fh.write(CCM_IN.format(order=order, cutoff=cutoff, A=A, matter=matter, rho=rho, nmax=nmax,
-                                  tnf="T" if tnf_approx >= 0 else "F", tnf_approx=max(tnf_approx, 0)))
-        cmd = [DRIVER, inp, tmp, inter, str(sub), str(step), repr(tol), str(maxit),
-               repr(tw[0]), repr(tw[1]), repr(tw[2]), str(dump), str(trip)]
+                                  tnf="T" if tnf_approx >= 0 else "F", tnf_approx=max(tnf_approx, 0),
+                                  bc="PBC", ntwist=1, my_twists=1, cc="CCD"))
+        tabc = tw[0] == "tabc"
+        twargs = ["0.0", "0.0", "0.0", str(tw[1]), str(tw[2])] if tabc else [repr(x) for x in tw]
+        cmd = [DRIVER, inp, tmp, inter, str(sub), str(step), repr(tol), str(maxit)] + twargs[:3] + \
+              [str(dump), str(trip)] + twargs[3:]
         if ranks > 1:
             cmd = ["/opt/conda/bin/mpiexec", "-n", str(ranks)] + cmd
         env = dict(os.environ, OMP_NUM_THREADS="1", MKL_NUM_THREADS="1")
@@ -94,7 +111,7 @@ def run_case(name):
                "tnf_approx": np.int64(tnf_approx),
                "matter": np.array(matter), "interaction": np.array(inter),
                "rho": np.float64(rho), "tol": np.float64(np.float32(tol)),
-               "twist": np.array(tw, np.float64)}
+               "twist": np.zeros(3)}
         its = []
         for line in open(os.path.join(tmp, "energies.txt")):
             f = line.split()
@@ -107,9 +124,26 @@ def run_case(name):
             elif f[0] == "box":
                 out["box"] = np.float64(float(f[1]))
                 out["volume"] = np.float64(float(f[2]))
+            elif f[0] == "twist":
+                out["twist"] = np.array([float(x) for x in f[1:4]], np.float64)
+                if tabc:
+                    out["tabc_weight"] = np.float64(float(f[4]))
             elif f[0] == "looptime":
                 out["looptime"] = np.float64(float(f[1]))
         out["energies"] = np.array(its, np.complex128)
+        if tabc:
+            # the reference program's own TABC branch for my_twists = k
+            deck = os.path.join(tmp, "ccm_in_tabc")
+            with open(deck, "w") as fh:
+                fh.write(CCM_IN.format(order=order, cutoff=cutoff, A=A, matter=matter, rho=rho, nmax=nmax,
+                                      tnf="T" if tnf_approx >= 0 else "F", tnf_approx=max(tnf_approx, 0),
+                                      bc="TABC", ntwist=tw[1], my_twists=tw[2], cc="mbpt2"))
+            res = subprocess.run([PROGRAM, deck], env=env, cwd=tmp, capture_output=True, text=True)
+            line = [x for x in res.stdout.splitlines() if "Twist averaged E0/N" in x]
+            if res.returncode != 0 or not line:
+                raise RuntimeError(f"{name}: reference program failed\n{res.stdout[-2000:]}")
+            re_, im = line[-1].split("(")[1].rstrip(")").split(",")
+            out["tabc_e0_per_n"] = np.complex128(complex(float(re_), float(im)))
         if dump:
             with open(os.path.join(tmp, "orbitals.bin"), "rb") as fh:
                 nsp, na = np.frombuffer(fh.read(8), np.int32)

@@ -6,7 +6,7 @@ import numpy as np
 import pytest
 
 import ccd_oracle as co
-from conftest import fixture_interaction, fixture_lattice, golden
+from conftest import GOLDEN, fixture_interaction, fixture_lattice, golden
 
 pytestmark = pytest.mark.gpu
 
@@ -97,6 +97,26 @@ def hand_over_blocks(ccnm, s, g, triples=False):
     if triples:
         # vnn_hppp records (k, d, c, b) hold <cb||kd>: rows (k,d), columns (c,b)
         hand_over(s, ccnm.BLOCK_HPPP, g["hppp_idx"].astype(np.int64), g["hppp_val"])
+
+
+DUMPED = sorted(f[:-4] for f in os.listdir(GOLDEN)
+                if f.endswith(".npz") and "orb_int" in np.load(os.path.join(GOLDEN, f)).files)
+
+
+@pytest.mark.parametrize("name", DUMPED)
+def test_native_basis_equals_reference_orbital_table(ccnm, name):
+    """The engine's own basis (ccgpu_create from the deck parameters) is the
+    reference's orbital table state for state, degenerate momenta in the
+    order of its quicksort."""
+    g = golden(name)
+    A, nmax = int(g["params"][0]), int(g["params"][1])
+    s = ccnm.Solver(A=A, matter=ccnm.PNM if str(g["matter"]) == "pnm" else ccnm.SNM,
+                    rho=float(g["rho"]), nmax=nmax, twist=tuple(g["twist"]))
+    o = s.orbitals()
+    oi, od = g["orb_int"], g["orb_dbl"]
+    for r, k in enumerate(("nx", "ny", "nz", "sz", "tz")):
+        assert np.array_equal(o[k], oi[r]), k
+    assert np.allclose(o["e"], od[3], rtol=1e-15, atol=0)
 
 
 @pytest.mark.parametrize("name", ["pnm14_n1_nnlo", "snm4_n1_nnlo", "pnm14_n1_minn"])

@@ -1,11 +1,46 @@
 """Pins the oracle (oracle/ccd_oracle.py) to the compiled reference: every fixture
 in tests/golden/ was written by the reference's own routines
 (oracle/ref_driver.f90 + make_golden.py)."""
+import os
+
 import numpy as np
 import pytest
 
 import ccd_oracle as co
-from conftest import fixture_interaction, fixture_lattice, golden
+from conftest import GOLDEN, fixture_interaction, fixture_lattice, golden
+
+# fixtures that carry the reference's orbital table
+DUMPED = sorted(f[:-4] for f in os.listdir(GOLDEN)
+                if f.endswith(".npz") and "orb_int" in np.load(os.path.join(GOLDEN, f)).files)
+
+
+@pytest.mark.parametrize("name", DUMPED)
+def test_oracle_basis_equals_reference_orbital_table(name):
+    """The oracle's basis, built from the deck parameters alone, is the
+    reference's orbital table state for state: degenerate momenta in the order
+    its quicksort leaves them (so open shells get the same holes), momenta and
+    energies to rounding."""
+    g = golden(name)
+    A, nmax = int(g["params"][0]), int(g["params"][1])
+    lat = co.Lattice.build(A, str(g["matter"]), float(g["rho"]), nmax, twist=tuple(g["twist"]))
+    oi, od = g["orb_int"], g["orb_dbl"]
+    assert np.array_equal(lat.n.T, oi[:3])
+    assert np.array_equal(lat.sz, oi[3]) and np.array_equal(lat.tz, oi[4])
+    assert np.allclose(lat.k.T, od[:3], rtol=0, atol=1e-15)
+    assert np.allclose(lat.e, od[3], rtol=1e-15, atol=0)
+
+
+def test_reference_index_order_sorts_with_ties():
+    """The restated quicksort returns a permutation that sorts its keys, also
+    with many equal keys and on the short arrays its insertion pass handles."""
+    rng = np.random.default_rng(3)
+    for n in (1, 2, 6, 7, 8, 30, 125, 343, 1000):
+        for levels in (3, 50, 10 ** 9):
+            arr = list(rng.integers(0, levels, n).astype(float))
+            order = co.reference_index_order(arr)
+            assert sorted(order) == list(range(n))
+            keys = [arr[i] for i in order]
+            assert keys == sorted(arr)
 
 
 @pytest.mark.parametrize("name", ["pnm14_n1_nnlo", "snm4_n1_nnlo", "pnm14_n1_minn", "snm4_n1_minn",
