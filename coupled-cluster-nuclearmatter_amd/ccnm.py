@@ -83,6 +83,10 @@ SIGNATURES = {
     "ccgpu_energy": [_I, _DP],
     "ccgpu_solve": [_I, C.POINTER(SolveOpts), C.POINTER(SolveResult)],
     "ccgpu_triples": [_I, _DP],
+    "ccgpu_triples_parts": [_I, _DP, _DP],
+    "ccgpu_setup_t3": [_I, _DP],
+    "ccgpu_t3_intermediate": [_I],
+    "ccgpu_t3_energy": [_I, _DP],
     "ccgpu_triples_stats": [_I, C.POINTER(C.c_longlong), _DP],
     "ccgpu_get_t2": [_I, _I, _IP, _IP, _IP, _IP, _DP],
     "ccgpu_set_t2": [_I, _I, _IP, _IP, _IP, _IP, _DP],
@@ -277,6 +281,26 @@ class Solver:
     def pert_triples2(self):
         x = np.zeros(2)
         _check(lib().ccgpu_triples(self.h, _dp(x)))
+        return complex(x[0], x[1])
+
+    def triples_parts(self):
+        """(et_4, et_5) of the last pert_triples2 call."""
+        a, b = np.zeros(2), np.zeros(2)
+        _check(lib().ccgpu_triples_parts(self.h, _dp(a), _dp(b)))
+        return complex(a[0], a[1]), complex(b[0], b[1])
+
+    # CCDT-1 (the CCDT1 branch of ccd_iter, ccm_iter.f90:26-92)
+    def setup_t3(self):
+        x = np.zeros(2)
+        _check(lib().ccgpu_setup_t3(self.h, _dp(x)))
+        return complex(x[0], x[1])
+
+    def t3_intermediate(self):
+        _check(lib().ccgpu_t3_intermediate(self.h))
+
+    def t3_energy(self):
+        x = np.zeros(2)
+        _check(lib().ccgpu_t3_energy(self.h, _dp(x)))
         return complex(x[0], x[1])
 
     def triples_stats(self):

@@ -61,7 +61,7 @@ void launch_t3_tables(const SolveDesc &d, const TriplesDesc &td, double volume, 
                       hipStream_t st);
 void launch_triples(const SolveDesc &d, const TriplesDesc &td, double volume, double2 *partial,
                     unsigned long long *count, int nblocks, double2 *out, unsigned long long *count_out,
-                    hipStream_t st);
+                    hipStream_t st, int mode);
 // mlp.hip
 void launch_mlp_forward(const float *params, const uint16_t *feat, int64_t n, float *y, hipStream_t st);
 void launch_mlp_loss_grad(const float *params, const uint16_t *feat, const float *target, const int64_t *idx,
@@ -308,12 +308,20 @@ struct Solve {
   ChiralModel chiral = chiral_model(3, 450.0, nullptr);   // constants of CCGPU_INT_CHIRAL
   ChiralDev chiral_dev{};  // its device tables (set by the generator)
   // three-nucleon force of CCGPU_INT_CHIRAL: -1 off, else the normal-ordering
-  // level (0 E0, 1 and Fock, 2 and the two-body blocks) and its constants
+  // level (0 E0, 1 and Fock, 2 and 3 and the two-body blocks; 3 also puts the
+  // three-body element into (T)) and its constants; tnf_set once the
+  // constants were given (they also define the residual three-body element
+  // W3 of CCDT-1 when the force is not normal-ordered)
   int tnf = -1;
+  bool tnf_set = false;
   double tnf_ce = 0.017, tnf_cd = 0.79, tnf_cutoff = 450.0;
   int tnf_nexp = 3;
   Chiral3Dev chiral3{};
   bool have_t3 = false;
+  bool have_w3 = false;          // t3.c3w holds the W3 constants
+  bool have_ccdt1 = false;       // setup_t3 done
+  double2 ccdt1_e3{0.0, 0.0};    // energy of the last t3_intermediate
+  double2 t3_et4{0.0, 0.0}, t3_et5{0.0, 0.0};   // parts of the last (T)
   TriplesDesc t3{};
   double2 *t3_partial = nullptr, *t3_out = nullptr;
   unsigned long long *t3_count = nullptr;   // per block, then the total at [8192]
@@ -1416,9 +1424,12 @@ int ccgpu_set_chiral(int handle, int order, double cutoff, const double *ci) {
 int ccgpu_set_3nf(int handle, int approx, double cE, double cD, double cutoff, int nexp) {
   return guarded([&] {
     Solve &S = get(handle);
-    if (approx < -1 || approx > 2) throw Error(fmt("3NF approximation %d: -1 off, 0, 1 or 2", approx));
-    if (approx >= 0 && (cutoff <= 0.0 || nexp < 1)) throw Error("3NF regulator needs cutoff > 0 and nexp >= 1");
+    if (approx < -1 || approx > 3) throw Error(fmt("3NF approximation %d: -1 off, 0, 1, 2 or 3", approx));
+    if (cutoff <= 0.0 || nexp < 1) throw Error("3NF regulator needs cutoff > 0 and nexp >= 1");
     S.tnf = approx;
+    S.tnf_set = true;
+    S.have_w3 = false;
+    S.have_ccdt1 = false;
     S.tnf_ce = cE;
     S.tnf_cd = cD;
     S.tnf_cutoff = cutoff;
@@ -1458,7 +1469,7 @@ int ccgpu_generate_interaction(int handle, int kind) {
     S.have_v = S.have_fock = S.derived = true;
     S.native_v = true;
     S.vkind = kind;
-    S.have_t3 = false;
+    S.have_t3 = S.have_w3 = S.have_ccdt1 = false;
     // V changed: an FP32 copy and cached work lists of this handle are stale
     if (S.desc.Vpppp32) {
       launch_to_f32(S.desc.Vpppp, S.desc.Vpppp32, P.nVpp, S.stream);
@@ -1588,7 +1599,7 @@ int ccgpu_set_block(int handle, int kind, int nbra, const int *bra_p, const int 
             }
           }
         }
-        S.have_t3 = false;
+        S.have_t3 = S.have_w3 = S.have_ccdt1 = false;
         break;
       }
       default:
@@ -1968,26 +1979,89 @@ void build_triples(Solve &S) {
   S.have_t3 = true;
 }
 
+// One sum over the triples (modes of k_triples_w): returns it and counts the
+// elements.
+double2 triples_sum(Solve &S, int mode) {
+  if (!S.have_t3) build_triples(S);
+  if (mode != 0 && !S.have_w3) {
+    if (!(S.native_v && S.vkind == CCGPU_INT_CHIRAL))
+      throw Error("the three-nucleon element W3 needs the built-in chiral interaction");
+    if (!S.tnf_set) throw Error("the three-nucleon element W3 needs its constants: call ccgpu_set_3nf");
+    const Plan &P = S.plan;
+    Chiral3Dev c = chiral3_constants(S.chiral, S.tnf_ce, S.tnf_cd, S.tnf_cutoff, S.tnf_nexp);
+    c.kx = S.desc.kx; c.ky = S.desc.ky; c.kz = S.desc.kz;
+    c.nx = S.desc.nx; c.ny = S.desc.ny; c.nz = S.desc.nz; c.sz = S.desc.sz; c.tz = S.desc.tz;
+    c.scale = std::pow(197.326968, 6) / (P.orb.volume * P.orb.volume);
+    S.t3.c3w = c;
+    S.have_w3 = true;
+  }
+  if (mode != 2 && !(S.t3.vhppp && S.t3.vhhhp) && mode != 0)
+    throw Error("CCDT-1 and the three-body (T) need the element tables: raise CCGPU_T3_TABLE_GB");
+  int nb = (int)std::min<int64_t>(8192, std::max<int64_t>(1, S.t3.npairs));
+  launch_triples(S.desc, S.t3, S.plan.orb.volume, S.t3_partial, S.t3_count, nb, S.t3_out, S.t3_count + 8192,
+                 S.stream, mode);
+  check_launch();
+  double2 r;
+  unsigned long long cnt = 0;
+  HIPCHK(hipMemcpyAsync(&r, S.t3_out, sizeof(double2), hipMemcpyDeviceToHost, S.stream));
+  HIPCHK(hipMemcpyAsync(&cnt, S.t3_count + 8192, sizeof(cnt), hipMemcpyDeviceToHost, S.stream));
+  HIPCHK(hipStreamSynchronize(S.stream));
+  S.t3_elements = (long long)cnt;
+  return r;
+}
+
 int ccgpu_triples(int handle, double *et) {
   return guarded([&] {
     Solve &S = get(handle);
     ensure_ready(S);
     if (!S.have_t) throw Error("no amplitudes: solve CCD first");
-    if (!S.have_t3) build_triples(S);
-    int nb = (int)std::min<int64_t>(8192, std::max<int64_t>(1, S.t3.npairs));
     auto t0 = std::chrono::steady_clock::now();
-    launch_triples(S.desc, S.t3, S.plan.orb.volume, S.t3_partial, S.t3_count, nb, S.t3_out, S.t3_count + 8192,
-                   S.stream);
-    check_launch();
-    double2 r;
-    unsigned long long cnt = 0;
-    HIPCHK(hipMemcpyAsync(&r, S.t3_out, sizeof(double2), hipMemcpyDeviceToHost, S.stream));
-    HIPCHK(hipMemcpyAsync(&cnt, S.t3_count + 8192, sizeof(cnt), hipMemcpyDeviceToHost, S.stream));
-    HIPCHK(hipStreamSynchronize(S.stream));
+    S.t3_et4 = triples_sum(S, 0);
+    S.t3_et5 = double2{0.0, 0.0};
+    if (S.tnf == 3) S.t3_et5 = triples_sum(S, 3);
     S.t3_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-    S.t3_elements = (long long)cnt;
+    const double2 r = S.tnf == 3 ? S.t3_et5 : S.t3_et4;
     et[0] = r.x;
     et[1] = r.y;
+  });
+}
+
+int ccgpu_triples_parts(int handle, double *et4, double *et5) {
+  return guarded([&] {
+    Solve &S = get(handle);
+    et4[0] = S.t3_et4.x; et4[1] = S.t3_et4.y;
+    et5[0] = S.t3_et5.x; et5[1] = S.t3_et5.y;
+  });
+}
+
+int ccgpu_setup_t3(int handle, double *e3) {
+  return guarded([&] {
+    Solve &S = get(handle);
+    ensure_ready(S);
+    const double2 r = triples_sum(S, 2);
+    S.have_ccdt1 = true;
+    S.ccdt1_e3 = r;
+    e3[0] = r.x;
+    e3[1] = r.y;
+  });
+}
+
+int ccgpu_t3_intermediate(int handle) {
+  return guarded([&] {
+    Solve &S = get(handle);
+    ensure_ready(S);
+    if (!S.have_ccdt1) throw Error("CCDT-1 not set up: call ccgpu_setup_t3 first");
+    if (!S.have_t) throw Error("no amplitudes: call ccgpu_mbpt2_start first");
+    S.ccdt1_e3 = triples_sum(S, 1);
+  });
+}
+
+int ccgpu_t3_energy(int handle, double *e3) {
+  return guarded([&] {
+    Solve &S = get(handle);
+    if (!S.have_ccdt1) throw Error("CCDT-1 not set up: call ccgpu_setup_t3 first");
+    e3[0] = S.ccdt1_e3.x;
+    e3[1] = S.ccdt1_e3.y;
   });
 }
 

@@ -104,6 +104,9 @@ struct TriplesDesc {
   // vnn_hhhp: ccm_iter.f90:1641-1690)
   int tnf;
   Chiral3Dev c3;
+  // the antisymmetrized three-body element W3 = <abc||ijk> (chiral_3nf_asym)
+  // of the CCDT-1 triples and of (T) at tnf_approx 3
+  Chiral3Dev c3w;
 };
 
 // Warm-start network: number of parameters (flat FP32 vector, layout in

@@ -275,11 +275,48 @@ __global__ __launch_bounds__(256) void k_triples(SolveDesc d, TriplesDesc td, do
   }
 }
 
+// chiral3_asym with the six ket permutations in a loop over one inlined copy
+// of the direct element (fewer registers); the same terms summed in the same
+// order, so the value is identical.
+__device__ __forceinline__ double2 w3_element(const Chiral3Dev &c, int a, int b, int cc, int i, int j, int k) {
+  const double fb = freg3_of(c, a, b, cc);
+  const int ket[3] = {i, j, k};
+  double2 r = make_double2(0.0, 0.0);
+#pragma unroll 1
+  for (int p = 0; p < 6; ++p) {
+    // (k,l,q) +, (l,k,q) -, (q,l,k) -, (k,q,l) -, (l,q,k) +, (q,k,l) +
+    const int x = p == 0 || p == 3 ? 0 : (p == 1 || p == 4 ? 1 : 2);
+    const int y = p == 1 || p == 5 ? 0 : (p == 0 || p == 2 ? 1 : 2);
+    const int z = 3 - x - y;
+    const double2 v = chiral3_direct(c, fb, a, b, cc, ket[x], ket[y], ket[z]);
+    if (p == 0 || p >= 4) {
+      r.x += v.x;
+      r.y += v.y;
+    } else {
+      r.x -= v.x;
+      r.y -= v.y;
+    }
+  }
+  return r;
+}
+
 // Table mode, one wave per particle triple: a block takes particle pairs
 // (a,b), its four waves take c = b+1+wave, b+5+wave, ..., and the 64 lanes of
 // a wave stride over the hole triples of that (K3,Tz3) channel.  Every lane of
 // a wave reads the same <c.||..> and <..||.c> table rows (cache-friendly), and
 // no lane waits for a longer hole-triple list than its wave's.
+//
+// With W the 18-term numerator (pert_triples2, ccm_triples.f90:611-700), W3 =
+// <abc||ijk> the antisymmetrized three-nucleon element (chiral_3nf_asym) and D
+// = f_i + f_j + f_k - f_a - f_b - f_c, the sum over a<b<c, i<j<k is
+//   MODE 0: |W|^2 / D                 (T), et_4
+//   MODE 1: (W3 + W) conj(W3) / D     CCDT-1 energy of T3 = (W3 + W)/D
+//           (t3_intermediate, ccm_t3_eqn.f90:1-192; t3_energy,
+//           ccm_t3full_channels.f90:570-667)
+//   MODE 2: |W3|^2 / D                CCDT-1 energy of the start T3 = W3/D
+//           (setup_t3, ccm_t3full_channels.f90:291-567)
+//   MODE 3: |W + W3|^2 / D            (T) at tnf_approx 3, et_5 (:656-722)
+template <int MODE>
 __global__ __launch_bounds__(256) void k_triples_w(SolveDesc d, TriplesDesc td, double volume, double2 *partial,
                                                    unsigned long long *count) {
   __shared__ double sre[256], sim[256];
@@ -303,22 +340,40 @@ __global__ __launch_bounds__(256) void k_triples_w(SolveDesc d, TriplesDesc td, 
       for (int h = h0 + lane; h < h1; h += 64) {
         const int i = td.hijk[3 * h], j = td.hijk[3 * h + 1], k = td.hijk[3 * h + 2];
         double2 w = make_double2(0.0, 0.0);
+        if constexpr (MODE != 2) {
 #pragma unroll 1
-        for (int t = 0; t < 18; ++t) {
-          const TermCode tc = kTerms[t];
-          const int x = sel3(tc.p0, a, b, c), y = sel3(tc.p1, a, b, c), z = sel3(tc.p2, a, b, c);
-          const int u = sel3(tc.h0, i, j, k), v = sel3(tc.h1, i, j, k), q = sel3(tc.h2, i, j, k);
-          const double2 f = tc.kind == 0 ? f_term<true>(d, td, volume, x, y, z, u, v, q)
-                                         : h_term<true>(d, td, volume, x, y, z, u, v, q);
-          w.x += (double)tc.sign * f.x;
-          w.y += (double)tc.sign * f.y;
+          for (int t = 0; t < 18; ++t) {
+            const TermCode tc = kTerms[t];
+            const int x = sel3(tc.p0, a, b, c), y = sel3(tc.p1, a, b, c), z = sel3(tc.p2, a, b, c);
+            const int u = sel3(tc.h0, i, j, k), v = sel3(tc.h1, i, j, k), q = sel3(tc.h2, i, j, k);
+            const double2 f = tc.kind == 0 ? f_term<true>(d, td, volume, x, y, z, u, v, q)
+                                           : h_term<true>(d, td, volume, x, y, z, u, v, q);
+            w.x += (double)tc.sign * f.x;
+            w.y += (double)tc.sign * f.y;
+          }
         }
         const double dre = d.fdiag[i].x + d.fdiag[j].x + d.fdiag[k].x - fa - fb - fc;
         const double dim = d.fdiag[i].y + d.fdiag[j].y + d.fdiag[k].y - fai - fbi - fci;
-        const double ww = w.x * w.x + w.y * w.y;
         const double den = dre * dre + dim * dim;
-        re += ww * dre / den;
-        im += -ww * dim / den;
+        // numerator n, then n / D = n conj(D) / |D|^2
+        double nre, nim = 0.0;
+        if constexpr (MODE == 0) {
+          nre = w.x * w.x + w.y * w.y;
+        } else {
+          const double2 w3 = w3_element(td.c3w, a, b, c, i, j, k);
+          if constexpr (MODE == 1) {
+            const double xr = w3.x + w.x, xi = w3.y + w.y;
+            nre = xr * w3.x + xi * w3.y;
+            nim = xi * w3.x - xr * w3.y;
+          } else if constexpr (MODE == 2) {
+            nre = w3.x * w3.x + w3.y * w3.y;
+          } else {
+            const double xr = w3.x + w.x, xi = w3.y + w.y;
+            nre = xr * xr + xi * xi;
+          }
+        }
+        re += (nre * dre + nim * dim) / den;
+        im += (nim * dre - nre * dim) / den;
       }
     }
   }
@@ -375,10 +430,16 @@ void launch_t3_tables(const SolveDesc &d, const TriplesDesc &td, double volume, 
 
 void launch_triples(const SolveDesc &d, const TriplesDesc &td, double volume, double2 *partial,
                     unsigned long long *count, int nblocks, double2 *out, unsigned long long *count_out,
-                    hipStream_t st) {
+                    hipStream_t st, int mode) {
   static const int variant = getenv("CCGPU_T3_KERNEL") ? atoi(getenv("CCGPU_T3_KERNEL")) : 1;
-  if (td.vhppp && td.vhhhp && variant == 1)
-    hipLaunchKernelGGL(k_triples_w, dim3(nblocks), dim3(256), 0, st, d, td, volume, partial, count);
+  if (mode == 1)
+    hipLaunchKernelGGL(k_triples_w<1>, dim3(nblocks), dim3(256), 0, st, d, td, volume, partial, count);
+  else if (mode == 2)
+    hipLaunchKernelGGL(k_triples_w<2>, dim3(nblocks), dim3(256), 0, st, d, td, volume, partial, count);
+  else if (mode == 3)
+    hipLaunchKernelGGL(k_triples_w<3>, dim3(nblocks), dim3(256), 0, st, d, td, volume, partial, count);
+  else if (td.vhppp && td.vhhhp && variant == 1)
+    hipLaunchKernelGGL(k_triples_w<0>, dim3(nblocks), dim3(256), 0, st, d, td, volume, partial, count);
   else if (td.vhppp && td.vhhhp)
     hipLaunchKernelGGL(k_triples<true>, dim3(nblocks), dim3(256), 0, st, d, td, volume, partial, count);
   else

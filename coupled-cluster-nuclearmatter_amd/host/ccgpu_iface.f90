@@ -116,6 +116,25 @@ module ccgpu_iface
        integer(c_int), value :: h
        complex(c_double_complex), intent(out) :: e
      end function ccgpu_triples
+     integer(c_int) function ccgpu_triples_parts(h, e4, e5) bind(C, name='ccgpu_triples_parts')
+       import :: c_int, c_double_complex
+       integer(c_int), value :: h
+       complex(c_double_complex), intent(out) :: e4, e5
+     end function ccgpu_triples_parts
+     integer(c_int) function ccgpu_setup_t3(h, e) bind(C, name='ccgpu_setup_t3')
+       import :: c_int, c_double_complex
+       integer(c_int), value :: h
+       complex(c_double_complex), intent(out) :: e
+     end function ccgpu_setup_t3
+     integer(c_int) function ccgpu_t3_intermediate(h) bind(C, name='ccgpu_t3_intermediate')
+       import :: c_int
+       integer(c_int), value :: h
+     end function ccgpu_t3_intermediate
+     integer(c_int) function ccgpu_t3_energy(h, e) bind(C, name='ccgpu_t3_energy')
+       import :: c_int, c_double_complex
+       integer(c_int), value :: h
+       complex(c_double_complex), intent(out) :: e
+     end function ccgpu_t3_energy
      integer(c_int) function ccgpu_last_error(buf, n) bind(C, name='ccgpu_last_error')
        import :: c_int, c_char
        character(kind=c_char), intent(out) :: buf(*)

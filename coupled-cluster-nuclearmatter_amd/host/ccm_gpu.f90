@@ -5,7 +5,9 @@
 ! and MBPT2 start through the C ABI, runs the ccd_iter loop (ccm_iter.f90:60-135)
 ! with one library call per reference routine (t2_intermediate, diis_t1t2,
 ! ccd_energy_save), optionally (T), and prints the reference's result block
-! (ccm_main.f90:322-348).
+! (ccm_main.f90:322-348).  cc_approx CCDT1 runs the reference's CCDT-1 branch
+! of ccd_iter (ccm_iter.f90:26-92): the CCD iteration of T2 with the energy of
+! the triples amplitude T3 = (W3 + W(T2))/D added every iteration.
 !
 ! One optional line after the reference deck sets engine options:
 !   ! interaction, DIIS subspace, DIIS step, tolerance, max iterations, precision, fp32 switch
@@ -39,7 +41,8 @@ program ccm_gpu
   real(4) :: tol4
   real(8), parameter :: pi = 3.14159265358979d0
   logical :: tnf_switch
-  complex(c_double_complex) :: e0, ener1, ener2, dener, et, mbpt2, eccsd, eccsdt
+  complex(c_double_complex) :: e0, ener1, ener2, dener, et, mbpt2, eccsd, eccsdt, e3, et4, et5
+  logical :: ccdt1
   type(ccgpu_lattice_params) :: p
   type(ccgpu_dims) :: dims
   real(8) :: t_start, t_end
@@ -78,8 +81,14 @@ program ccm_gpu
      ce = 0.d0
      cd = 0.d0
   end if
-  if (trim(cc_approx) /= 'CCD' .and. trim(cc_approx) /= 'CCD(T)' .and. trim(cc_approx) /= 'mbpt2') then
+  if (trim(cc_approx) /= 'CCD' .and. trim(cc_approx) /= 'CCD(T)' .and. trim(cc_approx) /= 'mbpt2' &
+       .and. trim(cc_approx) /= 'CCDT1') then
      write(0,'(2a)') 'ccm_gpu: unsupported cluster approximation ', trim(cc_approx)
+     error stop 3
+  end if
+  ccdt1 = trim(cc_approx) == 'CCDT1'
+  if (ccdt1 .and. trim(interaction) /= 'chiral') then
+     write(0,'(a)') 'ccm_gpu: CCDT1 needs the chiral three-nucleon force'
      error stop 3
   end if
 
@@ -115,6 +124,10 @@ program ccm_gpu
   tw_weight = 1.d0
   if (trim(boundary_conditions) == 'TABC') then
      if (my_twists == 0) then
+        if (ccdt1) then
+           write(0,'(a)') 'ccm_gpu: CCDT1 runs one twist per run (my_twists >= 1)'
+           error stop 3
+        end if
         call tabc_all()
         stop
      end if
@@ -139,6 +152,12 @@ program ccm_gpu
   call ccgpu_check(ccgpu_mbpt2_start(h, ener1), 'ccgpu_mbpt2_start')
   mbpt2 = e0 + ener1
   write(6,*) 'CCD energy', ener1 + e0
+  if (ccdt1) then
+     ! setup_t3 and the first t3_energy (ccm_iter.f90:26-40)
+     call ccgpu_check(ccgpu_setup_t3(h, e3), 'ccgpu_setup_t3')
+     write(6,*) 'CCDT-1 energy', e3
+     ener1 = ener1 + e3
+  end if
   eccsd = 0.d0
   eccsdt = 0.d0
 
@@ -155,6 +174,7 @@ program ccm_gpu
         if (abs(dener) > tol4r) then
            st = CCGPU_PREC_FP64
            if (fp32_phase) st = prec
+           if (ccdt1) call ccgpu_check(ccgpu_t3_intermediate(h), 'ccgpu_t3_intermediate')
            call ccgpu_check(ccgpu_residual(h, st), 'ccgpu_residual')
            count = count + 1
            nstep = nstep + 1
@@ -162,6 +182,11 @@ program ccm_gpu
            extrapolated = count == 0
            call ccgpu_check(ccgpu_energy(h, ener2), 'ccgpu_energy')
            write(6,*) 'CCD energy', ener2 + e0
+           if (ccdt1) then
+              call ccgpu_check(ccgpu_t3_energy(h, e3), 'ccgpu_t3_energy')
+              write(6,*) 'CCDT-1 energy', e3
+              ener2 = ener2 + e3
+           end if
            dener = ener2 - ener1
            ener1 = ener2
            write(6,'(a,2i5,2e16.8)') 'dener', 0, itimes, real(dener)
@@ -180,10 +205,12 @@ program ccm_gpu
      eccsd = e0 + ener2
      write(6,*) 'Total execution time CCD', t_end - t_start
      if (trim(cc_approx) == 'CCD(T)') then
+        ! et_4, or et_5 at tnf_approx 3 (ccm_triples.f90:725-732)
         call ccgpu_check(ccgpu_triples(h, et), 'ccgpu_triples')
+        call ccgpu_check(ccgpu_triples_parts(h, et4, et5), 'ccgpu_triples_parts')
         eccsdt = eccsd + et
         write(6,'(a,1x,2(F16.5,2x),a,1x,2(F16.5,3x))') 'Kf, Dens', dims%kf, dens, 'CCSD(T)', &
-             real(et / below_ef), 0.d0
+             real(et4 / below_ef), real(et5 / below_ef)
      end if
   end if
 
@@ -234,8 +261,14 @@ contains
             'ccgpu_set_chiral')
        ! tnf switch and approximation, 3NF regulator (ccm_main.f90:899-900): the
        ! normal ordering of normal_ordered_hamiltonian / setup_3nf_2bNO
-       if (tnf_switch) call ccgpu_check(ccgpu_set_3nf(hh, int(tnf_approx, c_int), real(ce, c_double), &
-            real(cd, c_double), real(cutoff3, c_double), int(nexp, c_int)), 'ccgpu_set_3nf')
+       if (tnf_switch) then
+          call ccgpu_check(ccgpu_set_3nf(hh, int(tnf_approx, c_int), real(ce, c_double), &
+               real(cd, c_double), real(cutoff3, c_double), int(nexp, c_int)), 'ccgpu_set_3nf')
+       else if (ccdt1) then
+          ! not normal-ordered; the constants of the CCDT-1 three-body element
+          call ccgpu_check(ccgpu_set_3nf(hh, -1_c_int, real(ce, c_double), &
+               real(cd, c_double), real(cutoff3, c_double), int(nexp, c_int)), 'ccgpu_set_3nf')
+       end if
        call ccgpu_check(ccgpu_generate_interaction(hh, CCGPU_INT_CHIRAL), 'ccgpu_generate_interaction')
     else
        call ccgpu_check(ccgpu_generate_interaction(hh, CCGPU_INT_MINNESOTA), 'ccgpu_generate_interaction')

@@ -49,6 +49,12 @@
  *   ccgpu_energy                      ccd_energy_save (ccm_energy.f90:4-71)
  *   ccgpu_solve                       ccd_iter loop (ccm_iter.f90:1-170)
  *   ccgpu_triples                     pert_triples2 (ccm_triples.f90:528-735)
+ *   ccgpu_triples_parts               its et_4 and et_5 (:706-732)
+ *   ccgpu_setup_t3                    setup_t3 + the first t3_energy of the
+ *                                     CCDT1 branch (ccm_iter.f90:26-40;
+ *                                     ccm_t3full_channels.f90:291-667)
+ *   ccgpu_t3_intermediate             t3_intermediate (ccm_t3_eqn.f90:1-192)
+ *   ccgpu_t3_energy                   t3_energy (ccm_t3full_channels.f90:570-667)
  *   ccgpu_get_t2 / ccgpu_set_t2       reads/writes of t2_ccm(ch)%val
  *   ccgpu_batch_solve                 no reference counterpart: the reference
  *                                     runs one twist per process
@@ -174,11 +180,13 @@ int ccgpu_generate_interaction(int handle, int kind);
 int ccgpu_set_chiral(int handle, int order, double cutoff, const double *ci);
 
 /* NNLO three-nucleon force of CCGPU_INT_CHIRAL, normal-ordered with respect to
- * the holes: approx -1 off (default), 0 E0, 1 E0 and Fock, 2 E0, Fock and the
- * two-body blocks (and the <cb||kd> elements of (T)), as tnf_approx of ccm_in.
- * cE, cD as the deck's second line (zero below NNLO); cutoff (MeV) and nexp
- * of its regulator as the deck's last line.  Call before
- * ccgpu_generate_interaction. */
+ * the holes: approx -1 not normal-ordered (default), 0 E0, 1 E0 and Fock, 2 E0,
+ * Fock and the two-body blocks (and the <cb||kd> elements of (T)), 3 as 2 and
+ * (T) with the three-body element in its numerator, as tnf_approx of ccm_in
+ * (tnf_switch F is -1).  cE, cD as the deck's second line (zero below NNLO);
+ * cutoff (MeV) and nexp of its regulator as the deck's last line.  The same
+ * constants define the three-body element <abc||ijk> of CCDT-1, also at -1.
+ * Call before ccgpu_generate_interaction. */
 int ccgpu_set_3nf(int handle, int approx, double cE, double cD, double cutoff, int nexp);
 
 /* <pq||rs> of the generated NN interaction (without the normal-ordered 3NF)
@@ -207,7 +215,23 @@ int ccgpu_residual(int handle, int precision);
 int ccgpu_update(int handle, int *count, int *nstep, int subspace, int diis_step);
 int ccgpu_energy(int handle, double *ecorr);                     /* ecorr[2] */
 int ccgpu_solve(int handle, const ccgpu_solve_opts *opts, ccgpu_solve_result *res);
+/* The (T) energy the reference adds to the CCD energy: et_4 = sum |W|^2/D, or
+ * at tnf_approx 3 et_5 = sum |W + W3|^2/D, over a<b<c, i<j<k with W3 =
+ * <abc||ijk> of the three-nucleon force. */
 int ccgpu_triples(int handle, double *et);                       /* et[2] */
+/* et_4 and et_5 of the last ccgpu_triples call (et_5 is zero below
+ * tnf_approx 3), the two numbers of the reference's 'CCSD(T)' line. */
+int ccgpu_triples_parts(int handle, double *et4, double *et5);  /* [2] each */
+/* CCDT-1 (cc_approx 'CCDT1'; needs the built-in chiral interaction and the
+ * 3NF constants of ccgpu_set_3nf): the iteration of T2 is that of CCD, and
+ * every energy adds sum T3 conj(W3) over a<b<c, i<j<k of the triples
+ * amplitude T3 = (W3 + W(T2))/D.  ccgpu_setup_t3 returns that energy for the
+ * start T3 = W3/D; ccgpu_t3_intermediate builds T3 from the current T2 (call
+ * it before ccgpu_residual, as the reference does) and ccgpu_t3_energy returns
+ * its energy (add it to the ccgpu_energy of the same iteration). */
+int ccgpu_setup_t3(int handle, double *e3);                      /* e3[2] */
+int ccgpu_t3_intermediate(int handle);
+int ccgpu_t3_energy(int handle, double *e3);                     /* e3[2] */
 /* Of the last ccgpu_triples call: the number of (abc, ijk) elements summed
  * (a<b<c particles, i<j<k holes, same total momentum and isospin) and its
  * wall time in seconds (kernels and the result copy). */

@@ -567,10 +567,22 @@ class CCD:
         return energies
 
     # ccm_triples.f90:174-294, 528-735 and ccm_t3full_channels.f90:4-284
-    def triples(self, T, vhppp=None):
-        """E(T) = sum over (K3,Tz3) channels of |W|^2 / D over a<b<c, i<j<k.
-        The particle term uses conj(<cb||kd>) as the reference does
-        (three_body_oneparticle_1 reads conj(vnn_hppp(kd,cb)))."""
+    def triples(self, T, vhppp=None, mode="t", w3=None):
+        """Sums over the (K3,Tz3) channels and a<b<c, i<j<k of the triples
+        amplitude numerator W(T2) (the 18 terms of pert_triples2) and D =
+        f_i + f_j + f_k - f_a - f_b - f_c.  The particle term uses
+        conj(<cb||kd>) as the reference does (three_body_oneparticle_1 reads
+        conj(vnn_hppp(kd,cb))).
+
+        mode "t":     |W|^2 / D, the (T) energy et_4 (ccm_triples.f90:528-735)
+        mode "td":    |W + W3|^2 / D, its et_5 at tnf_approx 3 (:656-722)
+        mode "ccdt1": (W3 + W) conj(W3) / D, the CCDT-1 energy of T3 =
+                      (W3 + W)/D (t3_intermediate, ccm_t3_eqn.f90:1-192;
+                      t3_energy, ccm_t3full_channels.f90:570-667)
+        mode "w3":    |W3|^2 / D, the CCDT-1 energy of the start T3 = W3/D
+                      (setup_t3, ccm_t3full_channels.f90:291-567)
+        with W3 = w3(a, b, c, i, j, k), the antisymmetrized three-body element
+        <abc||ijk> (chiral_3nf_asym)."""
         lat, ch = self.lat, self.ch
         A, nsp = lat.A, lat.nsp
         vh = self.v if vhppp is None else vhppp
@@ -636,6 +648,10 @@ class CCD:
             for (i, j, k) in hh3[key]:
                 for (a, b, c) in plist:
                     spen = f[i] + f[j] + f[k] - f[a] - f[b] - f[c]
+                    x3 = 0.0 if mode == "t" else complex(w3(a, b, c, i, j, k))
+                    if mode == "w3":
+                        et += x3 * np.conj(x3) / spen
+                        continue
                     s = (one_particle(a, b, c, i, j, k) - one_particle(b, a, c, i, j, k)
                          - one_particle(c, b, a, i, j, k) - one_particle(a, b, c, k, j, i)
                          - one_particle(a, b, c, i, k, j) + one_particle(c, b, a, i, k, j)
@@ -646,7 +662,12 @@ class CCD:
                           - one_hole(a, b, c, j, i, k) + one_hole(c, b, a, j, i, k)
                           + one_hole(c, b, a, k, j, i) + one_hole(a, c, b, j, i, k)
                           + one_hole(a, c, b, k, j, i))
-                    et += np.conj(s) * s / spen
+                    if mode == "t":
+                        et += np.conj(s) * s / spen
+                    elif mode == "td":
+                        et += np.conj(s + x3) * (s + x3) / spen
+                    else:
+                        et += (x3 + s) * np.conj(x3) / spen
         return et
 
 

@@ -53,6 +53,10 @@ CASES = {
     "snm4_n1_nnlo_3nf": (4, "snm", 0.16, 1, 3, "chiral", 6, 6, 1e-6, 100, (0, 0, 0), 1, 1, 1, 450, 2),
     "snm4_n1_nlo450_3nf": (4, "snm", 0.16, 1, 2, "chiral", 6, 6, 1e-6, 20, (0, 0, 0), 1, 0, 1, 450, 2),
     "pnm14_n1_nnlo_3nf1": (14, "pnm", 0.16, 1, 3, "chiral", 6, 6, 1e-6, 100, (0.3, 2.1, 1.2), 1, 0, 1, 450, 1),
+    # tnf_approx 3: two-body normal ordering, and (T) with the residual 3NF
+    # element in the triples amplitude (pert_triples2's et_5)
+    "snm4_n1_nnlo_3nf3": (4, "snm", 0.16, 1, 3, "chiral", 6, 6, 1e-6, 100, (0, 0, 0), 1, 1, 1, 450, 3),
+    "pnm14_n1_nnlo_3nf3": (14, "pnm", 0.16, 1, 3, "chiral", 6, 6, 1e-6, 100, (0.3, 1.1, 2.0), 0, 1, 1, 450, 3),
 }
 
 
@@ -64,6 +68,50 @@ for _k in range(1, 5):
                                           int(_k == 2), 1, 1)
 # one combination of the ntwist=3 deck (E0 and twist only)
 CASES["pnm14_n1_nnlo_tabc3_5"] = (14, "pnm", 0.16, 1, 3, "chiral", 6, 6, 1e-6, 100, ("tabc", 3, 5), 0, 0, 1)
+
+# Runs of the reference program itself (oracle/_ref/ccm.exe) on a deck, parsed
+# from its standard output: E0, every 'CCD energy' line (E0 + the T2
+# correlation energy: the MBPT2 start, then one per iteration), every
+# 'CCDT-1 energy' line (the T3 energy: the start, then one per iteration) and
+# the final 8-number result line.
+# name: (A, matter, rho, nmax, order, cutoff, tnf approximation, boundary
+#        conditions, ntwist, my_twists, cluster approximation)
+PROGRAM_CASES = {
+    "snm4_n1_nnlo_3nf_ccdt1": (4, "snm", 0.16, 1, 3, 450, 2, "PBC", 1, 1, "CCDT1"),
+    "pnm14_n1_nnlo_3nf_ccdt1": (14, "pnm", 0.16, 1, 3, 450, 2, "PBC", 1, 1, "CCDT1"),
+    "pnm14_n1_nnlo_3nf_ccdt1_tabc2_3": (14, "pnm", 0.16, 1, 3, 450, 2, "TABC", 2, 3, "CCDT1"),
+}
+
+
+def run_program_case(name):
+    A, matter, rho, nmax, order, cutoff, tnf_approx, bc, ntw, myt, cc = PROGRAM_CASES[name]
+    with tempfile.TemporaryDirectory() as tmp:
+        deck = os.path.join(tmp, "ccm_in")
+        with open(deck, "w") as fh:
+            fh.write(CCM_IN.format(order=order, cutoff=cutoff, A=A, matter=matter, rho=rho, nmax=nmax,
+                                  tnf="T" if tnf_approx >= 0 else "F", tnf_approx=max(tnf_approx, 0),
+                                  bc=bc, ntwist=ntw, my_twists=myt, cc=cc))
+        env = dict(os.environ, OMP_NUM_THREADS="8", MKL_NUM_THREADS="1")
+        res = subprocess.run([PROGRAM, deck], env=env, cwd=tmp, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError(f"{name}: reference program failed\n{res.stdout[-2000:]}")
+
+    def cplx(line):
+        re_, im = line.split("(")[1].split(")")[0].split(",")
+        return complex(float(re_), float(im))
+    lines = res.stdout.splitlines()
+    out = {"params": np.array([A, nmax, 6, 6, 100, order], np.int64), "cutoff": np.float64(cutoff),
+           "tnf_approx": np.int64(tnf_approx), "matter": np.array(matter), "interaction": np.array("chiral"),
+           "rho": np.float64(rho), "tol": np.float64(np.float32(1e-6)), "cc": np.array(cc),
+           "boundary": np.array(bc), "ntwist": np.int64(ntw), "my_twists": np.int64(myt)}
+    out["e0"] = np.complex128(cplx([x for x in lines if "Vacuum expectation value" in x][0]))
+    out["ccd_energies"] = np.array([cplx(x) for x in lines if x.strip().startswith("CCD energy")], np.complex128)
+    out["e3"] = np.array([cplx(x) for x in lines if "CCDT-1 energy" in x], np.complex128)
+    tail = [x for x in lines if x.startswith("     0.01700")]
+    out["result_line"] = np.array([float(v) for v in tail[-1].split()], np.float64)
+    np.savez_compressed(os.path.join(HERE, name + ".npz"), **out)
+    print(name, "iterations", len(out["e3"]) - 1, "final", out["ccd_energies"][-1] + out["e3"][-1])
+
 
 CCM_IN = """!Chiral order for Deltas(LO = 0,NLO=2,NNLO=3,N3LO=4) and cutoff
 {order}, {cutoff}
@@ -172,6 +220,9 @@ def run_case(name):
 
 
 if __name__ == "__main__":
-    names = sys.argv[1:] or list(CASES)
+    names = sys.argv[1:] or list(CASES) + list(PROGRAM_CASES)
     for n in names:
-        run_case(n)
+        if n in PROGRAM_CASES:
+            run_program_case(n)
+        else:
+            run_case(n)

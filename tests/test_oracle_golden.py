@@ -124,3 +124,47 @@ def test_oracle_triples_against_reference(name):
     et = cc.triples(T, vhppp=hv)
     ref = g["eccdt"] - g["eccd"]
     assert abs(et - ref) < 1e-12 * max(1.0, abs(ref))
+
+
+def test_oracle_ccdt1_energies_against_reference_program():
+    """The CCDT-1 triples energy (setup_t3, t3_intermediate, t3_energy) of the
+    oracle on the reference's own V, Fock and <cb||kd> blocks (SNM A=4,
+    |n_i|<=1, chiral NNLO with the normal-ordered NNLO 3NF) against the
+    'CCDT-1 energy' lines the reference program prints: the start T3 = W3/D,
+    and the first iteration's T3 = (W3 + W(T2))/D from the MBPT2 amplitudes
+    (this pins the sign of the cross term W conj(W3))."""
+    import chiral_oracle as ch
+    g = golden("snm4_n1_nnlo_3nf")
+    p = golden("snm4_n1_nnlo_3nf_ccdt1")
+    assert abs(p["result_line"][4] * 4 - g["e0"].real) < 4e-5
+    assert abs(p["ccd_energies"][0] - (g["e0"] + g["energies"][0])) < 1e-10 * abs(g["e0"])
+    lat = fixture_lattice(g)
+    cc = co.CCD(lat, fixture_interaction(g, lat), fock=g["fock"], e0=g["e0"])
+    T = cc.mbpt2_start()
+    h = g["hppp_idx"].astype(np.int64) - 1
+    hv = co.TableInteraction(lat.nsp, [(h[:, 2], h[:, 3], h[:, 0], h[:, 1], g["hppp_val"])])
+    c3 = ch.ThreeBodyConstants(ch.ChiralConstants(ch.NNLO, 450.0), 0.017, 0.79, 450.0, 3)
+    w3 = lambda a, b, c, i, j, k: ch.chiral3_asym(lat, c3, a, b, c, i, j, k)
+    e30 = cc.triples(T, vhppp=hv, mode="w3", w3=w3)
+    assert abs(e30 - p["e3"][0]) < 1e-9 * abs(p["e3"][0]), (e30, p["e3"][0])
+    e31 = cc.triples(T, vhppp=hv, mode="ccdt1", w3=w3)
+    assert abs(e31 - p["e3"][1]) < 1e-9 * abs(p["e3"][1]), (e31, p["e3"][1])
+
+
+def test_oracle_triples_with_residual_3nf_against_reference():
+    """(T) at tnf_approx 3 (pert_triples2's et_5: the antisymmetrized 3NF
+    element added to the triples numerator) on the reference's converged T2,
+    V and <cb||kd> blocks (SNM A=4, |n_i|<=1, NNLO with the normal-ordered
+    3NF)."""
+    import chiral_oracle as ch
+    g = golden("snm4_n1_nnlo_3nf3")
+    lat = fixture_lattice(g)
+    cc = co.CCD(lat, fixture_interaction(g, lat), fock=g["fock"], e0=g["e0"])
+    T = t_from_fixture(cc, g)
+    h = g["hppp_idx"].astype(np.int64) - 1
+    hv = co.TableInteraction(lat.nsp, [(h[:, 2], h[:, 3], h[:, 0], h[:, 1], g["hppp_val"])])
+    c3 = ch.ThreeBodyConstants(ch.ChiralConstants(ch.NNLO, 450.0), 0.017, 0.79, 450.0, 3)
+    w3 = lambda a, b, c, i, j, k: ch.chiral3_asym(lat, c3, a, b, c, i, j, k)
+    et5 = cc.triples(T, vhppp=hv, mode="td", w3=w3)
+    ref = g["eccdt"] - g["eccd"]
+    assert abs(et5 - ref) < 1e-10 * abs(ref), (et5, ref)
