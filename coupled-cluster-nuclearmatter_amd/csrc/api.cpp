@@ -1029,12 +1029,14 @@ void run_batch(Batch &B, const ccgpu_solve_opts &op, std::vector<ccgpu_solve_res
   // DIIS schedule) until
   //  (a) a DIIS extrapolation that follows an iteration with max |dE| <=
   //      fp32_switch (fp32_switch > 0), or
-  //  (b) every solve's low-precision |dE| <= tol: that stopping test is met.
+  //  (b) every solve has met the stopping test |dE| <= tol in low precision
+  //      at least once (solves that met it keep iterating meanwhile).
   // From the next iteration on the contractions are FP64 and the DIIS history
   // carries on; at least one FP64 iteration follows, so the stopping test that
   // ends a solve is always an FP64 one.  No solve ends inside the first phase.
   bool fp32_phase = op.precision != CCGPU_PREC_FP64;
   double prev_maxd = 1e300;
+  std::vector<char> reached(n, 0);
   for (int it = 1; it <= op.maxit; ++it) {
     bool any = false;
     for (int s = 0; s < n; ++s) {
@@ -1075,12 +1077,14 @@ void run_batch(Batch &B, const ccgpu_solve_opts &op, std::vector<ccgpu_solve_res
       iters[s] = it;
       if (prec != CCGPU_PREC_FP64) ++iters32[s];
       maxd = std::max(maxd, std::abs(dener[s]));
+      if (std::abs(dener[s]) <= op.tol) reached[s] = 1;
       if (op.verbose) printf("dener %5d %5d %16.8E  E %.12f%s\n", s, it, dener[s].real(), e2[s].real(),
                              prec == CCGPU_PREC_FP32 ? "  fp32" : prec == CCGPU_PREC_V32 ? "  v32" : "");
     }
     if (fp32_phase) {
       const bool at_boundary = op.fp32_switch > 0.0 && extrapolated && prev_maxd <= op.fp32_switch;
-      const bool at_tol = maxd <= op.tol;
+      bool at_tol = true;
+      for (int s = 0; s < n; ++s) at_tol = at_tol && (reached[s] || !B.h_descs[s].active);
       if (at_boundary || at_tol) {
         fp32_phase = false;
         // an FP32 |dE| is no FP64 convergence test: force at least one FP64 step
