@@ -972,6 +972,11 @@ void update_launches(Batch &B, int &count, int &nstep, int subspace, int diis_st
   }
 }
 
+}  // namespace
+// one sum over the triples, defined with the (T) entry points below
+extern "C" double2 triples_sum(Solve &S, int mode);
+namespace {
+
 std::vector<std::complex<double>> energies(Batch &B, bool timing) {
   const int n = (int)B.solves.size();
   {
@@ -1021,6 +1026,19 @@ void run_batch(Batch &B, const ccgpu_solve_opts &op, std::vector<ccgpu_solve_res
     res[s].embpt2[0] = S.e0.x + e1[s].real();
     res[s].embpt2[1] = S.e0.y + e1[s].imag();
   }
+  // CCDT-1 (the CCDT1 branch of ccd_iter, ccm_iter.f90:26-92): the start adds
+  // the energy of T3 = W3/D; every iteration builds T3 from the T2 before the
+  // update and adds its energy to that of the updated T2
+  const bool ccdt1 = op.ccdt1 != 0;
+  std::vector<std::complex<double>> e3(n, 0.0);
+  if (ccdt1)
+    for (int s = 0; s < n; ++s) {
+      Solve &S = *B.solves[s];
+      S.ccdt1_e3 = triples_sum(S, 2);
+      S.have_ccdt1 = true;
+      e1[s] += std::complex<double>(S.ccdt1_e3.x, S.ccdt1_e3.y);
+      elast[s] = e1[s];
+    }
   int count = 0, nstep = 0;
   auto t0 = std::chrono::steady_clock::now();
   bool graph_ok = op.use_graph != 0 && !timing && gemm_variant_override() < 0;
@@ -1050,6 +1068,13 @@ void run_batch(Batch &B, const ccgpu_solve_opts &op, std::vector<ccgpu_solve_res
       prec = op.precision;
       if (refine && it % refine == 0) prec = CCGPU_PREC_FP64;
     }
+    if (ccdt1)
+      for (int s = 0; s < n; ++s) {
+        if (!B.h_descs[s].active) continue;
+        Solve &S = *B.solves[s];
+        S.ccdt1_e3 = triples_sum(S, 1);
+        e3[s] = std::complex<double>(S.ccdt1_e3.x, S.ccdt1_e3.y);
+      }
     if (graph_ok) {
       const int q = prec;
       if (!B.gexec[q]) {
@@ -1072,6 +1097,7 @@ void run_batch(Batch &B, const ccgpu_solve_opts &op, std::vector<ccgpu_solve_res
     double maxd = 0;
     for (int s = 0; s < n; ++s) {
       if (!B.h_descs[s].active) continue;
+      if (ccdt1) e2[s] += e3[s];
       dener[s] = e2[s] - elast[s];
       elast[s] = e2[s];
       iters[s] = it;

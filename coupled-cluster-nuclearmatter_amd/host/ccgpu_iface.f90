@@ -31,6 +31,7 @@ module ccgpu_iface
      real(c_double) :: tol
      integer(c_int) :: maxit, precision, refine_every, use_graph, verbose
      real(c_double) :: fp32_switch
+     integer(c_int) :: ccdt1 = 0, pad_ = 0
   end type ccgpu_solve_opts
 
   type, bind(C) :: ccgpu_solve_result

@@ -25,7 +25,7 @@
 ! reference does (ccm_main.f90:163-245, where the MBPT2 and CCD sums are
 ! commented out and print as zero).  my_twists = 0 solves every combination in
 ! one batched solve and prints the weighted twist averages of E0, MBPT2, CCD
-! and CCD(T) (the accumulation the reference leaves commented out).
+! (or CCDT-1) and CCD(T) (the accumulation the reference leaves commented out).
 program ccm_gpu
   use iso_c_binding
   use ccgpu_iface
@@ -124,10 +124,6 @@ program ccm_gpu
   tw_weight = 1.d0
   if (trim(boundary_conditions) == 'TABC') then
      if (my_twists == 0) then
-        if (ccdt1) then
-           write(0,'(a)') 'ccm_gpu: CCDT1 runs one twist per run (my_twists >= 1)'
-           error stop 3
-        end if
         call tabc_all()
         stop
      end if
@@ -366,6 +362,8 @@ contains
        o%use_graph = 1
        o%verbose = 0
        o%fp32_switch = fp32_switch
+       o%ccdt1 = 0
+       if (ccdt1) o%ccdt1 = 1
        call ccgpu_check(ccgpu_batch_solve(int(ncomb, c_int), hs, o, res), 'ccgpu_batch_solve')
        do k = 1, ncomb
           cck(k) = cmplx(res(k)%eccd(1), res(k)%eccd(2), c_double_complex)

@@ -137,6 +137,11 @@ typedef struct {
                         fp32_switch (MeV; <= 0: never there), or once every
                         FP32 |dE| <= tol; FP64 afterwards with the DIIS history
                         kept, and at least one FP64 iteration before a stop */
+  int ccdt1;         /* 1: CCDT-1 energies (ccgpu_setup_t3 / t3_intermediate /
+                        t3_energy inside the loop, as the CCDT1 branch of
+                        ccd_iter); needs ccgpu_set_3nf and the chiral
+                        interaction.  0: CCD */
+  int pad_;
 } ccgpu_solve_opts;
 
 typedef struct {

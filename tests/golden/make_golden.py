@@ -79,8 +79,10 @@ CASES["pnm14_n1_nnlo_tabc3_5"] = (14, "pnm", 0.16, 1, 3, "chiral", 6, 6, 1e-6, 1
 PROGRAM_CASES = {
     "snm4_n1_nnlo_3nf_ccdt1": (4, "snm", 0.16, 1, 3, 450, 2, "PBC", 1, 1, "CCDT1"),
     "pnm14_n1_nnlo_3nf_ccdt1": (14, "pnm", 0.16, 1, 3, 450, 2, "PBC", 1, 1, "CCDT1"),
-    "pnm14_n1_nnlo_3nf_ccdt1_tabc2_3": (14, "pnm", 0.16, 1, 3, 450, 2, "TABC", 2, 3, "CCDT1"),
 }
+# every combination of the ntwist=2 TABC deck with CCDT-1
+for _k in range(1, 5):
+    PROGRAM_CASES[f"pnm14_n1_nnlo_3nf_ccdt1_tabc2_{_k}"] = (14, "pnm", 0.16, 1, 3, 450, 2, "TABC", 2, _k, "CCDT1")
 
 
 def run_program_case(name):

@@ -150,3 +150,61 @@ def test_triples_residual_3nf_native_twisted(ccnm):
     et = s.pert_triples2()
     ref = g["eccdt"] - g["eccd"]
     assert abs(et - ref) < 1e-6 * abs(ref), (et, ref)
+
+
+def test_ccdt1_inside_the_device_loop(ccnm):
+    """ccgpu_solve with ccdt1 = 1 (the whole CCDT1 loop on the device side of
+    the C ABI) against the reference program: iteration count and final
+    energy (PNM A=14, |n_i|<=1, NNLO with the normal-ordered 3NF)."""
+    p = golden("pnm14_n1_nnlo_3nf_ccdt1")
+    s = native_3nf(ccnm, 14, "pnm", 0.16, 1, (0.0, 0.0, 0.0), 2)
+    s.setup_t_amplitudes()
+    r = s.ccd_iter(subspace=6, diis_step=6, tol=TOL, maxit=100, ccdt1=1)
+    ref = p["ccd_energies"][-1] + p["e3"][-1]
+    assert r["iterations"] == len(p["e3"]) - 1, (r["iterations"], len(p["e3"]) - 1)
+    assert abs(r["eccd"] - ref) < 1e-7 * abs(p["ccd_energies"][0]) + 20 * TOL, (r["eccd"], ref)
+    assert abs(r["embpt2"] - p["ccd_energies"][0]) < 1e-7 * abs(p["ccd_energies"][0])
+
+
+def test_fortran_host_tabc_all_twists_ccdt1(tmp_path):
+    """my_twists = 0 with CCDT1: the four twist combinations of the ntwist=2
+    deck in one batched CCDT-1 solve; the printed twist-averaged CCD(T-1)
+    energy is the weighted sum of the reference program's per-twist results
+    and each twist takes the program's iteration count."""
+    import itertools
+    exe = os.path.join(ROOT, "coupled-cluster-nuclearmatter_amd", "host", "ccm_gpu")
+    if not os.path.exists(exe):
+        pytest.skip("Fortran host not built (flang absent)")
+    deck = tmp_path / "ccm_in"
+    deck.write_text("""!Chiral order for Deltas(LO = 0,NLO=2,NNLO=3,N3LO=4) and cutoff
+3, 450
+! cE and cD 3nf parameters:
+ 0.017, 0.79
+! number of particles
+14
+! specify: pnm/snm, input type: density/kfermi
+pnm, density
+! specify boundary conditions (PBC/TABC/TABCsp)
+TABC
+! dens/kf, ntwist,  nmax, my_twists
+0.16,2,1,0
+! specify cluster approximation: CCD, CCD(T)
+CCDT1
+! tnf switch (T/F) and specify 3nf approximation: 0=tnf0b, 1=tnf1b, 2=tnf2b
+T, 2
+! 3nf cutoff(MeV),non-local reg. exp
+450, 3
+""")
+    res = subprocess.run([exe, str(deck)], capture_output=True, text=True, timeout=120)
+    assert res.returncode == 0, res.stdout[-2000:] + res.stderr
+    x, w = np.polynomial.legendre.leggauss(2)
+    w = w * np.pi / 2
+    weights = [len(set(itertools.permutations((a, b, c)))) * w[a] * w[b] * w[c] / np.pi ** 3
+               for a in range(2) for b in range(a, 2) for c in range(b, 2)]
+    ps = [golden(f"pnm14_n1_nnlo_3nf_ccdt1_tabc2_{k}") for k in range(1, 5)]
+    cc = sum(wt * (p["ccd_energies"][-1] + p["e3"][-1]) for wt, p in zip(weights, ps)) / 14
+    line = [l for l in res.stdout.splitlines() if l.strip().startswith("Twist averaged CCSD")][-1]
+    got = complex(*(float(v) for v in line.split("(")[1].split(")")[0].split(",")))
+    assert abs(got - cc) < 1e-7 * abs(cc) + 20 * TOL / 14, (got, cc)
+    rows = [l.split() for l in res.stdout.splitlines() if l.startswith("twist")]
+    assert [int(r[-1]) for r in rows] == [len(p["e3"]) - 1 for p in ps]
