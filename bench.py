@@ -23,6 +23,8 @@ within +-20% of their defaults (the solver's own output); its inference is
 inside the timed region.  A "step" is one whole job
 of the fast arm (four converged solves).  The timed region starts with the
 interaction, Fock matrix and tables already in HBM.  Prints one JSON line.
+Beside the headline the JSON carries (T) on the symmetric-matter deck (SNM
+A=132 |n_i|<=3 with the normal-ordered 3NF) and CCDT-1 on PNM A=66 |n_i|<=2.
 """
 from __future__ import annotations
 
@@ -194,6 +196,33 @@ def triples_rate(ccnm):
             "e_ccd_per_n": round(r["eccd"].real / (2 * A), 8), "e_t_per_n": round(et.real / (2 * A), 8),
             "elements": n, "seconds": round(sec, 4), "tables_s": round(first - sec, 3),
             "elements_per_s": round(n / sec, 1)}
+
+
+def ccdt1_rate(ccnm):
+    """CCDT-1 (the reference's CCDT1 loop) on PNM A=66 |n_i|<=2, rho=0.16,
+    chiral NNLO(450) plus the NNLO 3NF normal-ordered to two-body: one T3
+    energy sum (the (T) numerator and the three-body element per (abc, ijk)
+    element) timed alone, then the whole solve to |dE| <= 1e-6 MeV with the
+    reference's DIIS 6/6."""
+    import numpy as np
+    s = ccnm.Solver(A=A, matter=ccnm.PNM, rho=0.16, nmax=2)
+    s.set_chiral(CHIRAL_ORDER, CUTOFF)
+    s.set_3nf(2, 0.017, 0.79, 450.0, 3)
+    s.generate_interaction(ccnm.INT_CHIRAL)
+    s.setup_t_amplitudes()
+    s.setup_t3()
+    n, _ = s.triples_stats()
+    t = time.perf_counter()
+    s.t3_intermediate()
+    sec = time.perf_counter() - t
+    s.setup_t_amplitudes()
+    t = time.perf_counter()
+    r = s.ccd_iter(subspace=6, diis_step=6, tol=float(np.float32(1e-6)), maxit=MAXIT, ccdt1=1)
+    solve = time.perf_counter() - t
+    return {"workload": "PNM A=66 |n_i|<=2, rho 0.16, chiral NNLO(450) + NNLO 3NF normal-ordered to two-body, "
+                        "CCDT-1 to 1e-6 MeV (DIIS 6/6)",
+            "elements": n, "t3_sum_s": round(sec, 4), "elements_per_s": round(n / sec, 1),
+            "solve_s": round(solve, 3), "iterations": r["iterations"], "e_per_n": round(r["eccd"].real / A, 8)}
 
 
 def learned_study(ccnm, solvers, net):
@@ -435,7 +464,11 @@ def main():
     study = learned_study(ccnm, solvers, net)
     twists = twist_batch(ccnm)
     t3 = triples_rate(ccnm)
+    t3["ccdt1"] = ccdt1_rate(ccnm)
     print(f"# (T): {t3['elements']:.3e} elements in {t3['seconds']:.3f} s", file=sys.stderr, flush=True)
+    c1 = t3["ccdt1"]
+    print(f"# CCDT-1: {c1['elements']:.3e} elements per T3 sum in {c1['t3_sum_s']:.3f} s; solve {c1['solve_s']:.2f} s, "
+          f"{c1['iterations']} iterations", file=sys.stderr, flush=True)
     print(f"# twists: sequential {twists['sequential_fp64_s']:.3f} s, batched FP64 {twists['batched_fp64_s']:.3f} s, "
           f"batched mixed {twists['batched_mixed_s']:.3f} s", file=sys.stderr, flush=True)
     cpu = None if args.no_cpu_baseline else cpu_baseline([r["iterations"] for r in plain])
