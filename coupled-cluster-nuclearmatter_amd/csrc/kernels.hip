@@ -703,10 +703,11 @@ __global__ __launch_bounds__(256) void k_zgemm_valu(const GemmGroup *__restrict_
 // block, shared by every row tile of the channel) comes through L2.  The four
 // wavefronts of a workgroup stack along M, each owning MB 16-row blocks and the
 // workgroup's NB 16-column blocks.  The sum over k of one 16x16x4 MFMA is
-// permutation invariant, so lane group g = lane>>4 takes the k indices
-// k0+4g..k0+4g+3 of a 16-wide k chunk (one 64-byte read of A per lane, a full
-// 128-byte line per row across four lanes) and the four MFMAs of the chunk run
-// over those four k values.
+// permutation invariant, so MFMA q of a 16-wide k chunk takes k0+4q..k0+4q+3,
+// lane group g = lane>>4 holding k0+4q+g: each load instruction then reads 64
+// contiguous bytes of every row (the four lane groups side by side), half the
+// cache lines per instruction of a per-lane-contiguous layout, which matters
+// because these loads are limited by the L1 tag path.
 // FAM only names the instantiation (1 = pp ladder, 2 = pp ladder on the FP32
 // copy of V with FP64 arithmetic) so profiles can tell the pp-ladder launches
 // from other families with the same tile grid.
@@ -747,7 +748,7 @@ __global__ __launch_bounds__(256) void k_zgemm_dl(const GemmGroup *__restrict__ 
       const int mc = min(m, g.M - 1);
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
-        const int k = k0 + lg * 4 + q;
+        const int k = k0 + 4 * q + lg;
         const double2 v = load_a<AT>(A + (int64_t)mc * g.lda + min(k, ke - 1));
         const bool ok = m < g.M && k < ke;
         a[x][q] = make_double2(ok ? v.x : 0.0, ok ? v.y : 0.0);
@@ -759,7 +760,7 @@ __global__ __launch_bounds__(256) void k_zgemm_dl(const GemmGroup *__restrict__ 
       const int nc = min(n, g.N - 1);
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
-        const int k = k0 + lg * 4 + q;
+        const int k = k0 + 4 * q + lg;
         const double2 v = gload(g.B + (int64_t)min(k, ke - 1) * g.ldb + nc);
         const bool ok = n < g.N && k < ke;
         b[y][q] = make_double2(ok ? v.x : 0.0, ok ? v.y : 0.0);
@@ -872,7 +873,7 @@ __global__ __launch_bounds__(256) void k_zgemm_lb(const GemmGroup *__restrict__ 
       const int mc = min(m, g.M - 1);
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
-        const int k = k0 + lg * 4 + q;
+        const int k = k0 + 4 * q + lg;
         const double2 v = ccgpu::load_a<AT>(A + (int64_t)mc * g.lda + min(k, ke - 1));
         const bool ok = m < g.M && k < ke;
         a[x][q] = make_double2(ok ? v.x : 0.0, ok ? v.y : 0.0);
@@ -903,7 +904,7 @@ __global__ __launch_bounds__(256) void k_zgemm_lb(const GemmGroup *__restrict__ 
     for (int q = 0; q < 4; ++q) {
       double2 b[NB];
 #pragma unroll
-      for (int y = 0; y < NB; ++y) b[y] = Bs[buf][lg * 4 + q][y * 16 + li];
+      for (int y = 0; y < NB; ++y) b[y] = Bs[buf][4 * q + lg][y * 16 + li];
 #pragma unroll
       for (int x = 0; x < MB; ++x) {
         const real ar = (real)a[x][q].x, ai = (real)a[x][q].y;
