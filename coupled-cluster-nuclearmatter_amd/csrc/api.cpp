@@ -972,10 +972,8 @@ void update_launches(Batch &B, int &count, int &nstep, int subspace, int diis_st
   }
 }
 
-}  // namespace
 // one sum over the triples, defined with the (T) entry points below
-extern "C" double2 triples_sum(Solve &S, int mode);
-namespace {
+double2 triples_sum(Solve &S, int mode);
 
 std::vector<std::complex<double>> energies(Batch &B, bool timing) {
   const int n = (int)B.solves.size();
@@ -1420,6 +1418,8 @@ int ccgpu_get_orbitals(int handle, int *nx, int *ny, int *nz, int *sz, int *tz, 
   });
 }
 
+}  // extern "C"
+namespace {
 // Device tables of the chiral NN for this lattice: radial functions over
 // m = |dn|^2 <= 3 (4 nmax)^2 and the per-side factors over j <= 3 (2 nmax)^2.
 void make_chiral_tables(Solve &S) {
@@ -1443,6 +1443,8 @@ void make_chiral_tables(Solve &S) {
   for (int i = 0; i < 7; ++i) d.cnlo[i] = c.cnlo[i];
   for (int i = 0; i < 2; ++i) { d.cg0[i] = c.cg0[i]; d.cg1[i] = c.cg1[i]; }
 }
+}  // namespace
+extern "C" {
 
 int ccgpu_set_chiral(int handle, int order, double cutoff, const double *ci) {
   return guarded([&] {
@@ -1881,6 +1883,8 @@ int ccgpu_get_t2_labels(int handle, int *abij) {
   });
 }
 
+}  // extern "C"
+namespace {
 // (K3,Tz3) hole-triple channels, orbital lookup and particle-pair list of
 // setup_t3_channel_structures (ccm_t3full_channels.f90:4-284), built once.
 void build_triples(Solve &S) {
@@ -2039,6 +2043,8 @@ double2 triples_sum(Solve &S, int mode) {
   S.t3_elements = (long long)cnt;
   return r;
 }
+}  // namespace
+extern "C" {
 
 int ccgpu_triples(int handle, double *et) {
   return guarded([&] {
