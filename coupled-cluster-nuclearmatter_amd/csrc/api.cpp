@@ -39,6 +39,8 @@ void launch_3nf(const SolveDesc &d, const Chiral3Dev &c, int approx, int nch, in
 void launch_derive(const SolveDesc &d, int nch, int nphc, int max_npp, int max_nhh, int max_nph, int max_nhp,
                    hipStream_t st);
 void launch_to_f32(const double2 *src, float2 *dst, int64_t n, hipStream_t st);
+void launch_to_f32_rows(const double2 *src, float2 *dst, const int64_t *off, const int64_t *off32, const int *npp,
+                        int nch, int max_npp, hipStream_t st);
 void launch_mbpt2(const SolveDesc *dd, int nsolve, int64_t maxT, hipStream_t st);
 void launch_energy(const SolveDesc *dd, int nsolve, int slot, hipStream_t st);
 void launch_i7(const SolveDesc *dd, int nsolve, int maxA, hipStream_t st);
@@ -317,6 +319,12 @@ struct Solve {
   double tnf_ce = 0.017, tnf_cd = 0.79, tnf_cutoff = 450.0;
   int tnf_nexp = 3;
   Chiral3Dev chiral3{};
+  // FP32 copy of V_pppp (Vpppp32): block c at off_vpp32[c], rows padded to
+  // even length; device copies of the offset tables
+  std::vector<int64_t> off_vpp32;
+  int64_t nVpp32 = 0;
+  int64_t *d_off_vpp = nullptr, *d_off_vpp32 = nullptr;
+  int *d_npp = nullptr;
   bool have_t3 = false;
   bool have_w3 = false;          // t3.c3w holds the W3 constants
   bool have_ccdt1 = false;       // setup_t3 done
@@ -737,15 +745,21 @@ std::unique_ptr<Batch> make_batch(const std::vector<Solve *> &ss, hipStream_t st
     B->maxA = std::max(B->maxA, S.plan.A);
     B->maxNp = std::max(B->maxNp, S.plan.np);
   }
-  // FP32 pp ladder list reads the float copy of Vpppp
+  // FP32 pp ladder list reads the float copy of Vpppp (its own padded rows)
   B->g_pp32 = B->g_pp;
   for (size_t g = 0; g < B->g_pp32.groups.size(); ++g) {
     int s = B->g_pp32.gsolve[g];
     Solve &S = *ss[s];
-    const double2 *base = S.desc.Vpppp;
-    const double2 *a = (const double2 *)B->g_pp32.groups[g].A;
-    if (S.desc.Vpppp32) B->g_pp32.groups[g].A = S.desc.Vpppp32 + (a - base);
-    else B->has_f32 = false;
+    if (!S.desc.Vpppp32) {
+      B->has_f32 = false;
+      continue;
+    }
+    const int64_t off = (const double2 *)B->g_pp32.groups[g].A - S.desc.Vpppp;
+    const auto &ov = S.plan.off_vpp;
+    const int c = (int)(std::upper_bound(ov.begin(), ov.end(), off) - ov.begin()) - 1;
+    if (c < 0 || c >= S.plan.nch || ov[c] != off) throw Error("pp ladder group does not start a V_pppp block");
+    B->g_pp32.groups[g].A = S.desc.Vpppp32 + S.off_vpp32[c];
+    B->g_pp32.groups[g].lda = (S.plan.npp[c] + 1) & ~1;
   }
   const int fams[6] = {0, 1, 2, 3, 4, 1};
   GemmList *lists[6] = {&B->g_x, &B->g_pp, &B->g_hh, &B->g_i4, &B->g_rt, &B->g_pp32};
@@ -1162,14 +1176,30 @@ void drop_single(int h) {
   }
 }
 
+// (Re)fill the FP32 copy of the pppp blocks from the FP64 blocks.
+void fill_f32(Solve &S) {
+  const Plan &P = S.plan;
+  launch_to_f32_rows(S.desc.Vpppp, S.desc.Vpppp32, S.d_off_vpp, S.d_off_vpp32, S.d_npp, P.nch, P.max_npp, S.stream);
+  check_launch();
+  HIPCHK(hipStreamSynchronize(S.stream));
+}
+
 // FP32 copy of the stored pppp blocks, made on the first FP32 request; the
 // cached single-solve batch is rebuilt so that its FP32 ladder list reads it.
 void ensure_f32(Solve &S, int handle) {
   if (S.desc.Vpppp32) return;
-  S.desc.Vpppp32 = dalloc<float2>(S.plan.nVpp, S.owned);
-  launch_to_f32(S.desc.Vpppp, S.desc.Vpppp32, S.plan.nVpp, S.stream);
-  check_launch();
-  HIPCHK(hipStreamSynchronize(S.stream));
+  const Plan &P = S.plan;
+  S.off_vpp32.assign(1, 0);
+  for (int c = 0; c < P.nch; ++c) {
+    const int64_t n = P.npp[c];
+    S.off_vpp32.push_back(S.off_vpp32.back() + n * ((n + 1) & ~1));
+  }
+  S.nVpp32 = S.off_vpp32.back();
+  S.d_off_vpp = dupload(P.off_vpp, S.owned);
+  S.d_off_vpp32 = dupload(S.off_vpp32, S.owned);
+  S.d_npp = dupload(P.npp, S.owned);
+  S.desc.Vpppp32 = dalloc<float2>(S.nVpp32, S.owned);
+  fill_f32(S);
   HIPCHK(hipMemcpy(S.d_desc, &S.desc, sizeof(SolveDesc), hipMemcpyHostToDevice));
   drop_single(handle);
 }
@@ -1503,11 +1533,7 @@ int ccgpu_generate_interaction(int handle, int kind) {
     S.vkind = kind;
     S.have_t3 = S.have_w3 = S.have_ccdt1 = false;
     // V changed: an FP32 copy and cached work lists of this handle are stale
-    if (S.desc.Vpppp32) {
-      launch_to_f32(S.desc.Vpppp, S.desc.Vpppp32, P.nVpp, S.stream);
-      check_launch();
-      HIPCHK(hipStreamSynchronize(S.stream));
-    }
+    if (S.desc.Vpppp32) fill_f32(S);
   });
 }
 

@@ -655,6 +655,25 @@ __device__ __forceinline__ double2 load_a<double2>(const double2 *p) { return gl
 template <>
 __device__ __forceinline__ double2 load_a<float2>(const float2 *p) { float2 v = gload(p); return make_double2(v.x, v.y); }
 
+// The FP32 copy of V_pppp has rows of even length and 16-byte aligned starts:
+// the element pair (k, k+1) at even k is one aligned 16-byte load.
+typedef float gfloat4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ void load_pair(const float2 *p, double2 &v0, double2 &v1) {
+  const gfloat4 v = *(const __attribute__((address_space(1))) gfloat4 *)p;
+  v0 = make_double2(v.x, v.y);
+  v1 = make_double2(v.z, v.w);
+}
+
+// k index of MFMA step q (0..3) of a 16-wide chunk held by lane group lg: the
+// four lane groups side by side (64 contiguous bytes of a row per FP64 load);
+// with FP32 storage each lane loads the pair (k, k+1) for steps 2h and 2h+1
+// (also 64 contiguous bytes per load).  Any assignment of the chunk's 16 k
+// values works as long as A and B use the same one.
+template <bool PAIR>
+__device__ __forceinline__ int kstep(int q, int lg) {
+  return PAIR ? 8 * (q >> 1) + 2 * lg + (q & 1) : 4 * q + lg;
+}
+
 // Plain FP64 VALU GEMM with the same contract, kept as an independent check of
 // the MFMA lane layouts (selected with CCGPU_GEMM=valu).
 __global__ __launch_bounds__(256) void k_zgemm_valu(const GemmGroup *__restrict__ groups,
@@ -741,17 +760,30 @@ __global__ __launch_bounds__(256) void k_zgemm_dl(const GemmGroup *__restrict__ 
   const int kb = tl.kb, ke = min(tl.ke, g.K);
   // branch-free loads: out-of-range indices are clamped to a valid element and
   // the value is zeroed, so all loads of a chunk issue back to back
+  constexpr bool PAIR = std::is_same<AT, float2>::value;
   auto load = [&](double2 (&a)[MB][4], double2 (&b)[NB][4], int k0) {
 #pragma unroll
     for (int x = 0; x < MB; ++x) {
       const int m = mw + x * 16 + li;
       const int mc = min(m, g.M - 1);
+      if constexpr (PAIR) {
 #pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        const int k = k0 + 4 * q + lg;
-        const double2 v = load_a<AT>(A + (int64_t)mc * g.lda + min(k, ke - 1));
-        const bool ok = m < g.M && k < ke;
-        a[x][q] = make_double2(ok ? v.x : 0.0, ok ? v.y : 0.0);
+        for (int h = 0; h < 2; ++h) {
+          const int k = k0 + 8 * h + 2 * lg;
+          double2 v0, v1;
+          load_pair((const float2 *)A + (int64_t)mc * g.lda + min(k, (ke - 1) & ~1), v0, v1);
+          const bool ok0 = m < g.M && k < ke, ok1 = m < g.M && k + 1 < ke;
+          a[x][2 * h] = make_double2(ok0 ? v0.x : 0.0, ok0 ? v0.y : 0.0);
+          a[x][2 * h + 1] = make_double2(ok1 ? v1.x : 0.0, ok1 ? v1.y : 0.0);
+        }
+      } else {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const int k = k0 + kstep<false>(q, lg);
+          const double2 v = load_a<AT>(A + (int64_t)mc * g.lda + min(k, ke - 1));
+          const bool ok = m < g.M && k < ke;
+          a[x][q] = make_double2(ok ? v.x : 0.0, ok ? v.y : 0.0);
+        }
       }
     }
 #pragma unroll
@@ -760,7 +792,7 @@ __global__ __launch_bounds__(256) void k_zgemm_dl(const GemmGroup *__restrict__ 
       const int nc = min(n, g.N - 1);
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
-        const int k = k0 + 4 * q + lg;
+        const int k = k0 + kstep<PAIR>(q, lg);
         const double2 v = gload(g.B + (int64_t)min(k, ke - 1) * g.ldb + nc);
         const bool ok = n < g.N && k < ke;
         b[y][q] = make_double2(ok ? v.x : 0.0, ok ? v.y : 0.0);
@@ -866,17 +898,30 @@ __global__ __launch_bounds__(256) void k_zgemm_lb(const GemmGroup *__restrict__ 
     }
   double2 a0[MB][4], a1[MB][4], bn[BPT];
   const int kb = tl.kb, ke = min(tl.ke, g.K);
+  constexpr bool PAIR = std::is_same<AT, float2>::value;
   auto load_a = [&](double2 (&a)[MB][4], int k0) {
 #pragma unroll
     for (int x = 0; x < MB; ++x) {
       const int m = mw + x * 16 + li;
       const int mc = min(m, g.M - 1);
+      if constexpr (PAIR) {
 #pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        const int k = k0 + 4 * q + lg;
-        const double2 v = ccgpu::load_a<AT>(A + (int64_t)mc * g.lda + min(k, ke - 1));
-        const bool ok = m < g.M && k < ke;
-        a[x][q] = make_double2(ok ? v.x : 0.0, ok ? v.y : 0.0);
+        for (int h = 0; h < 2; ++h) {
+          const int k = k0 + 8 * h + 2 * lg;
+          double2 v0, v1;
+          load_pair((const float2 *)A + (int64_t)mc * g.lda + min(k, (ke - 1) & ~1), v0, v1);
+          const bool ok0 = m < g.M && k < ke, ok1 = m < g.M && k + 1 < ke;
+          a[x][2 * h] = make_double2(ok0 ? v0.x : 0.0, ok0 ? v0.y : 0.0);
+          a[x][2 * h + 1] = make_double2(ok1 ? v1.x : 0.0, ok1 ? v1.y : 0.0);
+        }
+      } else {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const int k = k0 + kstep<false>(q, lg);
+          const double2 v = ccgpu::load_a<AT>(A + (int64_t)mc * g.lda + min(k, ke - 1));
+          const bool ok = m < g.M && k < ke;
+          a[x][q] = make_double2(ok ? v.x : 0.0, ok ? v.y : 0.0);
+        }
       }
     }
   };
@@ -904,7 +949,7 @@ __global__ __launch_bounds__(256) void k_zgemm_lb(const GemmGroup *__restrict__ 
     for (int q = 0; q < 4; ++q) {
       double2 b[NB];
 #pragma unroll
-      for (int y = 0; y < NB; ++y) b[y] = Bs[buf][4 * q + lg][y * 16 + li];
+      for (int y = 0; y < NB; ++y) b[y] = Bs[buf][kstep<PAIR>(q, lg)][y * 16 + li];
 #pragma unroll
       for (int x = 0; x < MB; ++x) {
         const real ar = (real)a[x][q].x, ai = (real)a[x][q].y;
@@ -1163,6 +1208,34 @@ void launch_derive(const SolveDesc &d, int nch, int nphc, int max_npp, int max_n
   auto gx = [](int64_t n) { return (unsigned)std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, 8192)); };
   hipLaunchKernelGGL(k_derive_r0, dim3(gx((int64_t)max_npp * max_nhh), nch), dim3(256), 0, st, d);
   hipLaunchKernelGGL(k_derive_vt, dim3(gx((int64_t)max_nhp * max_nph), nphc), dim3(256), 0, st, d);
+}
+
+// FP32 copy of the pppp blocks in the layout of the FP32-storage ladder: block
+// c (npp x npp) at dst + off32[c] with rows of even length ld = npp rounded up,
+// so every pair of elements (k, k+1) at even k is one aligned 16-byte load; the
+// padding element of an odd row is zero.
+__global__ void k_to_f32_rows(const double2 *src, float2 *dst, const int64_t *off, const int64_t *off32,
+                              const int *npp) {
+  const int c = blockIdx.y;
+  const int n = npp[c];
+  const int ld = (n + 1) & ~1;
+  const int64_t m = (int64_t)n * ld;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < m; e += (int64_t)gridDim.x * blockDim.x) {
+    const int r = (int)(e / ld), col = (int)(e % ld);
+    float2 v = make_float2(0.f, 0.f);
+    if (col < n) {
+      const double2 x = src[off[c] + (int64_t)r * n + col];
+      v = make_float2((float)x.x, (float)x.y);
+    }
+    dst[off32[c] + e] = v;
+  }
+}
+
+void launch_to_f32_rows(const double2 *src, float2 *dst, const int64_t *off, const int64_t *off32, const int *npp,
+                        int nch, int max_npp, hipStream_t st) {
+  const int64_t m = (int64_t)max_npp * ((max_npp + 1) & ~1);
+  const unsigned g = (unsigned)std::max<int64_t>(1, std::min<int64_t>((m + 255) / 256, 1024));
+  hipLaunchKernelGGL(k_to_f32_rows, dim3(g, nch), dim3(256), 0, st, src, dst, off, off32, npp);
 }
 
 void launch_to_f32(const double2 *src, float2 *dst, int64_t n, hipStream_t st) {
