@@ -156,7 +156,7 @@ void xcd_order_list(std::vector<GemmTile> &tiles, size_t ngroups) {
 // for the VALU cross-check kernel.
 struct GemmShape {
   int mb, nb;
-  int lb = 0;   // 1: B operand staged in LDS (same tiles, same results)
+  int lb = 0;   // 1: B operand staged in LDS; 2: A and B in LDS (mb 1); same tiles, same results
 };
 bool gemm_shape_ok(int mb, int nb) {
   return (mb == 0 && nb == 4) || (mb == 1 && nb >= 1 && nb <= 5) || (mb == 2 && (nb == 1 || nb == 2)) ||
@@ -622,6 +622,7 @@ std::vector<GemmShape> shape_candidates(int c) {
   v.push_back({4, 1});
   const size_t n = v.size();
   for (size_t i = 0; i < n; ++i) v.push_back({v[i].mb, v[i].nb, 1});   // LDS-staged B variants
+  for (int nb = 1; nb <= std::min(3, c + 1); ++nb) v.push_back({1, nb, 2});   // A and B in LDS
   return v;
 }
 
@@ -643,7 +644,8 @@ void load_tune_cache() {
     bool ok = true;
     for (int c = 0; c < GemmList::NCLS && ok; ++c) {
       int mb, nb, lb;
-      ok = fscanf(f, "%d %d %d", &mb, &nb, &lb) == 3 && (mb == 0 || gemm_shape_ok(mb, nb));
+      ok = fscanf(f, "%d %d %d", &mb, &nb, &lb) == 3 && (mb == 0 || gemm_shape_ok(mb, nb)) && lb >= 0 && lb <= 2 &&
+           (lb < 2 || (mb == 1 && nb <= 3));
       arr[c] = GemmShape{mb, nb, lb};
     }
     if (!ok) break;
@@ -710,7 +712,8 @@ void autotune(GemmList &L, int kind, const SolveDesc *d_descs, hipStream_t st) {
       }
     }
     L.shape[c] = best_shape;
-    rep += fmt(" c%d=%dx%d%s(%.1fus)", c, best_shape.mb, best_shape.nb, best_shape.lb ? "L" : "", 1e3f * best / 3.f);
+    rep += fmt(" c%d=%dx%d%s(%.1fus)", c, best_shape.mb, best_shape.nb,
+               best_shape.lb == 2 ? "A" : best_shape.lb ? "L" : "", 1e3f * best / 3.f);
   }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
@@ -828,7 +831,7 @@ void run_gemm(const GemmList &L, const Batch &B, int prec, bool a_f32, hipStream
       if (L.tiles[c].empty()) continue;
       char nm[48];
       snprintf(nm, sizeof nm, "fam%d.k%d.c%d.%dx%d%s", L.fam, kind, c, L.shape[c].mb, L.shape[c].nb,
-               L.shape[c].lb ? "L" : "");
+               L.shape[c].lb == 2 ? "A" : L.shape[c].lb ? "L" : "");
       Timer t(detail, st, nm);
       launch_gemm_shape(L.d_groups, L.d_tiles[c], (int)L.tiles[c].size(), B.d_descs, L.d_gsolve, kind, L.shape[c].mb,
                         L.shape[c].nb, L.shape[c].lb, st);

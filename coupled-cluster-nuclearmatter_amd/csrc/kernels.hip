@@ -1029,6 +1029,149 @@ __global__ __launch_bounds__(256) void k_zgemm_lb(const GemmGroup *__restrict__ 
       }
 }
 
+// The same grouped GEMM with both operands staged in LDS: the 64 x 16 A chunk
+// of a workgroup (MB = 1) is read row-coalesced from global memory (16
+// consecutive threads read one row's 16 consecutive k, 1-2 cache lines per
+// quarter-wave) and written to LDS, from where each lane reads its MFMA
+// operand; B as in k_zgemm_lb.  The k order is that of k_zgemm_dl and
+// k_zgemm_lb, so results are identical across the three forms.
+template <int MB, int NB, bool F32, typename AT, int FAM = 0>
+__global__ __launch_bounds__(256) void k_zgemm_la(const GemmGroup *__restrict__ groups,
+                                                  const GemmTile *__restrict__ tiles,
+                                                  const SolveDesc *__restrict__ descs,
+                                                  const int *__restrict__ gsolve) {
+  static_assert(MB == 1, "A chunk of 64 rows");
+  typedef typename std::conditional<F32, float, double>::type real;
+  typedef typename std::conditional<F32, f4, d4>::type acc4;
+  constexpr bool PAIR = std::is_same<AT, float2>::value;
+  constexpr int BC = 16 * NB;                 // tile columns
+  constexpr int BE = 16 * BC;                 // B elements per chunk
+  constexpr int BPT = (BE + 255) / 256;       // B elements per thread
+  __shared__ double2 Bs[2][16][BC + 1];
+  __shared__ double2 As[2][64][17];
+  const GemmTile tl = tiles[blockIdx.x];
+  if (descs && !descs[gsolve[tl.g]].active) return;
+  const GemmGroup g = groups[tl.g];
+  const AT *A = (const AT *)g.A;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int li = lane & 15, lg = lane >> 4;
+  const int mw = tl.m0 + w * 16;
+  acc4 cre[NB], cim[NB], cs[NB];
+#pragma unroll
+  for (int y = 0; y < NB; ++y) {
+    cre[y] = (acc4){0, 0, 0, 0};
+    cim[y] = (acc4){0, 0, 0, 0};
+    cs[y] = (acc4){0, 0, 0, 0};
+  }
+  double2 an[4], bn[BPT];
+  const int kb = tl.kb, ke = min(tl.ke, g.K);
+  // A chunk element e = threadIdx.x + 256 j: row e >> 4 of the tile, k0 + (e & 15)
+  auto load_a = [&](int k0) {
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int e = threadIdx.x + 256 * j;
+      const int m = tl.m0 + (e >> 4), k = k0 + (e & 15);
+      const double2 v = ccgpu::load_a<AT>(A + (int64_t)min(m, g.M - 1) * g.lda + min(k, ke - 1));
+      const bool ok = m < g.M && k < ke;
+      an[j] = make_double2(ok ? v.x : 0.0, ok ? v.y : 0.0);
+    }
+  };
+  auto store_a = [&](int buf) {
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int e = threadIdx.x + 256 * j;
+      As[buf][e >> 4][e & 15] = an[j];
+    }
+  };
+  auto load_b = [&](int k0) {
+#pragma unroll
+    for (int j = 0; j < BPT; ++j) {
+      const int e = threadIdx.x + 256 * j;
+      const int kr = e / BC, c = e % BC;
+      const int k = k0 + kr, n = tl.n0 + c;
+      const double2 v = gload(g.B + (int64_t)min(k, ke - 1) * g.ldb + min(n, g.N - 1));
+      const bool ok = e < BE && k < ke && n < g.N;
+      bn[j] = make_double2(ok ? v.x : 0.0, ok ? v.y : 0.0);
+    }
+  };
+  auto store_b = [&](int buf) {
+#pragma unroll
+    for (int j = 0; j < BPT; ++j) {
+      const int e = threadIdx.x + 256 * j;
+      if (e < BE) Bs[buf][e / BC][e % BC] = bn[j];
+    }
+  };
+  auto compute = [&](int buf) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int kq = kstep<PAIR>(q, lg);
+      const double2 a = As[buf][w * 16 + li][kq];
+      const real ar = (real)a.x, ai = (real)a.y;
+#pragma unroll
+      for (int y = 0; y < NB; ++y) {
+        const double2 b = Bs[buf][kq][y * 16 + li];
+        const real br = (real)b.x, bi = (real)b.y;
+        if constexpr (F32) {
+          cre[y] = __builtin_amdgcn_mfma_f32_16x16x4f32(ar, br, cre[y], 0, 0, 0);
+          cre[y] = __builtin_amdgcn_mfma_f32_16x16x4f32(-ai, bi, cre[y], 0, 0, 0);
+          cim[y] = __builtin_amdgcn_mfma_f32_16x16x4f32(ar, bi, cim[y], 0, 0, 0);
+          cim[y] = __builtin_amdgcn_mfma_f32_16x16x4f32(ai, br, cim[y], 0, 0, 0);
+        } else {
+          cre[y] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar, br, cre[y], 0, 0, 0);
+          cim[y] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai, bi, cim[y], 0, 0, 0);
+          cs[y] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar + ai, br + bi, cs[y], 0, 0, 0);
+        }
+      }
+    }
+  };
+  load_b(kb);
+  load_a(kb);
+  store_b(0);
+  store_a(0);
+  __syncthreads();
+  int buf = 0;
+  for (int k0 = kb; k0 < ke; k0 += 16) {
+    const bool more = k0 + 16 < ke;
+    if (more) {
+      load_b(k0 + 16);
+      load_a(k0 + 16);
+    }
+    compute(buf);
+    if (!more) break;
+    store_b(buf ^ 1);
+    store_a(buf ^ 1);
+    __syncthreads();
+    buf ^= 1;
+  }
+#pragma unroll
+  for (int y = 0; y < NB; ++y)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int row = F32 ? (lg * 4 + r) : (lg + 4 * r);
+      const int m = mw + row;
+      const int n = tl.n0 + y * 16 + li;
+      double vr, vi;
+      if constexpr (F32) {
+        vr = (double)cre[y][r];
+        vi = (double)cim[y][r];
+      } else {
+        vr = cre[y][r] - cim[y][r];
+        vi = cs[y][r] - cre[y][r] - cim[y][r];
+      }
+      if (m < g.M && n < g.N && g.nsplit > 1) {
+        g.P[((int64_t)tl.ks * g.M + m) * g.N + n] = make_double2(vr, vi);
+      } else if (m < g.M && n < g.N) {
+        double2 v = make_double2(g.alpha * vr, g.alpha * vi);
+        if (g.beta != 0.0) {
+          double2 c0 = g.C0[(int64_t)m * g.ldc0 + n];
+          v.x += g.beta * c0.x;
+          v.y += g.beta * c0.y;
+        }
+        g.C[(int64_t)m * g.ldc + n] = v;
+      }
+    }
+}
+
 // Thin GEMM for groups with N <= 4 columns (most pp-ladder channels with few
 // hole pairs): FP64 VALU instead of 16-column MFMA tiles, which would be 75% or
 // more padding.  A tile is 16 rows: each wave owns 4 rows, its 64 lanes walk
@@ -1324,6 +1467,21 @@ static void launch_shape_lb(const GemmGroup *groups, const GemmTile *tiles, int 
   }
 }
 
+// variant with A and B staged in LDS (k_zgemm_la), MB = 1.
+template <int NB>
+static void launch_shape_la(const GemmGroup *groups, const GemmTile *tiles, int ntiles, const SolveDesc *dd,
+                            const int *gsolve, int kind, hipStream_t st) {
+  switch (kind) {
+    case 0: hipLaunchKernelGGL((k_zgemm_la<1, NB, false, double2, 0>), dim3(ntiles), dim3(256), 0, st, groups, tiles, dd, gsolve); break;
+    case 1: hipLaunchKernelGGL((k_zgemm_la<1, NB, true, double2, 0>), dim3(ntiles), dim3(256), 0, st, groups, tiles, dd, gsolve); break;
+    case 2: hipLaunchKernelGGL((k_zgemm_la<1, NB, true, float2, 0>), dim3(ntiles), dim3(256), 0, st, groups, tiles, dd, gsolve); break;
+    case 3: hipLaunchKernelGGL((k_zgemm_la<1, NB, false, double2, 1>), dim3(ntiles), dim3(256), 0, st, groups, tiles, dd, gsolve); break;
+    case 4: hipLaunchKernelGGL((k_zgemm_la<1, NB, true, float2, 1>), dim3(ntiles), dim3(256), 0, st, groups, tiles, dd, gsolve); break;
+    case 5: hipLaunchKernelGGL((k_zgemm_la<1, NB, false, float2, 2>), dim3(ntiles), dim3(256), 0, st, groups, tiles, dd, gsolve); break;
+    default: break;
+  }
+}
+
 template <int MB, int NB>
 static void launch_shape(const GemmGroup *groups, const GemmTile *tiles, int ntiles, const SolveDesc *dd,
                          const int *gsolve, int kind, hipStream_t st) {
@@ -1341,6 +1499,15 @@ static void launch_shape(const GemmGroup *groups, const GemmTile *tiles, int nti
 void launch_gemm_shape(const GemmGroup *groups, const GemmTile *tiles, int ntiles, const SolveDesc *dd,
                        const int *gsolve, int kind, int mb, int nb, int lb, hipStream_t st) {
   if (ntiles <= 0) return;
+  if (lb == 2 && mb == 1) {
+    switch (nb) {
+      case 1: launch_shape_la<1>(groups, tiles, ntiles, dd, gsolve, kind, st); break;
+      case 2: launch_shape_la<2>(groups, tiles, ntiles, dd, gsolve, kind, st); break;
+      case 3: launch_shape_la<3>(groups, tiles, ntiles, dd, gsolve, kind, st); break;
+      default: break;
+    }
+    return;
+  }
   if (lb && mb > 0) {
     switch (mb * 10 + nb) {
       case 11: launch_shape_lb<1, 1>(groups, tiles, ntiles, dd, gsolve, kind, st); break;
