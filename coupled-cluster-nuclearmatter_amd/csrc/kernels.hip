@@ -1221,29 +1221,43 @@ __global__ __launch_bounds__(256) void k_zgemm_thin(const GemmGroup *__restrict_
         acc[r][n].y = fma(a[r].x, b[n].y, fma(a[r].y, b[n].x, acc[r][n].y));
       }
   }
+  // Butterfly reduce-scatter of the 16 sums over the 64 lanes: at the steps
+  // s = 32, 16, 8, 4 a lane keeps half of its values and adds its partner's
+  // copy of them, so lane L ends with value j = (L >> 2) & 15; the steps 2, 1
+  // finish it.  Every value goes through the same pairwise tree (lane, lane ^ s)
+  // as a full xor reduction of each value, so the sums are bit-identical to
+  // that, with 17 exchanges per component instead of 96.
+  double2 v[16];
 #pragma unroll
   for (int r = 0; r < 4; ++r)
 #pragma unroll
-    for (int n = 0; n < 4; ++n)
+    for (int n = 0; n < 4; ++n) v[4 * r + n] = acc[r][n];
 #pragma unroll
-      for (int s = 32; s > 0; s >>= 1) {
-        acc[r][n].x += __shfl_xor(acc[r][n].x, s);
-        acc[r][n].y += __shfl_xor(acc[r][n].y, s);
-      }
-  if (lane < 16) {
-    const int r = lane >> 2, n = lane & 3;
+  for (int st = 0; st < 4; ++st) {
+    const int s = 32 >> st, h = 8 >> st;
+    const bool up = (lane & s) != 0;
+#pragma unroll
+    for (int t = 0; t < h; ++t) {
+      const double2 lo = v[t], hi = v[t + h];
+      const double2 keep = up ? hi : lo, send = up ? lo : hi;
+      v[t].x = keep.x + __shfl_xor(send.x, s);
+      v[t].y = keep.y + __shfl_xor(send.y, s);
+    }
+  }
+#pragma unroll
+  for (int s = 2; s > 0; s >>= 1) {
+    v[0].x += __shfl_xor(v[0].x, s);
+    v[0].y += __shfl_xor(v[0].y, s);
+  }
+  if ((lane & 3) == 0) {
+    const int r = lane >> 4, n = (lane >> 2) & 3;
     const int m = m0 + r;
-    double2 v = make_double2(0.0, 0.0);
-#pragma unroll
-    for (int rr = 0; rr < 4; ++rr)
-#pragma unroll
-      for (int nn = 0; nn < 4; ++nn)
-        if (rr == r && nn == n) v = acc[rr][nn];
+    const double2 v0 = v[0];
     if (m < g.M && n < g.N) {
       if (g.nsplit > 1) {
-        g.P[((int64_t)tl.ks * g.M + m) * g.N + n] = v;
+        g.P[((int64_t)tl.ks * g.M + m) * g.N + n] = v0;
       } else {
-        double2 o = make_double2(g.alpha * v.x, g.alpha * v.y);
+        double2 o = make_double2(g.alpha * v0.x, g.alpha * v0.y);
         if (g.beta != 0.0) {
           const double2 c0 = g.C0[(int64_t)m * g.ldc0 + n];
           o.x += g.beta * c0.x;
