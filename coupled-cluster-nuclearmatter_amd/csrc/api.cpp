@@ -209,7 +209,8 @@ struct GemmList {
     return false;
   }
   void apply_env_shapes() {
-    // CCGPU_SHAPES="fam.class=mb.nb,..." (studies)
+    // CCGPU_SHAPES="fam.class=mb.nb[.lb],..." (studies and the tests of the
+    // kernel forms; lb 0 = B from L2, 1 = B in LDS, 2 = A and B in LDS)
     const char *e = getenv("CCGPU_SHAPES");
     if (!e) return;
     std::string spec(e);
@@ -217,10 +218,11 @@ struct GemmList {
     while (pos < spec.size()) {
       size_t end = spec.find(',', pos);
       if (end == std::string::npos) end = spec.size();
-      int f, c, mb, nb;
-      if (sscanf(spec.substr(pos, end - pos).c_str(), "%d.%d=%d.%d", &f, &c, &mb, &nb) == 4 && f == fam && c >= 0 &&
-          c < NCLS - 1 && mb > 0 && gemm_shape_ok(mb, nb))
-        shape[c] = GemmShape{mb, nb};
+      int f, c, mb, nb, lb = 0;
+      const int got = sscanf(spec.substr(pos, end - pos).c_str(), "%d.%d=%d.%d.%d", &f, &c, &mb, &nb, &lb);
+      if (got >= 4 && f == fam && c >= 0 && c < NCLS - 1 && mb > 0 && gemm_shape_ok(mb, nb) && lb >= 0 && lb <= 2 &&
+          (lb < 2 || (mb == 1 && nb <= 3)))
+        shape[c] = GemmShape{mb, nb, lb};
       pos = end + 1;
     }
   }
@@ -826,7 +828,7 @@ void run_gemm(const GemmList &L, const Batch &B, int prec, bool a_f32, hipStream
   } else {
     const int kind = gemm_kind(prec, a_f32, pp);
     // CCGPU_TIMING=2 also times every tile class of every family
-    static const bool detail = getenv("CCGPU_TIMING") && atoi(getenv("CCGPU_TIMING")) == 2;
+    const bool detail = getenv("CCGPU_TIMING") && atoi(getenv("CCGPU_TIMING")) == 2;
     for (int c = GemmList::NCLS - 1; c >= 0; --c) {
       if (L.tiles[c].empty()) continue;
       char nm[48];
