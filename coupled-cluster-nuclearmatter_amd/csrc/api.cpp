@@ -838,8 +838,10 @@ void run_gemm(const GemmList &L, const Batch &B, int prec, bool a_f32, hipStream
       launch_gemm_shape(L.d_groups, L.d_tiles[c], (int)L.tiles[c].size(), B.d_descs, L.d_gsolve, kind, L.shape[c].mb,
                         L.shape[c].nb, L.shape[c].lb, st);
     }
+    // split groups: the MFMA and thin kernels wrote partials; the VALU check
+    // above runs every group over its whole k range and writes C itself
+    launch_splitk_reduce(L.d_groups, L.d_split, (int)L.split.size(), L.maxmn, B.d_descs, L.d_gsolve, st);
   }
-  launch_splitk_reduce(L.d_groups, L.d_split, (int)L.split.size(), L.maxmn, B.d_descs, L.d_gsolve, st);
 }
 
 // Families in FP32 under CCGPU_PREC_FP32 (bit 0 X/I9 GEMM, 1 pp ladder, 2 hh
