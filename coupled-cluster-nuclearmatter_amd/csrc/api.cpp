@@ -233,7 +233,8 @@ struct GemmList {
   // batch), and the MFMA kernels sum k in the same order for every shape, so
   // results do not depend on the autotuner's choice or on batching.
   static int split_of(const GemmGroup &g) {
-    static const int64_t lim = getenv("CCGPU_SPLIT_TILES") ? atoi(getenv("CCGPU_SPLIT_TILES")) : 128;
+    const char *e = getenv("CCGPU_SPLIT_TILES");   // read per list build (tests force splits)
+    const int64_t lim = e && atoi(e) > 0 ? atoi(e) : 128;
     const int64_t ref_tiles = (int64_t)((g.M + 63) / 64) * ((g.N + 15) / 16);
     if (g.K < 512 || ref_tiles >= lim) return 1;
     return (int)std::min<int64_t>({(2 * lim + ref_tiles - 1) / ref_tiles, (int64_t)(g.K / 256), 8});

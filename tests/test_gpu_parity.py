@@ -185,18 +185,27 @@ def test_diis_trajectory_matches_oracle_when_well_conditioned(ccnm):
     assert np.max(np.abs(mine - theirs)) < 1e-8 * np.max(np.abs(theirs))
 
 
-def test_mfma_gemm_matches_valu_gemm(ccnm):
+@pytest.mark.parametrize("split", ["default", "all"])
+def test_mfma_gemm_matches_valu_gemm(ccnm, monkeypatch, split):
     """The MFMA grouped GEMM against the plain VALU GEMM (independent lane
-    layout) on the PNM A=66, nmax=2 lattice."""
+    layout, whole k range per tile) on the PNM A=66, nmax=2 lattice.  With
+    split="all", on the nmax=3 lattice (pp channels up to 620 pairs), every
+    group with k >= 512 is split (up to 8 ways), so the split-K partials and
+    their fixed-order sum are checked against the VALU result as well."""
+    nmax = 2
+    if split == "all":
+        monkeypatch.setenv("CCGPU_SPLIT_TILES", "1000000")
+        nmax = 3
     res = []
     for mode in ("mfma", "valu"):
         if mode == "valu":
             os.environ["CCGPU_GEMM"] = "valu"
         try:
-            s = ccnm.Solver(A=66, matter=ccnm.PNM, rho=0.16, nmax=2)
+            s = ccnm.Solver(A=66, matter=ccnm.PNM, rho=0.16, nmax=nmax)
             s.generate_interaction()
             s.setup_t_amplitudes()
             res.append(step_loop(s, 8, 8, 1e-8, 100))
+            s.close()
         finally:
             os.environ.pop("CCGPU_GEMM", None)
     a, b = res
