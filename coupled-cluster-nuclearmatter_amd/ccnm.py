@@ -22,6 +22,8 @@ INT_CHIRAL = 1
 LO, NLO, NNLO = 0, 2, 3
 BLOCK_HHHH, BLOCK_HHHP, BLOCK_HHPP, BLOCK_HPHP, BLOCK_HPPP, BLOCK_PPPP = 1, 2, 3, 4, 5, 6
 PREC_FP64, PREC_FP32, PREC_V32 = 0, 1, 2
+ACCEL_REFERENCE, ACCEL_ANDERSON = 0, 1
+STATUS_DIIS_FALLBACK, STATUS_NONFINITE, STATUS_MAXIT = 1, 2, 4
 
 
 class LatticeParams(C.Structure):
@@ -42,21 +44,21 @@ class SolveOpts(C.Structure):
     _fields_ = [("subspace", C.c_int), ("diis_step", C.c_int), ("tol", C.c_double),
                 ("maxit", C.c_int), ("precision", C.c_int), ("refine_every", C.c_int),
                 ("use_graph", C.c_int), ("verbose", C.c_int), ("fp32_switch", C.c_double),
-                ("ccdt1", C.c_int), ("pad_", C.c_int)]
+                ("ccdt1", C.c_int), ("accel", C.c_int), ("mix", C.c_double)]
 
 
 class SolveResult(C.Structure):
     _fields_ = [("iterations", C.c_int), ("converged", C.c_int), ("e0", C.c_double * 2),
                 ("embpt2", C.c_double * 2), ("eccd", C.c_double * 2),
                 ("ecorr_last", C.c_double * 2), ("seconds", C.c_double),
-                ("fp32_iterations", C.c_int), ("pad_", C.c_int)]
+                ("fp32_iterations", C.c_int), ("status", C.c_int)]
 
     def as_dict(self):
         z = lambda a: complex(a[0], a[1])
         return {"iterations": self.iterations, "converged": bool(self.converged),
                 "e0": z(self.e0), "embpt2": z(self.embpt2), "eccd": z(self.eccd),
                 "ecorr": z(self.ecorr_last), "seconds": self.seconds,
-                "fp32_iterations": self.fp32_iterations}
+                "fp32_iterations": self.fp32_iterations, "status": self.status}
 
 
 # every exported symbol with its argument types (kept in step with include/ccgpu.h)
@@ -95,6 +97,7 @@ SIGNATURES = {
     "ccgpu_set_t2_unique": [_I, _DP],
     "ccgpu_get_t2_labels": [_I, _IP],
     "ccgpu_batch_solve": [_I, _IP, C.POINTER(SolveOpts), C.POINTER(SolveResult)],
+    "ccgpu_solve_energies": [_I, _DP, _I, _IP],
     "ccgpu_kernel_stats": [C.c_char_p, _I, _DP, C.POINTER(C.c_longlong), _I],
     "ccgpu_debug_fetch": [_I, _I, _DP, C.c_longlong],
     "ccgpu_mlp_create": [C.c_ulonglong, _IP],
@@ -153,9 +156,10 @@ def plan(A, matter, rho, nmax, twist=(0.0, 0.0, 0.0)):
 
 
 def default_opts(subspace=6, diis_step=6, tol=1e-6, maxit=100, precision=PREC_FP64,
-                 refine_every=0, use_graph=0, verbose=0, fp32_switch=1.0, ccdt1=0):
+                 refine_every=0, use_graph=0, verbose=0, fp32_switch=1.0, ccdt1=0,
+                 accel=ACCEL_REFERENCE, mix=0.0):
     return SolveOpts(subspace, diis_step, tol, maxit, precision, refine_every, use_graph, verbose,
-                     fp32_switch, int(ccdt1), 0)
+                     fp32_switch, int(ccdt1), int(accel), float(mix))
 
 
 class Solver:
@@ -278,6 +282,15 @@ class Solver:
         r = SolveResult()
         _check(lib().ccgpu_solve(self.h, C.byref(o), C.byref(r)))
         return r.as_dict()
+
+    def solve_energies(self):
+        """Correlation energies of the last solve: the start, then one per
+        iteration."""
+        cnt = C.c_int()
+        _check(lib().ccgpu_solve_energies(self.h, None, 0, C.byref(cnt)))
+        out = np.zeros(2 * cnt.value)
+        _check(lib().ccgpu_solve_energies(self.h, _dp(out), cnt.value, C.byref(cnt)))
+        return out[0::2] + 1j * out[1::2]
 
     def pert_triples2(self):
         x = np.zeros(2)

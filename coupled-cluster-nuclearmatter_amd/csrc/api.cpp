@@ -51,6 +51,10 @@ void launch_ring_gather(const SolveDesc *dd, int nsolve, int64_t maxT, hipStream
 void launch_ring_scatter(const SolveDesc *dd, int nsolve, int64_t maxT, hipStream_t st);
 void launch_update(const SolveDesc *dd, int nsolve, int64_t maxT, const int *slot, hipStream_t st);
 void launch_gram(const SolveDesc *dd, int nsolve, const int *slots, int nstep, hipStream_t st);
+void launch_aa_update(const SolveDesc *dd, int nsolve, const int *slots, double beta, hipStream_t st);
+void launch_scatter_values(double2 *dst, const int64_t *idx, const double2 *val, int64_t n, hipStream_t st);
+void launch_aa_combine(const SolveDesc *dd, int nsolve, int64_t maxT, const int *slots, const double *coef,
+                       hipStream_t st);
 void launch_diis_axpy(const SolveDesc *dd, int nsolve, int64_t maxT, const int *slots, const double2 *coef,
                       int nvec, hipStream_t st);
 void launch_gemm_shape(const GemmGroup *groups, const GemmTile *tiles, int ntiles, const SolveDesc *dd,
@@ -156,11 +160,23 @@ void xcd_order_list(std::vector<GemmTile> &tiles, size_t ngroups) {
 // for the VALU cross-check kernel.
 struct GemmShape {
   int mb, nb;
-  int lb = 0;   // 1: B operand staged in LDS; 2: A and B in LDS (mb 1); same tiles, same results
+  int lb = 0;   // 1: B operand staged in LDS; 2: A and B in LDS (mb 1); 3: pp ladder with V and T
+                // streamed through LDS by direct-to-LDS loads (mb 1, nb 1-2); same tiles, same results
 };
 bool gemm_shape_ok(int mb, int nb) {
   return (mb == 0 && nb == 4) || (mb == 1 && nb >= 1 && nb <= 5) || (mb == 2 && (nb == 1 || nb == 2)) ||
          (mb == 4 && nb == 1);
+}
+// Forms a family's tile class may use.  Class 5 (N <= 4) has one form per
+// family, so its summation order never depends on the tuner: the thin VALU
+// kernel, or for the pp ladder the LDS-pipelined MFMA kernel.  The other
+// classes take any MFMA form; the LDS pipeline (lb 3) serves the pp ladder only.
+bool form_ok(int fam, int c, GemmShape s) {
+  if (c == 5) return fam == 1 ? (s.mb == 1 && s.nb == 1 && s.lb == 3) : (s.mb == 0 && s.nb == 4 && s.lb == 0);
+  if (s.mb <= 0 || !gemm_shape_ok(s.mb, s.nb) || s.lb < 0 || s.lb > 3) return false;
+  if (s.lb == 2) return s.mb == 1 && s.nb <= 3;
+  if (s.lb == 3) return fam == 1 && s.mb == 1 && s.nb <= 2;
+  return true;
 }
 // rows and columns of one tile of a shape; (0,4) is the thin VALU kernel
 int tile_rows(GemmShape s) { return s.mb == 0 ? 16 : 64 * s.mb; }
@@ -220,9 +236,8 @@ struct GemmList {
       if (end == std::string::npos) end = spec.size();
       int f, c, mb, nb, lb = 0;
       const int got = sscanf(spec.substr(pos, end - pos).c_str(), "%d.%d=%d.%d.%d", &f, &c, &mb, &nb, &lb);
-      if (got >= 4 && f == fam && c >= 0 && c < NCLS - 1 && mb > 0 && gemm_shape_ok(mb, nb) && lb >= 0 && lb <= 2 &&
-          (lb < 2 || (mb == 1 && nb <= 3)))
-        shape[c] = GemmShape{mb, nb, lb};
+      if (got >= 4 && f == fam && c >= 0 && c < NCLS && form_ok(fam, c, GemmShape{mb, nb, got >= 5 ? lb : 0}))
+        shape[c] = GemmShape{mb, nb, got >= 5 ? lb : 0};
       pos = end + 1;
     }
   }
@@ -232,12 +247,17 @@ struct GemmList {
   // on the group alone (not on the tile shape or on the other groups of a
   // batch), and the MFMA kernels sum k in the same order for every shape, so
   // results do not depend on the autotuner's choice or on batching.
+  bool nosplit = false;           // every group over its whole k range (the pp ladder)
   static int split_of(const GemmGroup &g) {
-    const char *e = getenv("CCGPU_SPLIT_TILES");   // read per list build (tests force splits)
+    // CCGPU_SPLIT_TILES and CCGPU_SPLIT_K are read per list build (tests force
+    // splits: a tile limit and a k quantum other than 128 and 256)
+    const char *e = getenv("CCGPU_SPLIT_TILES");
     const int64_t lim = e && atoi(e) > 0 ? atoi(e) : 128;
+    const char *q = getenv("CCGPU_SPLIT_K");
+    const int64_t quantum = q && atoi(q) >= 16 ? atoi(q) : 256;
     const int64_t ref_tiles = (int64_t)((g.M + 63) / 64) * ((g.N + 15) / 16);
     if (g.K < 512 || ref_tiles >= lim) return 1;
-    return (int)std::min<int64_t>({(2 * lim + ref_tiles - 1) / ref_tiles, (int64_t)(g.K / 256), 8});
+    return (int)std::min<int64_t>({(2 * lim + ref_tiles - 1) / ref_tiles, (int64_t)(g.K / quantum), 8});
   }
   void build_tiles() {
     for (int c = 0; c < NCLS; ++c) tiles[c].clear();
@@ -249,7 +269,7 @@ struct GemmList {
     for (size_t gi = 0; gi < groups.size(); ++gi) {
       GemmGroup &g = groups[gi];
       const int c = cls_of(g.N);
-      const int nsp = split_of(g);
+      const int nsp = nosplit ? 1 : split_of(g);
       int ksz = (g.K + nsp - 1) / nsp;
       ksz = (ksz + 15) / 16 * 16;
       g.nsplit = (g.K + ksz - 1) / ksz;
@@ -343,8 +363,11 @@ struct Solve {
   std::unordered_map<uint64_t, std::complex<double>> hphp;
   // <cb||kd> and <jk||lc> records of the drop-in path for (T): key (p,q,r,s)
   std::unordered_map<uint64_t, std::complex<double>> t3v;
+  // correlation energies of the last solve: the start, then one per iteration
+  std::vector<std::complex<double>> ehist;
   // DIIS state
   int depth = 0;
+  int depth_f = 0;         // columns of desc.histf (Anderson residual vectors)
   std::vector<int> ring;  // logical column -> physical slot
   int nstep = 0, count = 0;
   int *d_slot = nullptr;   // [1] physical slot for k_update
@@ -505,13 +528,23 @@ void build_device(Solve &S) {
   S.g_pp.bytes = 16.0 * (double)P.nVpp;
 }
 
-void alloc_history(Solve &S, int depth) {
+void alloc_history(Solve &S, int depth, bool anderson) {
   if (depth < 2 || depth > 9) throw Error(fmt("DIIS subspace %d outside [2, 9]", depth));
-  if (S.depth >= depth) return;
-  // history has subspace columns (diis_setup, ccm_diis.f90:284-288)
-  S.desc.hist = dalloc<double2>((size_t)depth * S.plan.nT, S.owned);
-  S.depth = depth;
-  HIPCHK(hipMemcpy(S.d_desc, &S.desc, sizeof(SolveDesc), hipMemcpyHostToDevice));
+  if (anderson && depth > 8) throw Error(fmt("Anderson subspace %d outside [2, 8]", depth));
+  bool changed = false;
+  if (S.depth < depth) {
+    // history has subspace columns (diis_setup, ccm_diis.f90:284-288)
+    S.desc.hist = dalloc<double2>((size_t)depth * S.plan.nT, S.owned);
+    S.depth = depth;
+    S.depth_f = 0;   // the FP32 residual vectors pair with the FP64 columns
+    changed = true;
+  }
+  if (anderson && S.depth_f < S.depth) {
+    S.desc.histf = dalloc<float2>((size_t)S.depth * S.plan.nT, S.owned);
+    S.depth_f = S.depth;
+    changed = true;
+  }
+  if (changed) HIPCHK(hipMemcpy(S.d_desc, &S.desc, sizeof(SolveDesc), hipMemcpyHostToDevice));
 }
 
 void ensure_ready(Solve &S) {
@@ -565,6 +598,21 @@ struct Batch {
   double2 *d_scal = nullptr;  // nsolve * 64
   std::vector<int> h_slot, h_slots;       // host staging of control arrays
   std::vector<double2> h_coef, h_scal;
+  std::vector<int> status;                // CCGPU_STATUS_* per solve of the current run
+  bool forced_done = false;               // CCGPU_FORCE_SINGULAR fired in this run
+  // Anderson update: per solve the held vectors (physical history slots,
+  // oldest first), their Gram matrix Re<f_j, f_k> and the energies of the
+  // mixed iterates y_j; device copies of the slots and coefficients
+  struct AA {
+    std::vector<int> slot;
+    std::vector<double> gram;   // kAAHeld x kAAHeld, logical order
+    std::vector<std::complex<double>> ey;
+  };
+  std::vector<AA> aa;
+  int *d_aaslots = nullptr;
+  double *d_aacoef = nullptr;
+  std::vector<int> h_aaslots;
+  std::vector<double> h_aacoef;
   std::vector<void *> owned;
   hipStream_t stream = nullptr;
   // second stream (default priority) for the ring branch of the residual, and
@@ -616,9 +664,13 @@ std::string list_signature(const GemmList &L, int kind) {
   return fmt("%d.%d.%zu.%016llx", L.fam, kind, L.groups.size(), (unsigned long long)h);
 }
 
-std::vector<GemmShape> shape_candidates(int c) {
+std::vector<GemmShape> shape_candidates(int fam, int c) {
   std::vector<GemmShape> v;
-  if (c == 5) return {{0, 4}};   // N <= 4: always the thin kernel (fixed summation order)
+  if (c == 5) return fam == 1 ? std::vector<GemmShape>{{1, 1, 3}} : std::vector<GemmShape>{{0, 4}};
+  if (fam == 1) {   // the LDS-pipelined form first: the tuner keeps the first of equal times
+    v.push_back({1, 1, 3});
+    if (c >= 1) v.push_back({1, 2, 3});
+  }
   for (int nb = 1; nb <= std::min(5, c + 1); ++nb) v.push_back({1, nb});
   v.push_back({2, 1});
   if (c >= 1) v.push_back({2, 2});
@@ -647,9 +699,10 @@ void load_tune_cache() {
     bool ok = true;
     for (int c = 0; c < GemmList::NCLS && ok; ++c) {
       int mb, nb, lb;
-      ok = fscanf(f, "%d %d %d", &mb, &nb, &lb) == 3 && (mb == 0 || gemm_shape_ok(mb, nb)) && lb >= 0 && lb <= 2 &&
-           (lb < 2 || (mb == 1 && nb <= 3));
+      ok = fscanf(f, "%d %d %d", &mb, &nb, &lb) == 3;
       arr[c] = GemmShape{mb, nb, lb};
+      int fam = -1;
+      ok = ok && sscanf(sig, "%d.", &fam) == 1 && form_ok(fam, c, arr[c]);
     }
     if (!ok) break;
     g_tuned[sig] = arr;
@@ -687,10 +740,10 @@ void autotune(GemmList &L, int kind, const SolveDesc *d_descs, hipStream_t st) {
   HIPCHK(hipEventCreate(&e1));
   std::string rep = fmt("fam%d/kind%d:", L.fam, kind);
   for (int c = 0; c < GemmList::NCLS; ++c) {
-    if (!L.has_class(c) || shape_candidates(c).size() < 2) continue;
+    if (!L.has_class(c) || shape_candidates(L.fam, c).size() < 2) continue;
     float best = 1e30f;
     GemmShape best_shape = L.shape[c];
-    for (const GemmShape cand : shape_candidates(c)) {
+    for (const GemmShape cand : shape_candidates(L.fam, c)) {
       GemmList T = L;   // copy of the groups; tiles for this candidate
       T.shape[c] = cand;
       std::vector<void *> tmp;
@@ -716,7 +769,7 @@ void autotune(GemmList &L, int kind, const SolveDesc *d_descs, hipStream_t st) {
     }
     L.shape[c] = best_shape;
     rep += fmt(" c%d=%dx%d%s(%.1fus)", c, best_shape.mb, best_shape.nb,
-               best_shape.lb == 2 ? "A" : best_shape.lb ? "L" : "", 1e3f * best / 3.f);
+               best_shape.lb == 3 ? "P" : best_shape.lb == 2 ? "A" : best_shape.lb ? "L" : "", 1e3f * best / 3.f);
   }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
@@ -773,6 +826,13 @@ std::unique_ptr<Batch> make_batch(const std::vector<Solve *> &ss, hipStream_t st
   for (int k = 0; k < 6; ++k) {
     lists[k]->fam = fams[k];
     lists[k]->xcd = fams[k] == 1;
+    if (fams[k] == 1) {
+      // the pp ladder has hundreds of 64-row tiles per class: no split-K, so
+      // no partial products go to HBM and back; N <= 4 on the LDS pipeline
+      lists[k]->nosplit = true;
+      lists[k]->shape[5] = GemmShape{1, 1, 3};
+      for (int c = 0; c < 5; ++c) lists[k]->shape[c] = GemmShape{1, c == 0 ? 1 : 2, 3};
+    }
     lists[k]->apply_env_shapes();
   }
   HIPCHK(hipStreamCreateWithFlags(&B->stream2, hipStreamNonBlocking));
@@ -781,6 +841,9 @@ std::unique_ptr<Batch> make_batch(const std::vector<Solve *> &ss, hipStream_t st
   B->d_slot = dalloc<int>(n, B->owned);
   B->d_slots = dalloc<int>((size_t)n * 16, B->owned);
   B->d_coef = dalloc<double2>((size_t)n * 16, B->owned);
+  B->d_aaslots = dalloc<int>((size_t)n * 16, B->owned);
+  B->d_aacoef = dalloc<double>((size_t)n * 16, B->owned);
+  B->status.assign(n, 0);
   return B;
 }
 
@@ -834,7 +897,7 @@ void run_gemm(const GemmList &L, const Batch &B, int prec, bool a_f32, hipStream
       if (L.tiles[c].empty()) continue;
       char nm[48];
       snprintf(nm, sizeof nm, "fam%d.k%d.c%d.%dx%d%s", L.fam, kind, c, L.shape[c].mb, L.shape[c].nb,
-               L.shape[c].lb == 2 ? "A" : L.shape[c].lb ? "L" : "");
+               L.shape[c].lb == 3 ? "P" : L.shape[c].lb == 2 ? "A" : L.shape[c].lb ? "L" : "");
       Timer t(detail, st, nm);
       launch_gemm_shape(L.d_groups, L.d_tiles[c], (int)L.tiles[c].size(), B.d_descs, L.d_gsolve, kind, L.shape[c].mb,
                         L.shape[c].nb, L.shape[c].lb, st);
@@ -929,6 +992,14 @@ void check_launch() {
   if (e != hipSuccess) throw Error(fmt("kernel launch failed: %s", hipGetErrorString(e)));
 }
 
+// CCGPU_FORCE_SINGULAR=s (tests of the failure handling): the first DIIS
+// extrapolation (or Anderson system of two or more pairs) of solve s of a
+// batch in a run is treated as singular, as a duplicate history would make it.
+int forced_singular() {
+  const char *e = getenv("CCGPU_FORCE_SINGULAR");
+  return e ? atoi(e) : -1;
+}
+
 // Update of every active solve (diis_t1t2, ccm_diis.f90:74-249); all active solves
 // of a batch share count/nstep because they start together.
 void update_launches(Batch &B, int &count, int &nstep, int subspace, int diis_step, bool timing) {
@@ -984,9 +1055,21 @@ void update_launches(Batch &B, int &count, int &nstep, int subspace, int diis_st
     rhs[ne] = -1.0;
     int info = 0;
     auto sol = zsolve(M, rhs, nstep, &info);
-    if (info != 0) throw Error(fmt("DIIS matrix singular (info %d) in solve %d", info, s));
+    const bool forced = s == forced_singular() && !B.forced_done;
+    if (forced) B.forced_done = true;
+    bool finite = info == 0 && !forced;
+    for (int p = 0; p < ne && finite; ++p) finite = std::isfinite(sol[p].real()) && std::isfinite(sol[p].imag());
+    if (!finite) {
+      // this solve keeps its Jacobi-updated amplitudes (the last history
+      // column) for this step; the other solves of the batch extrapolate
+      slots[(size_t)s * 16] = B.solves[s]->ring[nstep - 1];
+      coef[(size_t)s * 16] = double2{1.0, 0.0};
+      B.status[s] |= CCGPU_STATUS_DIIS_FALLBACK;
+      continue;
+    }
     for (int p = 0; p < ne; ++p) coef[(size_t)s * 16 + p] = double2{sol[p].real(), sol[p].imag()};
   }
+  HIPCHK(hipMemcpyAsync(B.d_slots, slots.data(), slots.size() * sizeof(int), hipMemcpyHostToDevice, st));
   HIPCHK(hipMemcpyAsync(B.d_coef, coef.data(), coef.size() * sizeof(double2), hipMemcpyHostToDevice, st));
   {
     Timer t(timing, st, "diis_axpy");
@@ -1013,19 +1096,172 @@ std::vector<std::complex<double>> energies(Batch &B, bool timing) {
   return e;
 }
 
+// Solution of min |sum_j c_j f_j|^2 subject to sum_j c_j = 1 for the Gram
+// matrix G (m x m, row-major): the system G c = 1 scaled by the diagonal, then
+// c normalized to unit sum.  Returns false when a pivot of the scaled matrix
+// falls below 1e-13 (near-parallel residual vectors) or the result is not
+// finite.
+bool anderson_coefficients(const std::vector<double> &G, int m, std::vector<double> &c) {
+  std::vector<double> sc(m), a((size_t)m * m), b(m, 0.0);
+  for (int j = 0; j < m; ++j) {
+    const double gjj = G[(size_t)j * m + j];
+    if (!(gjj > 0.0) || !std::isfinite(gjj)) return false;
+    sc[j] = 1.0 / std::sqrt(gjj);
+  }
+  for (int j = 0; j < m; ++j) {
+    b[j] = sc[j];
+    for (int k = 0; k < m; ++k) a[(size_t)j * m + k] = sc[j] * G[(size_t)j * m + k] * sc[k];
+  }
+  for (int k = 0; k < m; ++k) {
+    int piv = k;
+    for (int i = k + 1; i < m; ++i)
+      if (std::fabs(a[(size_t)i * m + k]) > std::fabs(a[(size_t)piv * m + k])) piv = i;
+    if (!(std::fabs(a[(size_t)piv * m + k]) > 1e-13)) return false;
+    if (piv != k) {
+      for (int j = 0; j < m; ++j) std::swap(a[(size_t)k * m + j], a[(size_t)piv * m + j]);
+      std::swap(b[k], b[piv]);
+    }
+    for (int i = k + 1; i < m; ++i) {
+      const double f = a[(size_t)i * m + k] / a[(size_t)k * m + k];
+      for (int j = k; j < m; ++j) a[(size_t)i * m + j] -= f * a[(size_t)k * m + j];
+      b[i] -= f * b[k];
+    }
+  }
+  c.assign(m, 0.0);
+  for (int i = m - 1; i >= 0; --i) {
+    double v = b[i];
+    for (int j = i + 1; j < m; ++j) v -= a[(size_t)i * m + j] * c[j];
+    c[i] = v / a[(size_t)i * m + i];
+  }
+  double sum = 0.0;
+  for (int j = 0; j < m; ++j) {
+    c[j] *= sc[j];
+    sum += c[j];
+  }
+  if (!std::isfinite(sum) || sum == 0.0) return false;
+  for (int j = 0; j < m; ++j) c[j] /= sum;
+  return true;
+}
+
+// One Anderson step of every active solve after its residual: k_aa_update
+// forms f, y and the new Gram row; the host appends the row, solves for the
+// coefficients (dropping the oldest vectors while the scaled system is near
+// singular) and uploads them for k_aa_combine, which forms the next iterate at
+// the start of the next iteration.  Returns the correlation energy of the next
+// iterate, sum_j c_j E(y_j) (the energy is linear in T).
+std::vector<std::complex<double>> anderson_step(Batch &B, int depth, double beta, bool timing) {
+  const int n = (int)B.solves.size();
+  hipStream_t st = B.stream;
+  B.h_aaslots.assign((size_t)n * 16, 0);
+  for (int s = 0; s < n; ++s) {
+    Batch::AA &q = B.aa[s];
+    int m = (int)q.slot.size();
+    int fresh = -1;
+    if (m == depth) {   // drop the oldest held vector, reuse its slot
+      fresh = q.slot[0];
+      std::vector<double> g((size_t)(m - 1) * (m - 1));
+      for (int j = 1; j < m; ++j)
+        for (int k = 1; k < m; ++k) g[(size_t)(j - 1) * (m - 1) + (k - 1)] = q.gram[(size_t)j * m + k];
+      q.gram.swap(g);
+      q.slot.erase(q.slot.begin());
+      q.ey.erase(q.ey.begin());
+      --m;
+    } else {
+      for (int c = 0; c < depth && fresh < 0; ++c)
+        if (std::find(q.slot.begin(), q.slot.end(), c) == q.slot.end()) fresh = c;
+    }
+    for (int j = 0; j < m; ++j) B.h_aaslots[(size_t)s * 16 + j] = q.slot[j];
+    B.h_aaslots[(size_t)s * 16 + 14] = fresh;
+    B.h_aaslots[(size_t)s * 16 + 15] = m;
+  }
+  HIPCHK(hipMemcpyAsync(B.d_aaslots, B.h_aaslots.data(), B.h_aaslots.size() * sizeof(int), hipMemcpyHostToDevice,
+                        st));
+  {
+    Timer t(timing, st, "anderson_update");
+    launch_aa_update(B.d_descs, n, B.d_aaslots, beta, st);
+  }
+  check_launch();
+  std::vector<double2> &sc = B.h_scal;
+  sc.assign((size_t)n * 64, double2{0, 0});
+  HIPCHK(hipMemcpyAsync(sc.data(), B.d_scal, sc.size() * sizeof(double2), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  std::vector<std::complex<double>> e(n, 0.0);
+  B.h_aacoef.assign((size_t)n * 16, 0.0);
+  for (int s = 0; s < n; ++s) {
+    if (!B.h_descs[s].active) continue;
+    Batch::AA &q = B.aa[s];
+    const int m = (int)q.slot.size();
+    const double2 *v = &sc[(size_t)s * 64 + 24];
+    std::vector<double> g((size_t)(m + 1) * (m + 1));
+    for (int j = 0; j < m; ++j)
+      for (int k = 0; k < m; ++k) g[(size_t)j * (m + 1) + k] = q.gram[(size_t)j * m + k];
+    for (int j = 0; j < m; ++j) g[(size_t)j * (m + 1) + m] = g[(size_t)m * (m + 1) + j] = v[j].x;
+    g[(size_t)m * (m + 1) + m] = v[m].x;
+    q.gram.swap(g);
+    q.slot.push_back(B.h_aaslots[(size_t)s * 16 + 14]);
+    q.ey.emplace_back(v[16].x, v[16].y);
+    std::vector<double> c;
+    const bool forced = s == forced_singular() && !B.forced_done && q.slot.size() >= 2;
+    if (forced) B.forced_done = true;
+    while (forced || !anderson_coefficients(q.gram, (int)q.slot.size(), c)) {
+      B.status[s] |= CCGPU_STATUS_DIIS_FALLBACK;
+      const int mm = (int)q.slot.size();
+      if (mm == 1) {   // a single vector: the plain mixed step
+        c.assign(1, 1.0);
+        break;
+      }
+      if (forced) {    // keep only the newest pair
+        c.assign(1, 1.0);
+        q.gram.assign(1, q.gram[(size_t)mm * mm - 1]);
+        q.slot.erase(q.slot.begin(), q.slot.end() - 1);
+        q.ey.erase(q.ey.begin(), q.ey.end() - 1);
+        break;
+      }
+      std::vector<double> g2((size_t)(mm - 1) * (mm - 1));
+      for (int j = 1; j < mm; ++j)
+        for (int k = 1; k < mm; ++k) g2[(size_t)(j - 1) * (mm - 1) + (k - 1)] = q.gram[(size_t)j * mm + k];
+      q.gram.swap(g2);
+      q.slot.erase(q.slot.begin());
+      q.ey.erase(q.ey.begin());
+    }
+    const int mm = (int)q.slot.size();
+    for (int j = 0; j < mm; ++j) {
+      B.h_aaslots[(size_t)s * 16 + j] = q.slot[j];
+      B.h_aacoef[(size_t)s * 16 + j] = c[j];
+      e[s] += c[j] * q.ey[j];
+    }
+    B.h_aaslots[(size_t)s * 16 + 15] = mm;
+  }
+  HIPCHK(hipMemcpyAsync(B.d_aaslots, B.h_aaslots.data(), B.h_aaslots.size() * sizeof(int), hipMemcpyHostToDevice,
+                        st));
+  HIPCHK(hipMemcpyAsync(B.d_aacoef, B.h_aacoef.data(), B.h_aacoef.size() * sizeof(double), hipMemcpyHostToDevice,
+                        st));
+  return e;
+}
+
 // The ccd_iter loop (ccm_iter.f90:60-135) over a batch of solves.
 void run_batch(Batch &B, const ccgpu_solve_opts &op, std::vector<ccgpu_solve_result> &res) {
   const int n = (int)B.solves.size();
   const bool timing = getenv("CCGPU_TIMING") && atoi(getenv("CCGPU_TIMING")) != 0;
   if (timing) { g_stats.clear(); g_stat_order.clear(); }
+  if (op.accel != CCGPU_ACCEL_REFERENCE && op.accel != CCGPU_ACCEL_ANDERSON)
+    throw Error(fmt("unknown accelerator %d", op.accel));
+  const bool aa = op.accel == CCGPU_ACCEL_ANDERSON;
+  if (aa && op.ccdt1) throw Error("CCDT-1 runs with the reference update (accel 0)");
+  if (aa && !(op.mix >= 0.0 && op.mix <= 1.0)) throw Error(fmt("mix %g outside [0, 1]", op.mix));
+  const double beta = op.mix > 0.0 ? op.mix : 1.0;
   for (Solve *S : B.solves) {
-    alloc_history(*S, op.subspace);
+    alloc_history(*S, op.subspace, aa);
     S->ring.resize(op.subspace);
     for (int k = 0; k < op.subspace; ++k) S->ring[k] = k;
   }
+  B.status.assign(n, 0);
+  B.forced_done = false;
+  B.aa.assign(n, Batch::AA{});
   // descs may have changed (history pointer)
   for (int s = 0; s < n; ++s) {
     B.h_descs[s].hist = B.solves[s]->desc.hist;
+    B.h_descs[s].histf = B.solves[s]->desc.histf;
     B.h_descs[s].active = 1;
   }
   HIPCHK(hipMemcpy(B.d_descs, B.h_descs.data(), n * sizeof(SolveDesc), hipMemcpyHostToDevice));
@@ -1059,6 +1295,7 @@ void run_batch(Batch &B, const ccgpu_solve_opts &op, std::vector<ccgpu_solve_res
       e1[s] += std::complex<double>(S.ccdt1_e3.x, S.ccdt1_e3.y);
       elast[s] = e1[s];
     }
+  for (int s = 0; s < n; ++s) B.solves[s]->ehist.assign(1, e1[s]);
   int count = 0, nstep = 0;
   auto t0 = std::chrono::steady_clock::now();
   bool graph_ok = op.use_graph != 0 && !timing && gemm_variant_override() < 0;
@@ -1074,12 +1311,22 @@ void run_batch(Batch &B, const ccgpu_solve_opts &op, std::vector<ccgpu_solve_res
   // ends a solve is always an FP64 one.  No solve ends inside the first phase.
   bool fp32_phase = op.precision != CCGPU_PREC_FP64;
   double prev_maxd = 1e300;
-  std::vector<char> reached(n, 0);
+  std::vector<char> reached(n, 0), failed(n, 0);
+  bool pending = false;   // Anderson: coefficients uploaded, next iterate not yet formed
   for (int it = 1; it <= op.maxit; ++it) {
+    if (pending) {
+      Timer t(timing, B.stream, "anderson_combine");
+      launch_aa_combine(B.d_descs, n, B.maxT, B.d_aaslots, B.d_aacoef, B.stream);
+      check_launch();
+      pending = false;
+    }
     bool any = false;
     for (int s = 0; s < n; ++s) {
-      bool act = fp32_phase ? B.h_descs[s].active != 0 : std::abs(dener[s]) > op.tol;
-      if (!act && B.h_descs[s].active) { set_active(B, s, 0); conv[s] = 1; }
+      bool act = !failed[s] && (fp32_phase ? B.h_descs[s].active != 0 : std::abs(dener[s]) > op.tol);
+      if (!act && B.h_descs[s].active) {
+        set_active(B, s, 0);
+        conv[s] = !failed[s];
+      }
       any = any || act;
     }
     if (!any) break;
@@ -1110,16 +1357,33 @@ void run_batch(Batch &B, const ccgpu_solve_opts &op, std::vector<ccgpu_solve_res
     check_launch();
     ++count;
     ++nstep;
-    update_launches(B, count, nstep, op.subspace, op.diis_step, timing);
-    check_launch();
-    const bool extrapolated = count == 0;
-    auto e2 = energies(B, timing);
+    std::vector<std::complex<double>> e2;
+    bool extrapolated;
+    if (aa) {
+      e2 = anderson_step(B, op.subspace, beta, timing);
+      pending = true;
+      extrapolated = true;
+    } else {
+      update_launches(B, count, nstep, op.subspace, op.diis_step, timing);
+      check_launch();
+      extrapolated = count == 0;
+      e2 = energies(B, timing);
+    }
     double maxd = 0;
     for (int s = 0; s < n; ++s) {
       if (!B.h_descs[s].active) continue;
       if (ccdt1) e2[s] += e3[s];
+      if (!std::isfinite(e2[s].real()) || !std::isfinite(e2[s].imag())) {
+        // a diverged solve stops here; the rest of the batch goes on
+        failed[s] = 1;
+        B.status[s] |= CCGPU_STATUS_NONFINITE;
+        iters[s] = it;
+        if (op.verbose) printf("solve %d: non-finite energy at iteration %d, stopped\n", s, it);
+        continue;
+      }
       dener[s] = e2[s] - elast[s];
       elast[s] = e2[s];
+      B.solves[s]->ehist.push_back(e2[s]);
       iters[s] = it;
       if (prec != CCGPU_PREC_FP64) ++iters32[s];
       maxd = std::max(maxd, std::abs(dener[s]));
@@ -1128,7 +1392,8 @@ void run_batch(Batch &B, const ccgpu_solve_opts &op, std::vector<ccgpu_solve_res
                              prec == CCGPU_PREC_FP32 ? "  fp32" : prec == CCGPU_PREC_V32 ? "  v32" : "");
     }
     if (fp32_phase) {
-      const bool at_boundary = op.fp32_switch > 0.0 && extrapolated && prev_maxd <= op.fp32_switch;
+      const bool at_boundary =
+          op.fp32_switch > 0.0 && extrapolated && (aa ? maxd : prev_maxd) <= op.fp32_switch;
       bool at_tol = true;
       for (int s = 0; s < n; ++s) at_tol = at_tol && (reached[s] || !B.h_descs[s].active);
       if (at_boundary || at_tol) {
@@ -1138,12 +1403,19 @@ void run_batch(Batch &B, const ccgpu_solve_opts &op, std::vector<ccgpu_solve_res
           if (B.h_descs[s].active) dener[s] = std::max(std::abs(dener[s]), 2.0 * op.tol);
       }
     }
-    if (!extrapolated) prev_maxd = maxd;
+    if (!extrapolated || aa) prev_maxd = maxd;
+  }
+  if (pending) {   // the iterate of the last Anderson step (iteration cap)
+    launch_aa_combine(B.d_descs, n, B.maxT, B.d_aaslots, B.d_aacoef, B.stream);
+    check_launch();
   }
   HIPCHK(hipStreamSynchronize(B.stream));
   double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   for (int s = 0; s < n; ++s) {
-    if (std::abs(dener[s]) <= op.tol) conv[s] = 1;
+    if (!failed[s] && std::abs(dener[s]) <= op.tol) conv[s] = 1;
+    if (failed[s]) conv[s] = 0;
+    if (!conv[s] && !failed[s]) B.status[s] |= CCGPU_STATUS_MAXIT;
+    res[s].status = B.status[s];
     Solve &S = *B.solves[s];
     res[s].iterations = iters[s];
     res[s].converged = conv[s];
@@ -1672,12 +1944,22 @@ int ccgpu_set_block(int handle, int kind, int nbra, const int *bra_p, const int 
         throw Error(fmt("block kind %d not accepted by ccgpu_set_block", kind));
     }
     if (dst && !up.empty()) {
-      // scatter on the host into a staging copy of the whole array
-      int64_t n = (kind == CCGPU_BLOCK_PPPP) ? P.nVpp : (kind == CCGPU_BLOCK_HHPP ? P.nT : P.nVhh);
-      std::vector<double2> h(n);
-      HIPCHK(hipMemcpy(h.data(), dst, n * sizeof(double2), hipMemcpyDeviceToHost));
-      for (auto &kv : up) h[kv.first] = kv.second;
-      HIPCHK(hipMemcpy(dst, h.data(), n * sizeof(double2), hipMemcpyHostToDevice));
+      // only the canonical (index, value) pairs cross PCIe; a kernel scatters
+      // them into the device array
+      std::vector<int64_t> idx(up.size());
+      std::vector<double2> val(up.size());
+      for (size_t e = 0; e < up.size(); ++e) {
+        idx[e] = up[e].first;
+        val[e] = up[e].second;
+      }
+      std::vector<void *> tmp;
+      int64_t *d_idx = dupload(idx, tmp);
+      double2 *d_val = dupload(val, tmp);
+      launch_scatter_values(dst, d_idx, d_val, (int64_t)up.size(), S.stream);
+      hipError_t e = hipGetLastError();
+      if (e == hipSuccess) e = hipStreamSynchronize(S.stream);
+      for (void *p : tmp) (void)hipFree(p);
+      if (e != hipSuccess) throw Error(fmt("block scatter failed: %s", hipGetErrorString(e)));
     }
     S.have_v = true;
     S.native_v = false;
@@ -1735,6 +2017,8 @@ int ccgpu_finalize_setup(int handle) {
     launch_derive(S.desc, P.nch, P.nphc, P.max_npp, P.max_nhh, max_nph, max_nhp, S.stream);
     check_launch();
     HIPCHK(hipStreamSynchronize(S.stream));
+    // the FP32 copy of V_pppp follows the blocks handed over since it was made
+    if (S.desc.Vpppp32) fill_f32(S);
     S.derived = true;
   });
 }
@@ -1791,7 +2075,7 @@ int ccgpu_update(int handle, int *count, int *nstep, int subspace, int diis_step
     ensure_ready(S);
     Batch &B = single(handle);
     if (S.depth < subspace) {
-      alloc_history(S, subspace);
+      alloc_history(S, subspace, false);
       B.h_descs[0].hist = S.desc.hist;
       HIPCHK(hipMemcpy(B.d_descs, B.h_descs.data(), sizeof(SolveDesc), hipMemcpyHostToDevice));
     }
@@ -1829,6 +2113,18 @@ int ccgpu_solve(int handle, const ccgpu_solve_opts *opts, ccgpu_solve_result *re
     std::vector<ccgpu_solve_result> r;
     run_batch(B, *opts, r);
     *res = r[0];
+  });
+}
+
+int ccgpu_solve_energies(int handle, double *e, int max, int *count) {
+  return guarded([&] {
+    Solve &S = get(handle);
+    const int n = (int)S.ehist.size();
+    *count = n;
+    for (int k = 0; k < std::min(n, max); ++k) {
+      e[2 * k] = S.ehist[k].real();
+      e[2 * k + 1] = S.ehist[k].imag();
+    }
   });
 }
 

@@ -46,8 +46,11 @@ struct SolveDesc {
   double2 *fdiag, *fpart;   // f(p,p) and f(p,partner(p))
   double2 *dh, *I7off, *dp, *I6off;
   double2 *e0;              // device E0
-  // DIIS history (depth x nT) and reductions
+  // DIIS history (depth x nT) and reductions.  With the Anderson update hist
+  // holds the mixed iterates y_j = T_j + beta f_j and histf the residual
+  // vectors f_j = R_j/D_j - T_j in FP32 (for the Gram matrix only).
   double2 *hist;
+  float2 *histf;
   double2 *red;             // reduction partials
   double2 *scal;            // small scalar outputs (energy, Gram)
 };

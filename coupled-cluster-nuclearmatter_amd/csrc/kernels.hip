@@ -316,6 +316,13 @@ __global__ void k_derive_vt(SolveDesc d) {
   }
 }
 
+// Drop-in hand-over: dst[idx[e]] = val[e] for the canonical elements of one
+// host block (ccgpu_set_block uploads only these pairs).
+__global__ void k_scatter_values(double2 *dst, const int64_t *idx, const double2 *val, int64_t n) {
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x)
+    dst[idx[e]] = val[e];
+}
+
 // ---------------------------------------------------------------------------
 // per-iteration kernels; blockIdx.y selects the solve of a batch
 // ---------------------------------------------------------------------------
@@ -654,6 +661,88 @@ __global__ __launch_bounds__(256) void k_gram_partial(const SolveDesc *descs, co
       if ((x & 7) == 0 && q < NP) d.red[(int64_t)blockIdx.x * 36 + q] = v[0];
     }
     __syncthreads();   // sv is rewritten by the next pass
+  }
+}
+
+// Anderson (Pulay) update of the fixed-point map g(T) = R(T)/D(T) with the
+// dynamic denominators D = dh_i + dh_j - dp_a - dp_b of the residual:
+// f = g(T) - T, y = T + beta f.  y goes to history slot sl[14] (FP64), f to
+// histf (FP32), and each block writes its partial sums of Re<f_j, f> over
+// the m = sl[15] held vectors j (slots sl[0..m-1], oldest first) and of
+// <f, f> (values 0..m, real part in .x), and of the correlation energy of y,
+// sum y conj(R0) (value 16).  The coefficients of the next iterate come from
+// the host's solve of the bordered Gram system.  No counterpart in the
+// reference, whose diis_t1t2 (ccm_diis.f90:74-249) mixes 0.75/0.25 every
+// iteration and extrapolates every diis_step iterations.
+constexpr int kAAMax = 9;   // held vectors + the new one
+__global__ __launch_bounds__(256) void k_aa_update(const SolveDesc *descs, const int *slots, double beta) {
+  const SolveDesc &d = descs[blockIdx.y];
+  if (!d.active) return;
+  const int *sl = slots + blockIdx.y * 16;
+  const int m = sl[15];
+  double2 *y = d.hist + (int64_t)sl[14] * d.nT;
+  float2 *fn = d.histf + (int64_t)sl[14] * d.nT;
+  double acc[kAAMax];
+#pragma unroll
+  for (int q = 0; q < kAAMax; ++q) acc[q] = 0.0;
+  double2 ey = make_double2(0.0, 0.0);
+  for (int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; u < d.nT; u += (int64_t)gridDim.x * blockDim.x) {
+    int c, r, s;
+    unique_labels(d, u, c, r, s);
+    int a = d.pp_a[d.off_pp[c] + r], b = d.pp_b[d.off_pp[c] + r];
+    int i = d.hh_i[d.off_hh[c] + s], j = d.hh_j[d.off_hh[c] + s];
+    double2 den = csub(csub(cadd(d.dh[i], d.dh[j]), d.dp[a - d.A]), d.dp[b - d.A]);
+    const double2 t = d.T[u];
+    const double2 f = csub(cdiv(d.R[u], den), t);
+    const double2 yv = cadd(t, cscale(beta, f));
+    y[u] = yv;
+    const float2 f32 = make_float2((float)f.x, (float)f.y);
+    fn[u] = f32;
+    const double fr = f32.x, fi = f32.y;
+#pragma unroll
+    for (int q = 0; q < kAAMax - 1; ++q)
+      if (q < m) {
+        const float2 g = d.histf[(int64_t)sl[q] * d.nT + u];
+        acc[q] += (double)g.x * fr + (double)g.y * fi;
+      }
+    acc[kAAMax - 1] += fr * fr + fi * fi;
+    ey = cadd(ey, cmul(yv, cconj(d.R0[u])));
+  }
+  // fixed-order block sums (pairwise tree over the 256 threads)
+  __shared__ double sv[kAAMax + 2][256];
+  const int x = threadIdx.x;
+#pragma unroll
+  for (int q = 0; q < kAAMax; ++q) sv[q][x] = acc[q];
+  sv[kAAMax][x] = ey.x;
+  sv[kAAMax + 1][x] = ey.y;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (x < w)
+#pragma unroll
+      for (int q = 0; q < kAAMax + 2; ++q) sv[q][x] += sv[q][x + w];
+    __syncthreads();
+  }
+  if (x < 17) {
+    double2 v = make_double2(0.0, 0.0);
+    if (x < m) v.x = sv[x][0];
+    else if (x == m) v.x = sv[kAAMax - 1][0];
+    else if (x == 16) v = make_double2(sv[kAAMax][0], sv[kAAMax + 1][0]);
+    d.red[(int64_t)blockIdx.x * 36 + x] = v;
+  }
+}
+
+// Next iterate of the Anderson update: T = sum_j c_j y_j over the m = sl[15]
+// held vectors (slots sl[0..m-1]); real coefficients coef[16 s + j].
+__global__ void k_aa_combine(const SolveDesc *descs, const int *slots, const double *coef) {
+  const SolveDesc &d = descs[blockIdx.y];
+  if (!d.active) return;
+  const int *sl = slots + blockIdx.y * 16;
+  const double *cf = coef + blockIdx.y * 16;
+  const int m = sl[15];
+  for (int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; u < d.nT; u += (int64_t)gridDim.x * blockDim.x) {
+    double2 acc = cscale(cf[0], d.hist[(int64_t)sl[0] * d.nT + u]);
+    for (int p = 1; p < m; ++p) acc = cadd(acc, cscale(cf[p], d.hist[(int64_t)sl[p] * d.nT + u]));
+    d.T[u] = acc;
   }
 }
 
@@ -1496,6 +1585,19 @@ void launch_gram(const SolveDesc *dd, int nsolve, const int *slots, int nstep, h
   int nv = ne * (ne + 1) / 2;
   hipLaunchKernelGGL(k_reduce_final, dim3(1, nsolve), dim3(256), 0, st, dd, kRedBlocks, 36, nv, 4);
 }
+void launch_scatter_values(double2 *dst, const int64_t *idx, const double2 *val, int64_t n, hipStream_t st) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_scatter_values, egrid(n, 1), dim3(256), 0, st, dst, idx, val, n);
+}
+void launch_aa_update(const SolveDesc *dd, int nsolve, const int *slots, double beta, hipStream_t st) {
+  hipLaunchKernelGGL(k_aa_update, dim3(kRedBlocks, nsolve), dim3(256), 0, st, dd, slots, beta);
+  // 17 values per solve into scal[24 .. 40]
+  hipLaunchKernelGGL(k_reduce_final, dim3(1, nsolve), dim3(256), 0, st, dd, kRedBlocks, 36, 17, 24);
+}
+void launch_aa_combine(const SolveDesc *dd, int nsolve, int64_t maxT, const int *slots, const double *coef,
+                       hipStream_t st) {
+  hipLaunchKernelGGL(k_aa_combine, egrid(maxT, nsolve), dim3(256), 0, st, dd, slots, coef);
+}
 void launch_diis_axpy(const SolveDesc *dd, int nsolve, int64_t maxT, const int *slots, const double2 *coef,
                       int nvec, hipStream_t st) {
   hipLaunchKernelGGL(k_diis_axpy, egrid(maxT, nsolve), dim3(256), 0, st, dd, slots, coef, nvec);
@@ -1517,6 +1619,223 @@ static void launch_shape_lb(const GemmGroup *groups, const GemmTile *tiles, int 
     case 3: hipLaunchKernelGGL((k_zgemm_lb<MB, NB, false, double2, 1>), dim3(ntiles), dim3(256), 0, st, groups, tiles, dd, gsolve); break;
     case 4: hipLaunchKernelGGL((k_zgemm_lb<MB, NB, true, float2, 1>), dim3(ntiles), dim3(256), 0, st, groups, tiles, dd, gsolve); break;
     case 5: hipLaunchKernelGGL((k_zgemm_lb<MB, NB, false, float2, 2>), dim3(ntiles), dim3(256), 0, st, groups, tiles, dd, gsolve); break;
+    default: break;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// pp ladder R += V_pppp T (ccm_t2_eqn.f90:574-607) with the V stream staged
+// through LDS by direct-to-LDS loads (global_load_lds_dwordx4): a tile is 64
+// rows x 16 NB columns over the whole k range of its channel (no split-K), and
+// every workgroup keeps three chunks of 16 KB of V (16 k of FP64, or 32 k of
+// the FP32 copy) plus the matching rows of T in flight, with no VGPRs spent on
+// staging.  Each 16 lanes of a load instruction read one V row's 256
+// contiguous bytes; the row's slots land XOR-swizzled by the row index, so the
+// MFMA reads (16 rows at one k) hit 16 different LDS slots.  The k order of
+// every sum is that of the other MFMA forms (step q takes k0 + 4q + lane
+// group), so all unsplit forms give the same bits.
+//   VK 0: FP64 V, FP64 MFMA (3 real products per complex MAC)
+//   VK 1: FP32-stored V, FP64 MFMA
+//   VK 2: FP32-stored V, FP32 MFMA (4 real products)
+// Waits are explicit: s_waitcnt vmcnt(n) for the chunk about to be read and
+// lgkmcnt(0) for this wave's LDS reads of the stage about to be refilled, then
+// a bare s_barrier (a workgroup fence would force vmcnt(0) and drain the
+// pipeline); the empty asm statements keep the compiler from moving memory
+// accesses across it.
+// ---------------------------------------------------------------------------
+typedef __attribute__((address_space(3))) void lds_void;
+template <int VM>
+__device__ __forceinline__ void ppl_sync() {
+  static_assert(VM >= 0 && VM < 64, "vmcnt range");
+  asm volatile("" ::: "memory");
+  __builtin_amdgcn_s_waitcnt((VM & 15) | ((VM >> 4) << 14) | (7 << 4) | (0 << 8));
+  __builtin_amdgcn_s_barrier();
+  asm volatile("" ::: "memory");
+}
+__device__ __forceinline__ void dma16(const void *src, char *lds) {
+  __builtin_amdgcn_global_load_lds(src, (lds_void *)lds, 16, 0, 0);
+}
+
+template <int NB, int VK>
+struct PplCfg {
+  static constexpr bool VF32 = VK != 0;
+  static constexpr int KC = VF32 ? 32 : 16;          // k per chunk: 64 rows x KC = 16 KB of V
+  static constexpr int NT = 16 * NB;
+  static constexpr int BBYTES = KC * NT * 16;        // T rows of a chunk (FP64)
+  static constexpr int BIW = BBYTES / 4096;          // T load instructions per wave per chunk
+  static constexpr int IPW = 4 + BIW;                // load instructions per wave per chunk
+  static constexpr int S = 3 * (16384 + BBYTES) <= 81920 ? 3 : 2;   // stages (two workgroups per CU)
+};
+
+template <int NB, int VK>
+__global__ __launch_bounds__(256, 2) void k_ppl(const GemmGroup *__restrict__ groups,
+                                              const GemmTile *__restrict__ tiles,
+                                              const SolveDesc *__restrict__ descs,
+                                              const int *__restrict__ gsolve) {
+  typedef PplCfg<NB, VK> C;
+  constexpr bool VF32 = C::VF32, MF32 = VK == 2;
+  constexpr int KC = C::KC, NT = C::NT, S = C::S, IPW = C::IPW, BIW = C::BIW;
+  static_assert(C::BBYTES % 4096 == 0, "T rows of a chunk in whole instructions per wave");
+  typedef typename std::conditional<MF32, float, double>::type real;
+  typedef typename std::conditional<MF32, f4, d4>::type acc4;
+  // one object per stage: the compiler's wait insertion then tells the
+  // stages apart and leaves the waits of the pipeline to the code below
+  __shared__ __attribute__((aligned(16))) char A0[16384], A1[16384], A2[S == 3 ? 16384 : 16];
+  __shared__ __attribute__((aligned(16))) char B0[C::BBYTES], B1[C::BBYTES], B2[S == 3 ? C::BBYTES : 16];
+
+  const GemmTile tl = tiles[blockIdx.x];
+  if (descs && !descs[gsolve[tl.g]].active) return;
+  const GemmGroup g = groups[tl.g];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int li = lane & 15, lg = lane >> 4;
+  const int K = g.K, nchunk = (K + KC - 1) / KC;
+  // per-lane source rows of the V loads: instruction i of wave w covers tile
+  // rows 16 w + 4 i + lg, slot li of each row holding element (or FP32 pair)
+  // li ^ (row & 15)
+  const char *arow[4];
+  int aslot[4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const int r = 16 * w + 4 * i + lg;
+    const int m = min(tl.m0 + r, g.M - 1);
+    aslot[i] = li ^ (r & 15);
+    arow[i] = (const char *)g.A + (int64_t)m * g.lda * (VF32 ? 8 : 16);
+  }
+  // T loads: element e = 64 (BIW w + j) + lane of the chunk's KC x NT block
+  int bkr[BIW], bcol[BIW];
+#pragma unroll
+  for (int j = 0; j < BIW; ++j) {
+    const int e = 64 * (BIW * w + j) + lane;
+    bkr[j] = e / NT;
+    bcol[j] = min(tl.n0 + e % NT, g.N - 1);
+  }
+  // chunks past the last one (the pipeline runs whole rounds of S) re-load the
+  // last chunk (L2 hits) and are masked to zero in compute
+  auto issue = [&](int c, char *as, char *bs) {
+    const int k0 = min(c, nchunk - 1) * KC;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const char *src;
+      if constexpr (VF32) src = arow[i] + 8 * (int64_t)min(k0 + 2 * aslot[i], ((K + 1) & ~1) - 2);
+      else src = arow[i] + 16 * (int64_t)min(k0 + aslot[i], K - 1);
+      dma16(src, as + (16 * w + 4 * i) * 256);
+    }
+#pragma unroll
+    for (int j = 0; j < BIW; ++j) {
+      const int k = min(k0 + bkr[j], K - 1);
+      dma16(g.B + (int64_t)k * g.ldb + bcol[j], bs + (BIW * w + j) * 1024);
+    }
+  };
+  acc4 cre[NB], cim[NB], cs[NB];
+#pragma unroll
+  for (int y = 0; y < NB; ++y) {
+    cre[y] = (acc4){0, 0, 0, 0};
+    cim[y] = (acc4){0, 0, 0, 0};
+    cs[y] = (acc4){0, 0, 0, 0};
+  }
+  const int row = 16 * w + li;
+  auto mac = [&](double2 a, int kk, const char *bs, int kvalid) {
+    const real ar = (real)a.x, ai = (real)a.y;
+#pragma unroll
+    for (int y = 0; y < NB; ++y) {
+      double2 b = *(const double2 *)(bs + (kk * NT + 16 * y + li) * 16);
+      if (kk >= kvalid) b = make_double2(0.0, 0.0);
+      const real br = (real)b.x, bi = (real)b.y;
+      if constexpr (MF32) {
+        cre[y] = __builtin_amdgcn_mfma_f32_16x16x4f32(ar, br, cre[y], 0, 0, 0);
+        cre[y] = __builtin_amdgcn_mfma_f32_16x16x4f32(-ai, bi, cre[y], 0, 0, 0);
+        cim[y] = __builtin_amdgcn_mfma_f32_16x16x4f32(ar, bi, cim[y], 0, 0, 0);
+        cim[y] = __builtin_amdgcn_mfma_f32_16x16x4f32(ai, br, cim[y], 0, 0, 0);
+      } else {
+        cre[y] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar, br, cre[y], 0, 0, 0);
+        cim[y] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai, bi, cim[y], 0, 0, 0);
+        cs[y] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar + ai, br + bi, cs[y], 0, 0, 0);
+      }
+    }
+  };
+  auto compute = [&](const char *as, const char *bs, int kvalid) {
+    if constexpr (VF32) {
+      // MFMA step (h, t) takes k = 2 (4 h + lane group) + t: each lane reads
+      // its FP32 pair once for two steps (same k order as kstep<true>)
+#pragma unroll
+      for (int h = 0; h < 4; ++h) {
+        const int pr = 4 * h + lg;
+        const float4 a4 = *(const float4 *)(as + row * 256 + ((pr ^ (row & 15)) * 16));
+        mac(make_double2(a4.x, a4.y), 2 * pr, bs, kvalid);
+        mac(make_double2(a4.z, a4.w), 2 * pr + 1, bs, kvalid);
+      }
+    } else {
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int kk = 4 * q + lg;
+        const double2 a = *(const double2 *)(as + row * 256 + ((kk ^ (row & 15)) * 16));
+        mac(a, kk, bs, kvalid);
+      }
+    }
+  };
+  // One step of the pipeline: chunk c from stage (ac, bc); the stage (an, bn)
+  // read in the previous step is refilled with chunk c + S - 1.  The loop body
+  // is branch-free (whole rounds of S chunks, every step issues and waits for
+  // the same count), so the compiler's wait insertion tracks each stage's
+  // loads and adds no wait of its own before the LDS reads.
+  auto step = [&](int c, const char *ac, const char *bc, char *an, char *bn) {
+    ppl_sync<(S - 2) * IPW>();
+    issue(c + S - 1, an, bn);
+    compute(ac, bc, max(0, min(KC, K - c * KC)));
+  };
+  issue(0, A0, B0);
+  if constexpr (S == 3) {
+    issue(1, A1, B1);
+    for (int c = 0; c < nchunk; c += 3) {
+      step(c, A0, B0, A2, B2);
+      step(c + 1, A1, B1, A0, B0);
+      step(c + 2, A2, B2, A1, B1);
+    }
+  } else {
+    for (int c = 0; c < nchunk; c += 2) {
+      step(c, A0, B0, A1, B1);
+      step(c + 1, A1, B1, A0, B0);
+    }
+  }
+  // no load into LDS may outlive the workgroup
+  ppl_sync<0>();
+  // result layouts of the 16x16x4 MFMAs: register r of lane l holds column l&15
+  // and row (l>>4)+4r for f64, row 4(l>>4)+r for f32
+#pragma unroll
+  for (int y = 0; y < NB; ++y)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int rr = MF32 ? (lg * 4 + r) : (lg + 4 * r);
+      const int m = tl.m0 + 16 * w + rr;
+      const int n = tl.n0 + y * 16 + li;
+      double vr, vi;
+      if constexpr (MF32) {
+        vr = (double)cre[y][r];
+        vi = (double)cim[y][r];
+      } else {
+        vr = cre[y][r] - cim[y][r];
+        vi = cs[y][r] - cre[y][r] - cim[y][r];
+      }
+      if (m < g.M && n < g.N) {
+        double2 v = make_double2(g.alpha * vr, g.alpha * vi);
+        if (g.beta != 0.0) {
+          const double2 c0 = g.C0[(int64_t)m * g.ldc0 + n];
+          v.x += g.beta * c0.x;
+          v.y += g.beta * c0.y;
+        }
+        g.C[(int64_t)m * g.ldc + n] = v;
+      }
+    }
+}
+
+template <int NB>
+static void launch_shape_ppl(const GemmGroup *groups, const GemmTile *tiles, int ntiles, const SolveDesc *dd,
+                             const int *gsolve, int kind, hipStream_t st) {
+  // kinds of the pp ladder: 3 FP64, 4 FP32 MFMA on the FP32 copy, 5 FP32 copy with FP64 MFMA
+  switch (kind) {
+    case 3: hipLaunchKernelGGL((k_ppl<NB, 0>), dim3(ntiles), dim3(256), 0, st, groups, tiles, dd, gsolve); break;
+    case 5: hipLaunchKernelGGL((k_ppl<NB, 1>), dim3(ntiles), dim3(256), 0, st, groups, tiles, dd, gsolve); break;
+    case 4: hipLaunchKernelGGL((k_ppl<NB, 2>), dim3(ntiles), dim3(256), 0, st, groups, tiles, dd, gsolve); break;
     default: break;
   }
 }
@@ -1553,6 +1872,14 @@ static void launch_shape(const GemmGroup *groups, const GemmTile *tiles, int nti
 void launch_gemm_shape(const GemmGroup *groups, const GemmTile *tiles, int ntiles, const SolveDesc *dd,
                        const int *gsolve, int kind, int mb, int nb, int lb, hipStream_t st) {
   if (ntiles <= 0) return;
+  if (lb == 3 && mb == 1) {   // pp ladder through the LDS load pipeline
+    switch (nb) {
+      case 1: launch_shape_ppl<1>(groups, tiles, ntiles, dd, gsolve, kind, st); break;
+      case 2: launch_shape_ppl<2>(groups, tiles, ntiles, dd, gsolve, kind, st); break;
+      default: break;
+    }
+    return;
+  }
   if (lb == 2 && mb == 1) {
     switch (nb) {
       case 1: launch_shape_la<1>(groups, tiles, ntiles, dd, gsolve, kind, st); break;

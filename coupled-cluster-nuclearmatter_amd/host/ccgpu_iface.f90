@@ -31,13 +31,14 @@ module ccgpu_iface
      real(c_double) :: tol
      integer(c_int) :: maxit, precision, refine_every, use_graph, verbose
      real(c_double) :: fp32_switch
-     integer(c_int) :: ccdt1 = 0, pad_ = 0
+     integer(c_int) :: ccdt1 = 0, accel = 0
+     real(c_double) :: mix = 0.d0
   end type ccgpu_solve_opts
 
   type, bind(C) :: ccgpu_solve_result
      integer(c_int) :: iterations, converged
      real(c_double) :: e0(2), embpt2(2), eccd(2), ecorr_last(2), seconds
-     integer(c_int) :: fp32_iterations, pad_
+     integer(c_int) :: fp32_iterations, status
   end type ccgpu_solve_result
 
   interface

@@ -141,8 +141,30 @@ typedef struct {
                         t3_energy inside the loop, as the CCDT1 branch of
                         ccd_iter); needs ccgpu_set_3nf and the chiral
                         interaction.  0: CCD */
-  int pad_;
+  int accel;         /* CCGPU_ACCEL_REFERENCE: the update and DIIS of
+                        diis_t1t2 (0.75/0.25 mixing every iteration, DIIS every
+                        diis_step iterations over subspace iterates);
+                        CCGPU_ACCEL_ANDERSON: Anderson/Pulay update every
+                        iteration on the residual vectors f = R/D - T, with at
+                        most subspace held vectors (diis_step unused) */
+  double mix;        /* CCGPU_ACCEL_ANDERSON: T_next = sum_j c_j (T_j + mix f_j);
+                        (0, 1], 0 selects 1 */
 } ccgpu_solve_opts;
+
+/* convergence accelerators (ccgpu_solve_opts.accel) */
+#define CCGPU_ACCEL_REFERENCE 0
+#define CCGPU_ACCEL_ANDERSON 1
+
+/* per-solve status flags (ccgpu_solve_result.status) */
+#define CCGPU_STATUS_DIIS_FALLBACK 1   /* a singular DIIS/Anderson system: that
+                                          solve kept its plain update for the
+                                          step (the reference prints ZGESV's
+                                          info and carries on,
+                                          ccm_diis.f90:173-176) */
+#define CCGPU_STATUS_NONFINITE 2       /* non-finite energy: the solve was
+                                          stopped, the others of a batch go on */
+#define CCGPU_STATUS_MAXIT 4           /* iteration cap reached before |dE| <= tol
+                                          (ccm_iter.f90:71-74 stops silently) */
 
 typedef struct {
   int iterations;
@@ -153,7 +175,7 @@ typedef struct {
   double ecorr_last[2];  /* last correlation energy */
   double seconds;        /* wall time of the iteration loop */
   int fp32_iterations;   /* iterations whose contractions ran in FP32 or V32 */
-  int pad_;
+  int status;            /* CCGPU_STATUS_* flags of this solve */
 } ccgpu_solve_result;
 
 /* Host-only planning of a lattice: no GPU work.  Fills dims. */
@@ -252,6 +274,12 @@ int ccgpu_get_t2_unique(int handle, double *t);
 int ccgpu_set_t2_unique(int handle, const double *t);
 /* Orbital quadruples (a,b,i,j) of the storage order, 4 ints per element. */
 int ccgpu_get_t2_labels(int handle, int *abij);
+
+/* Correlation energies of the last ccgpu_solve / ccgpu_batch_solve of a handle:
+ * the start (MBPT2 or learned), then one per iteration, as the reference's
+ * per-iteration 'dener' lines follow them (ccm_iter.f90:88-94); up to max
+ * complex values into e[2 max], the total number into count. */
+int ccgpu_solve_energies(int handle, double *e, int max, int *count);
 
 /* Solve several handles together: every kernel launch covers all of them. */
 int ccgpu_batch_solve(int n, const int *handles, const ccgpu_solve_opts *opts,

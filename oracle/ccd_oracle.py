@@ -16,6 +16,8 @@ are relative to deltafull-CCMkspace/ of the reference):
   update + DIIS  ccm_diis.f90:74-293 (diis_t1t2, diis_setup)
   energy         ccm_energy.f90:4-71 (ccd_energy_save)
   driver loop    ccm_iter.f90:1-170 (ccd_iter)
+  Anderson       the engine's own accelerator (no reference counterpart),
+                 restated in solve_anderson so the GPU trajectory has a checker
   (T)            ccm_triples.f90:174-294, 528-735; ccm_t3full_channels.f90:4-284
   Minnesota      chiral_module_andreas_with_delta.f90:8980-9111 (vmom_minnesota)
 
@@ -412,36 +414,54 @@ class CCD:
 
     def partners(self, x):
         """States with the same momentum and isospin as x (including x)."""
-        lat = self.lat
-        return np.where(np.all(lat.n == lat.n[x], 1) & (lat.tz == lat.tz[x]))[0]
+        cache = self.__dict__.setdefault("_partner_cache", {})
+        got = cache.get(int(x))
+        if got is None:
+            lat = self.lat
+            got = np.where(np.all(lat.n == lat.n[x], 1) & (lat.tz == lat.tz[x]))[0]
+            cache[int(x)] = got
+        return got
+
+    def partner_table(self):
+        """(nsp, P) table of partners(x), padded with -1."""
+        pt = self.__dict__.get("_ptab")
+        if pt is None:
+            lists = [self.partners(x) for x in range(self.lat.nsp)]
+            pt = -np.ones((self.lat.nsp, max(len(v) for v in lists)), np.int64)
+            for x, v in enumerate(lists):
+                pt[x, :len(v)] = v
+            self._ptab = pt
+        return pt
 
     # ccm_t2_eqn.f90:1312-1518
     def i6i7(self, T):
         lat, ch = self.lat, self.ch
         A, nsp = lat.A, lat.nsp
+        pt = self.partner_table()
         I7 = np.zeros((A, A), np.complex128)
         I6 = np.zeros((nsp, nsp), np.complex128)
         for c in range(len(ch.keys)):
             h, p = ch.hh[c], ch.pp[c]
             # I7(k,i) += 0.5 * V(kl,cd) T(cd,il) for (i,l) in channel, k partner of i
             X = self.v_hhpp[c] @ T[c]          # X[(k,l),(i,l')] = sum_cd V(kl,cd) T(cd,il')
-            for col, (i, l) in enumerate(h):
-                for k in self.partners(i):
-                    if k >= A:
-                        continue
-                    row = ch.hh_row[k, l]
-                    if ch.hh_ch[k, l] != c:
-                        continue
-                    I7[k, i] += 0.5 * X[row, col]
+            i, l = h[:, 0], h[:, 1]
+            cols = np.arange(len(h))
+            for slot in range(pt.shape[1]):
+                k = pt[i, slot]
+                ok = (k >= 0) & (k < A)
+                kk = np.where(ok, k, 0)
+                ok &= ch.hh_ch[kk, l] == c
+                np.add.at(I7, (k[ok], i[ok]), 0.5 * X[ch.hh_row[kk, l][ok], cols[ok]])
             # I6(a,c) += -0.5 V(kl,dc) T(da,kl) for (d,a) in channel, c partner of a
             Y = T[c] @ self.v_hhpp[c]          # Y[(d,a),(d',c)] = sum_kl T(da,kl) V(kl,d'c)
-            for row, (d, a) in enumerate(p):
-                for cc in self.partners(a):
-                    if cc < A:
-                        continue
-                    if ch.pp_ch[d, cc] != c:
-                        continue
-                    I6[a, cc] += -0.5 * Y[row, ch.pp_row[d, cc]]
+            d, a = p[:, 0], p[:, 1]
+            rows = np.arange(len(p))
+            for slot in range(pt.shape[1]):
+                cc = pt[a, slot]
+                ok = cc >= A
+                ccs = np.where(ok, cc, A)
+                ok &= ch.pp_ch[d, ccs] == c
+                np.add.at(I6, (a[ok], cc[ok]), -0.5 * Y[rows[ok], ch.pp_row[d, ccs][ok]])
         I7 = I7 + self.fock[:A, :A]
         I6[A:, A:] = I6[A:, A:] + self.fock[A:, A:]
         return I6, I7
@@ -460,24 +480,33 @@ class CCD:
         np.fill_diagonal(I6, 0.0)
         R = [np.conj(self.v_hhpp[c]).T.copy() for c in range(nC)]
         # one-body terms with P(ab) and P(ij) (ccm_t2_eqn.f90:144-211)
+        pt = self.partner_table()
         for c in range(nC if terms & 1 else 0):
             h, p = ch.hh[c], ch.pp[c]
-            for row, (a, b) in enumerate(p):
-                s = np.zeros(len(h), np.complex128)
-                for cc in self.partners(b):
-                    if cc < A or ch.pp_ch[a, cc] != c:
-                        continue
-                    s += I6[b, cc] * T[c][ch.pp_row[a, cc], :]
-                R[c][row, :] += s
-                R[c][ch.pp_row[b, a], :] -= s
-            for col, (i, j) in enumerate(h):
-                s = np.zeros(len(p), np.complex128)
-                for k in self.partners(j):
-                    if k >= A or ch.hh_ch[i, k] != c:
-                        continue
-                    s -= I7[k, j] * T[c][:, ch.hh_row[i, k]]
-                R[c][:, col] += s
-                R[c][:, ch.hh_row[j, i]] -= s
+            # row (a,b): s = sum_{c' partner of b} I6(b,c') T(ac',:), added to
+            # row (a,b) and subtracted from row (b,a)
+            a, b = p[:, 0], p[:, 1]
+            S = np.zeros((len(p), len(h)), np.complex128)
+            for slot in range(pt.shape[1]):
+                cc = pt[b, slot]
+                ok = cc >= A
+                ccs = np.where(ok, cc, A)
+                ok &= ch.pp_ch[a, ccs] == c
+                S[ok] += I6[b[ok], cc[ok]][:, None] * T[c][ch.pp_row[a[ok], cc[ok]], :]
+            R[c] += S
+            R[c][ch.pp_row[b, a], :] -= S
+            # column (i,j): s = -sum_{k partner of j} I7(k,j) T(:,ik), added to
+            # column (i,j) and subtracted from column (j,i)
+            i, j = h[:, 0], h[:, 1]
+            S = np.zeros((len(p), len(h)), np.complex128)
+            for slot in range(pt.shape[1]):
+                k = pt[j, slot]
+                ok = (k >= 0) & (k < A)
+                kk = np.where(ok, k, 0)
+                ok &= ch.hh_ch[i, kk] == c
+                S[:, ok] -= I7[k[ok], j[ok]][None, :] * T[c][:, ch.hh_row[i[ok], k[ok]]]
+            R[c] += S
+            R[c][:, ch.hh_row[j, i]] -= S
         # I9 and the two ladders (ccm_t2_eqn.f90:423-647)
         for c in range(nC):
             I9 = self.v_hhhh[c] + 0.5 * (self.v_hhpp[c] @ T[c])
@@ -564,6 +593,89 @@ class CCD:
                 callback(it, e2)
         self.T = T
         self.ecc = self.e0 + e2
+        return energies
+
+    @staticmethod
+    def anderson_coefficients(G):
+        """min |sum_j c_j f_j| with sum_j c_j = 1 for the Gram matrix G: G c = 1
+        scaled by the diagonal, then c normalized; None when a scaled pivot falls
+        below 1e-13 (Gaussian elimination with partial pivoting) or the result is
+        not finite.  The engine's accelerator, not a reference routine."""
+        m = len(G)
+        d = np.diag(G).copy()
+        if not np.all(np.isfinite(d)) or np.any(d <= 0.0):
+            return None
+        sc = 1.0 / np.sqrt(d)
+        a = sc[:, None] * G * sc[None, :]
+        b = sc.copy()
+        for k in range(m):
+            piv = k + int(np.argmax(np.abs(a[k:, k])))
+            if not abs(a[piv, k]) > 1e-13:
+                return None
+            if piv != k:
+                a[[k, piv]] = a[[piv, k]]
+                b[[k, piv]] = b[[piv, k]]
+            for i in range(k + 1, m):
+                f = a[i, k] / a[k, k]
+                a[i, k:] -= f * a[k, k:]
+                b[i] -= f * b[k]
+        c = np.zeros(m)
+        for i in range(m - 1, -1, -1):
+            c[i] = (b[i] - a[i, i + 1:] @ c[i + 1:]) / a[i, i]
+        c *= sc
+        tot = c.sum()
+        if not np.isfinite(tot) or tot == 0.0:
+            return None
+        return c / tot
+
+    def solve_anderson(self, depth=8, beta=1.0, tol=1e-8, maxit=100, T0=None):
+        """The engine's Anderson (Pulay) update (ccgpu_solve_opts.accel 1), which
+        replaces the reference's diis_t1t2 schedule: with the residual map
+        g(T) = R(T)/D(T) of the dynamic denominators, every iteration forms
+        f = g(T) - T (kept rounded to complex64 for the Gram matrix) and
+        y = T + beta f, holds the last `depth` pairs, and takes
+        T = sum_j c_j y_j with the coefficients of anderson_coefficients on
+        G_jk = Re<f_j, f_k>, dropping the oldest pairs while that system is
+        near singular.  The stopping test is the reference's |dE| <= tol
+        (ccm_iter.f90:71-88).  Returns the energies (start, then one per
+        iteration)."""
+        T = self.mbpt2_start() if T0 is None else [t.copy() for t in T0]
+        x = self.flat(T)
+        e1 = self.energy(T)
+        energies = [e1]
+        ys, fs, eys = [], [], []
+        dener = 4.0
+        for it in range(1, maxit + 1):
+            if not abs(dener) > tol:
+                break
+            R, d_h, d_p = self.residual(self.unflat(x))
+            D = self.flat(self.denominators(d_h, d_p))
+            Rf = self.flat(R)
+            with np.errstate(divide="ignore", invalid="ignore"):
+                f = np.where(D != 0.0, Rf / np.where(D != 0.0, D, 1.0), 0.0) - x
+            f32 = f.astype(np.complex64).astype(np.complex128)
+            y = x + beta * f
+            if len(ys) == depth:
+                ys.pop(0), fs.pop(0), eys.pop(0)
+            ys.append(y)
+            fs.append(f32)
+            eys.append(self.energy(self.unflat(y)))
+            while True:
+                G = np.array([[np.real(np.vdot(fj, fk)) for fk in fs] for fj in fs])
+                c = self.anderson_coefficients(G)
+                if c is not None:
+                    break
+                if len(ys) == 1:
+                    c = np.ones(1)
+                    break
+                ys.pop(0), fs.pop(0), eys.pop(0)
+            x = sum(cj * yj for cj, yj in zip(c, ys))
+            e2 = sum(cj * ej for cj, ej in zip(c, eys))
+            dener = e2 - e1
+            e1 = e2
+            energies.append(e2)
+        self.T = self.unflat(x)
+        self.ecc = self.e0 + e1
         return energies
 
     # ccm_triples.f90:174-294, 528-735 and ccm_t3full_channels.f90:4-284

@@ -242,7 +242,7 @@ def _pair_dot(m1, m2, m3, m4):
 
 def chiral_direct(lat, cc: ChiralConstants, p, q, r, s):
     """chiral_NN_with_delta(p,q,r,s) (:6304-6668), complex, in MeV."""
-    p, q, r, s = (np.asarray(x, np.int64) for x in (p, q, r, s))
+    p, q, r, s = np.broadcast_arrays(*(np.asarray(x, np.int64) for x in (p, q, r, s)))
     k1, k2, k3, k4 = (lat.k[x] * HBARC for x in (p, q, r, s))
     m1, m2, m3, m4 = (lat.sz[x] for x in (p, q, r, s))
     t1, t2, t3, t4 = (lat.tz[x] for x in (p, q, r, s))

@@ -68,6 +68,12 @@ for _k in range(1, 5):
                                           int(_k == 2), 1, 1)
 # one combination of the ntwist=3 deck (E0 and twist only)
 CASES["pnm14_n1_nnlo_tabc3_5"] = (14, "pnm", 0.16, 1, 3, "chiral", 6, 6, 1e-6, 100, ("tabc", 3, 5), 0, 0, 1)
+# PNM A=66 |n_i|<=4 (1458 states), chiral NNLO(450) NN, DIIS 8/8, tol 1e-8 MeV,
+# at the four densities of the large-basis study: E0, the MBPT2 start and every
+# iteration's correlation energy (no dumps: V_pppp alone is 7.7 GB ordered)
+for _rho in (0.08, 0.12, 0.16, 0.20):
+    CASES[f"pnm66_n4_nnlo_r{int(round(_rho * 100)):03d}"] = (66, "pnm", _rho, 4, 3, "chiral", 8, 8, 1e-8, 100,
+                                                            (0, 0, 0), 0, 0, 8)
 
 # Runs of the reference program itself (oracle/_ref/ccm.exe) on a deck, parsed
 # from its standard output: E0, every 'CCD energy' line (E0 + the T2
@@ -80,6 +86,11 @@ PROGRAM_CASES = {
     "snm4_n1_nnlo_3nf_ccdt1": (4, "snm", 0.16, 1, 3, 450, 2, "PBC", 1, 1, "CCDT1"),
     "pnm14_n1_nnlo_3nf_ccdt1": (14, "pnm", 0.16, 1, 3, 450, 2, "PBC", 1, 1, "CCDT1"),
 }
+# the reference's own sample deck (deltafull-CCMkspace/ccm_in: SNM A=28,
+# rho=0.165, |n_i|<=2, chiral NNLO(450) plus the 3NF normal-ordered to
+# two-body, CCD), run unmodified from where it lies
+PROGRAM_CASES["ccm_in_sample"] = (28, "snm", 0.165, 2, 3, 450, 2, "PBC", 1, 1, "CCD")
+SAMPLE_DECK = "/root/reference/deltafull-CCMkspace/ccm_in"
 # every combination of the ntwist=2 TABC deck with CCDT-1
 for _k in range(1, 5):
     PROGRAM_CASES[f"pnm14_n1_nnlo_3nf_ccdt1_tabc2_{_k}"] = (14, "pnm", 0.16, 1, 3, 450, 2, "TABC", 2, _k, "CCDT1")
@@ -88,11 +99,14 @@ for _k in range(1, 5):
 def run_program_case(name):
     A, matter, rho, nmax, order, cutoff, tnf_approx, bc, ntw, myt, cc = PROGRAM_CASES[name]
     with tempfile.TemporaryDirectory() as tmp:
-        deck = os.path.join(tmp, "ccm_in")
-        with open(deck, "w") as fh:
-            fh.write(CCM_IN.format(order=order, cutoff=cutoff, A=A, matter=matter, rho=rho, nmax=nmax,
-                                  tnf="T" if tnf_approx >= 0 else "F", tnf_approx=max(tnf_approx, 0),
-                                  bc=bc, ntwist=ntw, my_twists=myt, cc=cc))
+        if name == "ccm_in_sample":
+            deck = SAMPLE_DECK
+        else:
+            deck = os.path.join(tmp, "ccm_in")
+            with open(deck, "w") as fh:
+                fh.write(CCM_IN.format(order=order, cutoff=cutoff, A=A, matter=matter, rho=rho, nmax=nmax,
+                                      tnf="T" if tnf_approx >= 0 else "F", tnf_approx=max(tnf_approx, 0),
+                                      bc=bc, ntwist=ntw, my_twists=myt, cc=cc))
         env = dict(os.environ, OMP_NUM_THREADS="8", MKL_NUM_THREADS="1")
         res = subprocess.run([PROGRAM, deck], env=env, cwd=tmp, capture_output=True, text=True)
     if res.returncode != 0:
@@ -111,8 +125,14 @@ def run_program_case(name):
     out["e3"] = np.array([cplx(x) for x in lines if "CCDT-1 energy" in x], np.complex128)
     tail = [x for x in lines if x.startswith("     0.01700")]
     out["result_line"] = np.array([float(v) for v in tail[-1].split()], np.float64)
+    for key, label in (("e0_per_n", "Twist averaged E0/N"), ("mbpt2_per_n", "Twist averaged mbpt2"),
+                       ("eccd_per_n", "Twist averaged CCSD")):
+        hit = [x for x in lines if label in x]
+        if hit:
+            out[key] = np.complex128(cplx(hit[-1]))
     np.savez_compressed(os.path.join(HERE, name + ".npz"), **out)
-    print(name, "iterations", len(out["e3"]) - 1, "final", out["ccd_energies"][-1] + out["e3"][-1])
+    e3 = out["e3"][-1] if len(out["e3"]) else 0.0
+    print(name, "iterations", len(out["ccd_energies"]) - 1, "final", out["ccd_energies"][-1] + e3)
 
 
 CCM_IN = """!Chiral order for Deltas(LO = 0,NLO=2,NNLO=3,N3LO=4) and cutoff

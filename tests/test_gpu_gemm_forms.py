@@ -7,18 +7,24 @@ tile class: B read from L2, B staged in LDS, or A and B staged in LDS, with
 split k by the group alone, so the amplitudes must not depend on the choice.
 Each case forces one form on every class of every family (CCGPU_SHAPES, which
 also switches the autotuner off) and compares T2 after a few iterations with
-the first form's, element for element."""
+the first form's, element for element.  The pp ladder family has no split-K
+and, for N <= 4, one form of its own (the LDS-pipelined MFMA kernel); the
+pipelined form (lb 3: V and T streamed into LDS by direct-to-LDS loads) is
+forced on the pp ladder alone, the other families keeping B from L2."""
 import numpy as np
 import pytest
 
 pytestmark = pytest.mark.gpu
 
-# mb.nb.lb: 64*mb rows x 16*nb columns; lb 0 = B from L2, 1 = B in LDS, 2 = A and B in LDS
-FORMS = ["1.1.0", "1.1.1", "1.1.2", "1.2.2", "1.3.2", "2.1.0", "2.2.1", "4.1.0", "1.5.1"]
+# mb.nb.lb: 64*mb rows x 16*nb columns; lb 0 = B from L2, 1 = B in LDS, 2 = A and B in LDS,
+# 3 = pp ladder through the LDS load pipeline
+FORMS = ["1.1.0", "1.1.1", "1.1.2", "1.2.2", "1.3.2", "2.1.0", "2.2.1", "4.1.0", "1.5.1", "1.1.3", "1.2.3"]
+TAGS = ("", "L", "A", "P")
 
 
 def t2_after(ccnm, monkeypatch, form, precision):
-    spec = ",".join(f"{f}.{c}={form}" for f in range(5) for c in range(5))
+    other = form if not form.endswith(".3") else "1.1.0"
+    spec = ",".join(f"{f}.{c}={form if f == 1 else other}" for f in range(5) for c in range(5))
     monkeypatch.setenv("CCGPU_SHAPES", spec)
     monkeypatch.setenv("CCGPU_TIMING", "2")   # names every launched class with its shape
     s = ccnm.Solver(A=66, matter=ccnm.PNM, rho=0.16, nmax=3)
@@ -28,11 +34,14 @@ def t2_after(ccnm, monkeypatch, form, precision):
         r = s.ccd_iter(subspace=8, diis_step=8, tol=0.0, maxit=4, precision=precision)
         assert r["iterations"] == 4
         # the forced form ran: every MFMA class of the five families carries it
-        mb, nb, lb = (int(x) for x in form.split("."))
-        tag = f"{mb}x{nb}" + ("", "L", "A")[lb]
+        def tag(f):
+            mb, nb, lb = (int(x) for x in f.split("."))
+            return f"{mb}x{nb}" + TAGS[lb]
         launched = [k for k in ccnm.kernel_stats() if k.startswith("fam") and ".c5." not in k]
         assert {k.split(".")[0] for k in launched} == {f"fam{f}" for f in range(5)}, launched
-        assert all(k.endswith("." + tag) for k in launched), (tag, launched)
+        for k in launched:
+            want = tag(form) if k.startswith("fam1.") else tag(other)
+            assert k.endswith("." + want), (want, launched)
         return s.t2_unique()
     finally:
         s.close()

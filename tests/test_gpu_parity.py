@@ -189,12 +189,14 @@ def test_diis_trajectory_matches_oracle_when_well_conditioned(ccnm):
 def test_mfma_gemm_matches_valu_gemm(ccnm, monkeypatch, split):
     """The MFMA grouped GEMM against the plain VALU GEMM (independent lane
     layout, whole k range per tile) on the PNM A=66, nmax=2 lattice.  With
-    split="all", on the nmax=3 lattice (pp channels up to 620 pairs), every
-    group with k >= 512 is split (up to 8 ways), so the split-K partials and
-    their fixed-order sum are checked against the VALU result as well."""
+    split="all", on the nmax=3 lattice (pp pairs up to 620 per channel), every
+    group with k >= 512 outside the pp ladder (which never splits) is split 8
+    ways (split quantum lowered to 64 k), so the split-K partials and their
+    fixed-order sum are checked against the VALU result as well."""
     nmax = 2
     if split == "all":
         monkeypatch.setenv("CCGPU_SPLIT_TILES", "1000000")
+        monkeypatch.setenv("CCGPU_SPLIT_K", "64")
         nmax = 3
     res = []
     for mode in ("mfma", "valu"):
