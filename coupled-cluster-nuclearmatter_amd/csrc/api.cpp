@@ -131,15 +131,21 @@ void xcd_order_list(std::vector<GemmTile> &tiles, size_t ngroups) {
   std::vector<std::vector<GemmTile>> bins(nx);
   std::vector<double> load(nx, 0.0);
   std::vector<std::vector<GemmTile>> per(ngroups);
-  for (auto &t : tiles) per[t.g].push_back(t);
+  std::vector<double> cost(ngroups, 0.0);
+  for (auto &t : tiles) {
+    per[t.g].push_back(t);
+    cost[t.g] += (double)(t.ke - t.kb);   // a tile's time goes with its k range
+  }
+  // groups largest first onto the least-loaded XCD: the longest tiles start
+  // first and the tail of the launch is made of short ones
   std::vector<int> order(ngroups);
   for (size_t i = 0; i < order.size(); ++i) order[i] = (int)i;
-  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return per[a].size() > per[b].size(); });
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return cost[a] > cost[b]; });
   for (int g : order) {
     if (per[g].empty()) continue;
     int best = (int)(std::min_element(load.begin(), load.end()) - load.begin());
     for (auto &t : per[g]) bins[best].push_back(t);
-    load[best] += (double)per[g].size();
+    load[best] += cost[g];
   }
   std::vector<GemmTile> out;
   size_t maxlen = 0;
@@ -161,7 +167,8 @@ void xcd_order_list(std::vector<GemmTile> &tiles, size_t ngroups) {
 struct GemmShape {
   int mb, nb;
   int lb = 0;   // 1: B operand staged in LDS; 2: A and B in LDS (mb 1); 3: pp ladder with V and T
-                // streamed through LDS by direct-to-LDS loads (mb 1, nb 1-2); same tiles, same results
+                // streamed through LDS by direct-to-LDS loads (mb 1, nb 1-2); 4: the same with eight
+                // waves and one workgroup per CU (mb 1, nb 2); same tiles, same results
 };
 bool gemm_shape_ok(int mb, int nb) {
   return (mb == 0 && nb == 4) || (mb == 1 && nb >= 1 && nb <= 5) || (mb == 2 && (nb == 1 || nb == 2)) ||
@@ -172,10 +179,11 @@ bool gemm_shape_ok(int mb, int nb) {
 // kernel, or for the pp ladder the LDS-pipelined MFMA kernel.  The other
 // classes take any MFMA form; the LDS pipeline (lb 3) serves the pp ladder only.
 bool form_ok(int fam, int c, GemmShape s) {
-  if (c == 5) return fam == 1 ? (s.mb == 1 && s.nb == 1 && s.lb == 3) : (s.mb == 0 && s.nb == 4 && s.lb == 0);
-  if (s.mb <= 0 || !gemm_shape_ok(s.mb, s.nb) || s.lb < 0 || s.lb > 3) return false;
+  const bool ppl = s.mb == 1 && ((s.nb <= 2 && s.lb == 3) || (s.nb == 2 && s.lb == 4));
+  if (c == 5) return fam == 1 ? ppl : (s.mb == 0 && s.nb == 4 && s.lb == 0);
+  if (s.mb <= 0 || !gemm_shape_ok(s.mb, s.nb) || s.lb < 0 || s.lb > 4) return false;
   if (s.lb == 2) return s.mb == 1 && s.nb <= 3;
-  if (s.lb == 3) return fam == 1 && s.mb == 1 && s.nb <= 2;
+  if (s.lb >= 3) return fam == 1 && ppl;
   return true;
 }
 // rows and columns of one tile of a shape; (0,4) is the thin VALU kernel
@@ -248,6 +256,7 @@ struct GemmList {
   // batch), and the MFMA kernels sum k in the same order for every shape, so
   // results do not depend on the autotuner's choice or on batching.
   bool nosplit = false;           // every group over its whole k range (the pp ladder)
+  bool merged = false;            // one launch per distinct tile shape instead of per class
   static int split_of(const GemmGroup &g) {
     // CCGPU_SPLIT_TILES and CCGPU_SPLIT_K are read per list build (tests force
     // splits: a tile limit and a k quantum other than 128 and 256)
@@ -287,6 +296,15 @@ struct GemmList {
             tiles[c].push_back(GemmTile{(int)gi, m, n, ks, ks * ksz, std::min(g.K, (ks + 1) * ksz), 0, 0});
       for (int m = 0; m < g.M; m += 64)
         for (int n = 0; n < g.N; n += 64) tiles64.push_back(GemmTile{(int)gi, m, n, 0, 0, g.K, 0, 0});
+    }
+    if (merged) {   // classes of one shape share one launch (that of the first class of the shape)
+      for (int c = 1; c < NCLS; ++c)
+        for (int c2 = 0; c2 < c; ++c2)
+          if (shape[c2].mb == shape[c].mb && shape[c2].nb == shape[c].nb && shape[c2].lb == shape[c].lb) {
+            tiles[c2].insert(tiles[c2].end(), tiles[c].begin(), tiles[c].end());
+            tiles[c].clear();
+            break;
+          }
     }
     if (xcd)
       for (int c = 0; c < NCLS; ++c) xcd_order_list(tiles[c], groups.size());
@@ -451,6 +469,17 @@ void build_device(Solve &S) {
   d.off_vpp = dupload(P.off_vpp, o); d.off_vhh = dupload(P.off_vhh, o);
   d.pp_a = dupload(P.pp_a, o); d.pp_b = dupload(P.pp_b, o);
   d.hh_i = dupload(P.hh_i, o); d.hh_j = dupload(P.hh_j, o);
+  {
+    std::vector<uint64_t> lab(P.nT);
+    for (int c = 0; c < P.nch; ++c)
+      for (int r = 0; r < P.npp[c]; ++r)
+        for (int q = 0; q < P.nhh[c]; ++q) {
+          const int64_t pr = P.off_pp[c] + r, hs = P.off_hh[c] + q;
+          lab[P.off_t[c] + (int64_t)r * P.nhh[c] + q] = (uint64_t)P.pp_a[pr] | ((uint64_t)P.pp_b[pr] << 16) |
+                                                        ((uint64_t)P.hh_i[hs] << 32) | ((uint64_t)P.hh_j[hs] << 48);
+        }
+    d.lab = dupload(lab, o);
+  }
   d.ob_rowB = dupload(P.ob_rowB, o); d.ob_rowA = dupload(P.ob_rowA, o);
   d.ob_colJ = dupload(P.ob_colJ, o); d.ob_colI = dupload(P.ob_colI, o);
   d.nrows = (int64_t)P.pp_a.size();
@@ -491,7 +520,7 @@ void build_device(Solve &S) {
   d.dp = dalloc<double2>(P.np, o);
   d.I6off = dalloc<double2>(P.np, o);
   d.e0 = dalloc<double2>(1, o);
-  d.red = dalloc<double2>(512 * 36, o);
+  d.red = dalloc<double2>(1024 * 36, o);
   d.scal = dalloc<double2>(64, o);
   d.hist = nullptr;
   S.scal = d.scal;
@@ -667,8 +696,9 @@ std::string list_signature(const GemmList &L, int kind) {
 std::vector<GemmShape> shape_candidates(int fam, int c) {
   std::vector<GemmShape> v;
   if (c == 5) return fam == 1 ? std::vector<GemmShape>{{1, 1, 3}} : std::vector<GemmShape>{{0, 4}};
-  if (fam == 1) {   // the LDS-pipelined form first: the tuner keeps the first of equal times
+  if (fam == 1) {   // the LDS-pipelined forms first: the tuner keeps the first of equal times
     v.push_back({1, 1, 3});
+    v.push_back({1, 2, 4});
     if (c >= 1) v.push_back({1, 2, 3});
   }
   for (int nb = 1; nb <= std::min(5, c + 1); ++nb) v.push_back({1, nb});
@@ -727,7 +757,7 @@ bool autotune_enabled() {
 }
 
 void autotune(GemmList &L, int kind, const SolveDesc *d_descs, hipStream_t st) {
-  if (!autotune_enabled() || L.groups.empty()) return;
+  if (!autotune_enabled() || L.groups.empty() || L.merged) return;
   const std::string sig = list_signature(L, kind);
   load_tune_cache();
   auto it = g_tuned.find(sig);
@@ -769,7 +799,8 @@ void autotune(GemmList &L, int kind, const SolveDesc *d_descs, hipStream_t st) {
     }
     L.shape[c] = best_shape;
     rep += fmt(" c%d=%dx%d%s(%.1fus)", c, best_shape.mb, best_shape.nb,
-               best_shape.lb == 3 ? "P" : best_shape.lb == 2 ? "A" : best_shape.lb ? "L" : "", 1e3f * best / 3.f);
+               best_shape.lb == 4 ? "W" : best_shape.lb == 3 ? "P" : best_shape.lb == 2 ? "A" : best_shape.lb ? "L" : "",
+               1e3f * best / 3.f);
   }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
@@ -827,11 +858,17 @@ std::unique_ptr<Batch> make_batch(const std::vector<Solve *> &ss, hipStream_t st
     lists[k]->fam = fams[k];
     lists[k]->xcd = fams[k] == 1;
     if (fams[k] == 1) {
-      // the pp ladder has hundreds of 64-row tiles per class: no split-K, so
-      // no partial products go to HBM and back; N <= 4 on the LDS pipeline
+      // the pp ladder on 64 x 16 tiles of the LDS-pipelined kernel (four
+      // waves, two workgroups per CU) for every class, all in one launch:
+      // thousands of tiles, so no split-K and no partial products to HBM and
+      // back, and the launch's tail is short tiles instead of one class's
+      // last round.  Measured at PNM A=66 |n_i|<=4: 0.48 ms against 0.61 ms
+      // per class with split-K; one launch per form (64 x 32 eight-wave
+      // tiles for N > 16, which stream V once) took 0.54 ms, the wide launch
+      // being too short to fill the GPU.
       lists[k]->nosplit = true;
-      lists[k]->shape[5] = GemmShape{1, 1, 3};
-      for (int c = 0; c < 5; ++c) lists[k]->shape[c] = GemmShape{1, c == 0 ? 1 : 2, 3};
+      for (int c = 0; c < GemmList::NCLS; ++c) lists[k]->shape[c] = GemmShape{1, 1, 3};
+      lists[k]->merged = true;
     }
     lists[k]->apply_env_shapes();
   }
@@ -897,7 +934,7 @@ void run_gemm(const GemmList &L, const Batch &B, int prec, bool a_f32, hipStream
       if (L.tiles[c].empty()) continue;
       char nm[48];
       snprintf(nm, sizeof nm, "fam%d.k%d.c%d.%dx%d%s", L.fam, kind, c, L.shape[c].mb, L.shape[c].nb,
-               L.shape[c].lb == 3 ? "P" : L.shape[c].lb == 2 ? "A" : L.shape[c].lb ? "L" : "");
+               L.shape[c].lb == 4 ? "W" : L.shape[c].lb == 3 ? "P" : L.shape[c].lb == 2 ? "A" : L.shape[c].lb ? "L" : "");
       Timer t(detail, st, nm);
       launch_gemm_shape(L.d_groups, L.d_tiles[c], (int)L.tiles[c].size(), B.d_descs, L.d_gsolve, kind, L.shape[c].mb,
                         L.shape[c].nb, L.shape[c].lb, st);

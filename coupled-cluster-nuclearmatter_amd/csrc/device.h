@@ -26,6 +26,8 @@ struct SolveDesc {
   const int *t_ch, *npp, *nhh;
   const int64_t *off_t, *off_pp, *off_hh, *off_vpp, *off_vhh;
   const int *pp_a, *pp_b, *hh_i, *hh_j;
+  // per unique amplitude its orbitals a | b << 16 | i << 32 | j << 48
+  const uint64_t *lab;
   const int *ob_rowB, *ob_rowA, *ob_colJ, *ob_colI;
   // I6 by rows: channel of each unique pp row, CSR lists of partials per particle
   int64_t nrows;

@@ -326,6 +326,21 @@ __global__ void k_scatter_values(double2 *dst, const int64_t *idx, const double2
 // ---------------------------------------------------------------------------
 // per-iteration kernels; blockIdx.y selects the solve of a batch
 // ---------------------------------------------------------------------------
+// orbitals of unique amplitude u from its packed label
+__device__ __forceinline__ void amp_orbitals(const SolveDesc &d, int64_t u, int &a, int &b, int &i, int &j) {
+  const uint64_t l = d.lab[u];
+  a = (int)(l & 0xffff);
+  b = (int)((l >> 16) & 0xffff);
+  i = (int)((l >> 32) & 0xffff);
+  j = (int)(l >> 48);
+}
+// dynamic energy denominator dh_i + dh_j - dp_a - dp_b of unique amplitude u
+__device__ __forceinline__ double2 amp_denominator(const SolveDesc &d, int64_t u) {
+  int a, b, i, j;
+  amp_orbitals(d, u, a, b, i, j);
+  const double2 hi = d.dh[i], hj = d.dh[j], pa = d.dp[a - d.A], pb = d.dp[b - d.A];
+  return make_double2(hi.x + hj.x - pa.x - pb.x, hi.y + hj.y - pa.y - pb.y);
+}
 __device__ __forceinline__ void unique_labels(const SolveDesc &d, int64_t u, int &c, int &r, int &s) {
   c = d.t_ch[u];
   int64_t loc = u - d.off_t[c];
@@ -339,10 +354,8 @@ __device__ __forceinline__ void unique_labels(const SolveDesc &d, int64_t u, int
 __global__ void k_mbpt2(const SolveDesc *descs) {
   const SolveDesc &d = descs[blockIdx.y];
   for (int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; u < d.nT; u += (int64_t)gridDim.x * blockDim.x) {
-    int c, r, s;
-    unique_labels(d, u, c, r, s);
-    int a = d.pp_a[d.off_pp[c] + r], b = d.pp_b[d.off_pp[c] + r];
-    int i = d.hh_i[d.off_hh[c] + s], j = d.hh_j[d.off_hh[c] + s];
+    int a, b, i, j;
+    amp_orbitals(d, u, a, b, i, j);
     double den = -d.fdiag[a].x - d.fdiag[b].x + d.fdiag[i].x + d.fdiag[j].x;
     double2 t = make_double2(0.0, 0.0);
     if (fabs(den) != 0.0) { double2 v = d.R0[u]; t = make_double2(v.x / den, v.y / den); }
@@ -575,11 +588,7 @@ __global__ void k_update(const SolveDesc *descs, const int *slot) {
   if (!d.active) return;
   double2 *h = d.hist + (int64_t)slot[blockIdx.y] * d.nT;
   for (int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; u < d.nT; u += (int64_t)gridDim.x * blockDim.x) {
-    int c, r, s;
-    unique_labels(d, u, c, r, s);
-    int a = d.pp_a[d.off_pp[c] + r], b = d.pp_b[d.off_pp[c] + r];
-    int i = d.hh_i[d.off_hh[c] + s], j = d.hh_j[d.off_hh[c] + s];
-    double2 den = csub(csub(cadd(d.dh[i], d.dh[j]), d.dp[a - d.A]), d.dp[b - d.A]);
+    const double2 den = amp_denominator(d, u);
     double2 t = cadd(cscale(0.75, d.T[u]), cscale(0.25, cdiv(d.R[u], den)));
     d.T[u] = t;
     h[u] = t;
@@ -687,11 +696,7 @@ __global__ __launch_bounds__(256) void k_aa_update(const SolveDesc *descs, const
   for (int q = 0; q < kAAMax; ++q) acc[q] = 0.0;
   double2 ey = make_double2(0.0, 0.0);
   for (int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; u < d.nT; u += (int64_t)gridDim.x * blockDim.x) {
-    int c, r, s;
-    unique_labels(d, u, c, r, s);
-    int a = d.pp_a[d.off_pp[c] + r], b = d.pp_b[d.off_pp[c] + r];
-    int i = d.hh_i[d.off_hh[c] + s], j = d.hh_j[d.off_hh[c] + s];
-    double2 den = csub(csub(cadd(d.dh[i], d.dh[j]), d.dp[a - d.A]), d.dp[b - d.A]);
+    const double2 den = amp_denominator(d, u);
     const double2 t = d.T[u];
     const double2 f = csub(cdiv(d.R[u], den), t);
     const double2 yv = cadd(t, cscale(beta, f));
@@ -1590,9 +1595,10 @@ void launch_scatter_values(double2 *dst, const int64_t *idx, const double2 *val,
   hipLaunchKernelGGL(k_scatter_values, egrid(n, 1), dim3(256), 0, st, dst, idx, val, n);
 }
 void launch_aa_update(const SolveDesc *dd, int nsolve, const int *slots, double beta, hipStream_t st) {
-  hipLaunchKernelGGL(k_aa_update, dim3(kRedBlocks, nsolve), dim3(256), 0, st, dd, slots, beta);
+  constexpr int kAABlocks = 1024;   // partials: SolveDesc::red holds 1024 x 36
+  hipLaunchKernelGGL(k_aa_update, dim3(kAABlocks, nsolve), dim3(256), 0, st, dd, slots, beta);
   // 17 values per solve into scal[24 .. 40]
-  hipLaunchKernelGGL(k_reduce_final, dim3(1, nsolve), dim3(256), 0, st, dd, kRedBlocks, 36, 17, 24);
+  hipLaunchKernelGGL(k_reduce_final, dim3(1, nsolve), dim3(256), 0, st, dd, kAABlocks, 36, 17, 24);
 }
 void launch_aa_combine(const SolveDesc *dd, int nsolve, int64_t maxT, const int *slots, const double *coef,
                        hipStream_t st) {
@@ -1627,8 +1633,8 @@ static void launch_shape_lb(const GemmGroup *groups, const GemmTile *tiles, int 
 // pp ladder R += V_pppp T (ccm_t2_eqn.f90:574-607) with the V stream staged
 // through LDS by direct-to-LDS loads (global_load_lds_dwordx4): a tile is 64
 // rows x 16 NB columns over the whole k range of its channel (no split-K), and
-// every workgroup keeps three chunks of 16 KB of V (16 k of FP64, or 32 k of
-// the FP32 copy) plus the matching rows of T in flight, with no VGPRs spent on
+// every workgroup keeps up to three chunks of 16 KB of V (16 k of FP64, or 32 k
+// of the FP32 copy) plus the matching rows of T in flight, with no VGPRs spent on
 // staging.  Each 16 lanes of a load instruction read one V row's 256
 // contiguous bytes; the row's slots land XOR-swizzled by the row index, so the
 // MFMA reads (16 rows at one k) hit 16 different LDS slots.  The k order of
@@ -1664,7 +1670,8 @@ struct PplCfg {
   static constexpr int BBYTES = KC * NT * 16;        // T rows of a chunk (FP64)
   static constexpr int BIW = BBYTES / 4096;          // T load instructions per wave per chunk
   static constexpr int IPW = 4 + BIW;                // load instructions per wave per chunk
-  static constexpr int S = 3 * (16384 + BBYTES) <= 81920 ? 3 : 2;   // stages (two workgroups per CU)
+  // stages: as many as two workgroups per CU (160 KB of LDS) allow, at most 4
+  static constexpr int S = 4 * (16384 + BBYTES) <= 81920 ? 4 : 3 * (16384 + BBYTES) <= 81920 ? 3 : 2;
 };
 
 template <int NB, int VK>
@@ -1680,8 +1687,10 @@ __global__ __launch_bounds__(256, 2) void k_ppl(const GemmGroup *__restrict__ gr
   typedef typename std::conditional<MF32, f4, d4>::type acc4;
   // one object per stage: the compiler's wait insertion then tells the
   // stages apart and leaves the waits of the pipeline to the code below
-  __shared__ __attribute__((aligned(16))) char A0[16384], A1[16384], A2[S == 3 ? 16384 : 16];
-  __shared__ __attribute__((aligned(16))) char B0[C::BBYTES], B1[C::BBYTES], B2[S == 3 ? C::BBYTES : 16];
+  __shared__ __attribute__((aligned(16))) char A0[16384], A1[16384], A2[S >= 3 ? 16384 : 16],
+      A3[S >= 4 ? 16384 : 16];
+  __shared__ __attribute__((aligned(16))) char B0[C::BBYTES], B1[C::BBYTES], B2[S >= 3 ? C::BBYTES : 16],
+      B3[S >= 4 ? C::BBYTES : 16];
 
   const GemmTile tl = tiles[blockIdx.x];
   if (descs && !descs[gsolve[tl.g]].active) return;
@@ -1784,7 +1793,16 @@ __global__ __launch_bounds__(256, 2) void k_ppl(const GemmGroup *__restrict__ gr
     compute(ac, bc, max(0, min(KC, K - c * KC)));
   };
   issue(0, A0, B0);
-  if constexpr (S == 3) {
+  if constexpr (S == 4) {
+    issue(1, A1, B1);
+    issue(2, A2, B2);
+    for (int c = 0; c < nchunk; c += 4) {
+      step(c, A0, B0, A3, B3);
+      step(c + 1, A1, B1, A0, B0);
+      step(c + 2, A2, B2, A1, B1);
+      step(c + 3, A3, B3, A2, B2);
+    }
+  } else if constexpr (S == 3) {
     issue(1, A1, B1);
     for (int c = 0; c < nchunk; c += 3) {
       step(c, A0, B0, A2, B2);
@@ -1826,6 +1844,172 @@ __global__ __launch_bounds__(256, 2) void k_ppl(const GemmGroup *__restrict__ gr
         g.C[(int64_t)m * g.ldc + n] = v;
       }
     }
+}
+
+// The same pipeline with eight waves and one workgroup per CU: a tile is 64
+// rows x 32 columns, waves 0-3 take columns 0-15 and waves 4-7 columns 16-31 of
+// the same V chunk, so a channel with up to 32 hole pairs streams its V_pppp
+// block once (16-column tiles stream it once per column tile).  With the
+// whole LDS to itself the workgroup keeps five chunks in flight (FP64 V; three
+// for the FP32 copy).  Same k order, so the same bits as the other forms.
+template <int VK>
+struct Ppl8Cfg {
+  static constexpr bool VF32 = VK != 0;
+  static constexpr int KC = VF32 ? 32 : 16;
+  static constexpr int BBYTES = KC * 32 * 16;         // T rows of a chunk: KC x 32 columns
+  static constexpr int BIW = BBYTES / 8192;           // T load instructions per wave per chunk
+  static constexpr int IPW = 2 + BIW;                 // V: 16 instructions over 8 waves
+  static constexpr int S = VF32 ? 4 : 6;              // stages (S x (16 KB + BBYTES) <= 160 KB)
+};
+
+template <int VK>
+__global__ __launch_bounds__(512, 2) void k_ppl8(const GemmGroup *__restrict__ groups,
+                                               const GemmTile *__restrict__ tiles,
+                                               const SolveDesc *__restrict__ descs,
+                                               const int *__restrict__ gsolve) {
+  typedef Ppl8Cfg<VK> C;
+  constexpr bool VF32 = C::VF32, MF32 = VK == 2;
+  constexpr int KC = C::KC, S = C::S, IPW = C::IPW, BIW = C::BIW;
+  static_assert(C::BBYTES % 8192 == 0 && S * (16384 + C::BBYTES) <= 163840, "LDS budget");
+  typedef typename std::conditional<MF32, float, double>::type real;
+  typedef typename std::conditional<MF32, f4, d4>::type acc4;
+  __shared__ __attribute__((aligned(16))) char A0[16384], A1[16384], A2[16384], A3[16384],
+      A4[S >= 5 ? 16384 : 16], A5[S >= 6 ? 16384 : 16];
+  __shared__ __attribute__((aligned(16))) char B0[C::BBYTES], B1[C::BBYTES], B2[C::BBYTES], B3[C::BBYTES],
+      B4[S >= 5 ? C::BBYTES : 16], B5[S >= 6 ? C::BBYTES : 16];
+  const GemmTile tl = tiles[blockIdx.x];
+  if (descs && !descs[gsolve[tl.g]].active) return;
+  const GemmGroup g = groups[tl.g];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int li = lane & 15, lg = lane >> 4;
+  const int wr = w & 3, wc = w >> 2;                 // row block and column block of this wave
+  const int K = g.K, nchunk = (K + KC - 1) / KC;
+  const char *arow[2];
+  int aslot[2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i) {
+    const int r = 4 * (2 * w + i) + lg;
+    aslot[i] = li ^ (r & 15);
+    arow[i] = (const char *)g.A + (int64_t)min(tl.m0 + r, g.M - 1) * g.lda * (VF32 ? 8 : 16);
+  }
+  int bkr[BIW], bcol[BIW];
+#pragma unroll
+  for (int j = 0; j < BIW; ++j) {
+    const int e = 64 * (BIW * w + j) + lane;
+    bkr[j] = e >> 5;
+    bcol[j] = min(tl.n0 + (e & 31), g.N - 1);
+  }
+  auto issue = [&](int c, char *as, char *bs) {
+    const int k0 = min(c, nchunk - 1) * KC;
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+      const char *src;
+      if constexpr (VF32) src = arow[i] + 8 * (int64_t)min(k0 + 2 * aslot[i], ((K + 1) & ~1) - 2);
+      else src = arow[i] + 16 * (int64_t)min(k0 + aslot[i], K - 1);
+      dma16(src, as + (2 * w + i) * 1024);
+    }
+#pragma unroll
+    for (int j = 0; j < BIW; ++j) {
+      const int k = min(k0 + bkr[j], K - 1);
+      dma16(g.B + (int64_t)k * g.ldb + bcol[j], bs + (BIW * w + j) * 1024);
+    }
+  };
+  acc4 cre = (acc4){0, 0, 0, 0}, cim = cre, cs = cre;
+  const int row = 16 * wr + li;
+  auto mac = [&](double2 a, int kk, const char *bs, int kvalid) {
+    const real ar = (real)a.x, ai = (real)a.y;
+    double2 b = *(const double2 *)(bs + (kk * 32 + 16 * wc + li) * 16);
+    if (kk >= kvalid) b = make_double2(0.0, 0.0);
+    const real br = (real)b.x, bi = (real)b.y;
+    if constexpr (MF32) {
+      cre = __builtin_amdgcn_mfma_f32_16x16x4f32(ar, br, cre, 0, 0, 0);
+      cre = __builtin_amdgcn_mfma_f32_16x16x4f32(-ai, bi, cre, 0, 0, 0);
+      cim = __builtin_amdgcn_mfma_f32_16x16x4f32(ar, bi, cim, 0, 0, 0);
+      cim = __builtin_amdgcn_mfma_f32_16x16x4f32(ai, br, cim, 0, 0, 0);
+    } else {
+      cre = __builtin_amdgcn_mfma_f64_16x16x4f64(ar, br, cre, 0, 0, 0);
+      cim = __builtin_amdgcn_mfma_f64_16x16x4f64(ai, bi, cim, 0, 0, 0);
+      cs = __builtin_amdgcn_mfma_f64_16x16x4f64(ar + ai, br + bi, cs, 0, 0, 0);
+    }
+  };
+  auto compute = [&](const char *as, const char *bs, int kvalid) {
+    if constexpr (VF32) {
+#pragma unroll
+      for (int h = 0; h < 4; ++h) {
+        const int pr = 4 * h + lg;
+        const float4 a4 = *(const float4 *)(as + row * 256 + ((pr ^ (row & 15)) * 16));
+        mac(make_double2(a4.x, a4.y), 2 * pr, bs, kvalid);
+        mac(make_double2(a4.z, a4.w), 2 * pr + 1, bs, kvalid);
+      }
+    } else {
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int kk = 4 * q + lg;
+        mac(*(const double2 *)(as + row * 256 + ((kk ^ (row & 15)) * 16)), kk, bs, kvalid);
+      }
+    }
+  };
+  auto step = [&](int c, const char *ac, const char *bc, char *an, char *bn) {
+    ppl_sync<(S - 2) * IPW>();
+    issue(c + S - 1, an, bn);
+    compute(ac, bc, max(0, min(KC, K - c * KC)));
+  };
+  issue(0, A0, B0);
+  issue(1, A1, B1);
+  issue(2, A2, B2);
+  if constexpr (S == 6) {
+    issue(3, A3, B3);
+    issue(4, A4, B4);
+    for (int c = 0; c < nchunk; c += 6) {
+      step(c, A0, B0, A5, B5);
+      step(c + 1, A1, B1, A0, B0);
+      step(c + 2, A2, B2, A1, B1);
+      step(c + 3, A3, B3, A2, B2);
+      step(c + 4, A4, B4, A3, B3);
+      step(c + 5, A5, B5, A4, B4);
+    }
+  } else {
+    for (int c = 0; c < nchunk; c += 4) {
+      step(c, A0, B0, A3, B3);
+      step(c + 1, A1, B1, A0, B0);
+      step(c + 2, A2, B2, A1, B1);
+      step(c + 3, A3, B3, A2, B2);
+    }
+  }
+  ppl_sync<0>();
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const int rr = MF32 ? (lg * 4 + r) : (lg + 4 * r);
+    const int m = tl.m0 + 16 * wr + rr;
+    const int n = tl.n0 + 16 * wc + li;
+    double vr, vi;
+    if constexpr (MF32) {
+      vr = (double)cre[r];
+      vi = (double)cim[r];
+    } else {
+      vr = cre[r] - cim[r];
+      vi = cs[r] - cre[r] - cim[r];
+    }
+    if (m < g.M && n < g.N) {
+      double2 v = make_double2(g.alpha * vr, g.alpha * vi);
+      if (g.beta != 0.0) {
+        const double2 c0 = g.C0[(int64_t)m * g.ldc0 + n];
+        v.x += g.beta * c0.x;
+        v.y += g.beta * c0.y;
+      }
+      g.C[(int64_t)m * g.ldc + n] = v;
+    }
+  }
+}
+
+static void launch_shape_ppl8(const GemmGroup *groups, const GemmTile *tiles, int ntiles, const SolveDesc *dd,
+                              const int *gsolve, int kind, hipStream_t st) {
+  switch (kind) {
+    case 3: hipLaunchKernelGGL((k_ppl8<0>), dim3(ntiles), dim3(512), 0, st, groups, tiles, dd, gsolve); break;
+    case 5: hipLaunchKernelGGL((k_ppl8<1>), dim3(ntiles), dim3(512), 0, st, groups, tiles, dd, gsolve); break;
+    case 4: hipLaunchKernelGGL((k_ppl8<2>), dim3(ntiles), dim3(512), 0, st, groups, tiles, dd, gsolve); break;
+    default: break;
+  }
 }
 
 template <int NB>
@@ -1872,6 +2056,10 @@ static void launch_shape(const GemmGroup *groups, const GemmTile *tiles, int nti
 void launch_gemm_shape(const GemmGroup *groups, const GemmTile *tiles, int ntiles, const SolveDesc *dd,
                        const int *gsolve, int kind, int mb, int nb, int lb, hipStream_t st) {
   if (ntiles <= 0) return;
+  if (lb == 4 && mb == 1 && nb == 2) {   // pp ladder, eight-wave LDS pipeline, 64 x 32 tiles
+    launch_shape_ppl8(groups, tiles, ntiles, dd, gsolve, kind, st);
+    return;
+  }
   if (lb == 3 && mb == 1) {   // pp ladder through the LDS load pipeline
     switch (nb) {
       case 1: launch_shape_ppl<1>(groups, tiles, ntiles, dd, gsolve, kind, st); break;

@@ -17,13 +17,14 @@ import pytest
 pytestmark = pytest.mark.gpu
 
 # mb.nb.lb: 64*mb rows x 16*nb columns; lb 0 = B from L2, 1 = B in LDS, 2 = A and B in LDS,
-# 3 = pp ladder through the LDS load pipeline
-FORMS = ["1.1.0", "1.1.1", "1.1.2", "1.2.2", "1.3.2", "2.1.0", "2.2.1", "4.1.0", "1.5.1", "1.1.3", "1.2.3"]
-TAGS = ("", "L", "A", "P")
+# 3 = pp ladder through the LDS load pipeline, 4 = the same with eight waves (64 x 32 tiles)
+FORMS = ["1.1.0", "1.1.1", "1.1.2", "1.2.2", "1.3.2", "2.1.0", "2.2.1", "4.1.0", "1.5.1", "1.1.3", "1.2.3",
+         "1.2.4"]
+TAGS = ("", "L", "A", "P", "W")
 
 
 def t2_after(ccnm, monkeypatch, form, precision):
-    other = form if not form.endswith(".3") else "1.1.0"
+    other = form if not (form.endswith(".3") or form.endswith(".4")) else "1.1.0"
     spec = ",".join(f"{f}.{c}={form if f == 1 else other}" for f in range(5) for c in range(5))
     monkeypatch.setenv("CCGPU_SHAPES", spec)
     monkeypatch.setenv("CCGPU_TIMING", "2")   # names every launched class with its shape
