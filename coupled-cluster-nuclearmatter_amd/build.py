@@ -22,7 +22,7 @@ HIP_FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-I", os.path.join(ROOT, "include"),
 ]
-SOURCES = ["plan.cpp", "chiral.cpp", "kernels.hip", "triples.hip", "mlp.hip", "api.cpp"]
+SOURCES = ["plan.cpp", "chiral.cpp", "kernels.hip", "triples.hip", "mlp.hip", "noalloc.hip", "api.cpp"]
 
 
 def _run(cmd, cwd=None):

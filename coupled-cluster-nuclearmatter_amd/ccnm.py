@@ -111,6 +111,7 @@ SIGNATURES = {
     "ccgpu_mlp_loss_grad": [_I, _I, _FP, _FP, _DP, _FP],
     "ccgpu_mlp_features": [_I, _I, _FP, _FP],
     "ccgpu_autotune_report": [C.c_char_p, _I],
+    "ccgpu_energy_noalloc": [C.c_longlong, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p],
     "ccgpu_last_error": [C.c_char_p, _I],
 }
 
@@ -365,6 +366,13 @@ def batch_solve(solvers, **kw):
     res = (SolveResult * len(solvers))()
     _check(lib().ccgpu_batch_solve(len(solvers), _ip(hs), C.byref(o), res))
     return [r.as_dict() for r in res]
+
+
+def energy_noalloc(n, t_ptr, r0_ptr, out_ptr, stream=None):
+    """ccgpu_energy_noalloc on device pointers (ints, e.g. torch data_ptr());
+    stream-ordered, no synchronization, no allocation."""
+    _check(lib().ccgpu_energy_noalloc(int(n), C.c_void_p(t_ptr), C.c_void_p(r0_ptr), C.c_void_p(out_ptr),
+                                      C.c_void_p(stream or 0)))
 
 
 def kernel_stats():

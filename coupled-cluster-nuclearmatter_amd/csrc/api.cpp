@@ -83,8 +83,14 @@ void launch_mlp_apply(const SolveDesc &d, const float *y, float scale, float ef,
 using namespace ccgpu;
 
 namespace {
-
 thread_local std::string g_err;
+}  // namespace
+
+namespace ccgpu {
+void set_last_error(const std::string &msg) { g_err = msg; }
+}  // namespace ccgpu
+
+namespace {
 
 struct Error : std::runtime_error {
   explicit Error(const std::string &m) : std::runtime_error(m) {}

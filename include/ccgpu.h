@@ -328,6 +328,18 @@ int ccgpu_mlp_forward(int mlp, int n, const float *x, float *y);
 int ccgpu_mlp_loss_grad(int mlp, int n, const float *x, const float *t, double *loss, float *grad);
 int ccgpu_mlp_features(int mlp, int handle, float *feat, float *target);
 
+/* Correlation energy sum t conj(r0) of n unique amplitudes (the sum of
+ * ccgpu_energy) on caller-owned device buffers t, r0 (n complex each),
+ * written to the device buffer ecorr (one complex), launched on `stream`
+ * (a hipStream_t; null for the default stream) without synchronizing.  No
+ * counterpart in the reference: it is for callers that cannot allocate device
+ * memory (code under hipGraph capture, serving loops).  It allocates nothing:
+ * its block partials live in a static per-device array of 4096 complex values,
+ * one per span of 65536 amplitudes, so n is limited to 268 435 456; larger n
+ * is rejected with an error.  Calls that overlap on one GPU share that array
+ * and race. */
+int ccgpu_energy_noalloc(long long n, const double *t, const double *r0, double *ecorr, void *stream);
+
 int ccgpu_last_error(char *buf, int len);
 
 #ifdef __cplusplus
