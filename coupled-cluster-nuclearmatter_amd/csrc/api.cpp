@@ -1355,6 +1355,12 @@ void run_batch(Batch &B, const ccgpu_solve_opts &op, std::vector<ccgpu_solve_res
   bool fp32_phase = op.precision != CCGPU_PREC_FP64;
   double prev_maxd = 1e300;
   std::vector<char> reached(n, 0), failed(n, 0);
+  // stopping test: |dE| <= tol on `need` consecutive iterations.  The
+  // reference's update stops at the first (ccm_iter.f90:71-74); the Anderson
+  // update waits for a second one, since one small energy change can fall
+  // between two larger ones before the iterate has settled
+  const int need = aa ? 2 : 1;
+  std::vector<int> small(n, 0);
   bool pending = false;   // Anderson: coefficients uploaded, next iterate not yet formed
   for (int it = 1; it <= op.maxit; ++it) {
     if (pending) {
@@ -1365,7 +1371,7 @@ void run_batch(Batch &B, const ccgpu_solve_opts &op, std::vector<ccgpu_solve_res
     }
     bool any = false;
     for (int s = 0; s < n; ++s) {
-      bool act = !failed[s] && (fp32_phase ? B.h_descs[s].active != 0 : std::abs(dener[s]) > op.tol);
+      bool act = !failed[s] && (fp32_phase ? B.h_descs[s].active != 0 : small[s] < need);
       if (!act && B.h_descs[s].active) {
         set_active(B, s, 0);
         conv[s] = !failed[s];
@@ -1426,6 +1432,7 @@ void run_batch(Batch &B, const ccgpu_solve_opts &op, std::vector<ccgpu_solve_res
       }
       dener[s] = e2[s] - elast[s];
       elast[s] = e2[s];
+      small[s] = std::abs(dener[s]) <= op.tol ? small[s] + 1 : 0;
       B.solves[s]->ehist.push_back(e2[s]);
       iters[s] = it;
       if (prec != CCGPU_PREC_FP64) ++iters32[s];
@@ -1441,9 +1448,12 @@ void run_batch(Batch &B, const ccgpu_solve_opts &op, std::vector<ccgpu_solve_res
       for (int s = 0; s < n; ++s) at_tol = at_tol && (reached[s] || !B.h_descs[s].active);
       if (at_boundary || at_tol) {
         fp32_phase = false;
-        // an FP32 |dE| is no FP64 convergence test: force at least one FP64 step
+        // an FP32 |dE| is no FP64 convergence test: force FP64 steps
         for (int s = 0; s < n; ++s)
-          if (B.h_descs[s].active) dener[s] = std::max(std::abs(dener[s]), 2.0 * op.tol);
+          if (B.h_descs[s].active) {
+            dener[s] = std::max(std::abs(dener[s]), 2.0 * op.tol);
+            small[s] = 0;
+          }
       }
     }
     if (!extrapolated || aa) prev_maxd = maxd;
@@ -1455,7 +1465,7 @@ void run_batch(Batch &B, const ccgpu_solve_opts &op, std::vector<ccgpu_solve_res
   HIPCHK(hipStreamSynchronize(B.stream));
   double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   for (int s = 0; s < n; ++s) {
-    if (!failed[s] && std::abs(dener[s]) <= op.tol) conv[s] = 1;
+    if (!failed[s] && small[s] >= need) conv[s] = 1;
     if (failed[s]) conv[s] = 0;
     if (!conv[s] && !failed[s]) B.status[s] |= CCGPU_STATUS_MAXIT;
     res[s].status = B.status[s];

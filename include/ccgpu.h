@@ -146,7 +146,8 @@ typedef struct {
                         diis_step iterations over subspace iterates);
                         CCGPU_ACCEL_ANDERSON: Anderson/Pulay update every
                         iteration on the residual vectors f = R/D - T, with at
-                        most subspace held vectors (diis_step unused) */
+                        most subspace held vectors (diis_step unused); it stops
+                        after two consecutive iterations with |dE| <= tol */
   double mix;        /* CCGPU_ACCEL_ANDERSON: T_next = sum_j c_j (T_j + mix f_j);
                         (0, 1], 0 selects 1 */
 } ccgpu_solve_opts;

@@ -636,17 +636,17 @@ class CCD:
         y = T + beta f, holds the last `depth` pairs, and takes
         T = sum_j c_j y_j with the coefficients of anderson_coefficients on
         G_jk = Re<f_j, f_k>, dropping the oldest pairs while that system is
-        near singular.  The stopping test is the reference's |dE| <= tol
-        (ccm_iter.f90:71-88).  Returns the energies (start, then one per
-        iteration)."""
+        near singular.  It stops once |dE| <= tol on two consecutive
+        iterations (the reference's test, ccm_iter.f90:71-88, asks for one).
+        Returns the energies (start, then one per iteration)."""
         T = self.mbpt2_start() if T0 is None else [t.copy() for t in T0]
         x = self.flat(T)
         e1 = self.energy(T)
         energies = [e1]
         ys, fs, eys = [], [], []
-        dener = 4.0
+        small = 0
         for it in range(1, maxit + 1):
-            if not abs(dener) > tol:
+            if small >= 2:
                 break
             R, d_h, d_p = self.residual(self.unflat(x))
             D = self.flat(self.denominators(d_h, d_p))
@@ -671,7 +671,7 @@ class CCD:
                 ys.pop(0), fs.pop(0), eys.pop(0)
             x = sum(cj * yj for cj, yj in zip(c, ys))
             e2 = sum(cj * ej for cj, ej in zip(c, eys))
-            dener = e2 - e1
+            small = small + 1 if abs(e2 - e1) <= tol else 0
             e1 = e2
             energies.append(e2)
         self.T = self.unflat(x)
