@@ -88,8 +88,10 @@ def run_plain(ccnm, solvers, tol=TOL):
 #   fp64      FP64 contractions throughout
 #   v32       pp ladder on the complex64 copy of V_pppp (FP64 arithmetic)
 #   fp32      FP32 MFMA contractions, pp ladder on the complex64 copy
-FAST_PLANS = ("fp32", "v32", "fp64")
-PREC = {"fp64": 0, "fp32": 1, "v32": 2}
+#   bf16x3    X/I9, hh ladder and ring on split bf16 products (hi*hi + hi*lo +
+#             lo*hi, FP32 sums), pp ladder as in fp32
+FAST_PLANS = ("bf16x3", "fp32", "v32", "fp64")
+PREC = {"fp64": 0, "fp32": 1, "v32": 2, "bf16x3": 3}
 
 
 def run_fast(ccnm, solvers, net, tol=TOL_FAST, plan="fp32", learned=True, accel=1, batched=True):
@@ -472,6 +474,7 @@ def main():
     # roofline of the dominant kernel: the pp ladder (FP64 in the plain arm)
     ms64, n64, bytes64, flops64, st64 = ladder_roofline(ccnm, solvers[2], ccnm.PREC_FP64)
     ms32, n32, bytes32, flops32, st32 = ladder_roofline(ccnm, solvers[2], ccnm.PREC_FP32)
+    stbf = ladder_roofline(ccnm, solvers[2], ccnm.PREC_BF16X3)[4]
     gbs64 = bytes64 / (ms64 * 1e-3) / 1e9
     tfs64 = flops64 / (ms64 * 1e-3) / 1e12
     ai64 = flops64 / bytes64
@@ -550,6 +553,7 @@ def main():
         "roofline_fp32_ladder": roof32,
         "kernel_ms_fp64": {k: round(v[0], 4) for k, v in st64.items()},
         "kernel_ms_fp32": {k: round(v[0], 4) for k, v in st32.items()},
+        "kernel_ms_bf16x3": {k: round(v[0], 4) for k, v in stbf.items()},
         "tile_shapes": shapes,
         "cpu_baseline": cpu,
     }

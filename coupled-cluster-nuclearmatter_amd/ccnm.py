@@ -21,7 +21,7 @@ INT_MINNESOTA = 0
 INT_CHIRAL = 1
 LO, NLO, NNLO = 0, 2, 3
 BLOCK_HHHH, BLOCK_HHHP, BLOCK_HHPP, BLOCK_HPHP, BLOCK_HPPP, BLOCK_PPPP = 1, 2, 3, 4, 5, 6
-PREC_FP64, PREC_FP32, PREC_V32 = 0, 1, 2
+PREC_FP64, PREC_FP32, PREC_V32, PREC_BF16X3 = 0, 1, 2, 3
 ACCEL_REFERENCE, ACCEL_ANDERSON = 0, 1
 STATUS_DIIS_FALLBACK, STATUS_NONFINITE, STATUS_MAXIT = 1, 2, 4
 

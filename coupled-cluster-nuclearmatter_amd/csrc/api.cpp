@@ -654,10 +654,10 @@ struct Batch {
   // the fork/join events
   hipStream_t stream2 = nullptr;
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
-  hipGraph_t graph[3] = {nullptr, nullptr, nullptr};        // per precision mode
-  hipGraphExec_t gexec[3] = {nullptr, nullptr, nullptr};
+  hipGraph_t graph[4] = {nullptr, nullptr, nullptr, nullptr};        // per precision mode
+  hipGraphExec_t gexec[4] = {nullptr, nullptr, nullptr, nullptr};
   ~Batch() {
-    for (int q = 0; q < 3; ++q) {
+    for (int q = 0; q < 4; ++q) {
       if (gexec[q]) (void)hipGraphExecDestroy(gexec[q]);
       if (graph[q]) (void)hipGraphDestroy(graph[q]);
     }
@@ -922,11 +922,16 @@ int term_mask() {
 }
 
 // One GEMM family: three launches (one per tile class), or the VALU check.
-// prec here is the arithmetic of this family: FP64, FP32, or V32 (pp ladder
-// only: FP32-stored A, FP64 arithmetic)
+// prec here is the arithmetic of this family: FP64, FP32, V32 (pp ladder
+// only: FP32-stored A, FP64 arithmetic) or BF16x3 (every family but the pp
+// ladder, FP64-stored A)
 int gemm_kind(int prec, bool a_f32, bool pp) {
   if (pp) return prec == CCGPU_PREC_FP64 ? 3 : prec == CCGPU_PREC_V32 ? 5 : 4;
+  if (prec == CCGPU_PREC_BF16X3) return 6;
   return prec == CCGPU_PREC_FP64 ? 0 : (a_f32 ? 2 : 1);
+}
+bool known_precision(int p) {
+  return p == CCGPU_PREC_FP64 || p == CCGPU_PREC_FP32 || p == CCGPU_PREC_V32 || p == CCGPU_PREC_BF16X3;
 }
 
 void run_gemm(const GemmList &L, const Batch &B, int prec, bool a_f32, hipStream_t st, bool pp = false) {
@@ -951,9 +956,10 @@ void run_gemm(const GemmList &L, const Batch &B, int prec, bool a_f32, hipStream
   }
 }
 
-// Families in FP32 under CCGPU_PREC_FP32 (bit 0 X/I9 GEMM, 1 pp ladder, 2 hh
-// ladder, 3 ring GEMMs); all by default, CCGPU_FP32_FAMILIES narrows it down
-// for precision studies.
+// Families in low precision under CCGPU_PREC_FP32 and CCGPU_PREC_BF16X3 (bit 0
+// X/I9 GEMM, 1 pp ladder, 2 hh ladder, 3 ring GEMMs); all by default,
+// CCGPU_FP32_FAMILIES narrows it down for precision studies.  Under BF16x3 the
+// pp ladder's bit selects the FP32 ladder on the FP32 copy of V_pppp.
 int fp32_families() {
   const char *e = getenv("CCGPU_FP32_FAMILIES");
   return e ? atoi(e) : 15;
@@ -969,9 +975,10 @@ void residual_launches(Batch &B, int prec, bool timing) {
   hipStream_t st = B.stream;
   const int terms = term_mask();
   if (prec != CCGPU_PREC_FP64 && !B.has_f32) throw Error("FP32/V32 residual without the FP32 copy of Vpppp");
-  const int fam = prec == CCGPU_PREC_FP32 ? fp32_families() : 0;
-  auto pf = [&](int bit) { return (fam >> bit) & 1 ? CCGPU_PREC_FP32 : CCGPU_PREC_FP64; };
-  const int pp_prec = prec == CCGPU_PREC_V32 ? CCGPU_PREC_V32 : pf(1);
+  const bool low = prec == CCGPU_PREC_FP32 || prec == CCGPU_PREC_BF16X3;
+  const int fam = low ? fp32_families() : 0;
+  auto pf = [&](int bit) { return (fam >> bit) & 1 ? prec : CCGPU_PREC_FP64; };
+  const int pp_prec = prec == CCGPU_PREC_V32 ? CCGPU_PREC_V32 : (fam >> 1) & 1 ? CCGPU_PREC_FP32 : CCGPU_PREC_FP64;
   {
     Timer t(timing, st, "gemm_x_vhhpp_t");
     run_gemm(B.g_x, B, pf(0), false, st);
@@ -1308,8 +1315,7 @@ void run_batch(Batch &B, const ccgpu_solve_opts &op, std::vector<ccgpu_solve_res
     B.h_descs[s].active = 1;
   }
   HIPCHK(hipMemcpy(B.d_descs, B.h_descs.data(), n * sizeof(SolveDesc), hipMemcpyHostToDevice));
-  if (op.precision != CCGPU_PREC_FP64 && op.precision != CCGPU_PREC_FP32 && op.precision != CCGPU_PREC_V32)
-    throw Error(fmt("unknown precision %d", op.precision));
+  if (!known_precision(op.precision)) throw Error(fmt("unknown precision %d", op.precision));
   if (op.precision != CCGPU_PREC_FP64) {
     for (Solve *S : B.solves)
       if (!S->desc.Vpppp32) throw Error("FP32/V32 ladder requested before the FP32 copy of Vpppp was made");
@@ -1439,7 +1445,10 @@ void run_batch(Batch &B, const ccgpu_solve_opts &op, std::vector<ccgpu_solve_res
       maxd = std::max(maxd, std::abs(dener[s]));
       if (std::abs(dener[s]) <= op.tol) reached[s] = 1;
       if (op.verbose) printf("dener %5d %5d %16.8E  E %.12f%s\n", s, it, dener[s].real(), e2[s].real(),
-                             prec == CCGPU_PREC_FP32 ? "  fp32" : prec == CCGPU_PREC_V32 ? "  v32" : "");
+                             prec == CCGPU_PREC_FP32     ? "  fp32"
+                             : prec == CCGPU_PREC_V32    ? "  v32"
+                             : prec == CCGPU_PREC_BF16X3 ? "  bf16x3"
+                                                         : "");
     }
     if (fp32_phase) {
       const bool at_boundary =
@@ -2111,8 +2120,7 @@ int ccgpu_residual(int handle, int precision) {
     Solve &S = get(handle);
     ensure_ready(S);
     if (!S.have_t) throw Error("no amplitudes: call ccgpu_mbpt2_start or ccgpu_set_t2 first");
-    if (precision != CCGPU_PREC_FP64 && precision != CCGPU_PREC_FP32 && precision != CCGPU_PREC_V32)
-      throw Error(fmt("unknown precision %d", precision));
+    if (!known_precision(precision)) throw Error(fmt("unknown precision %d", precision));
     if (precision != CCGPU_PREC_FP64) ensure_f32(S, handle);
     Batch &B = single(handle);
     prepare_lists(B, precision);

@@ -1306,6 +1306,186 @@ __global__ __launch_bounds__(256) void k_zgemm_la(const GemmGroup *__restrict__ 
     }
 }
 
+// BF16x3 grouped complex GEMM: every real operand x (FP64 or FP32 in memory)
+// is split once into two bfloat16 numbers, xh = bf16(x) and xl = bf16(x - xh)
+// (both rounded to nearest, x - xh exact in FP32), and a product is summed as
+// xh yh + xh yl + xl yh on v_mfma_f32_16x16x32_bf16 with FP32 accumulation; the
+// dropped xl yl and the two roundings leave |x y - sum| <= 3 * 2^-16 |x y|.
+// A complex MAC packs (re, im) as neighbouring k of one bf16 MFMA: lane slot
+// 2c holds ar and 2c+1 ai of complex index c, the B operand (br, -bi) gives the
+// real part and (bi, br) the imaginary part, so 16 complex k are one MFMA and
+// three MFMAs per part cover the split.
+// Both operands are staged in LDS already split and in MFMA lane order: the
+// 64*MB x 16 A chunk row-coalesced (16 threads per row), B column-wise with
+// the pair (br, -bi); each wave rebuilds (bi, br) from it with a rotate and a
+// sign flip.  Loads run two 16-wide k chunks ahead of the MFMAs in registers,
+// issued unconditionally (indices clamped, values zeroed past the k range) so
+// the count of loads in flight is the same on every path, and the barriers
+// wait for LDS only (a workgroup fence would drain the loads in flight).
+// The k order is fixed by the tile, so repeated runs give the same bits.
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ void bf3_split(float x, float y, unsigned &hi, unsigned &lo) {
+  const __bf16 xh = (__bf16)x, yh = (__bf16)y;
+  const __bf16 xl = (__bf16)(x - (float)xh), yl = (__bf16)(y - (float)yh);
+  hi = __builtin_bit_cast(unsigned, (bf16x2){xh, yh});
+  lo = __builtin_bit_cast(unsigned, (bf16x2){xl, yl});
+}
+// (br, -bi) -> (bi, br): swap the halves, then flip the sign of the new low half
+__device__ __forceinline__ u32x4 bf3_im_pairs(u32x4 v) {
+  u32x4 r;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) r[q] = ((v[q] >> 16) | (v[q] << 16)) ^ 0x8000u;
+  return r;
+}
+__device__ __forceinline__ void bf3_lds_sync() {
+  asm volatile("" ::: "memory");
+  __builtin_amdgcn_s_waitcnt((15) | (3 << 14) | (7 << 4) | (0 << 8));   // lgkmcnt(0) only
+  __builtin_amdgcn_s_barrier();
+  asm volatile("" ::: "memory");
+}
+
+template <int MB, int NB, typename AT>
+__global__ __launch_bounds__(256) void k_zgemm_bf3(const GemmGroup *__restrict__ groups,
+                                                   const GemmTile *__restrict__ tiles,
+                                                   const SolveDesc *__restrict__ descs,
+                                                   const int *__restrict__ gsolve) {
+  constexpr int ROWS = 64 * MB;               // tile rows
+  constexpr int BC = 16 * NB;                 // tile columns
+  constexpr int APT = ROWS * 16 / 256;        // A elements per thread per chunk
+  constexpr int BPT = (16 * BC + 255) / 256;  // B elements per thread per chunk
+  constexpr int LD = 20;                      // dwords per LDS row (16 + 4 pad: conflict-free b128 reads)
+  __shared__ __align__(16) unsigned Ah[2][ROWS][LD], Al[2][ROWS][LD];
+  __shared__ __align__(16) unsigned Bh[2][BC][LD], Bl[2][BC][LD];
+  const GemmTile tl = tiles[blockIdx.x];
+  if (descs && !descs[gsolve[tl.g]].active) return;
+  const GemmGroup g = groups[tl.g];
+  const AT *A = (const AT *)g.A;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int li = lane & 15, lg = lane >> 4;
+  f4 cre[MB][NB], cim[MB][NB];
+#pragma unroll
+  for (int x = 0; x < MB; ++x)
+#pragma unroll
+    for (int y = 0; y < NB; ++y) {
+      cre[x][y] = (f4){0, 0, 0, 0};
+      cim[x][y] = (f4){0, 0, 0, 0};
+    }
+  const int kb = tl.kb, ke = min(tl.ke, g.K);
+  // A element e = threadIdx.x + 256 j: tile row e >> 4, k0 + (e & 15);
+  // B element e: k0 + e / BC, tile column e % BC.  The loads only fetch (from
+  // clamped, valid addresses); elements outside the block are zeroed when the
+  // chunk is split into LDS, so no instruction touches a load's result before
+  // the chunk is needed.
+  auto load = [&](double2 (&a)[APT], double2 (&b)[BPT], int k0) {
+#pragma unroll
+    for (int j = 0; j < APT; ++j) {
+      const int e = threadIdx.x + 256 * j;
+      const int m = tl.m0 + (e >> 4), k = k0 + (e & 15);
+      a[j] = ccgpu::load_a<AT>(A + (int64_t)min(m, g.M - 1) * g.lda + min(k, ke - 1));
+    }
+#pragma unroll
+    for (int j = 0; j < BPT; ++j) {
+      const int e = threadIdx.x + 256 * j;
+      const int k = k0 + e / BC, n = tl.n0 + e % BC;
+      b[j] = gload(g.B + (int64_t)min(k, ke - 1) * g.ldb + min(n, g.N - 1));
+    }
+  };
+  auto store = [&](const double2 (&a)[APT], const double2 (&b)[BPT], int k0, int buf) {
+#pragma unroll
+    for (int j = 0; j < APT; ++j) {
+      const int e = threadIdx.x + 256 * j;
+      const bool ok = tl.m0 + (e >> 4) < g.M && k0 + (e & 15) < ke;
+      unsigned h, l;
+      bf3_split(ok ? (float)a[j].x : 0.f, ok ? (float)a[j].y : 0.f, h, l);
+      Ah[buf][e >> 4][e & 15] = h;
+      Al[buf][e >> 4][e & 15] = l;
+    }
+#pragma unroll
+    for (int j = 0; j < BPT; ++j) {
+      const int e = threadIdx.x + 256 * j;
+      if (16 * BC % 256 == 0 || e < 16 * BC) {
+        const bool ok = k0 + e / BC < ke && tl.n0 + e % BC < g.N;
+        unsigned h, l;
+        bf3_split(ok ? (float)b[j].x : 0.f, ok ? -(float)b[j].y : 0.f, h, l);
+        Bh[buf][e % BC][e / BC] = h;
+        Bl[buf][e % BC][e / BC] = l;
+      }
+    }
+  };
+  auto compute = [&](int buf) {
+    u32x4 ah[MB], al[MB];
+#pragma unroll
+    for (int x = 0; x < MB; ++x) {
+      const int r = (w * MB + x) * 16 + li;
+      ah[x] = *(const u32x4 *)&Ah[buf][r][4 * lg];
+      al[x] = *(const u32x4 *)&Al[buf][r][4 * lg];
+    }
+#pragma unroll
+    for (int y = 0; y < NB; ++y) {
+      const u32x4 brh = *(const u32x4 *)&Bh[buf][y * 16 + li][4 * lg];
+      const u32x4 brl = *(const u32x4 *)&Bl[buf][y * 16 + li][4 * lg];
+      const u32x4 bih = bf3_im_pairs(brh), bil = bf3_im_pairs(brl);
+#pragma unroll
+      for (int x = 0; x < MB; ++x) {
+        const bf16x8 xh = __builtin_bit_cast(bf16x8, ah[x]), xl = __builtin_bit_cast(bf16x8, al[x]);
+        cre[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(xl, __builtin_bit_cast(bf16x8, brh), cre[x][y], 0, 0, 0);
+        cre[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(xh, __builtin_bit_cast(bf16x8, brl), cre[x][y], 0, 0, 0);
+        cre[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(xh, __builtin_bit_cast(bf16x8, brh), cre[x][y], 0, 0, 0);
+        cim[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(xl, __builtin_bit_cast(bf16x8, bih), cim[x][y], 0, 0, 0);
+        cim[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(xh, __builtin_bit_cast(bf16x8, bil), cim[x][y], 0, 0, 0);
+        cim[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(xh, __builtin_bit_cast(bf16x8, bih), cim[x][y], 0, 0, 0);
+      }
+    }
+  };
+  double2 a0[APT], b0[BPT], a1[APT], b1[BPT];
+  // pairs of chunks; the loads and LDS stores of a pair run unconditionally
+  // (past the k range they fetch valid addresses and store zeros), only the
+  // MFMAs of a second chunk past the end are skipped
+  const int nchunk = (ke - kb + 15) >> 4;
+  load(a0, b0, kb);
+  load(a1, b1, kb + 16);
+  store(a0, b0, kb, 0);
+  bf3_lds_sync();
+  for (int c = 0, k0 = kb; c < nchunk; c += 2, k0 += 32) {
+    // chunk k0 in LDS buffer 0, chunk k0 + 16 in flight in a1/b1
+    load(a0, b0, k0 + 32);
+    compute(0);
+    store(a1, b1, k0 + 16, 1);
+    bf3_lds_sync();
+    // chunk k0 + 16 in LDS buffer 1, chunk k0 + 32 in flight in a0/b0
+    load(a1, b1, k0 + 48);
+    if (c + 1 < nchunk) compute(1);
+    store(a0, b0, k0 + 32, 0);
+    bf3_lds_sync();
+  }
+  // result layout of the 16x16 FP32 MFMAs: register r of lane l holds column
+  // l & 15 and row 4 (l >> 4) + r
+#pragma unroll
+  for (int x = 0; x < MB; ++x)
+#pragma unroll
+    for (int y = 0; y < NB; ++y)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int m = tl.m0 + (w * MB + x) * 16 + lg * 4 + r;
+        const int n = tl.n0 + y * 16 + li;
+        const double vr = (double)cre[x][y][r], vi = (double)cim[x][y][r];
+        if (m < g.M && n < g.N && g.nsplit > 1) {
+          g.P[((int64_t)tl.ks * g.M + m) * g.N + n] = make_double2(vr, vi);
+        } else if (m < g.M && n < g.N) {
+          double2 v = make_double2(g.alpha * vr, g.alpha * vi);
+          if (g.beta != 0.0) {
+            double2 c0 = g.C0[(int64_t)m * g.ldc0 + n];
+            v.x += g.beta * c0.x;
+            v.y += g.beta * c0.y;
+          }
+          g.C[(int64_t)m * g.ldc + n] = v;
+        }
+      }
+}
+
 // Thin GEMM for groups with N <= 4 columns (most pp-ladder channels with few
 // hole pairs): FP64 VALU instead of 16-column MFMA tiles, which would be 75% or
 // more padding.  A tile is 16 rows: each wave owns 4 rows, its 64 lanes walk
@@ -1610,7 +1790,8 @@ void launch_diis_axpy(const SolveDesc *dd, int nsolve, int64_t maxT, const int *
 }
 
 // Kernel kinds: 0 FP64, 1 FP32 (A double), 2 FP32 (A float), 3 FP64 pp ladder,
-// 4 FP32 pp ladder (A float), 5 pp ladder with A float and FP64 arithmetic.
+// 4 FP32 pp ladder (A float), 5 pp ladder with A float and FP64 arithmetic,
+// 6 BF16x3 split products (A double; the thin N <= 4 class stays FP64).
 // Tile shapes (MB, NB): MB 16-row blocks per wave (4 waves stacked along M),
 // NB 16-column blocks: (1,1) (1,2) (1,3) (1,4) (1,5) (2,1) (2,2) (4,1); and
 // (0,4): the thin VALU kernel (16-row tiles, N <= 4).  lb = 1 selects the
@@ -2053,9 +2234,30 @@ static void launch_shape(const GemmGroup *groups, const GemmTile *tiles, int nti
   }
 }
 
+// kind 6: BF16x3 split products (k_zgemm_bf3), one form for every lb
+template <int MB, int NB>
+static void launch_shape_bf3(const GemmGroup *groups, const GemmTile *tiles, int ntiles, const SolveDesc *dd,
+                             const int *gsolve, hipStream_t st) {
+  hipLaunchKernelGGL((k_zgemm_bf3<MB, NB, double2>), dim3(ntiles), dim3(256), 0, st, groups, tiles, dd, gsolve);
+}
+
 void launch_gemm_shape(const GemmGroup *groups, const GemmTile *tiles, int ntiles, const SolveDesc *dd,
                        const int *gsolve, int kind, int mb, int nb, int lb, hipStream_t st) {
   if (ntiles <= 0) return;
+  if (kind == 6 && mb > 0) {
+    switch (mb * 10 + nb) {
+      case 11: launch_shape_bf3<1, 1>(groups, tiles, ntiles, dd, gsolve, st); break;
+      case 12: launch_shape_bf3<1, 2>(groups, tiles, ntiles, dd, gsolve, st); break;
+      case 13: launch_shape_bf3<1, 3>(groups, tiles, ntiles, dd, gsolve, st); break;
+      case 14: launch_shape_bf3<1, 4>(groups, tiles, ntiles, dd, gsolve, st); break;
+      case 15: launch_shape_bf3<1, 5>(groups, tiles, ntiles, dd, gsolve, st); break;
+      case 21: launch_shape_bf3<2, 1>(groups, tiles, ntiles, dd, gsolve, st); break;
+      case 22: launch_shape_bf3<2, 2>(groups, tiles, ntiles, dd, gsolve, st); break;
+      case 41: launch_shape_bf3<4, 1>(groups, tiles, ntiles, dd, gsolve, st); break;
+      default: break;
+    }
+    return;
+  }
   if (lb == 4 && mb == 1 && nb == 2) {   // pp ladder, eight-wave LDS pipeline, 64 x 32 tiles
     launch_shape_ppl8(groups, tiles, ntiles, dd, gsolve, kind, st);
     return;
@@ -2093,7 +2295,7 @@ void launch_gemm_shape(const GemmGroup *groups, const GemmTile *tiles, int ntile
   }
   if (mb == 0) {   // thin VALU kernel, N <= 4, FP64 arithmetic
     const bool af = kind == 2 || kind == 4 || kind == 5;
-    const bool pp = kind >= 3;
+    const bool pp = kind >= 3 && kind <= 5;
     if (af && pp) hipLaunchKernelGGL((k_zgemm_thin<float2, 1>), dim3(ntiles), dim3(256), 0, st, groups, tiles, dd, gsolve);
     else if (af) hipLaunchKernelGGL((k_zgemm_thin<float2, 0>), dim3(ntiles), dim3(256), 0, st, groups, tiles, dd, gsolve);
     else if (pp) hipLaunchKernelGGL((k_zgemm_thin<double2, 1>), dim3(ntiles), dim3(256), 0, st, groups, tiles, dd, gsolve);

@@ -7,7 +7,7 @@ module ccgpu_iface
 
   integer(c_int), parameter :: CCGPU_PNM = 0, CCGPU_SNM = 1
   integer(c_int), parameter :: CCGPU_INT_MINNESOTA = 0, CCGPU_INT_CHIRAL = 1
-  integer(c_int), parameter :: CCGPU_PREC_FP64 = 0, CCGPU_PREC_FP32 = 1, CCGPU_PREC_V32 = 2
+  integer(c_int), parameter :: CCGPU_PREC_FP64 = 0, CCGPU_PREC_FP32 = 1, CCGPU_PREC_V32 = 2, CCGPU_PREC_BF16X3 = 3
   integer(c_int), parameter :: CCGPU_BLOCK_HHHH = 1, CCGPU_BLOCK_HHHP = 2, CCGPU_BLOCK_HHPP = 3, &
        CCGPU_BLOCK_HPHP = 4, CCGPU_BLOCK_HPPP = 5, CCGPU_BLOCK_PPPP = 6
 

@@ -14,7 +14,8 @@
 !   chiral, 6, 6, 1e-6, 100, fp64, 1.0
 ! The interaction is chiral (the Delta-full chiral NN of the deck's first line,
 ! chiral order and cutoff, as the reference) or minnesota.
-! With precision fp32 (or v32: FP32-stored pppp elements, FP64 arithmetic) the contractions run in FP32 up to the first DIIS
+! With precision fp32 (or v32: FP32-stored pppp elements, FP64 arithmetic; or bf16x3: split
+! bfloat16 products with FP32 sums) the contractions run in low precision up to the first DIIS
 ! extrapolation that follows an iteration with |dE| <= the switch (MeV; 0: never
 ! there) or until the FP32 |dE| meets the tolerance, and in FP64 afterwards
 ! (DIIS history kept), so the stopping test is always an FP64 one.
@@ -121,6 +122,7 @@ program ccm_gpu
   prec = CCGPU_PREC_FP64
   if (trim(precname) == 'fp32') prec = CCGPU_PREC_FP32
   if (trim(precname) == 'v32') prec = CCGPU_PREC_V32
+  if (trim(precname) == 'bf16x3') prec = CCGPU_PREC_BF16X3
   tw_weight = 1.d0
   if (trim(boundary_conditions) == 'TABC') then
      if (my_twists == 0) then

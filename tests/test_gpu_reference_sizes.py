@@ -106,18 +106,19 @@ def warm_net(ccnm):
 
 
 @pytest.mark.parametrize("accel", [0, 1])
-@pytest.mark.parametrize("prec", ["fp64", "v32", "fp32"])
+@pytest.mark.parametrize("prec", ["fp64", "v32", "fp32", "bf16x3"])
 def test_fast_arm_nmax4_within_1e5_per_nucleon(ccnm, lattices, warm_net, prec, accel):
     """Every plan of the fast path (learned start, the four densities in one
-    batch replayed from a hipGraph, FP64 / V32 / FP32 contractions with the
-    FP64 stopping test, the reference update or Anderson) lands within
-    1e-5 MeV per nucleon of the reference's converged E/N."""
+    batch replayed from a hipGraph, FP64 / V32 / FP32 / BF16x3 contractions
+    with the FP64 stopping test, the reference update or Anderson) lands
+    within 1e-5 MeV per nucleon of the reference's converged E/N."""
     sols = [lattices[n] for n in NAMES]
     for s in sols:
         warm_net.warm_start(s)
-    p = {"fp64": ccnm.PREC_FP64, "v32": ccnm.PREC_V32, "fp32": ccnm.PREC_FP32}[prec]
+    p = {"fp64": ccnm.PREC_FP64, "v32": ccnm.PREC_V32, "fp32": ccnm.PREC_FP32, "bf16x3": ccnm.PREC_BF16X3}[prec]
     res = ccnm.batch_solve(sols, subspace=8, diis_step=8, tol=1e-8, maxit=100, precision=p,
-                           fp32_switch=1.0 if prec == "fp32" else 0.0, use_graph=1, accel=accel, mix=1.0)
+                           fp32_switch=1.0 if prec in ("fp32", "bf16x3") else 0.0, use_graph=1, accel=accel,
+                           mix=1.0)
     for r, nm in zip(res, NAMES):
         g = golden(nm)
         assert r["converged"], (nm, r)
@@ -239,3 +240,100 @@ def test_residual_terms_elementwise_nmax2(ccnm, oracle_n2, monkeypatch, terms, s
     assert np.max(np.abs(gdh - dh)) <= 1e-12 * np.max(np.abs(dh))
     assert np.max(np.abs(gdp - dp[A:])) <= 1e-12 * np.max(np.abs(dp[A:]))
     s.close()
+
+
+def _abs_oracle(cc):
+    """The oracle with every interaction block replaced by its modulus: its
+    ladder terms of |T| bound, element by element, the sums of |products| that
+    the ladder contractions of the true residual add up."""
+    import copy
+    ca = copy.copy(cc)
+    for nm in ("v_hhpp", "v_pppp", "v_hhhh", "vtil", "w0"):
+        setattr(ca, nm, [np.abs(x) for x in getattr(cc, nm)])
+    return ca
+
+
+def _ring_abs(cc, xabs):
+    """Sum of |products| behind every element of the ring term (the oracle's
+    Pandya chain T~ -> I4 = 1/2 V~ T~ + W0 -> R~ = T~ I4 -> four scattered
+    adds, ccm_t2_eqn.f90:657-1185) with moduli throughout and the four adds
+    taken with sign +1, so cancellations between them do not hide the error
+    each one carries."""
+    ch = cc.ch
+    out = np.zeros(ch.size)
+    for K in range(len(ch.ph_keys)):
+        ph, hp = ch.ph[K], ch.hp[K]
+        a, i = ph[:, 0][:, None], ph[:, 1][:, None]
+        j, b = hp[:, 0][None, :], hp[:, 1][None, :]
+        idx = ch.flat_index(a, b, i, j)
+        tt = np.where(idx >= 0, xabs[np.maximum(idx, 0)], 0.0)
+        rt = tt @ (0.5 * (np.abs(cc.vtil[K]) @ tt) + np.abs(cc.w0[K]))
+        for aa, bb, ii, jj in ((a, b, i, j), (b, a, i, j), (a, b, j, i), (b, a, j, i)):
+            idx2 = ch.flat_index(aa, bb, ii, jj)
+            m = idx2 >= 0
+            np.add.at(out, idx2[m], rt[m])
+    return out
+
+
+@pytest.mark.parametrize("terms", [2, 4, 8])
+def test_bf16x3_residual_within_split_product_bound(ccnm, oracle_n2, monkeypatch, terms):
+    """The BF16x3 residual (X/I9, hh ladder and both ring GEMMs on split bf16
+    products with FP32 sums; the pp ladder in FP32 on the FP32 copy of V)
+    against the FP64 residual, element by element, on PNM A=66 |n_i|<=2
+    (chiral NNLO(450)) at amplitudes after 12 Anderson iterations.
+
+    Bound: a real product x*y summed as xh*yh + xh*yl + xl*yh, with
+    xh = bf16(x) and xl = bf16(x - xh), is within 3 * 2^-16 |x y| of x*y
+    (2^-16 |x| from each split, 2^-16 |x y| for the dropped xl*yl); an FP32
+    sum of K products adds at most K * 2^-24 of the sum of their moduli, and
+    the FP64 -> FP32 conversion 2^-24 per operand.  The hh ladder and the ring
+    are two GEMMs in a row (I9 or I4, then R), which doubles the bound, so
+        |R_bf16x3 - R_fp64| <= 2 eps |R|(|V|, |T|),
+        eps = 3 * 2^-16 + (K + 2) * 2^-24,
+    where |R|(|V|, |T|) is the oracle's residual with every block and
+    amplitude replaced by its modulus and K the longest contraction.  The pp
+    ladder (one FP32 GEMM on FP32-rounded V): (K + 3) * 2^-24 |R|(|V|, |T|).
+    The BF16x3 kernels must have run, and the error must be far below that of
+    plain bf16 products (2^-8)."""
+    cc = oracle_n2
+    s = ccnm.Solver(A=A, matter=ccnm.PNM, rho=0.16, nmax=2)
+    s.set_chiral(ccnm.NNLO, 450.0)
+    s.generate_interaction(ccnm.INT_CHIRAL)
+    s.setup_t_amplitudes()
+    s.ccd_iter(subspace=8, diis_step=8, tol=0.0, maxit=12, accel=ccnm.ACCEL_ANDERSON, mix=1.0)
+    t = s.t2_unique()
+    lab = s.t2_labels()
+    a, b, i, j = lab[:, 0], lab[:, 1], lab[:, 2], lab[:, 3]
+    x = np.zeros(cc.ch.size, np.complex128)
+    for (p, q, r_, u), sg in (((a, b, i, j), 1.0), ((b, a, i, j), -1.0), ((a, b, j, i), -1.0), ((b, a, j, i), 1.0)):
+        x[cc.ch.flat_index(p, q, r_, u)] = sg * t
+    if terms == 8:
+        bound_abs = _ring_abs(cc, np.abs(x))[cc.ch.flat_index(a, b, i, j)]
+    else:
+        ca = _abs_oracle(cc)
+        Rabs, _, _ = ca.residual(ca.unflat(np.abs(x)), terms)
+        bound_abs = np.real(ca.flat(Rabs)[cc.ch.flat_index(a, b, i, j)])
+    monkeypatch.setenv("CCGPU_TERMS", str(terms))
+    s.set_t2_unique(t)
+    s.t2_intermediate(ccnm.PREC_FP64)
+    r64 = s.debug_fetch("R", len(t))
+    monkeypatch.setenv("CCGPU_TIMING", "2")   # names every launched GEMM class with its kernel kind
+    s.t2_intermediate(ccnm.PREC_BF16X3)
+    rbf = s.debug_fetch("R", len(t))
+    launched = list(ccnm.kernel_stats())
+    monkeypatch.delenv("CCGPU_TIMING")
+    if terms == 2:
+        eps = (s.dims["max_npp"] + 3) * 2.0**-24
+    else:
+        # longest contraction: a pp block (X = V_hhpp T), or a particle-hole
+        # channel (both ring GEMMs run over ph or hp pairs of one channel)
+        kring = max(max(len(x) for x in cc.ch.ph), max(len(x) for x in cc.ch.hp))
+        kmax = max(s.dims["max_npp"], kring)
+        eps = 2 * (3 * 2.0**-16 + (kmax + 2) * 2.0**-24)
+        assert any(".k6." in k for k in launched), launched
+    err = np.abs(rbf - r64)
+    assert np.all(np.isfinite(rbf))
+    assert np.all(err <= eps * bound_abs + 1e-12 * np.abs(r64).max()), (err / np.maximum(bound_abs, 1e-300)).max()
+    assert err.max() < 2.0**-12 * np.abs(r64).max()
+    s.close()
+
