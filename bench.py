@@ -376,7 +376,7 @@ def main():
                          "normal-ordered 3NF), its tables and one (T) kernel; no JSON line")
     ap.add_argument("--profile-run", action="store_true",
                     help="short run for counter collection: 6 FP64 iterations of the rho=0.16 lattice, "
-                         "then 6 FP32 iterations of the four-density batch; no JSON line")
+                         "then 6 FP32 and 6 BF16x3 iterations of the four-density batch; no JSON line")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this engine runs on one GPU; --gpus above 1 is not supported "
@@ -399,8 +399,13 @@ def main():
             s.setup_t_amplitudes()
         r32 = ccnm.batch_solve(solvers, subspace=DEPTH, diis_step=DEPTH, tol=1e-30, maxit=6,
                                precision=ccnm.PREC_FP32, fp32_switch=0.0, use_graph=1, accel=1, mix=1.0)
+        for s in solvers:
+            s.setup_t_amplitudes()
+        rbf = ccnm.batch_solve(solvers, subspace=DEPTH, diis_step=DEPTH, tol=1e-30, maxit=6,
+                               precision=ccnm.PREC_BF16X3, fp32_switch=0.0, use_graph=1, accel=1, mix=1.0)
         print(f"# profile run: FP64 {r64['seconds']*1e3:.1f} ms / 6 iterations, "
-              f"FP32 batch {r32[0]['seconds']*1e3:.1f} ms / 6 iterations", file=sys.stderr, flush=True)
+              f"FP32 batch {r32[0]['seconds']*1e3:.1f} ms / 6 iterations, "
+              f"BF16x3 batch {rbf[0]['seconds']*1e3:.1f} ms / 6 iterations", file=sys.stderr, flush=True)
         return
 
     # plain arm: the FP64 answer of every density and the time to it

@@ -180,11 +180,19 @@ bool gemm_shape_ok(int mb, int nb) {
   return (mb == 0 && nb == 4) || (mb == 1 && nb >= 1 && nb <= 5) || (mb == 2 && (nb == 1 || nb == 2)) ||
          (mb == 4 && nb == 1);
 }
+// Shapes of the BF16x3 kernel (its accumulators are FP32 and its operands
+// are staged in LDS already split, so it affords wider tiles): 64 rows by
+// 16..128 columns, or 128 rows by 16..64 columns; (0,4) is the thin kernel.
+bool bf3_shape_ok(int mb, int nb) {
+  return (mb == 0 && nb == 4) || (mb == 1 && nb >= 1 && nb <= 8 && nb != 7) || (mb == 2 && nb >= 1 && nb <= 4);
+}
 // Forms a family's tile class may use.  Class 5 (N <= 4) has one form per
 // family, so its summation order never depends on the tuner: the thin VALU
 // kernel, or for the pp ladder the LDS-pipelined MFMA kernel.  The other
 // classes take any MFMA form; the LDS pipeline (lb 3) serves the pp ladder only.
+// List ids 10 + fam are the BF16x3 lists of a family (lb 0 only).
 bool form_ok(int fam, int c, GemmShape s) {
+  if (fam >= 10) return c == 5 ? (s.mb == 0 && s.nb == 4 && s.lb == 0) : (s.mb > 0 && s.lb == 0 && bf3_shape_ok(s.mb, s.nb));
   const bool ppl = s.mb == 1 && ((s.nb <= 2 && s.lb == 3) || (s.nb == 2 && s.lb == 4));
   if (c == 5) return fam == 1 ? ppl : (s.mb == 0 && s.nb == 4 && s.lb == 0);
   if (s.mb <= 0 || !gemm_shape_ok(s.mb, s.nb) || s.lb < 0 || s.lb > 4) return false;
@@ -199,6 +207,8 @@ int tile_cols(GemmShape s) { return s.mb == 0 ? 4 : 16 * s.nb; }
 struct GemmList {
   static constexpr int NCLS = 6;
   int fam = 0;   // 0 X = V_hhpp T, 1 pp ladder, 2 hh ladder, 3 ring I4, 4 ring R~
+  bool bf3 = false;   // the list of the BF16x3 kernel: its own classes, shapes and tuning
+  int id() const { return bf3 ? 10 + fam : fam; }
   std::vector<GemmGroup> groups;
   std::vector<int> gsolve;
   GemmShape shape[NCLS] = {{2, 1}, {2, 2}, {1, 3}, {1, 4}, {1, 5}, {0, 4}};
@@ -216,9 +226,14 @@ struct GemmList {
   double2 *d_part = nullptr;
   double flops = 0;       // algorithmic real flops per launch
   double bytes = 0;       // algorithmic bytes of the A operand per launch
-  // class 5: N <= 4; 0: N <= 16; 1: <= 32; 2: <= 48; 3: <= 64 or > 80; 4: <= 80
+  // class 5: N <= 4; 0: N <= 16; 1: <= 32; 2: <= 48; 3: <= 64 or > 80; 4: <= 80.
+  // BF16x3 lists: 5: N <= 4; 0: <= 16; 1: <= 32; 2: <= 64; 4: <= 96; 3: wider
   static int cls_of(int N) {
     return N <= 4 ? 5 : N <= 16 ? 0 : N <= 32 ? 1 : N <= 48 ? 2 : N <= 64 ? 3 : N <= 80 ? 4 : 3;
+  }
+  int cls(int N) const {
+    if (!bf3) return cls_of(N);
+    return N <= 4 ? 5 : N <= 16 ? 0 : N <= 32 ? 1 : N <= 64 ? 2 : N <= 96 ? 4 : 3;
   }
   void add(GemmGroup g, int solve) {
     if (g.M <= 0 || g.N <= 0 || g.K <= 0) return;
@@ -235,7 +250,7 @@ struct GemmList {
   }
   bool has_class(int c) const {
     for (const GemmGroup &g : groups)
-      if (cls_of(g.N) == c) return true;
+      if (cls(g.N) == c) return true;
     return false;
   }
   void apply_env_shapes() {
@@ -250,7 +265,7 @@ struct GemmList {
       if (end == std::string::npos) end = spec.size();
       int f, c, mb, nb, lb = 0;
       const int got = sscanf(spec.substr(pos, end - pos).c_str(), "%d.%d=%d.%d.%d", &f, &c, &mb, &nb, &lb);
-      if (got >= 4 && f == fam && c >= 0 && c < NCLS && form_ok(fam, c, GemmShape{mb, nb, got >= 5 ? lb : 0}))
+      if (got >= 4 && f == id() && c >= 0 && c < NCLS && form_ok(id(), c, GemmShape{mb, nb, got >= 5 ? lb : 0}))
         shape[c] = GemmShape{mb, nb, got >= 5 ? lb : 0};
       pos = end + 1;
     }
@@ -283,7 +298,7 @@ struct GemmList {
     maxmn = 0;
     for (size_t gi = 0; gi < groups.size(); ++gi) {
       GemmGroup &g = groups[gi];
-      const int c = cls_of(g.N);
+      const int c = cls(g.N);
       const int nsp = nosplit ? 1 : split_of(g);
       int ksz = (g.K + nsp - 1) / nsp;
       ksz = (ksz + 15) / 16 * 16;
@@ -625,6 +640,7 @@ struct Batch {
   SolveDesc *d_descs = nullptr;  // device array
   std::vector<SolveDesc> h_descs;
   GemmList g_x, g_pp, g_hh, g_i4, g_rt, g_pp32;
+  GemmList g_xb, g_hhb, g_i4b, g_rtb;   // the same groups for the BF16x3 kernel
   bool has_f32 = true;   // every solve has the FP32 copy of its pppp blocks
   int64_t maxT = 0, maxVhh = 0, maxRows = 0;
   int maxA = 0, maxNp = 0;
@@ -696,11 +712,19 @@ std::string list_signature(const GemmList &L, int kind) {
     mix((uint64_t)g.N);
     mix((uint64_t)g.K);
   }
-  return fmt("%d.%d.%zu.%016llx", L.fam, kind, L.groups.size(), (unsigned long long)h);
+  return fmt("%d.%d.%zu.%016llx", L.id(), kind, L.groups.size(), (unsigned long long)h);
 }
 
 std::vector<GemmShape> shape_candidates(int fam, int c) {
   std::vector<GemmShape> v;
+  if (fam >= 10) {   // BF16x3 lists: up to the widest column count of the class
+    if (c == 5) return {{0, 4}};
+    const int w = c == 0 ? 1 : c == 1 ? 2 : c == 2 ? 4 : c == 4 ? 6 : 8;
+    for (int nb = 1; nb <= w; ++nb)
+      if (bf3_shape_ok(1, nb)) v.push_back({1, nb});
+    for (int nb = 1; nb <= std::min(4, w); ++nb) v.push_back({2, nb});
+    return v;
+  }
   if (c == 5) return fam == 1 ? std::vector<GemmShape>{{1, 1, 3}} : std::vector<GemmShape>{{0, 4}};
   if (fam == 1) {   // the LDS-pipelined forms first: the tuner keeps the first of equal times
     v.push_back({1, 1, 3});
@@ -774,12 +798,12 @@ void autotune(GemmList &L, int kind, const SolveDesc *d_descs, hipStream_t st) {
   hipEvent_t e0, e1;
   HIPCHK(hipEventCreate(&e0));
   HIPCHK(hipEventCreate(&e1));
-  std::string rep = fmt("fam%d/kind%d:", L.fam, kind);
+  std::string rep = fmt("fam%d/kind%d:", L.id(), kind);
   for (int c = 0; c < GemmList::NCLS; ++c) {
-    if (!L.has_class(c) || shape_candidates(L.fam, c).size() < 2) continue;
+    if (!L.has_class(c) || shape_candidates(L.id(), c).size() < 2) continue;
     float best = 1e30f;
     GemmShape best_shape = L.shape[c];
-    for (const GemmShape cand : shape_candidates(L.fam, c)) {
+    for (const GemmShape cand : shape_candidates(L.id(), c)) {
       GemmList T = L;   // copy of the groups; tiles for this candidate
       T.shape[c] = cand;
       std::vector<void *> tmp;
@@ -857,12 +881,25 @@ std::unique_ptr<Batch> make_batch(const std::vector<Solve *> &ss, hipStream_t st
     B->g_pp32.groups[g].A = S.desc.Vpppp32 + S.off_vpp32[c];
     B->g_pp32.groups[g].lda = (S.plan.npp[c] + 1) & ~1;
   }
-  const int fams[6] = {0, 1, 2, 3, 4, 1};
-  GemmList *lists[6] = {&B->g_x, &B->g_pp, &B->g_hh, &B->g_i4, &B->g_rt, &B->g_pp32};
+  // BF16x3 lists: the groups of the X, hh and ring families, tiled for the
+  // BF16x3 kernel (up to 128 columns, so a ring channel's Vt rows are read
+  // once rather than once per 32 columns) and ordered channel-per-XCD
+  B->g_xb = B->g_x;
+  B->g_hhb = B->g_hh;
+  B->g_i4b = B->g_i4;
+  B->g_rtb = B->g_rt;
+  const int fams[10] = {0, 1, 2, 3, 4, 1, 0, 2, 3, 4};
+  GemmList *lists[10] = {&B->g_x, &B->g_pp, &B->g_hh, &B->g_i4, &B->g_rt, &B->g_pp32,
+                         &B->g_xb, &B->g_hhb, &B->g_i4b, &B->g_rtb};
   B->d_descs = dupload(B->h_descs, B->owned);
-  for (int k = 0; k < 6; ++k) {
+  for (int k = 0; k < 10; ++k) {
     lists[k]->fam = fams[k];
-    lists[k]->xcd = fams[k] == 1;
+    lists[k]->bf3 = k >= 6;
+    lists[k]->xcd = fams[k] == 1 || k >= 6;
+    if (k >= 6) {
+      const GemmShape def[GemmList::NCLS] = {{1, 1}, {1, 2}, {1, 4}, {1, 8}, {1, 6}, {0, 4}};
+      for (int c = 0; c < GemmList::NCLS; ++c) lists[k]->shape[c] = def[c];
+    }
     if (fams[k] == 1) {
       // the pp ladder on 64 x 16 tiles of the LDS-pipelined kernel (four
       // waves, two workgroups per CU) for every class, all in one launch:
@@ -895,12 +932,13 @@ std::unique_ptr<Batch> make_batch(const std::vector<Solve *> &ss, hipStream_t st
 // never contracts (a single-solve batch used only for its MBPT2 start or
 // energy, the FP32 ladder of an FP64 solve) are never tuned.
 void prepare_lists(Batch &B, int prec) {
-  const int kinds[6] = {0, 3, 0, 0, 0, 4};   // the kernel kind each list is tuned for
-  GemmList *lists[6] = {&B.g_x, &B.g_pp, &B.g_hh, &B.g_i4, &B.g_rt, &B.g_pp32};
-  for (int k = 0; k < 6; ++k) {
+  const int kinds[10] = {0, 3, 0, 0, 0, 4, 6, 6, 6, 6};   // the kernel kind each list is tuned for
+  GemmList *lists[10] = {&B.g_x, &B.g_pp, &B.g_hh, &B.g_i4, &B.g_rt, &B.g_pp32, &B.g_xb, &B.g_hhb, &B.g_i4b, &B.g_rtb};
+  for (int k = 0; k < 10; ++k) {
     GemmList &L = *lists[k];
     if (L.ready) continue;
     if (k == 5 && (prec == CCGPU_PREC_FP64 || !B.has_f32)) continue;
+    if (k >= 6 && prec != CCGPU_PREC_BF16X3) continue;
     autotune(L, kinds[k], B.d_descs, B.stream);
     L.build_tiles();
     L.upload(B.owned);
@@ -979,9 +1017,13 @@ void residual_launches(Batch &B, int prec, bool timing) {
   const int fam = low ? fp32_families() : 0;
   auto pf = [&](int bit) { return (fam >> bit) & 1 ? prec : CCGPU_PREC_FP64; };
   const int pp_prec = prec == CCGPU_PREC_V32 ? CCGPU_PREC_V32 : (fam >> 1) & 1 ? CCGPU_PREC_FP32 : CCGPU_PREC_FP64;
+  // the BF16x3 kernel runs its own lists (wider tiles)
+  auto list = [&](GemmList &L64, GemmList &Lbf, int bit) -> const GemmList & {
+    return pf(bit) == CCGPU_PREC_BF16X3 ? Lbf : L64;
+  };
   {
     Timer t(timing, st, "gemm_x_vhhpp_t");
-    run_gemm(B.g_x, B, pf(0), false, st);
+    run_gemm(list(B.g_x, B.g_xb, 0), B, pf(0), false, st);
   }
   {
     Timer t(timing, st, "i7_i9_i6");
@@ -1011,11 +1053,11 @@ void residual_launches(Batch &B, int prec, bool timing) {
     }
     {
       Timer t(timing, sr, "gemm_ring_i4");
-      run_gemm(B.g_i4, B, pf(3), false, sr);
+      run_gemm(list(B.g_i4, B.g_i4b, 3), B, pf(3), false, sr);
     }
     {
       Timer t(timing, sr, "gemm_ring_rt");
-      run_gemm(B.g_rt, B, pf(3), false, sr);
+      run_gemm(list(B.g_rt, B.g_rtb, 3), B, pf(3), false, sr);
     }
   }
   if (terms & 2) {
@@ -1025,7 +1067,7 @@ void residual_launches(Batch &B, int prec, bool timing) {
   }
   if (terms & 4) {
     Timer t(timing, st, "gemm_hh_ladder");
-    run_gemm(B.g_hh, B, pf(2), false, st);
+    run_gemm(list(B.g_hh, B.g_hhb, 2), B, pf(2), false, st);
   }
   if (fork) {
     HIPCHK(hipEventRecord(B.ev_join, sr));

@@ -1791,7 +1791,8 @@ void launch_diis_axpy(const SolveDesc *dd, int nsolve, int64_t maxT, const int *
 
 // Kernel kinds: 0 FP64, 1 FP32 (A double), 2 FP32 (A float), 3 FP64 pp ladder,
 // 4 FP32 pp ladder (A float), 5 pp ladder with A float and FP64 arithmetic,
-// 6 BF16x3 split products (A double; the thin N <= 4 class stays FP64).
+// 6 BF16x3 split products (A double; the thin N <= 4 class stays FP64), tile
+// shapes (1,1..6), (1,8), (2,1..4), (4,1).
 // Tile shapes (MB, NB): MB 16-row blocks per wave (4 waves stacked along M),
 // NB 16-column blocks: (1,1) (1,2) (1,3) (1,4) (1,5) (2,1) (2,2) (4,1); and
 // (0,4): the thin VALU kernel (16-row tiles, N <= 4).  lb = 1 selects the
@@ -2251,8 +2252,12 @@ void launch_gemm_shape(const GemmGroup *groups, const GemmTile *tiles, int ntile
       case 13: launch_shape_bf3<1, 3>(groups, tiles, ntiles, dd, gsolve, st); break;
       case 14: launch_shape_bf3<1, 4>(groups, tiles, ntiles, dd, gsolve, st); break;
       case 15: launch_shape_bf3<1, 5>(groups, tiles, ntiles, dd, gsolve, st); break;
+      case 16: launch_shape_bf3<1, 6>(groups, tiles, ntiles, dd, gsolve, st); break;
+      case 18: launch_shape_bf3<1, 8>(groups, tiles, ntiles, dd, gsolve, st); break;
       case 21: launch_shape_bf3<2, 1>(groups, tiles, ntiles, dd, gsolve, st); break;
       case 22: launch_shape_bf3<2, 2>(groups, tiles, ntiles, dd, gsolve, st); break;
+      case 23: launch_shape_bf3<2, 3>(groups, tiles, ntiles, dd, gsolve, st); break;
+      case 24: launch_shape_bf3<2, 4>(groups, tiles, ntiles, dd, gsolve, st); break;
       case 41: launch_shape_bf3<4, 1>(groups, tiles, ntiles, dd, gsolve, st); break;
       default: break;
     }
