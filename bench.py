@@ -149,6 +149,107 @@ def train_warm_start(ccnm, nsolves=28, seed=2024):
     return net, n, losses, [round(r, 4) for r in rhos]
 
 
+def warm_start_protocol(ccnm, ntrain=200, nheld=50, seed=5150):
+    """The learned start on held-out lattices, against the MBPT2 start.
+
+    Training: ntrain seeded solves of PNM A=66 |n_i|<=3 (chiral NNLO(450)),
+    rho ~ U[0.06, 0.22] fm^-3 at least 0.01 from every bench density, twists
+    ~ U[0, pi]^3, c1 and c3 ~ default x U[0.8, 1.2], each converged to
+    |dE| <= 1e-8 MeV.  Evaluation: nheld lattices at |n_i|<=4 drawn the same
+    way from another seed (none of them a training lattice: other sizes,
+    densities and twists), each solved from MBPT2 and from the learned start
+    with the reference's update under DIIS 8/8 and 6/6 and with the Anderson
+    update, FP64, to |dE| <= 1e-8 MeV.  Reports iterations, seconds and the
+    start error ||T0 - T||/||T|| of every held-out solve and their means."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+
+    def draw():
+        while True:
+            rho = rng.uniform(0.06, 0.22)
+            if min(abs(rho - d) for d in DENSITIES) >= 0.01:
+                break
+        ci = list(CI_DEFAULT)
+        f1, f3 = rng.uniform(0.8, 1.2, 2)
+        ci[0] *= f1
+        ci[2] *= f3
+        return rho, tuple(rng.uniform(0.0, 3.14159265358979, 3)), ci, (round(f1, 4), round(f3, 4))
+
+    net = ccnm.WarmStart(seed=13)
+    t = time.perf_counter()
+    n = 0
+    for k in range(ntrain):
+        rho, tw, ci, _ = draw()
+        s = lattice(ccnm, rho, 3, tw, ci)
+        s.setup_t_amplitudes()
+        s.ccd_iter(subspace=DEPTH, diis_step=DEPTH, tol=TOL, maxit=MAXIT, use_graph=1, accel=1, mix=1.0)
+        n = net.add_solve(s)
+        s.close()
+        if (k + 1) % 50 == 0:
+            print(f"# warm-start protocol: {k + 1} training solves", file=sys.stderr, flush=True)
+    solve_s = time.perf_counter() - t
+    t = time.perf_counter()
+    losses = [net.train(steps, batch=16384, lr=lr, seed=steps) for lr, steps in ((2e-3, 3000), (5e-4, 1500), (1e-4, 1000))]
+    train_s = time.perf_counter() - t
+    rng = np.random.default_rng(seed + 1)
+    rows = []
+    schedules = (("ref88", dict(subspace=8, diis_step=8)), ("ref66", dict(subspace=6, diis_step=6)),
+                 ("anderson8", dict(subspace=8, diis_step=8, accel=1, mix=1.0)))
+    for k in range(nheld):
+        rho, tw, ci, f = draw()
+        s = lattice(ccnm, rho, NMAX, tw, ci)
+        nt = s.dims["t2_unique"]
+        row = {"rho": round(rho, 4), "twist": [round(x, 4) for x in tw], "c1c3_factors": f}
+        for kw in (dict(), dict(accel=1, mix=1.0)):   # untimed: tile tuning and graph capture of this lattice
+            s.setup_t_amplitudes()
+            s.ccd_iter(subspace=DEPTH, diis_step=DEPTH, tol=0.0, maxit=2, use_graph=1, **kw)
+        conv = None
+        for name, kw in schedules:
+            out = []
+            for learned in (False, True):
+                t0 = time.perf_counter()
+                if learned:
+                    net.warm_start(s)
+                else:
+                    s.setup_t_amplitudes()
+                dt = time.perf_counter() - t0   # the start (inference or MBPT2), then the solve
+                start = s.debug_fetch("T", nt) if name == "ref88" else None
+                t0 = time.perf_counter()
+                r = s.ccd_iter(tol=TOL, maxit=MAXIT, use_graph=1, **kw)
+                dt += time.perf_counter() - t0
+                out.append((r["iterations"], dt, r["converged"], r["eccd"].real / A, start))
+                if name == "ref88" and not learned:
+                    conv = s.debug_fetch("T", nt)
+            row[f"iters_{name}"] = [out[0][0], out[1][0]]
+            row[f"s_{name}"] = [round(out[0][1], 5), round(out[1][1], 5)]
+            row[f"converged_{name}"] = [bool(out[0][2]), bool(out[1][2])]
+            row[f"e_per_n_diff_{name}"] = abs(out[0][3] - out[1][3])
+            if name == "ref88":
+                nc = np.linalg.norm(conv)
+                row["start_err"] = [round(float(np.linalg.norm(out[0][4] - conv) / nc), 4),
+                                    round(float(np.linalg.norm(out[1][4] - conv) / nc), 4)]
+        s.close()
+        rows.append(row)
+        if (k + 1) % 10 == 0:
+            print(f"# warm-start protocol: {k + 1} held-out lattices", file=sys.stderr, flush=True)
+    summ = {}
+    for name, _ in schedules:
+        it = np.array([r[f"iters_{name}"] for r in rows], float)
+        sec = np.array([r[f"s_{name}"] for r in rows], float)
+        summ[name] = {"mean_iterations_mbpt2": round(float(it[:, 0].mean()), 3),
+                      "mean_iterations_learned": round(float(it[:, 1].mean()), 3),
+                      "mean_s_mbpt2": round(float(sec[:, 0].mean()), 5),
+                      "mean_s_learned": round(float(sec[:, 1].mean()), 5),
+                      "fewer_iterations": int((it[:, 1] < it[:, 0]).sum()),
+                      "more_iterations": int((it[:, 1] > it[:, 0]).sum()),
+                      "max_e_per_n_diff": max(r[f"e_per_n_diff_{name}"] for r in rows)}
+    se = np.array([r["start_err"] for r in rows])
+    summ["mean_start_err"] = [round(float(se[:, 0].mean()), 4), round(float(se[:, 1].mean()), 4)]
+    return {"train": {"solves": ntrain, "amplitudes": n, "solve_s": round(solve_s, 2), "train_s": round(train_s, 2),
+                      "losses": losses},
+            "held_out": nheld, "summary": summ, "lattices": rows}
+
+
 def twist_leg(ccnm, net, plan, ntwist, nmax, seed):
     """Twist averaging at rho=0.16: ntwist twists theta ~ U[0, pi]^3 (seeded),
     each its own lattice with its own hole set and channel sizes.  Plain:
@@ -374,6 +475,10 @@ def main():
     ap.add_argument("--profile-triples", action="store_true",
                     help="short run for counter collection: the (T) workload (SNM A=132 |n_i|<=3, chiral NNLO + "
                          "normal-ordered 3NF), its tables and one (T) kernel; no JSON line")
+    ap.add_argument("--warm-start-study", action="store_true",
+                    help="the learned start's protocol instead of the bench: 200 training solves at |n_i|<=3, "
+                         "50 held-out lattices at |n_i|<=4, iterations and time against MBPT2 (DIIS 8/8, 6/6, "
+                         "Anderson); prints its own JSON line")
     ap.add_argument("--profile-run", action="store_true",
                     help="short run for counter collection: 6 FP64 iterations of the rho=0.16 lattice, "
                          "then 6 FP32 and 6 BF16x3 iterations of the four-density batch; no JSON line")
@@ -384,6 +489,9 @@ def main():
         sys.exit(2)
 
     ccnm = load_ccnm()
+    if args.warm_start_study:
+        print(json.dumps({"warm_start_protocol": warm_start_protocol(ccnm)}))
+        return
     if args.profile_triples:
         t3 = triples_rate(ccnm)
         print(f"# profile (T): {t3['elements']:.3e} elements in {t3['seconds']:.3f} s", file=sys.stderr, flush=True)
