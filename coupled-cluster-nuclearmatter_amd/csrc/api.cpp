@@ -53,6 +53,11 @@ void launch_update(const SolveDesc *dd, int nsolve, int64_t maxT, const int *slo
 void launch_gram(const SolveDesc *dd, int nsolve, const int *slots, int nstep, hipStream_t st);
 void launch_aa_update(const SolveDesc *dd, int nsolve, const int *slots, double beta, hipStream_t st);
 void launch_scatter_values(double2 *dst, const int64_t *idx, const double2 *val, int64_t n, hipStream_t st);
+void launch_fill_vpppp(const SolveDesc &d, double volume, const ChiralDev *c, int nch, int max_npp, hipStream_t st);
+void launch_share_gather(const SolveDesc *dd, int nsolve, int64_t maxT, const uint32_t *const *maps, double2 *Tu,
+                         hipStream_t st);
+void launch_share_scatter(const SolveDesc *dd, int nsolve, int64_t maxT, const uint32_t *const *maps,
+                          const double2 *Ru, hipStream_t st);
 void launch_aa_combine(const SolveDesc *dd, int nsolve, int64_t maxT, const int *slots, const double *coef,
                        hipStream_t st);
 void launch_diis_axpy(const SolveDesc *dd, int nsolve, int64_t maxT, const int *slots, const double2 *coef,
@@ -206,7 +211,7 @@ int tile_cols(GemmShape s) { return s.mb == 0 ? 4 : 16 * s.nb; }
 
 struct GemmList {
   static constexpr int NCLS = 6;
-  int fam = 0;   // 0 X = V_hhpp T, 1 pp ladder, 2 hh ladder, 3 ring I4, 4 ring R~
+  int fam = 0;   // 0 X = V_hhpp T, 1 pp ladder, 2 hh ladder, 3 ring I4, 4 ring R~, 5 shared pp ladder
   bool bf3 = false;   // the list of the BF16x3 kernel: its own classes, shapes and tuning
   int id() const { return bf3 ? 10 + fam : fam; }
   std::vector<GemmGroup> groups;
@@ -368,6 +373,7 @@ struct Solve {
   hipStream_t stream = nullptr;
   bool have_v = false, have_fock = false, derived = false, have_t = false;
   bool native_v = false;   // V from the built-in generator (needed by (T))
+  uint64_t vgen = 0;       // counts changes of V (a shared pp ladder refills its union blocks)
   int vkind = -1;          // CCGPU_INT_* of the generated V
   ChiralModel chiral = chiral_model(3, 450.0, nullptr);   // constants of CCGPU_INT_CHIRAL
   ChiralDev chiral_dev{};  // its device tables (set by the generator)
@@ -641,6 +647,29 @@ struct Batch {
   std::vector<SolveDesc> h_descs;
   GemmList g_x, g_pp, g_hh, g_i4, g_rt, g_pp32;
   GemmList g_xb, g_hhb, g_i4b, g_rtb;   // the same groups for the BF16x3 kernel
+  // Shared pp ladder of a batch of twists of one lattice and interaction: the
+  // NN element <ab||cd> depends on the lattice labels of a, b, c, d only (the
+  // twist cancels in every momentum difference), so one V_pppp over the union
+  // of the twists' particle pairs serves all of them.  Per (K, Tz) channel one
+  // GEMM R_u = V_u T_u whose columns are the hole pairs of every twist side
+  // by side; T_u holds each twist's amplitudes at its own pairs (zero where a
+  // twist owns no pair), R_u goes back into each twist's R.  V_pppp is read
+  // once per iteration instead of once per twist, and narrow per-twist
+  // GEMMs become one wide GEMM per channel.
+  struct Shared {
+    bool on = false;
+    std::vector<uint64_t> vgen;   // V generations the union blocks were filled from
+    int nu = 0, max_npp = 0;
+    std::vector<int> npp, ncol, pa, pb;   // union pairs in solve 0's orbital numbering
+    std::vector<int64_t> off_pp, off_v, off_t;
+    int64_t nV = 0, nTu = 0;
+    double2 *V = nullptr, *Tu = nullptr, *Ru = nullptr;
+    float2 *V32 = nullptr;
+    int *d_npp = nullptr, *d_pa = nullptr, *d_pb = nullptr;
+    int64_t *d_off_pp = nullptr, *d_off_v = nullptr;
+    uint32_t **d_maps = nullptr;   // per solve: unique amplitude -> 2 * position + (sign < 0)
+    GemmList g64, gbf;             // FP64 GEMMs on V_u, BF16x3 GEMMs on its FP32 copy
+  } share;
   bool has_f32 = true;   // every solve has the FP32 copy of its pppp blocks
   int64_t maxT = 0, maxVhh = 0, maxRows = 0;
   int maxA = 0, maxNp = 0;
@@ -842,6 +871,11 @@ void autotune(GemmList &L, int kind, const SolveDesc *d_descs, hipStream_t st) {
   g_tune_report += rep;
 }
 
+bool share_enabled();
+bool share_eligible(const std::vector<Solve *> &ss);
+void build_shared(Batch &B);
+void check_launch();
+
 std::unique_ptr<Batch> make_batch(const std::vector<Solve *> &ss, hipStream_t st) {
   auto B = std::make_unique<Batch>();
   B->solves = ss;
@@ -924,7 +958,199 @@ std::unique_ptr<Batch> make_batch(const std::vector<Solve *> &ss, hipStream_t st
   B->d_aaslots = dalloc<int>((size_t)n * 16, B->owned);
   B->d_aacoef = dalloc<double>((size_t)n * 16, B->owned);
   B->status.assign(n, 0);
+  if (share_enabled() && share_eligible(ss)) build_shared(*B);
   return B;
+}
+
+// ---------------------------------------------------------------------------
+// Shared pp ladder (see Batch::Shared).  A batch qualifies when it holds two
+// or more solves of one lattice (A, matter, density, nmax; twists free) whose
+// V comes from the same built-in interaction with no three-nucleon part in
+// V_pppp.  CCGPU_SHARE_V=0 keeps one ladder per solve.
+// ---------------------------------------------------------------------------
+bool share_enabled() {
+  const char *e = getenv("CCGPU_SHARE_V");
+  return !(e && atoi(e) == 0);
+}
+
+bool same_chiral(const ChiralModel &a, const ChiralModel &b) {
+  auto eq = [](const double *x, const double *y, int n) {
+    for (int i = 0; i < n; ++i)
+      if (x[i] != y[i]) return false;
+    return true;
+  };
+  return a.order == b.order && a.cutoff == b.cutoff && a.nexp == b.nexp && a.c1 == b.c1 && a.c2 == b.c2 &&
+         a.c3 == b.c3 && a.c4 == b.c4 && eq(a.cs, b.cs, 3) && eq(a.ct, b.ct, 3) && eq(a.cnlo, b.cnlo, 7) &&
+         eq(a.cg0, b.cg0, 2) && eq(a.cg1, b.cg1, 2);
+}
+
+bool share_eligible(const std::vector<Solve *> &ss) {
+  if (ss.size() < 2) return false;
+  const Solve &a = *ss[0];
+  const Orbitals &oa = a.plan.orb;
+  for (const Solve *p : ss) {
+    const Solve &b = *p;
+    const Orbitals &ob = b.plan.orb;
+    if (!b.native_v || b.vkind != a.vkind || b.tnf >= 2 || a.tnf >= 2) return false;
+    if (ob.A != oa.A || ob.nsp != oa.nsp || ob.nmax != oa.nmax || ob.rho != oa.rho || ob.kf != oa.kf ||
+        ob.volume != oa.volume)
+      return false;
+    if (b.vkind == CCGPU_INT_CHIRAL && !same_chiral(a.chiral, b.chiral)) return false;
+  }
+  return true;
+}
+
+// Union channels, pairs and position maps; the lists of the wide GEMMs.
+void build_shared(Batch &B) {
+  Batch::Shared &X = B.share;
+  const std::vector<Solve *> &ss = B.solves;
+  const int n = (int)ss.size();
+  const Orbitals &o0 = ss[0]->plan.orb;
+  std::map<std::array<int, 5>, int> lab0;
+  for (int x = 0; x < o0.nsp; ++x) lab0[{o0.nx[x], o0.ny[x], o0.nz[x], o0.sz[x], o0.tz[x]}] = x;
+  std::vector<std::vector<int>> to0(n);
+  for (int s = 0; s < n; ++s) {
+    const Orbitals &o = ss[s]->plan.orb;
+    to0[s].resize(o.nsp);
+    for (int x = 0; x < o.nsp; ++x) {
+      auto it = lab0.find({o.nx[x], o.ny[x], o.nz[x], o.sz[x], o.tz[x]});
+      if (it == lab0.end()) throw Error("shared ladder: orbital labels differ between the solves");
+      to0[s][x] = it->second;
+    }
+  }
+  // union channels in key order, their pairs (x < y in solve 0's numbering)
+  std::map<std::array<int, 4>, int> uch;
+  std::vector<std::vector<int>> chof(n);   // solve channel -> union channel
+  for (int s = 0; s < n; ++s) {
+    const Plan &P = ss[s]->plan;
+    chof[s].resize(P.nch);
+    for (int c = 0; c < P.nch; ++c) uch.emplace(std::array<int, 4>{P.key_tz[c], P.key_kx[c], P.key_ky[c], P.key_kz[c]}, 0);
+  }
+  int u = 0;
+  for (auto &kv : uch) kv.second = u++;
+  X.nu = u;
+  std::vector<std::vector<uint64_t>> pairs(X.nu);
+  const uint64_t W = (uint64_t)o0.nsp;
+  for (int s = 0; s < n; ++s) {
+    const Plan &P = ss[s]->plan;
+    for (int c = 0; c < P.nch; ++c) {
+      const int uc = uch.at({P.key_tz[c], P.key_kx[c], P.key_ky[c], P.key_kz[c]});
+      chof[s][c] = uc;
+      for (int r = 0; r < P.npp[c]; ++r) {
+        int x = to0[s][P.pp_a[P.off_pp[c] + r]], y = to0[s][P.pp_b[P.off_pp[c] + r]];
+        if (x > y) std::swap(x, y);
+        pairs[uc].push_back((uint64_t)x * W + y);
+      }
+    }
+  }
+  X.npp.assign(X.nu, 0);
+  X.ncol.assign(X.nu, 0);
+  X.off_pp = {0};
+  X.off_v = {0};
+  X.off_t = {0};
+  X.pa.clear();
+  X.pb.clear();
+  std::vector<std::unordered_map<uint64_t, int>> row(X.nu);
+  std::vector<std::vector<int>> col(X.nu, std::vector<int>(n, -1));
+  for (int s = 0; s < n; ++s) {
+    const Plan &P = ss[s]->plan;
+    for (int c = 0; c < P.nch; ++c) {
+      const int uc = chof[s][c];
+      col[uc][s] = X.ncol[uc];
+      X.ncol[uc] += P.nhh[c];
+    }
+  }
+  X.max_npp = 0;
+  for (int uc = 0; uc < X.nu; ++uc) {
+    auto &pl = pairs[uc];
+    std::sort(pl.begin(), pl.end());
+    pl.erase(std::unique(pl.begin(), pl.end()), pl.end());
+    X.npp[uc] = (int)pl.size();
+    X.max_npp = std::max(X.max_npp, X.npp[uc]);
+    for (size_t r = 0; r < pl.size(); ++r) {
+      row[uc][pl[r]] = (int)r;
+      X.pa.push_back((int)(pl[r] / W));
+      X.pb.push_back((int)(pl[r] % W));
+    }
+    X.off_pp.push_back(X.off_pp.back() + X.npp[uc]);
+    X.off_v.push_back(X.off_v.back() + (int64_t)X.npp[uc] * X.npp[uc]);
+    X.off_t.push_back(X.off_t.back() + (int64_t)X.npp[uc] * X.ncol[uc]);
+  }
+  X.nV = X.off_v.back();
+  X.nTu = X.off_t.back();
+  if (X.nTu >= (int64_t)1 << 31) throw Error("shared ladder: union blocks above 2^31 elements");
+  // position maps
+  std::vector<uint32_t *> maps(n);
+  for (int s = 0; s < n; ++s) {
+    const Plan &P = ss[s]->plan;
+    std::vector<uint32_t> m(P.nT);
+    for (int c = 0; c < P.nch; ++c) {
+      const int uc = chof[s][c], nh = P.nhh[c];
+      for (int r = 0; r < P.npp[c]; ++r) {
+        const int x = to0[s][P.pp_a[P.off_pp[c] + r]], y = to0[s][P.pp_b[P.off_pp[c] + r]];
+        const int ur = row[uc].at((uint64_t)std::min(x, y) * W + std::max(x, y));
+        const int64_t base = X.off_t[uc] + (int64_t)ur * X.ncol[uc] + col[uc][s];
+        for (int j = 0; j < nh; ++j) m[P.off_t[c] + (int64_t)r * nh + j] = (uint32_t)(2 * (base + j) + (x > y));
+      }
+    }
+    maps[s] = dupload(m, B.owned);
+  }
+  X.d_maps = dupload(maps, B.owned);
+  X.d_npp = dupload(X.npp, B.owned);
+  X.d_pa = dupload(X.pa, B.owned);
+  X.d_pb = dupload(X.pb, B.owned);
+  X.d_off_pp = dupload(X.off_pp, B.owned);
+  X.d_off_v = dupload(X.off_v, B.owned);
+  X.V = dalloc<double2>(std::max<int64_t>(X.nV, 1), B.owned);
+  X.V32 = dalloc<float2>(std::max<int64_t>(X.nV, 1), B.owned);
+  X.Tu = dalloc<double2>(std::max<int64_t>(X.nTu, 1), B.owned);
+  X.Ru = dalloc<double2>(std::max<int64_t>(X.nTu, 1), B.owned);
+  HIPCHK(hipMemset(X.Tu, 0, std::max<int64_t>(X.nTu, 1) * sizeof(double2)));   // positions no solve owns
+  // the wide GEMMs: R_u = V_u T_u per union channel
+  X.g64 = GemmList();
+  for (int uc = 0; uc < X.nu; ++uc) {
+    const int m_ = X.npp[uc], nc = X.ncol[uc];
+    X.g64.add(mkgroup(X.V + X.off_v[uc], X.Tu + X.off_t[uc], X.Ru + X.off_t[uc], nullptr, m_, nc, m_, m_, nc, nc, 0,
+                      1.0, 0.0),
+              0);
+  }
+  X.gbf = X.g64;
+  for (size_t g = 0; g < X.gbf.groups.size(); ++g)
+    X.gbf.groups[g].A = X.V32 + (((const double2 *)X.g64.groups[g].A) - X.V);
+  X.g64.fam = X.gbf.fam = 5;
+  X.gbf.bf3 = true;
+  X.g64.xcd = X.gbf.xcd = true;
+  const GemmShape def[GemmList::NCLS] = {{1, 1}, {1, 2}, {1, 4}, {1, 8}, {1, 6}, {0, 4}};
+  for (int c = 0; c < GemmList::NCLS; ++c) X.gbf.shape[c] = def[c];
+  X.g64.apply_env_shapes();
+  X.gbf.apply_env_shapes();
+  X.vgen.assign(n, ~0ull);
+  X.on = true;
+}
+
+// (Re)fill the union V_pppp and its FP32 copy when a solve's V changed.
+void refresh_shared(Batch &B) {
+  Batch::Shared &X = B.share;
+  if (!X.on) return;
+  bool stale = false;
+  for (size_t s = 0; s < B.solves.size(); ++s) stale = stale || X.vgen[s] != B.solves[s]->vgen;
+  if (!stale) return;
+  if (!share_eligible(B.solves)) throw Error("shared pp ladder: the solves no longer share one interaction");
+  Solve &S0 = *B.solves[0];
+  SolveDesc dv = S0.desc;
+  dv.npp = X.d_npp;
+  dv.off_pp = X.d_off_pp;
+  dv.pp_a = X.d_pa;
+  dv.pp_b = X.d_pb;
+  dv.off_vpp = X.d_off_v;
+  dv.Vpppp = X.V;
+  launch_fill_vpppp(dv, S0.plan.orb.volume, S0.vkind == CCGPU_INT_CHIRAL ? &S0.chiral_dev : nullptr, X.nu,
+                    X.max_npp, B.stream);
+  check_launch();
+  launch_to_f32(X.V, X.V32, X.nV, B.stream);
+  check_launch();
+  HIPCHK(hipStreamSynchronize(B.stream));
+  for (size_t s = 0; s < B.solves.size(); ++s) X.vgen[s] = B.solves[s]->vgen;
 }
 
 // Shapes, tiles and device copies of the work lists a residual in precision
@@ -943,6 +1169,19 @@ void prepare_lists(Batch &B, int prec) {
     L.build_tiles();
     L.upload(B.owned);
     L.ready = true;
+  }
+  if (B.share.on) {
+    refresh_shared(B);
+    GemmList *sl[2] = {&B.share.g64, &B.share.gbf};
+    const int sk[2] = {0, 7};
+    for (int k = 0; k < 2; ++k) {
+      GemmList &L = *sl[k];
+      if (L.ready || (k == 1 && prec == CCGPU_PREC_FP64)) continue;
+      autotune(L, sk[k], nullptr, B.stream);
+      L.build_tiles();
+      L.upload(B.owned);
+      L.ready = true;
+    }
   }
 }
 
@@ -972,11 +1211,16 @@ bool known_precision(int p) {
   return p == CCGPU_PREC_FP64 || p == CCGPU_PREC_FP32 || p == CCGPU_PREC_V32 || p == CCGPU_PREC_BF16X3;
 }
 
-void run_gemm(const GemmList &L, const Batch &B, int prec, bool a_f32, hipStream_t st, bool pp = false) {
+void run_gemm(const GemmList &L, const Batch &B, int prec, bool a_f32, hipStream_t st, bool pp = false,
+              int kind_override = -1) {
+  // the shared pp ladder's groups span every solve of the batch: no per-solve skip
+  const SolveDesc *dd = L.fam == 5 ? nullptr : B.d_descs;
   if (gemm_variant_override() == 3) {
-    launch_gemm_valu(L.d_groups, L.d_tiles64, (int)L.tiles64.size(), B.d_descs, L.d_gsolve, st);
+    // the VALU check reads FP64-stored operands only
+    if (a_f32) throw Error("CCGPU_GEMM=valu checks contractions on FP64-stored operands only");
+    launch_gemm_valu(L.d_groups, L.d_tiles64, (int)L.tiles64.size(), dd, L.d_gsolve, st);
   } else {
-    const int kind = gemm_kind(prec, a_f32, pp);
+    const int kind = kind_override >= 0 ? kind_override : gemm_kind(prec, a_f32, pp);
     // CCGPU_TIMING=2 also times every tile class of every family
     const bool detail = getenv("CCGPU_TIMING") && atoi(getenv("CCGPU_TIMING")) == 2;
     for (int c = GemmList::NCLS - 1; c >= 0; --c) {
@@ -985,12 +1229,12 @@ void run_gemm(const GemmList &L, const Batch &B, int prec, bool a_f32, hipStream
       snprintf(nm, sizeof nm, "fam%d.k%d.c%d.%dx%d%s", L.fam, kind, c, L.shape[c].mb, L.shape[c].nb,
                L.shape[c].lb == 4 ? "W" : L.shape[c].lb == 3 ? "P" : L.shape[c].lb == 2 ? "A" : L.shape[c].lb ? "L" : "");
       Timer t(detail, st, nm);
-      launch_gemm_shape(L.d_groups, L.d_tiles[c], (int)L.tiles[c].size(), B.d_descs, L.d_gsolve, kind, L.shape[c].mb,
+      launch_gemm_shape(L.d_groups, L.d_tiles[c], (int)L.tiles[c].size(), dd, L.d_gsolve, kind, L.shape[c].mb,
                         L.shape[c].nb, L.shape[c].lb, st);
     }
     // split groups: the MFMA and thin kernels wrote partials; the VALU check
     // above runs every group over its whole k range and writes C itself
-    launch_splitk_reduce(L.d_groups, L.d_split, (int)L.split.size(), L.maxmn, B.d_descs, L.d_gsolve, st);
+    launch_splitk_reduce(L.d_groups, L.d_split, (int)L.split.size(), L.maxmn, dd, L.d_gsolve, st);
   }
 }
 
@@ -1060,7 +1304,23 @@ void residual_launches(Batch &B, int prec, bool timing) {
       run_gemm(list(B.g_rt, B.g_rtb, 3), B, pf(3), false, sr);
     }
   }
-  if (terms & 2) {
+  if ((terms & 2) && B.share.on) {
+    // shared pp ladder: FP64 GEMMs on V_u, or BF16x3 on its FP32 copy in every
+    // low-precision mode
+    {
+      Timer t(timing, st, "pp_share_gather");
+      launch_share_gather(B.d_descs, n, B.maxT, B.share.d_maps, B.share.Tu, st);
+    }
+    {
+      Timer t(timing, st, "gemm_pp_ladder");
+      if (pp_prec == CCGPU_PREC_FP64) run_gemm(B.share.g64, B, CCGPU_PREC_FP64, false, st, false, 0);
+      else run_gemm(B.share.gbf, B, CCGPU_PREC_BF16X3, true, st, false, 7);
+    }
+    {
+      Timer t(timing, st, "pp_share_scatter");
+      launch_share_scatter(B.d_descs, n, B.maxT, B.share.d_maps, B.share.Ru, st);
+    }
+  } else if (terms & 2) {
     Timer t(timing, st, "gemm_pp_ladder");
     if (pp_prec != CCGPU_PREC_FP64) run_gemm(B.g_pp32, B, pp_prec, true, st, true);
     else run_gemm(B.g_pp, B, pp_prec, false, st, true);
@@ -1914,6 +2174,7 @@ int ccgpu_generate_interaction(int handle, int kind) {
     HIPCHK(hipStreamSynchronize(S.stream));
     S.have_v = S.have_fock = S.derived = true;
     S.native_v = true;
+    ++S.vgen;
     S.vkind = kind;
     S.have_t3 = S.have_w3 = S.have_ccdt1 = false;
     // V changed: an FP32 copy and cached work lists of this handle are stale
@@ -2067,6 +2328,7 @@ int ccgpu_set_block(int handle, int kind, int nbra, const int *bra_p, const int 
     }
     S.have_v = true;
     S.native_v = false;
+    ++S.vgen;
     S.vkind = -1;
     S.derived = false;
   });

@@ -351,6 +351,41 @@ __device__ __forceinline__ void unique_labels(const SolveDesc &d, int64_t u, int
 
 // MBPT2 start t(ab,ij) = conj(<ij||ab>)/(f_ii+f_jj-f_aa-f_bb), zero denominators
 // skipped (setup_t_amplitudes, ccm_iter.f90:2351-2367).
+// Shared pp ladder of a batch of twists of one lattice (one V_pppp over the
+// union of their particle pairs, per (K, Tz) channel one GEMM whose columns
+// are the hole pairs of every twist side by side).  maps[s][u] = 2 p + (sign
+// < 0): the position p of solve s's unique amplitude u in the union block
+// (row: the pair's union row, column: the solve's column range), with the
+// orientation sign of the pair there.  Positions no solve owns stay zero.
+__global__ void k_share_gather(const SolveDesc *descs, const uint32_t *const *maps, double2 *Tu) {
+  const SolveDesc &d = descs[blockIdx.y];
+  const uint32_t *m = maps[blockIdx.y];
+  for (int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; u < d.nT; u += (int64_t)gridDim.x * blockDim.x) {
+    const uint32_t w = m[u];
+    const double2 t = d.T[u];
+    Tu[w >> 1] = (w & 1) ? make_double2(-t.x, -t.y) : t;
+  }
+}
+// R += the union product at each amplitude's position (active solves only)
+__global__ void k_share_scatter(const SolveDesc *descs, const uint32_t *const *maps, const double2 *Ru) {
+  const SolveDesc &d = descs[blockIdx.y];
+  if (!d.active) return;
+  const uint32_t *m = maps[blockIdx.y];
+  for (int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; u < d.nT; u += (int64_t)gridDim.x * blockDim.x) {
+    const uint32_t w = m[u];
+    const double2 r = Ru[w >> 1];
+    double2 x = d.R[u];
+    if (w & 1) {
+      x.x -= r.x;
+      x.y -= r.y;
+    } else {
+      x.x += r.x;
+      x.y += r.y;
+    }
+    d.R[u] = x;
+  }
+}
+
 __global__ void k_mbpt2(const SolveDesc *descs) {
   const SolveDesc &d = descs[blockIdx.y];
   for (int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; u < d.nT; u += (int64_t)gridDim.x * blockDim.x) {
@@ -1645,6 +1680,21 @@ void launch_fill_blocks_chiral(const SolveDesc &d, const ChiralDev &c, int nch, 
   fill_blocks(d, v, nch, nphc, max_npp, max_nhh, max_nhp, st);
 }
 
+// V_pppp blocks only (the union blocks of a shared pp ladder)
+void launch_fill_vpppp(const SolveDesc &d, double volume, const ChiralDev *c, int nch, int max_npp, hipStream_t st) {
+  const unsigned gx = (unsigned)std::max<int64_t>(1, std::min<int64_t>(((int64_t)max_npp * max_npp + 255) / 256, 8192));
+  if (c) {
+    ChiralV v;
+    v.c = *c;
+    v.nx = d.nx; v.ny = d.ny; v.nz = d.nz; v.sz = d.sz; v.tz = d.tz;
+    hipLaunchKernelGGL(k_fill_vpppp<ChiralV>, dim3(gx, nch), dim3(256), 0, st, d, v);
+  } else {
+    MinnesotaV v;
+    v.o.kx = d.kx; v.o.ky = d.ky; v.o.kz = d.kz; v.o.sz = d.sz; v.o.tz = d.tz; v.o.volume = volume;
+    hipLaunchKernelGGL(k_fill_vpppp<MinnesotaV>, dim3(gx, nch), dim3(256), 0, st, d, v);
+  }
+}
+
 void launch_elements(const SolveDesc &d, double volume, const ChiralDev *c, const int4 *pqrs, int n, double2 *out,
                      hipStream_t st) {
   const dim3 g((unsigned)std::max(1, (n + 255) / 256));
@@ -1770,6 +1820,14 @@ void launch_gram(const SolveDesc *dd, int nsolve, const int *slots, int nstep, h
   int nv = ne * (ne + 1) / 2;
   hipLaunchKernelGGL(k_reduce_final, dim3(1, nsolve), dim3(256), 0, st, dd, kRedBlocks, 36, nv, 4);
 }
+void launch_share_gather(const SolveDesc *dd, int nsolve, int64_t maxT, const uint32_t *const *maps, double2 *Tu,
+                         hipStream_t st) {
+  hipLaunchKernelGGL(k_share_gather, egrid(maxT, nsolve), dim3(256), 0, st, dd, maps, Tu);
+}
+void launch_share_scatter(const SolveDesc *dd, int nsolve, int64_t maxT, const uint32_t *const *maps,
+                          const double2 *Ru, hipStream_t st) {
+  hipLaunchKernelGGL(k_share_scatter, egrid(maxT, nsolve), dim3(256), 0, st, dd, maps, Ru);
+}
 void launch_scatter_values(double2 *dst, const int64_t *idx, const double2 *val, int64_t n, hipStream_t st) {
   if (n <= 0) return;
   hipLaunchKernelGGL(k_scatter_values, egrid(n, 1), dim3(256), 0, st, dst, idx, val, n);
@@ -1792,7 +1850,7 @@ void launch_diis_axpy(const SolveDesc *dd, int nsolve, int64_t maxT, const int *
 // Kernel kinds: 0 FP64, 1 FP32 (A double), 2 FP32 (A float), 3 FP64 pp ladder,
 // 4 FP32 pp ladder (A float), 5 pp ladder with A float and FP64 arithmetic,
 // 6 BF16x3 split products (A double; the thin N <= 4 class stays FP64), tile
-// shapes (1,1..6), (1,8), (2,1..4), (4,1).
+// shapes (1,1..6), (1,8), (2,1..4), (4,1); 7 the same with A float.
 // Tile shapes (MB, NB): MB 16-row blocks per wave (4 waves stacked along M),
 // NB 16-column blocks: (1,1) (1,2) (1,3) (1,4) (1,5) (2,1) (2,2) (4,1); and
 // (0,4): the thin VALU kernel (16-row tiles, N <= 4).  lb = 1 selects the
@@ -2238,27 +2296,30 @@ static void launch_shape(const GemmGroup *groups, const GemmTile *tiles, int nti
 // kind 6: BF16x3 split products (k_zgemm_bf3), one form for every lb
 template <int MB, int NB>
 static void launch_shape_bf3(const GemmGroup *groups, const GemmTile *tiles, int ntiles, const SolveDesc *dd,
-                             const int *gsolve, hipStream_t st) {
-  hipLaunchKernelGGL((k_zgemm_bf3<MB, NB, double2>), dim3(ntiles), dim3(256), 0, st, groups, tiles, dd, gsolve);
+                             const int *gsolve, int kind, hipStream_t st) {
+  if (kind == 7)
+    hipLaunchKernelGGL((k_zgemm_bf3<MB, NB, float2>), dim3(ntiles), dim3(256), 0, st, groups, tiles, dd, gsolve);
+  else
+    hipLaunchKernelGGL((k_zgemm_bf3<MB, NB, double2>), dim3(ntiles), dim3(256), 0, st, groups, tiles, dd, gsolve);
 }
 
 void launch_gemm_shape(const GemmGroup *groups, const GemmTile *tiles, int ntiles, const SolveDesc *dd,
                        const int *gsolve, int kind, int mb, int nb, int lb, hipStream_t st) {
   if (ntiles <= 0) return;
-  if (kind == 6 && mb > 0) {
+  if ((kind == 6 || kind == 7) && mb > 0) {
     switch (mb * 10 + nb) {
-      case 11: launch_shape_bf3<1, 1>(groups, tiles, ntiles, dd, gsolve, st); break;
-      case 12: launch_shape_bf3<1, 2>(groups, tiles, ntiles, dd, gsolve, st); break;
-      case 13: launch_shape_bf3<1, 3>(groups, tiles, ntiles, dd, gsolve, st); break;
-      case 14: launch_shape_bf3<1, 4>(groups, tiles, ntiles, dd, gsolve, st); break;
-      case 15: launch_shape_bf3<1, 5>(groups, tiles, ntiles, dd, gsolve, st); break;
-      case 16: launch_shape_bf3<1, 6>(groups, tiles, ntiles, dd, gsolve, st); break;
-      case 18: launch_shape_bf3<1, 8>(groups, tiles, ntiles, dd, gsolve, st); break;
-      case 21: launch_shape_bf3<2, 1>(groups, tiles, ntiles, dd, gsolve, st); break;
-      case 22: launch_shape_bf3<2, 2>(groups, tiles, ntiles, dd, gsolve, st); break;
-      case 23: launch_shape_bf3<2, 3>(groups, tiles, ntiles, dd, gsolve, st); break;
-      case 24: launch_shape_bf3<2, 4>(groups, tiles, ntiles, dd, gsolve, st); break;
-      case 41: launch_shape_bf3<4, 1>(groups, tiles, ntiles, dd, gsolve, st); break;
+      case 11: launch_shape_bf3<1, 1>(groups, tiles, ntiles, dd, gsolve, kind, st); break;
+      case 12: launch_shape_bf3<1, 2>(groups, tiles, ntiles, dd, gsolve, kind, st); break;
+      case 13: launch_shape_bf3<1, 3>(groups, tiles, ntiles, dd, gsolve, kind, st); break;
+      case 14: launch_shape_bf3<1, 4>(groups, tiles, ntiles, dd, gsolve, kind, st); break;
+      case 15: launch_shape_bf3<1, 5>(groups, tiles, ntiles, dd, gsolve, kind, st); break;
+      case 16: launch_shape_bf3<1, 6>(groups, tiles, ntiles, dd, gsolve, kind, st); break;
+      case 18: launch_shape_bf3<1, 8>(groups, tiles, ntiles, dd, gsolve, kind, st); break;
+      case 21: launch_shape_bf3<2, 1>(groups, tiles, ntiles, dd, gsolve, kind, st); break;
+      case 22: launch_shape_bf3<2, 2>(groups, tiles, ntiles, dd, gsolve, kind, st); break;
+      case 23: launch_shape_bf3<2, 3>(groups, tiles, ntiles, dd, gsolve, kind, st); break;
+      case 24: launch_shape_bf3<2, 4>(groups, tiles, ntiles, dd, gsolve, kind, st); break;
+      case 41: launch_shape_bf3<4, 1>(groups, tiles, ntiles, dd, gsolve, kind, st); break;
       default: break;
     }
     return;
@@ -2299,7 +2360,7 @@ void launch_gemm_shape(const GemmGroup *groups, const GemmTile *tiles, int ntile
     return;
   }
   if (mb == 0) {   // thin VALU kernel, N <= 4, FP64 arithmetic
-    const bool af = kind == 2 || kind == 4 || kind == 5;
+    const bool af = kind == 2 || kind == 4 || kind == 5 || kind == 7;   // A stored as float2
     const bool pp = kind >= 3 && kind <= 5;
     if (af && pp) hipLaunchKernelGGL((k_zgemm_thin<float2, 1>), dim3(ntiles), dim3(256), 0, st, groups, tiles, dd, gsolve);
     else if (af) hipLaunchKernelGGL((k_zgemm_thin<float2, 0>), dim3(ntiles), dim3(256), 0, st, groups, tiles, dd, gsolve);
