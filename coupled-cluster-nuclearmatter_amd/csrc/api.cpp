@@ -47,8 +47,8 @@ void launch_i7(const SolveDesc *dd, int nsolve, int maxA, hipStream_t st);
 void launch_i6(const SolveDesc *dd, int nsolve, int maxnp, int64_t maxrows, hipStream_t st);
 void launch_i9(const SolveDesc *dd, int nsolve, int64_t maxvhh, hipStream_t st);
 void launch_resid_init(const SolveDesc *dd, int nsolve, int64_t maxT, bool onebody, hipStream_t st);
-void launch_ring_gather(const SolveDesc *dd, int nsolve, int64_t maxT, hipStream_t st);
-void launch_ring_scatter(const SolveDesc *dd, int nsolve, int64_t maxT, hipStream_t st);
+void launch_ring_gather(const SolveDesc *dd, int nsolve, int64_t maxT, bool f32, hipStream_t st);
+void launch_ring_scatter(const SolveDesc *dd, int nsolve, int64_t maxT, bool f32, hipStream_t st);
 void launch_update(const SolveDesc *dd, int nsolve, int64_t maxT, const int *slot, hipStream_t st);
 void launch_gram(const SolveDesc *dd, int nsolve, const int *slots, int nstep, hipStream_t st);
 void launch_aa_update(const SolveDesc *dd, int nsolve, const int *slots, double beta, hipStream_t st);
@@ -526,7 +526,11 @@ void build_device(Solve &S) {
   d.off_rt = dupload(P.off_rt, o); d.off_i4 = dupload(P.off_i4, o); d.off_vt = dupload(P.off_vt, o);
   d.ph_a = dupload(P.ph_a, o); d.ph_i = dupload(P.ph_i, o);
   d.hp_j = dupload(P.hp_j, o); d.hp_b = dupload(P.hp_b, o);
-  d.ring4 = dupload(P.ring4, o);
+  {
+    if (P.nRt >= ((int64_t)1 << 31)) throw Error("ring blocks above 2^31 elements");
+    std::vector<int> r4(P.ring4.begin(), P.ring4.end());
+    d.ring4 = dupload(r4, o);
+  }
   d.T = dalloc<double2>(P.nT, o);
   d.R = dalloc<double2>(P.nT, o);
   d.R0 = dalloc<double2>(P.nT, o);
@@ -647,6 +651,8 @@ struct Batch {
   std::vector<SolveDesc> h_descs;
   GemmList g_x, g_pp, g_hh, g_i4, g_rt, g_pp32;
   GemmList g_xb, g_hhb, g_i4b, g_rtb;   // the same groups for the BF16x3 kernel
+  GemmList g_i4f, g_rtf;   // BF16x3 ring GEMMs on the FP32 ring blocks (when every solve has them)
+  bool ring32 = false;
   // Shared pp ladder of a batch of twists of one lattice and interaction: the
   // NN element <ab||cd> depends on the lattice labels of a, b, c, d only (the
   // twist cancels in every momentum difference), so one V_pppp over the union
@@ -922,14 +928,46 @@ std::unique_ptr<Batch> make_batch(const std::vector<Solve *> &ss, hipStream_t st
   B->g_hhb = B->g_hh;
   B->g_i4b = B->g_i4;
   B->g_rtb = B->g_rt;
-  const int fams[10] = {0, 1, 2, 3, 4, 1, 0, 2, 3, 4};
-  GemmList *lists[10] = {&B->g_x, &B->g_pp, &B->g_hh, &B->g_i4, &B->g_rt, &B->g_pp32,
-                         &B->g_xb, &B->g_hhb, &B->g_i4b, &B->g_rtb};
+  // the ring GEMMs of the BF16x3 residual on the FP32 ring blocks: the same
+  // groups with every pointer moved to the FP32 copy (same offsets); half the
+  // ring's bytes, no change to the products (BF16x3 splits FP32 values)
+  B->ring32 = true;
+  for (Solve *S : ss) B->ring32 = B->ring32 && S->desc.Tt32;
+  if (B->ring32) {
+    B->g_i4f = B->g_i4;
+    B->g_rtf = B->g_rt;
+    auto to32 = [&](GemmList &L) {
+      for (size_t g = 0; g < L.groups.size(); ++g) {
+        const SolveDesc &d = ss[L.gsolve[g]]->desc;
+        GemmGroup &G = L.groups[g];
+        auto mv = [&](const void *p) -> const void * {
+          const double2 *q = (const double2 *)p;
+          if (!q) return nullptr;
+          if (q >= d.Tt && q < d.Tt + ss[L.gsolve[g]]->plan.nRt) return d.Tt32 + (q - d.Tt);
+          if (q >= d.Vt && q < d.Vt + ss[L.gsolve[g]]->plan.nRt) return d.Vt32 + (q - d.Vt);
+          if (q >= d.Rt && q < d.Rt + ss[L.gsolve[g]]->plan.nRt) return d.Rt32 + (q - d.Rt);
+          if (q >= d.I4 && q < d.I4 + ss[L.gsolve[g]]->plan.nI4) return d.I4_32 + (q - d.I4);
+          if (q >= d.W0 && q < d.W0 + ss[L.gsolve[g]]->plan.nI4) return d.W0_32 + (q - d.W0);
+          throw Error("ring GEMM operand outside the ring blocks");
+        };
+        G.A = mv(G.A);
+        G.B = (const double2 *)mv(G.B);
+        G.C = (double2 *)mv(G.C);
+        G.C0 = (const double2 *)mv(G.C0);
+      }
+    };
+    to32(B->g_i4f);
+    to32(B->g_rtf);
+  }
+  const int fams[12] = {0, 1, 2, 3, 4, 1, 0, 2, 3, 4, 3, 4};
+  GemmList *lists[12] = {&B->g_x, &B->g_pp, &B->g_hh, &B->g_i4, &B->g_rt, &B->g_pp32,
+                         &B->g_xb, &B->g_hhb, &B->g_i4b, &B->g_rtb, &B->g_i4f, &B->g_rtf};
   B->d_descs = dupload(B->h_descs, B->owned);
-  for (int k = 0; k < 10; ++k) {
+  for (int k = 0; k < 12; ++k) {
     lists[k]->fam = fams[k];
     lists[k]->bf3 = k >= 6;
     lists[k]->xcd = fams[k] == 1 || k >= 6;
+    lists[k]->nosplit = lists[k]->nosplit || k >= 10;   // FP32 outputs: no FP64 partials
     if (k >= 6) {
       const GemmShape def[GemmList::NCLS] = {{1, 1}, {1, 2}, {1, 4}, {1, 8}, {1, 6}, {0, 4}};
       for (int c = 0; c < GemmList::NCLS; ++c) lists[k]->shape[c] = def[c];
@@ -1158,13 +1196,15 @@ void refresh_shared(Batch &B) {
 // never contracts (a single-solve batch used only for its MBPT2 start or
 // energy, the FP32 ladder of an FP64 solve) are never tuned.
 void prepare_lists(Batch &B, int prec) {
-  const int kinds[10] = {0, 3, 0, 0, 0, 4, 6, 6, 6, 6};   // the kernel kind each list is tuned for
-  GemmList *lists[10] = {&B.g_x, &B.g_pp, &B.g_hh, &B.g_i4, &B.g_rt, &B.g_pp32, &B.g_xb, &B.g_hhb, &B.g_i4b, &B.g_rtb};
-  for (int k = 0; k < 10; ++k) {
+  const int kinds[12] = {0, 3, 0, 0, 0, 4, 6, 6, 6, 6, 8, 8};   // the kernel kind each list is tuned for
+  GemmList *lists[12] = {&B.g_x, &B.g_pp, &B.g_hh, &B.g_i4, &B.g_rt, &B.g_pp32, &B.g_xb, &B.g_hhb, &B.g_i4b, &B.g_rtb,
+                         &B.g_i4f, &B.g_rtf};
+  for (int k = 0; k < 12; ++k) {
     GemmList &L = *lists[k];
     if (L.ready) continue;
     if (k == 5 && (prec == CCGPU_PREC_FP64 || !B.has_f32)) continue;
     if (k >= 6 && prec != CCGPU_PREC_BF16X3) continue;
+    if (k >= 10 && !B.ring32) continue;
     autotune(L, kinds[k], B.d_descs, B.stream);
     L.build_tiles();
     L.upload(B.owned);
@@ -1290,18 +1330,22 @@ void residual_launches(Batch &B, int prec, bool timing) {
     HIPCHK(hipEventRecord(B.ev_fork, st));
     HIPCHK(hipStreamWaitEvent(sr, B.ev_fork, 0));
   }
+  // BF16x3 ring on the FP32 ring blocks when every solve has them
+  const bool ring32 = pf(3) == CCGPU_PREC_BF16X3 && B.ring32;
   if (terms & 8) {
     {
       Timer t(timing, sr, "ring_gather");
-      launch_ring_gather(B.d_descs, n, B.maxT, sr);
+      launch_ring_gather(B.d_descs, n, B.maxT, ring32, sr);
     }
     {
       Timer t(timing, sr, "gemm_ring_i4");
-      run_gemm(list(B.g_i4, B.g_i4b, 3), B, pf(3), false, sr);
+      if (ring32) run_gemm(B.g_i4f, B, pf(3), true, sr, false, 8);
+      else run_gemm(list(B.g_i4, B.g_i4b, 3), B, pf(3), false, sr);
     }
     {
       Timer t(timing, sr, "gemm_ring_rt");
-      run_gemm(list(B.g_rt, B.g_rtb, 3), B, pf(3), false, sr);
+      if (ring32) run_gemm(B.g_rtf, B, pf(3), true, sr, false, 8);
+      else run_gemm(list(B.g_rt, B.g_rtb, 3), B, pf(3), false, sr);
     }
   }
   if ((terms & 2) && B.share.on) {
@@ -1335,7 +1379,7 @@ void residual_launches(Batch &B, int prec, bool timing) {
   }
   if (terms & 8) {
     Timer t(timing, st, "ring_scatter");
-    launch_ring_scatter(B.d_descs, n, B.maxT, st);
+    launch_ring_scatter(B.d_descs, n, B.maxT, ring32, st);
   }
 }
 
@@ -1825,6 +1869,11 @@ void fill_f32(Solve &S) {
   const Plan &P = S.plan;
   launch_to_f32_rows(S.desc.Vpppp, S.desc.Vpppp32, S.d_off_vpp, S.d_off_vpp32, S.d_npp, P.nch, P.max_npp, S.stream);
   check_launch();
+  if (S.desc.Vt32) {
+    launch_to_f32(S.desc.Vt, S.desc.Vt32, P.nRt, S.stream);
+    launch_to_f32(S.desc.W0, S.desc.W0_32, P.nI4, S.stream);
+    check_launch();
+  }
   HIPCHK(hipStreamSynchronize(S.stream));
 }
 
@@ -1843,6 +1892,13 @@ void ensure_f32(Solve &S, int handle) {
   S.d_off_vpp32 = dupload(S.off_vpp32, S.owned);
   S.d_npp = dupload(P.npp, S.owned);
   S.desc.Vpppp32 = dalloc<float2>(S.nVpp32, S.owned);
+  // FP32 ring blocks of the BF16x3 residual; T~ entries with a=b or i=j stay zero
+  S.desc.Tt32 = dalloc<float2>(std::max<int64_t>(P.nRt, 1), S.owned);
+  S.desc.Rt32 = dalloc<float2>(std::max<int64_t>(P.nRt, 1), S.owned);
+  S.desc.Vt32 = dalloc<float2>(std::max<int64_t>(P.nRt, 1), S.owned);
+  S.desc.I4_32 = dalloc<float2>(std::max<int64_t>(P.nI4, 1), S.owned);
+  S.desc.W0_32 = dalloc<float2>(std::max<int64_t>(P.nI4, 1), S.owned);
+  HIPCHK(hipMemset(S.desc.Tt32, 0, std::max<int64_t>(P.nRt, 1) * sizeof(float2)));
   fill_f32(S);
   HIPCHK(hipMemcpy(S.d_desc, &S.desc, sizeof(SolveDesc), hipMemcpyHostToDevice));
   drop_single(handle);

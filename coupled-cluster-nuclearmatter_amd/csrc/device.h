@@ -39,11 +39,14 @@ struct SolveDesc {
   const int *nph, *nhp;
   const int64_t *off_ph, *off_hp, *off_rt, *off_i4, *off_vt;
   const int *ph_a, *ph_i, *hp_j, *hp_b;
-  const int64_t *ring4;
+  const int *ring4;   // four Pandya positions per unique amplitude (every ring block below 2^31 elements)
   // amplitudes and blocks
   double2 *T, *R, *R0, *Vhhpp, *Vhhhh, *X, *Vpppp;
   float2 *Vpppp32;
   double2 *Tt, *Rt, *I4, *W0, *Vt;
+  // FP32 ring blocks of the BF16x3 residual (null until the FP32 copies are
+  // made): T~, R~, I4 per iteration, V~ and W0 copied from the FP64 blocks
+  float2 *Tt32, *Rt32, *I4_32, *W0_32, *Vt32;
   // one-body
   double2 *fdiag, *fpart;   // f(p,p) and f(p,partner(p))
   double2 *dh, *I7off, *dp, *I6off;

@@ -590,28 +590,54 @@ __global__ void k_resid_init(const SolveDesc *descs, int onebody) {
   }
 }
 
-// Pandya gather T~((a,i),(j,b)) = t(ab,ij) (ccm_t2_eqn.f90:37-62).
+// Pandya gather T~((a,i),(j,b)) = t(ab,ij) (ccm_t2_eqn.f90:37-62); F32: into
+// the FP32 blocks of the BF16x3 residual.  The four positions of an amplitude
+// are one 16-byte load.
+template <bool F32>
 __global__ void k_ring_gather(const SolveDesc *descs) {
   const SolveDesc &d = descs[blockIdx.y];
   if (!d.active) return;
   for (int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; u < d.nT; u += (int64_t)gridDim.x * blockDim.x) {
-    double2 t = d.T[u];
-    double2 m = make_double2(-t.x, -t.y);
-    const int64_t *p = d.ring4 + 4 * u;
-    d.Tt[p[0]] = t;
-    d.Tt[p[1]] = m;
-    d.Tt[p[2]] = m;
-    d.Tt[p[3]] = t;
+    const double2 t = d.T[u];
+    const int4 p = *(const int4 *)(d.ring4 + 4 * u);
+    if constexpr (F32) {
+      const float2 tf = make_float2((float)t.x, (float)t.y), mf = make_float2(-tf.x, -tf.y);
+      d.Tt32[p.x] = tf;
+      d.Tt32[p.y] = mf;
+      d.Tt32[p.z] = mf;
+      d.Tt32[p.w] = tf;
+    } else {
+      const double2 m = make_double2(-t.x, -t.y);
+      d.Tt[p.x] = t;
+      d.Tt[p.y] = m;
+      d.Tt[p.z] = m;
+      d.Tt[p.w] = t;
+    }
   }
 }
 
-// Ring scatter with P(ab)P(ij) (ccm_t2_eqn.f90:1133-1185), owner-computes.
+// Ring scatter with P(ab)P(ij) (ccm_t2_eqn.f90:1133-1185), owner-computes;
+// F32: from the FP32 R~ of the BF16x3 residual (summed in FP64).
+template <bool F32>
 __global__ void k_ring_scatter(const SolveDesc *descs) {
   const SolveDesc &d = descs[blockIdx.y];
   if (!d.active) return;
   for (int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; u < d.nT; u += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t *p = d.ring4 + 4 * u;
-    double2 v = csub(csub(cadd(d.Rt[p[0]], d.Rt[p[3]]), d.Rt[p[1]]), d.Rt[p[2]]);
+    const int4 p = *(const int4 *)(d.ring4 + 4 * u);
+    double2 r0, r1, r2, r3;
+    if constexpr (F32) {
+      const float2 a = d.Rt32[p.x], b = d.Rt32[p.y], c = d.Rt32[p.z], e = d.Rt32[p.w];
+      r0 = make_double2(a.x, a.y);
+      r1 = make_double2(b.x, b.y);
+      r2 = make_double2(c.x, c.y);
+      r3 = make_double2(e.x, e.y);
+    } else {
+      r0 = d.Rt[p.x];
+      r1 = d.Rt[p.y];
+      r2 = d.Rt[p.z];
+      r3 = d.Rt[p.w];
+    }
+    const double2 v = csub(csub(cadd(r0, r3), r1), r2);
     d.R[u] = cadd(d.R[u], v);
   }
 }
@@ -1382,7 +1408,8 @@ __device__ __forceinline__ void bf3_lds_sync() {
   asm volatile("" ::: "memory");
 }
 
-template <int MB, int NB, typename AT>
+// F32IO: B, C and C0 are FP32 blocks (float2 behind the group's pointers).
+template <int MB, int NB, typename AT, bool F32IO = false>
 __global__ __launch_bounds__(256) void k_zgemm_bf3(const GemmGroup *__restrict__ groups,
                                                    const GemmTile *__restrict__ tiles,
                                                    const SolveDesc *__restrict__ descs,
@@ -1414,21 +1441,23 @@ __global__ __launch_bounds__(256) void k_zgemm_bf3(const GemmGroup *__restrict__
   // clamped, valid addresses); elements outside the block are zeroed when the
   // chunk is split into LDS, so no instruction touches a load's result before
   // the chunk is needed.
-  auto load = [&](double2 (&a)[APT], double2 (&b)[BPT], int k0) {
+  typedef typename std::conditional<F32IO, float2, double2>::type BT;
+  const BT *Bp = (const BT *)g.B;
+  auto load = [&](AT (&a)[APT], BT (&b)[BPT], int k0) {
 #pragma unroll
     for (int j = 0; j < APT; ++j) {
       const int e = threadIdx.x + 256 * j;
       const int m = tl.m0 + (e >> 4), k = k0 + (e & 15);
-      a[j] = ccgpu::load_a<AT>(A + (int64_t)min(m, g.M - 1) * g.lda + min(k, ke - 1));
+      a[j] = gload(A + (int64_t)min(m, g.M - 1) * g.lda + min(k, ke - 1));
     }
 #pragma unroll
     for (int j = 0; j < BPT; ++j) {
       const int e = threadIdx.x + 256 * j;
       const int k = k0 + e / BC, n = tl.n0 + e % BC;
-      b[j] = gload(g.B + (int64_t)min(k, ke - 1) * g.ldb + min(n, g.N - 1));
+      b[j] = gload(Bp + (int64_t)min(k, ke - 1) * g.ldb + min(n, g.N - 1));
     }
   };
-  auto store = [&](const double2 (&a)[APT], const double2 (&b)[BPT], int k0, int buf) {
+  auto store = [&](const AT (&a)[APT], const BT (&b)[BPT], int k0, int buf) {
 #pragma unroll
     for (int j = 0; j < APT; ++j) {
       const int e = threadIdx.x + 256 * j;
@@ -1475,7 +1504,8 @@ __global__ __launch_bounds__(256) void k_zgemm_bf3(const GemmGroup *__restrict__
       }
     }
   };
-  double2 a0[APT], b0[BPT], a1[APT], b1[BPT];
+  AT a0[APT], a1[APT];
+  BT b0[BPT], b1[BPT];
   // pairs of chunks; the loads and LDS stores of a pair run unconditionally
   // (past the k range they fetch valid addresses and store zeros), only the
   // MFMAs of a second chunk past the end are skipped
@@ -1511,12 +1541,21 @@ __global__ __launch_bounds__(256) void k_zgemm_bf3(const GemmGroup *__restrict__
           g.P[((int64_t)tl.ks * g.M + m) * g.N + n] = make_double2(vr, vi);
         } else if (m < g.M && n < g.N) {
           double2 v = make_double2(g.alpha * vr, g.alpha * vi);
-          if (g.beta != 0.0) {
-            double2 c0 = g.C0[(int64_t)m * g.ldc0 + n];
-            v.x += g.beta * c0.x;
-            v.y += g.beta * c0.y;
+          if constexpr (F32IO) {
+            if (g.beta != 0.0) {
+              const float2 c0 = ((const float2 *)g.C0)[(int64_t)m * g.ldc0 + n];
+              v.x += g.beta * c0.x;
+              v.y += g.beta * c0.y;
+            }
+            ((float2 *)g.C)[(int64_t)m * g.ldc + n] = make_float2((float)v.x, (float)v.y);
+          } else {
+            if (g.beta != 0.0) {
+              double2 c0 = g.C0[(int64_t)m * g.ldc0 + n];
+              v.x += g.beta * c0.x;
+              v.y += g.beta * c0.y;
+            }
+            g.C[(int64_t)m * g.ldc + n] = v;
           }
-          g.C[(int64_t)m * g.ldc + n] = v;
         }
       }
 }
@@ -1527,7 +1566,8 @@ __global__ __launch_bounds__(256) void k_zgemm_bf3(const GemmGroup *__restrict__
 // the k range 64 consecutive elements at a time (one coalesced 1 KB read of a
 // row per instruction), each lane reuses its B row (4 columns) for the 4 rows,
 // and the 64 partial sums of every output are combined with a fixed xor tree.
-template <typename AT, int FAM>
+// F32IO: B, C and C0 are FP32 blocks (float2 behind the group's pointers).
+template <typename AT, int FAM, bool F32IO = false>
 __global__ __launch_bounds__(256) void k_zgemm_thin(const GemmGroup *__restrict__ groups,
                                                     const GemmTile *__restrict__ tiles,
                                                     const SolveDesc *__restrict__ descs,
@@ -1551,7 +1591,13 @@ __global__ __launch_bounds__(256) void k_zgemm_thin(const GemmGroup *__restrict_
     double2 b[4], a[4];
 #pragma unroll
     for (int n = 0; n < 4; ++n) {
-      const double2 v = gload(g.B + (int64_t)kc * g.ldb + min(n, g.N - 1));
+      double2 v;
+      if constexpr (F32IO) {
+        const float2 f = gload((const float2 *)g.B + (int64_t)kc * g.ldb + min(n, g.N - 1));
+        v = make_double2(f.x, f.y);
+      } else {
+        v = gload(g.B + (int64_t)kc * g.ldb + min(n, g.N - 1));
+      }
       const bool ok = okk && n < g.N;
       b[n] = make_double2(ok ? v.x : 0.0, ok ? v.y : 0.0);
     }
@@ -1607,12 +1653,21 @@ __global__ __launch_bounds__(256) void k_zgemm_thin(const GemmGroup *__restrict_
         g.P[((int64_t)tl.ks * g.M + m) * g.N + n] = v0;
       } else {
         double2 o = make_double2(g.alpha * v0.x, g.alpha * v0.y);
-        if (g.beta != 0.0) {
-          const double2 c0 = g.C0[(int64_t)m * g.ldc0 + n];
-          o.x += g.beta * c0.x;
-          o.y += g.beta * c0.y;
+        if constexpr (F32IO) {
+          if (g.beta != 0.0) {
+            const float2 c0 = ((const float2 *)g.C0)[(int64_t)m * g.ldc0 + n];
+            o.x += g.beta * c0.x;
+            o.y += g.beta * c0.y;
+          }
+          ((float2 *)g.C)[(int64_t)m * g.ldc + n] = make_float2((float)o.x, (float)o.y);
+        } else {
+          if (g.beta != 0.0) {
+            const double2 c0 = g.C0[(int64_t)m * g.ldc0 + n];
+            o.x += g.beta * c0.x;
+            o.y += g.beta * c0.y;
+          }
+          g.C[(int64_t)m * g.ldc + n] = o;
         }
-        g.C[(int64_t)m * g.ldc + n] = o;
       }
     }
   }
@@ -1794,11 +1849,13 @@ void launch_i9(const SolveDesc *dd, int nsolve, int64_t maxvhh, hipStream_t st) 
 void launch_resid_init(const SolveDesc *dd, int nsolve, int64_t maxT, bool onebody, hipStream_t st) {
   hipLaunchKernelGGL(k_resid_init, egrid(maxT, nsolve), dim3(256), 0, st, dd, onebody ? 1 : 0);
 }
-void launch_ring_gather(const SolveDesc *dd, int nsolve, int64_t maxT, hipStream_t st) {
-  hipLaunchKernelGGL(k_ring_gather, egrid(maxT, nsolve), dim3(256), 0, st, dd);
+void launch_ring_gather(const SolveDesc *dd, int nsolve, int64_t maxT, bool f32, hipStream_t st) {
+  if (f32) hipLaunchKernelGGL(k_ring_gather<true>, egrid(maxT, nsolve), dim3(256), 0, st, dd);
+  else hipLaunchKernelGGL(k_ring_gather<false>, egrid(maxT, nsolve), dim3(256), 0, st, dd);
 }
-void launch_ring_scatter(const SolveDesc *dd, int nsolve, int64_t maxT, hipStream_t st) {
-  hipLaunchKernelGGL(k_ring_scatter, egrid(maxT, nsolve), dim3(256), 0, st, dd);
+void launch_ring_scatter(const SolveDesc *dd, int nsolve, int64_t maxT, bool f32, hipStream_t st) {
+  if (f32) hipLaunchKernelGGL(k_ring_scatter<true>, egrid(maxT, nsolve), dim3(256), 0, st, dd);
+  else hipLaunchKernelGGL(k_ring_scatter<false>, egrid(maxT, nsolve), dim3(256), 0, st, dd);
 }
 void launch_update(const SolveDesc *dd, int nsolve, int64_t maxT, const int *slot, hipStream_t st) {
   hipLaunchKernelGGL(k_update, egrid(maxT, nsolve), dim3(256), 0, st, dd, slot);
@@ -1850,7 +1907,8 @@ void launch_diis_axpy(const SolveDesc *dd, int nsolve, int64_t maxT, const int *
 // Kernel kinds: 0 FP64, 1 FP32 (A double), 2 FP32 (A float), 3 FP64 pp ladder,
 // 4 FP32 pp ladder (A float), 5 pp ladder with A float and FP64 arithmetic,
 // 6 BF16x3 split products (A double; the thin N <= 4 class stays FP64), tile
-// shapes (1,1..6), (1,8), (2,1..4), (4,1); 7 the same with A float.
+// shapes (1,1..6), (1,8), (2,1..4), (4,1); 7 the same with A float; 8 with A,
+// B, C and C0 float (the FP32 ring blocks).
 // Tile shapes (MB, NB): MB 16-row blocks per wave (4 waves stacked along M),
 // NB 16-column blocks: (1,1) (1,2) (1,3) (1,4) (1,5) (2,1) (2,2) (4,1); and
 // (0,4): the thin VALU kernel (16-row tiles, N <= 4).  lb = 1 selects the
@@ -2297,7 +2355,9 @@ static void launch_shape(const GemmGroup *groups, const GemmTile *tiles, int nti
 template <int MB, int NB>
 static void launch_shape_bf3(const GemmGroup *groups, const GemmTile *tiles, int ntiles, const SolveDesc *dd,
                              const int *gsolve, int kind, hipStream_t st) {
-  if (kind == 7)
+  if (kind == 8)
+    hipLaunchKernelGGL((k_zgemm_bf3<MB, NB, float2, true>), dim3(ntiles), dim3(256), 0, st, groups, tiles, dd, gsolve);
+  else if (kind == 7)
     hipLaunchKernelGGL((k_zgemm_bf3<MB, NB, float2>), dim3(ntiles), dim3(256), 0, st, groups, tiles, dd, gsolve);
   else
     hipLaunchKernelGGL((k_zgemm_bf3<MB, NB, double2>), dim3(ntiles), dim3(256), 0, st, groups, tiles, dd, gsolve);
@@ -2306,7 +2366,7 @@ static void launch_shape_bf3(const GemmGroup *groups, const GemmTile *tiles, int
 void launch_gemm_shape(const GemmGroup *groups, const GemmTile *tiles, int ntiles, const SolveDesc *dd,
                        const int *gsolve, int kind, int mb, int nb, int lb, hipStream_t st) {
   if (ntiles <= 0) return;
-  if ((kind == 6 || kind == 7) && mb > 0) {
+  if ((kind == 6 || kind == 7 || kind == 8) && mb > 0) {
     switch (mb * 10 + nb) {
       case 11: launch_shape_bf3<1, 1>(groups, tiles, ntiles, dd, gsolve, kind, st); break;
       case 12: launch_shape_bf3<1, 2>(groups, tiles, ntiles, dd, gsolve, kind, st); break;
@@ -2357,6 +2417,10 @@ void launch_gemm_shape(const GemmGroup *groups, const GemmTile *tiles, int ntile
       case 41: launch_shape_lb<4, 1>(groups, tiles, ntiles, dd, gsolve, kind, st); break;
       default: break;
     }
+    return;
+  }
+  if (mb == 0 && kind == 8) {   // thin VALU kernel on FP32 blocks
+    hipLaunchKernelGGL((k_zgemm_thin<float2, 0, true>), dim3(ntiles), dim3(256), 0, st, groups, tiles, dd, gsolve);
     return;
   }
   if (mb == 0) {   // thin VALU kernel, N <= 4, FP64 arithmetic
