@@ -141,14 +141,24 @@ __device__ double2 h_term(const SolveDesc &d, const TriplesDesc &td, double vol,
 // Dense element tables for (T): every <cb||kd> (particles c, b, hole k, the
 // particle d of either spin that conserves momentum and isospin) and every
 // <jk||lc>, evaluated once by the same element source as the in-place path.
+// <cb||kd> = -<bc||kd> (the NN element and the normal-ordered 3NF sum are
+// both antisymmetric in the bra): the fill evaluates c < b and writes both
+// entries, c = b is zero.  Grid stride over the half (c < b) only.
 __global__ __launch_bounds__(256) void k_t3_fill_hppp(SolveDesc d, TriplesDesc td, double vol, double2 *out) {
-  const int64_t n = (int64_t)d.np * d.np * d.A * 2;
+  const int np = d.np;
+  const int64_t npair = (int64_t)np * (np - 1) / 2;   // c < b, in the order of pairs (b, c) with c < b
+  const int64_t per = (int64_t)d.A * 2;
+  const int64_t n = npair * per;
   for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x) {
-    const int sd = (int)(e & 1) * 2 - 1;
-    const int64_t r = e >> 1;
-    const int k = (int)(r % d.A);
-    const int64_t cb = r / d.A;
-    const int b = (int)(cb % d.np) + d.A, c = (int)(cb / d.np) + d.A;
+    const int64_t q = e / per;
+    const int rem = (int)(e - q * per);
+    const int sd = (rem & 1) * 2 - 1, k = rem >> 1;
+    // pair index q -> (c < b): b - A = largest x with x (x - 1) / 2 <= q
+    int bx = (int)((1.0 + sqrt(1.0 + 8.0 * (double)q)) * 0.5);
+    while ((int64_t)bx * (bx - 1) / 2 > q) --bx;
+    while ((int64_t)(bx + 1) * bx / 2 <= q) ++bx;
+    const int cx = (int)(q - (int64_t)bx * (bx - 1) / 2);
+    const int b = bx + d.A, c = cx + d.A;
     const int dd = orb_lookup(td, d.nx[c] + d.nx[b] - d.nx[k], d.ny[c] + d.ny[b] - d.ny[k],
                               d.nz[c] + d.nz[b] - d.nz[k], sd, d.tz[c] + d.tz[b] - d.tz[k]);
     double2 v = make_double2(0.0, 0.0);
@@ -160,7 +170,15 @@ __global__ __launch_bounds__(256) void k_t3_fill_hppp(SolveDesc d, TriplesDesc t
         v.y += w.y;
       }
     }
-    out[e] = v;
+    const int64_t tail = (int64_t)k * 2 + (sd + 1) / 2;
+    out[((int64_t)cx * np + bx) * per + tail] = v;
+    out[((int64_t)bx * np + cx) * per + tail] = make_double2(-v.x, -v.y);
+  }
+  // the diagonal c = b
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < (int64_t)np * per;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t x = e / per;
+    out[(x * np + x) * per + (e - x * per)] = make_double2(0.0, 0.0);
   }
 }
 
