@@ -72,7 +72,8 @@ void launch_t3_tables(const SolveDesc &d, const TriplesDesc &td, double volume, 
                       hipStream_t st);
 void launch_triples(const SolveDesc &d, const TriplesDesc &td, double volume, double2 *partial,
                     unsigned long long *count, int nblocks, double2 *out, unsigned long long *count_out,
-                    hipStream_t st, int mode);
+                    hipStream_t st, int mode, const double2 *tdt, int maxnhh);
+void launch_t3_td(const SolveDesc &d, const TriplesDesc &td, double2 *tdt, hipStream_t st);
 // mlp.hip
 void launch_mlp_forward(const float *params, const uint16_t *feat, int64_t n, float *y, hipStream_t st);
 void launch_mlp_loss_grad(const float *params, const uint16_t *feat, const float *target, const int64_t *idx,
@@ -400,6 +401,7 @@ struct Solve {
   double2 t3_et4{0.0, 0.0}, t3_et5{0.0, 0.0};   // parts of the last (T)
   TriplesDesc t3{};
   double2 *t3_partial = nullptr, *t3_out = nullptr;
+  double2 *t3_td = nullptr;   // t(x d, u v) of the F terms, refilled from T for every (T) sum
   unsigned long long *t3_count = nullptr;   // per block, then the total at [8192]
   long long t3_elements = 0;
   double t3_seconds = 0.0;
@@ -2783,8 +2785,23 @@ double2 triples_sum(Solve &S, int mode) {
   if (mode != 2 && !(S.t3.vhppp && S.t3.vhhhp) && mode != 0)
     throw Error("CCDT-1 and the three-body (T) need the element tables: raise CCGPU_T3_TABLE_GB");
   int nb = (int)std::min<int64_t>(8192, std::max<int64_t>(1, S.t3.npairs));
+  // the staged (T) kernel: the F terms' amplitude table (np A^2 x 2 spins),
+  // made once per handle when the element tables exist and it fits with them
+  // in CCGPU_T3_TABLE_GB, refilled from the current T for every sum
+  const Plan &P = S.plan;
+  if (mode == 0 && S.t3.vhppp && S.t3.vhhhp && !S.t3_td) {
+    const double lim = (getenv("CCGPU_T3_TABLE_GB") ? atof(getenv("CCGPU_T3_TABLE_GB")) : 16.0) * 1e9;
+    const int64_t ntab = (int64_t)P.np * P.np * P.A * 2 + (int64_t)P.A * P.A * P.np * 2;
+    const int64_t ntd = (int64_t)P.np * P.A * P.A * 2;
+    if ((double)(ntab + ntd) * sizeof(double2) <= lim && (size_t)4 * 3 * P.max_nhh * sizeof(double2) <= 65536)
+      S.t3_td = dalloc<double2>(ntd, S.owned);
+  }
+  if (mode == 0 && S.t3_td) {
+    launch_t3_td(S.desc, S.t3, S.t3_td, S.stream);
+    check_launch();
+  }
   launch_triples(S.desc, S.t3, S.plan.orb.volume, S.t3_partial, S.t3_count, nb, S.t3_out, S.t3_count + 8192,
-                 S.stream, mode);
+                 S.stream, mode, mode == 0 ? S.t3_td : nullptr, P.max_nhh);
   check_launch();
   double2 r;
   unsigned long long cnt = 0;

@@ -413,6 +413,146 @@ __global__ __launch_bounds__(256) void k_triples_w(SolveDesc d, TriplesDesc td, 
   }
 }
 
+// Amplitude factor of every F term, t(x d, u v) for each particle x, ordered
+// hole pair (u, v) and both spins of the particle d that conserves momentum
+// and isospin (zero where d is a hole or absent): out[((x - A) A + u) A + v]
+// holds the two spins side by side.  Rebuilt from T for every (T) sum.
+__global__ __launch_bounds__(256) void k_t3_fill_td(SolveDesc d, TriplesDesc td, double2 *out) {
+  const int64_t n = (int64_t)d.np * d.A * d.A * 2;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x) {
+    const int sd = (int)(e & 1) * 2 - 1;
+    const int64_t r = e >> 1;
+    const int v = (int)(r % d.A), u = (int)((r / d.A) % d.A);
+    const int x = (int)(r / ((int64_t)d.A * d.A)) + d.A;
+    const int dd = orb_lookup(td, d.nx[u] + d.nx[v] - d.nx[x], d.ny[u] + d.ny[v] - d.ny[x],
+                              d.nz[u] + d.nz[v] - d.nz[x], sd, d.tz[u] + d.tz[v] - d.tz[x]);
+    out[e] = dd >= d.A ? t_of(d, x, dd, u, v) : make_double2(0.0, 0.0);
+  }
+}
+
+// (T) from the element tables with the amplitudes staged: the same sum, terms
+// and order as k_triples_w<0> (the same bits), with fewer gathers per
+// element.  A wave's particle triple (a,b,c) fixes the three particle pairs
+// (a,b), (a,c), (c,b) of the H terms: their rows of T (one per pair, the hole
+// pairs of its channel) are copied into the wave's slice of LDS, and an H term
+// reads t(xy, u l) there at the column of the hole pair (u, l).  The F terms
+// read t(x d, u v) for both spins of d from the table above (one 32-byte
+// load instead of the momentum lookup and three index loads per spin).
+__device__ __forceinline__ void t3_wave_sync() {
+  __builtin_amdgcn_s_waitcnt((15) | (3 << 14) | (7 << 4) | (0 << 8));   // lgkmcnt(0)
+  __builtin_amdgcn_wave_barrier();
+}
+
+__global__ __launch_bounds__(256) void k_triples_s(SolveDesc d, TriplesDesc td, const double2 *__restrict__ tdt,
+                                                   int maxnhh, double2 *partial, unsigned long long *count) {
+  extern __shared__ double2 t3rows[];   // 4 waves x 3 pairs x maxnhh
+  __shared__ double sre[256], sim[256];
+  __shared__ unsigned long long scnt[256];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  double2 *rows = t3rows + (int64_t)wave * 3 * maxnhh;
+  const int A = d.A, np = d.np;
+  double re = 0.0, im = 0.0;
+  unsigned long long nel = 0;
+  for (int64_t pr = blockIdx.x; pr < td.npairs; pr += gridDim.x) {
+    const int a = td.pair_a[pr], b = td.pair_b[pr];
+    for (int c = b + 1 + wave; c < d.nsp; c += 4) {
+      const int kx = d.nx[a] + d.nx[b] + d.nx[c], ky = d.ny[a] + d.ny[b] + d.ny[c], kz = d.nz[a] + d.nz[b] + d.nz[c];
+      const int tz = d.tz[a] + d.tz[b] + d.tz[c];
+      const int ix = kx - td.k0, iy = ky - td.k0, iz = kz - td.k0, it = tz - td.tz0;
+      if (ix < 0 || iy < 0 || iz < 0 || ix >= td.nk || iy >= td.nk || iz >= td.nk || it < 0 || it >= td.ntz) continue;
+      const int ch = td.chan[((it * td.nk + ix) * td.nk + iy) * td.nk + iz];
+      if (ch < 0) continue;
+      const int h0 = td.hptr[ch], h1 = td.hptr[ch + 1];
+      if (lane == 0) nel += (unsigned long long)(h1 - h0);
+      // the pairs (a,b), (a,c), (c,b): channel, row, orientation; their rows
+      int pch[3], psg[3];
+#pragma unroll
+      for (int p = 0; p < 3; ++p) {
+        const int x = p == 2 ? c : a, y = p == 0 ? b : (p == 1 ? c : b);
+        const DPair q = d.ppref[(size_t)(x - A) * np + (y - A)];
+        pch[p] = q.ch;
+        psg[p] = q.v & 1;
+        if (q.ch >= 0) {
+          const int nh = d.nhh[q.ch];
+          const double2 *src = d.T + d.off_t[q.ch] + (int64_t)(q.v >> 1) * nh;
+          for (int col = lane; col < nh; col += 64) rows[p * maxnhh + col] = src[col];
+        }
+      }
+      t3_wave_sync();
+      const double fa = d.fdiag[a].x, fb = d.fdiag[b].x, fc = d.fdiag[c].x;
+      const double fai = d.fdiag[a].y, fbi = d.fdiag[b].y, fci = d.fdiag[c].y;
+      for (int h = h0 + lane; h < h1; h += 64) {
+        const int i = td.hijk[3 * h], j = td.hijk[3 * h + 1], k = td.hijk[3 * h + 2];
+        double2 w = make_double2(0.0, 0.0);
+#pragma unroll 1
+        for (int t = 0; t < 18; ++t) {
+          const TermCode tc = kTerms[t];
+          const int x = sel3(tc.p0, a, b, c), y = sel3(tc.p1, a, b, c), z = sel3(tc.p2, a, b, c);
+          const int u = sel3(tc.h0, i, j, k), v = sel3(tc.h1, i, j, k), q = sel3(tc.h2, i, j, k);
+          double2 f = make_double2(0.0, 0.0);
+          if (tc.kind == 0) {
+            // F = sum_d conj(<zy||qd>) t(x d, u v)
+            const double2 *vv = td.vhppp + 2 * (((int64_t)(z - A) * np + (y - A)) * A + q);
+            const double2 *tt = tdt + 2 * (((int64_t)(x - A) * A + u) * A + v);
+#pragma unroll
+            for (int sd = 0; sd < 2; ++sd) {
+              const double2 vx = vv[sd], tx = tt[sd];
+              f.x += vx.x * tx.x + vx.y * tx.y;
+              f.y += vx.x * tx.y - vx.y * tx.x;
+            }
+          } else {
+            // H = sum_l conj(<vq||lz>) t(xy, u l)
+            const int slot = tc.p0 == 2 ? 2 : (tc.p1 == 1 ? 0 : 1);
+            const int nx = d.nx[q] + d.nx[v] - d.nx[z], ny = d.ny[q] + d.ny[v] - d.ny[z];
+            const int nz = d.nz[q] + d.nz[v] - d.nz[z], tl = d.tz[q] + d.tz[v] - d.tz[z];
+            const double2 *vv = td.vhhhp + 2 * (((int64_t)v * A + q) * np + (z - A));
+#pragma unroll
+            for (int sd = 0; sd < 2; ++sd) {
+              const int l = orb_lookup(td, nx, ny, nz, 2 * sd - 1, tl);
+              if (l < 0 || l >= A) continue;
+              double2 tx = make_double2(0.0, 0.0);
+              const DPair hr = d.hhref[(size_t)u * A + l];
+              if (pch[slot] >= 0 && hr.ch == pch[slot]) {
+                tx = rows[slot * maxnhh + (hr.v >> 1)];
+                if ((psg[slot] ^ hr.v) & 1) tx = make_double2(-tx.x, -tx.y);
+              }
+              const double2 vx = vv[sd];
+              f.x += vx.x * tx.x + vx.y * tx.y;
+              f.y += vx.x * tx.y - vx.y * tx.x;
+            }
+          }
+          w.x += (double)tc.sign * f.x;
+          w.y += (double)tc.sign * f.y;
+        }
+        const double dre = d.fdiag[i].x + d.fdiag[j].x + d.fdiag[k].x - fa - fb - fc;
+        const double dim = d.fdiag[i].y + d.fdiag[j].y + d.fdiag[k].y - fai - fbi - fci;
+        const double den = dre * dre + dim * dim;
+        const double nre = w.x * w.x + w.y * w.y, nim = 0.0;
+        re += (nre * dre + nim * dim) / den;
+        im += (nim * dre - nre * dim) / den;
+      }
+      // the rows are overwritten for the next triple only after every lane read them
+      t3_wave_sync();
+    }
+  }
+  sre[threadIdx.x] = re;
+  sim[threadIdx.x] = im;
+  scnt[threadIdx.x] = nel;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) {
+      sre[threadIdx.x] += sre[threadIdx.x + s];
+      sim[threadIdx.x] += sim[threadIdx.x + s];
+      scnt[threadIdx.x] += scnt[threadIdx.x + s];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    partial[blockIdx.x] = make_double2(sre[0], sim[0]);
+    count[blockIdx.x] = scnt[0];
+  }
+}
+
 __global__ void k_triples_count(const unsigned long long *count, int n, unsigned long long *out) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
   unsigned long long s = 0;
@@ -446,17 +586,27 @@ void launch_t3_tables(const SolveDesc &d, const TriplesDesc &td, double volume, 
   hipLaunchKernelGGL(k_t3_fill_hhhp, dim3(1024), dim3(256), 0, st, d, td, volume, hhhp);
 }
 
+void launch_t3_td(const SolveDesc &d, const TriplesDesc &td, double2 *tdt, hipStream_t st) {
+  hipLaunchKernelGGL(k_t3_fill_td, dim3(4096), dim3(256), 0, st, d, td, tdt);
+}
+
+// CCGPU_T3_KERNEL (studies and the tests that compare the forms): 1 (default)
+// the staged kernel when its amplitude table tdt exists, else the table
+// kernel k_triples_w; 3 k_triples_w; other values the thread-per-triple form.
 void launch_triples(const SolveDesc &d, const TriplesDesc &td, double volume, double2 *partial,
                     unsigned long long *count, int nblocks, double2 *out, unsigned long long *count_out,
-                    hipStream_t st, int mode) {
-  static const int variant = getenv("CCGPU_T3_KERNEL") ? atoi(getenv("CCGPU_T3_KERNEL")) : 1;
-  if (mode == 1)
+                    hipStream_t st, int mode, const double2 *tdt, int maxnhh) {
+  const int variant = getenv("CCGPU_T3_KERNEL") ? atoi(getenv("CCGPU_T3_KERNEL")) : 1;
+  if (mode == 0 && variant == 1 && tdt && td.vhppp && td.vhhhp) {
+    const size_t lds = (size_t)4 * 3 * maxnhh * sizeof(double2);
+    hipLaunchKernelGGL(k_triples_s, dim3(nblocks), dim3(256), lds, st, d, td, tdt, maxnhh, partial, count);
+  } else if (mode == 1)
     hipLaunchKernelGGL(k_triples_w<1>, dim3(nblocks), dim3(256), 0, st, d, td, volume, partial, count);
   else if (mode == 2)
     hipLaunchKernelGGL(k_triples_w<2>, dim3(nblocks), dim3(256), 0, st, d, td, volume, partial, count);
   else if (mode == 3)
     hipLaunchKernelGGL(k_triples_w<3>, dim3(nblocks), dim3(256), 0, st, d, td, volume, partial, count);
-  else if (td.vhppp && td.vhhhp && variant == 1)
+  else if (td.vhppp && td.vhhhp && (variant == 1 || variant == 3))
     hipLaunchKernelGGL(k_triples_w<0>, dim3(nblocks), dim3(256), 0, st, d, td, volume, partial, count);
   else if (td.vhppp && td.vhhhp)
     hipLaunchKernelGGL(k_triples<true>, dim3(nblocks), dim3(256), 0, st, d, td, volume, partial, count);

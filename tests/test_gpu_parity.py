@@ -423,3 +423,23 @@ minnesota, 8, 8, 1e-8, 100, PREC, 1.0
     ref = [0.017, 0.79, float(g["kf"]), 0.16, g["e0"].real / A, (g["e0"] + g["energies"][0]).real / A,
            g["eccd"].real / A, g["eccdt"].real / A]
     assert np.allclose(vals, ref, atol=2e-5), (vals, ref)
+
+
+@pytest.mark.parametrize("matter", ["pnm", "snm"])
+def test_triples_staged_kernel_equals_table_kernel(ccnm, monkeypatch, matter):
+    """(T) of the staged kernel (F-term amplitudes from a per-call table, H-term
+    rows of T in LDS) against the table kernel that gathers every amplitude:
+    the same 18 terms in the same order per element, so the same bits."""
+    A = 66 if matter == "pnm" else 28
+    s = ccnm.Solver(A=A, matter=ccnm.PNM if matter == "pnm" else ccnm.SNM, rho=0.16, nmax=2)
+    s.set_chiral(ccnm.NNLO, 450.0)
+    s.generate_interaction(ccnm.INT_CHIRAL)
+    s.setup_t_amplitudes()
+    assert s.ccd_iter(subspace=8, diis_step=8, tol=1e-8, maxit=100)["converged"]
+    out = {}
+    for variant in ("3", "1"):
+        monkeypatch.setenv("CCGPU_T3_KERNEL", variant)
+        out[variant] = s.pert_triples2()
+    s.close()
+    assert out["1"] == out["3"], out
+    assert out["1"] != 0
