@@ -1242,10 +1242,10 @@ int term_mask() {
 
 // One GEMM family: three launches (one per tile class), or the VALU check.
 // prec here is the arithmetic of this family: FP64, FP32, V32 (pp ladder
-// only: FP32-stored A, FP64 arithmetic) or BF16x3 (every family but the pp
-// ladder, FP64-stored A)
+// only: FP32-stored A, FP64 arithmetic) or BF16x3 (the pp ladder on the FP32
+// copy of V, the other families on FP64-stored A)
 int gemm_kind(int prec, bool a_f32, bool pp) {
-  if (pp) return prec == CCGPU_PREC_FP64 ? 3 : prec == CCGPU_PREC_V32 ? 5 : 4;
+  if (pp) return prec == CCGPU_PREC_FP64 ? 3 : prec == CCGPU_PREC_V32 ? 5 : prec == CCGPU_PREC_BF16X3 ? 9 : 4;
   if (prec == CCGPU_PREC_BF16X3) return 6;
   return prec == CCGPU_PREC_FP64 ? 0 : (a_f32 ? 2 : 1);
 }
@@ -1282,8 +1282,8 @@ void run_gemm(const GemmList &L, const Batch &B, int prec, bool a_f32, hipStream
 
 // Families in low precision under CCGPU_PREC_FP32 and CCGPU_PREC_BF16X3 (bit 0
 // X/I9 GEMM, 1 pp ladder, 2 hh ladder, 3 ring GEMMs); all by default,
-// CCGPU_FP32_FAMILIES narrows it down for precision studies.  Under BF16x3 the
-// pp ladder's bit selects the FP32 ladder on the FP32 copy of V_pppp.
+// CCGPU_FP32_FAMILIES narrows it down for precision studies.  The low-precision
+// pp ladder reads the FP32 copy of V_pppp.
 int fp32_families() {
   const char *e = getenv("CCGPU_FP32_FAMILIES");
   return e ? atoi(e) : 15;
@@ -1302,7 +1302,7 @@ void residual_launches(Batch &B, int prec, bool timing) {
   const bool low = prec == CCGPU_PREC_FP32 || prec == CCGPU_PREC_BF16X3;
   const int fam = low ? fp32_families() : 0;
   auto pf = [&](int bit) { return (fam >> bit) & 1 ? prec : CCGPU_PREC_FP64; };
-  const int pp_prec = prec == CCGPU_PREC_V32 ? CCGPU_PREC_V32 : (fam >> 1) & 1 ? CCGPU_PREC_FP32 : CCGPU_PREC_FP64;
+  const int pp_prec = prec == CCGPU_PREC_V32 ? CCGPU_PREC_V32 : pf(1);
   // the BF16x3 kernel runs its own lists (wider tiles)
   auto list = [&](GemmList &L64, GemmList &Lbf, int bit) -> const GemmList & {
     return pf(bit) == CCGPU_PREC_BF16X3 ? Lbf : L64;

@@ -1906,6 +1906,7 @@ void launch_diis_axpy(const SolveDesc *dd, int nsolve, int64_t maxT, const int *
 
 // Kernel kinds: 0 FP64, 1 FP32 (A double), 2 FP32 (A float), 3 FP64 pp ladder,
 // 4 FP32 pp ladder (A float), 5 pp ladder with A float and FP64 arithmetic,
+// 9 pp ladder on BF16x3 split products (A float);
 // 6 BF16x3 split products (A double; the thin N <= 4 class stays FP64), tile
 // shapes (1,1..6), (1,8), (2,1..4), (4,1); 7 the same with A float; 8 with A,
 // B, C and C0 float (the FP32 ring blocks).
@@ -1941,6 +1942,7 @@ static void launch_shape_lb(const GemmGroup *groups, const GemmTile *tiles, int 
 //   VK 0: FP64 V, FP64 MFMA (3 real products per complex MAC)
 //   VK 1: FP32-stored V, FP64 MFMA
 //   VK 2: FP32-stored V, FP32 MFMA (4 real products)
+//   VK 3: FP32-stored V, BF16x3 split products on bf16 MFMA (FP32 sums)
 // Waits are explicit: s_waitcnt vmcnt(n) for the chunk about to be read and
 // lgkmcnt(0) for this wave's LDS reads of the stage about to be refilled, then
 // a bare s_barrier (a workgroup fence would force vmcnt(0) and drain the
@@ -1978,7 +1980,7 @@ __global__ __launch_bounds__(256, 2) void k_ppl(const GemmGroup *__restrict__ gr
                                               const SolveDesc *__restrict__ descs,
                                               const int *__restrict__ gsolve) {
   typedef PplCfg<NB, VK> C;
-  constexpr bool VF32 = C::VF32, MF32 = VK == 2;
+  constexpr bool VF32 = C::VF32, MF32 = VK == 2 || VK == 3, BF3 = VK == 3;
   constexpr int KC = C::KC, NT = C::NT, S = C::S, IPW = C::IPW, BIW = C::BIW;
   static_assert(C::BBYTES % 4096 == 0, "T rows of a chunk in whole instructions per wave");
   typedef typename std::conditional<MF32, float, double>::type real;
@@ -2061,7 +2063,43 @@ __global__ __launch_bounds__(256, 2) void k_ppl(const GemmGroup *__restrict__ gr
     }
   };
   auto compute = [&](const char *as, const char *bs, int kvalid) {
-    if constexpr (VF32) {
+    if constexpr (BF3) {
+      // two bf16 MFMA steps of 16 complex k: lane group lg takes complex
+      // k = 16 s + 4 lg + 0..3, i.e. the FP32 pairs 8 s + 2 lg and 8 s + 2 lg + 1
+      // of its row; A and B split into bf16 hi/lo, (re, im) side by side
+#pragma unroll
+      for (int st = 0; st < 2; ++st) {
+        const int p0 = 8 * st + 2 * lg;
+        const float4 x0 = *(const float4 *)(as + row * 256 + ((p0 ^ (row & 15)) * 16));
+        const float4 x1 = *(const float4 *)(as + row * 256 + (((p0 + 1) ^ (row & 15)) * 16));
+        unsigned hh[4], ll[4];
+        bf3_split(x0.x, x0.y, hh[0], ll[0]);
+        bf3_split(x0.z, x0.w, hh[1], ll[1]);
+        bf3_split(x1.x, x1.y, hh[2], ll[2]);
+        bf3_split(x1.z, x1.w, hh[3], ll[3]);
+        const u32x4 ah = {hh[0], hh[1], hh[2], hh[3]}, al = {ll[0], ll[1], ll[2], ll[3]};
+        const bf16x8 xh = __builtin_bit_cast(bf16x8, ah), xl = __builtin_bit_cast(bf16x8, al);
+#pragma unroll
+        for (int y = 0; y < NB; ++y) {
+          unsigned bhh[4], bll[4];
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            const int kk = 16 * st + 4 * lg + q;
+            double2 b = *(const double2 *)(bs + (kk * NT + 16 * y + li) * 16);
+            if (kk >= kvalid) b = make_double2(0.0, 0.0);
+            bf3_split((float)b.x, -(float)b.y, bhh[q], bll[q]);
+          }
+          const u32x4 bh = {bhh[0], bhh[1], bhh[2], bhh[3]}, bl = {bll[0], bll[1], bll[2], bll[3]};
+          const u32x4 ih = bf3_im_pairs(bh), il = bf3_im_pairs(bl);
+          cre[y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(xl, __builtin_bit_cast(bf16x8, bh), cre[y], 0, 0, 0);
+          cre[y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(xh, __builtin_bit_cast(bf16x8, bl), cre[y], 0, 0, 0);
+          cre[y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(xh, __builtin_bit_cast(bf16x8, bh), cre[y], 0, 0, 0);
+          cim[y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(xl, __builtin_bit_cast(bf16x8, ih), cim[y], 0, 0, 0);
+          cim[y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(xh, __builtin_bit_cast(bf16x8, il), cim[y], 0, 0, 0);
+          cim[y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(xh, __builtin_bit_cast(bf16x8, ih), cim[y], 0, 0, 0);
+        }
+      }
+    } else if constexpr (VF32) {
       // MFMA step (h, t) takes k = 2 (4 h + lane group) + t: each lane reads
       // its FP32 pair once for two steps (same k order as kstep<true>)
 #pragma unroll
@@ -2306,6 +2344,8 @@ static void launch_shape_ppl8(const GemmGroup *groups, const GemmTile *tiles, in
     case 3: hipLaunchKernelGGL((k_ppl8<0>), dim3(ntiles), dim3(512), 0, st, groups, tiles, dd, gsolve); break;
     case 5: hipLaunchKernelGGL((k_ppl8<1>), dim3(ntiles), dim3(512), 0, st, groups, tiles, dd, gsolve); break;
     case 4: hipLaunchKernelGGL((k_ppl8<2>), dim3(ntiles), dim3(512), 0, st, groups, tiles, dd, gsolve); break;
+    // BF16x3: the four-wave kernel on the same 64 x 32 tiles
+    case 9: hipLaunchKernelGGL((k_ppl<2, 3>), dim3(ntiles), dim3(256), 0, st, groups, tiles, dd, gsolve); break;
     default: break;
   }
 }
@@ -2318,6 +2358,7 @@ static void launch_shape_ppl(const GemmGroup *groups, const GemmTile *tiles, int
     case 3: hipLaunchKernelGGL((k_ppl<NB, 0>), dim3(ntiles), dim3(256), 0, st, groups, tiles, dd, gsolve); break;
     case 5: hipLaunchKernelGGL((k_ppl<NB, 1>), dim3(ntiles), dim3(256), 0, st, groups, tiles, dd, gsolve); break;
     case 4: hipLaunchKernelGGL((k_ppl<NB, 2>), dim3(ntiles), dim3(256), 0, st, groups, tiles, dd, gsolve); break;
+    case 9: hipLaunchKernelGGL((k_ppl<NB, 3>), dim3(ntiles), dim3(256), 0, st, groups, tiles, dd, gsolve); break;
     default: break;
   }
 }

@@ -96,10 +96,11 @@ extern "C" {
  * V32   the pp ladder reads the complex64 copy of V_pppp (half the bytes) and
  *       computes in FP64; the rounding of V is a fixed perturbation of the
  *       problem, so the iteration and DIIS behave as in FP64
- * BF16X3 the X/I9 GEMM, the hh ladder and both ring GEMMs split every operand
- *       into two bfloat16 numbers (hi = bf16(x), lo = bf16(x - hi)) and sum
- *       hi*hi + hi*lo + lo*hi on bf16 MFMA with FP32 accumulation (each real
- *       product within 3 * 2^-16 of exact); the pp ladder as in FP32 */
+ * BF16X3 every contraction splits each operand into two bfloat16 numbers
+ *       (hi = bf16(x), lo = bf16(x - hi)) and sums hi*hi + hi*lo + lo*hi on
+ *       bf16 MFMA with FP32 accumulation (each real product within 3 * 2^-16
+ *       of exact); the pp ladder reads the complex64 copy of V_pppp and the
+ *       ring keeps its Pandya blocks in FP32 */
 #define CCGPU_PREC_FP64 0
 #define CCGPU_PREC_FP32 1
 #define CCGPU_PREC_V32 2
