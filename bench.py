@@ -47,7 +47,7 @@ DENSITIES = [0.08, 0.12, 0.16, 0.20]
 A, NMAX = 66, 4
 TOL, DEPTH, MAXIT = 1e-8, 8, 100
 TARGET_PER_N = 1e-5  # MeV per nucleon: the accuracy the fast arm must reach
-TOL_FAST = 66 * TARGET_PER_N   # MeV: |dE/N| <= 1e-5 MeV on two consecutive iterations (Anderson)
+TOL_FAST = A * TARGET_PER_N    # MeV: |dE/N| <= 1e-5 MeV on two consecutive iterations (Anderson)
 LOW_SWITCH = 3e-2    # MeV: the low-precision phase ends once every |dE| is below this
 CHIRAL_ORDER, CUTOFF = 3, 450.0   # NNLO, 450 MeV: the first line of the reference's ccm_in
 CI_DEFAULT = (-0.74, -0.49, -0.65, 0.96)   # c1..c4 (GeV^-1) of that order
@@ -479,6 +479,9 @@ def main():
                     help="the learned start's protocol instead of the bench: 200 training solves at |n_i|<=3, "
                          "50 held-out lattices at |n_i|<=4, iterations and time against MBPT2 (DIIS 8/8, 6/6, "
                          "Anderson); prints its own JSON line")
+    ap.add_argument("--profile-twists", action="store_true",
+                    help="short run for counter collection: 8 twists of PNM A=66 |n_i|<=4 at rho=0.16 in one "
+                         "batch (shared pp ladder), 6 FP64 and 6 BF16x3 iterations; no JSON line")
     ap.add_argument("--profile-run", action="store_true",
                     help="short run for counter collection: 6 FP64 iterations of the rho=0.16 lattice, "
                          "then 6 FP32 and 6 BF16x3 iterations of the four-density batch; no JSON line")
@@ -491,6 +494,20 @@ def main():
     ccnm = load_ccnm()
     if args.warm_start_study:
         print(json.dumps({"warm_start_protocol": warm_start_protocol(ccnm)}))
+        return
+    if args.profile_twists:
+        import numpy as np
+        tws = np.random.default_rng(20241203).uniform(0.0, 3.14159265358979, (8, 3))
+        sols = [lattice(ccnm, 0.16, NMAX, tw) for tw in tws]
+        out = []
+        for prec in (ccnm.PREC_FP64, ccnm.PREC_BF16X3):
+            for s in sols:
+                s.setup_t_amplitudes()
+            r = ccnm.batch_solve(sols, subspace=DEPTH, diis_step=DEPTH, tol=1e-30, maxit=6, precision=prec,
+                                 fp32_switch=0.0, use_graph=1, accel=1, mix=1.0)
+            out.append(f"{r[0]['seconds'] * 1e3:.1f} ms")
+        print(f"# profile twists: FP64 batch {out[0]} / 6 iterations, BF16x3 batch {out[1]} / 6 iterations",
+              file=sys.stderr, flush=True)
         return
     if args.profile_triples:
         t3 = triples_rate(ccnm)
