@@ -44,7 +44,7 @@ class SolveOpts(C.Structure):
     _fields_ = [("subspace", C.c_int), ("diis_step", C.c_int), ("tol", C.c_double),
                 ("maxit", C.c_int), ("precision", C.c_int), ("refine_every", C.c_int),
                 ("use_graph", C.c_int), ("verbose", C.c_int), ("fp32_switch", C.c_double),
-                ("ccdt1", C.c_int), ("accel", C.c_int), ("mix", C.c_double)]
+                ("ccdt1", C.c_int), ("accel", C.c_int), ("mix", C.c_double), ("tail", C.c_int)]
 
 
 class SolveResult(C.Structure):
@@ -158,9 +158,9 @@ def plan(A, matter, rho, nmax, twist=(0.0, 0.0, 0.0)):
 
 def default_opts(subspace=6, diis_step=6, tol=1e-6, maxit=100, precision=PREC_FP64,
                  refine_every=0, use_graph=0, verbose=0, fp32_switch=1.0, ccdt1=0,
-                 accel=ACCEL_REFERENCE, mix=0.0):
+                 accel=ACCEL_REFERENCE, mix=0.0, tail=PREC_FP64):
     return SolveOpts(subspace, diis_step, tol, maxit, precision, refine_every, use_graph, verbose,
-                     fp32_switch, int(ccdt1), int(accel), float(mix))
+                     fp32_switch, int(ccdt1), int(accel), float(mix), int(tail))
 
 
 class Solver:

@@ -1664,11 +1664,14 @@ void run_batch(Batch &B, const ccgpu_solve_opts &op, std::vector<ccgpu_solve_res
   }
   HIPCHK(hipMemcpy(B.d_descs, B.h_descs.data(), n * sizeof(SolveDesc), hipMemcpyHostToDevice));
   if (!known_precision(op.precision)) throw Error(fmt("unknown precision %d", op.precision));
-  if (op.precision != CCGPU_PREC_FP64) {
+  if (op.tail != CCGPU_PREC_FP64 && op.tail != CCGPU_PREC_V32)
+    throw Error(fmt("tail precision %d: FP64 or V32 only", op.tail));
+  if (op.precision != CCGPU_PREC_FP64 || op.tail != CCGPU_PREC_FP64) {
     for (Solve *S : B.solves)
       if (!S->desc.Vpppp32) throw Error("FP32/V32 ladder requested before the FP32 copy of Vpppp was made");
   }
   prepare_lists(B, op.precision);
+  if (op.tail != CCGPU_PREC_FP64) prepare_lists(B, op.tail);
   auto e1 = energies(B, false);
   res.assign(n, ccgpu_solve_result{});
   std::vector<std::complex<double>> dener(n, 4.0), elast = e1;
@@ -1733,10 +1736,16 @@ void run_batch(Batch &B, const ccgpu_solve_opts &op, std::vector<ccgpu_solve_res
       any = any || act;
     }
     if (!any) break;
-    int prec = CCGPU_PREC_FP64;
+    // the low-precision phase, then the tail precision (FP64, or V32)
+    int prec = op.tail;
+    bool low = false;
     if (fp32_phase) {
       prec = op.precision;
-      if (refine && it % refine == 0) prec = CCGPU_PREC_FP64;
+      low = true;
+      if (refine && it % refine == 0) {
+        prec = CCGPU_PREC_FP64;
+        low = false;
+      }
     }
     if (ccdt1)
       for (int s = 0; s < n; ++s) {
@@ -1789,7 +1798,7 @@ void run_batch(Batch &B, const ccgpu_solve_opts &op, std::vector<ccgpu_solve_res
       small[s] = std::abs(dener[s]) <= op.tol ? small[s] + 1 : 0;
       B.solves[s]->ehist.push_back(e2[s]);
       iters[s] = it;
-      if (prec != CCGPU_PREC_FP64) ++iters32[s];
+      if (low) ++iters32[s];
       maxd = std::max(maxd, std::abs(dener[s]));
       if (std::abs(dener[s]) <= op.tol) reached[s] = 1;
       if (op.verbose) printf("dener %5d %5d %16.8E  E %.12f%s\n", s, it, dener[s].real(), e2[s].real(),
@@ -2531,7 +2540,7 @@ int ccgpu_solve(int handle, const ccgpu_solve_opts *opts, ccgpu_solve_result *re
     Solve &S = get(handle);
     ensure_ready(S);
     if (!S.have_t) throw Error("no amplitudes: call ccgpu_mbpt2_start or ccgpu_set_t2 first");
-    if (opts->precision != CCGPU_PREC_FP64) ensure_f32(S, handle);
+    if (opts->precision != CCGPU_PREC_FP64 || opts->tail != CCGPU_PREC_FP64) ensure_f32(S, handle);
     Batch &B = single(handle);
     std::vector<ccgpu_solve_result> r;
     run_batch(B, *opts, r);
@@ -2559,7 +2568,7 @@ int ccgpu_batch_solve(int n, const int *handles, const ccgpu_solve_opts *opts, c
       Solve &S = get(handles[s]);
       ensure_ready(S);
       if (!S.have_t) throw Error(fmt("handle %d has no amplitudes", handles[s]));
-      if (opts->precision != CCGPU_PREC_FP64) ensure_f32(S, handles[s]);
+      if (opts->precision != CCGPU_PREC_FP64 || opts->tail != CCGPU_PREC_FP64) ensure_f32(S, handles[s]);
       ss.push_back(&S);
     }
     // the work lists, tiles and captured graphs of a handle set are kept for

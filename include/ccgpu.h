@@ -156,6 +156,11 @@ typedef struct {
                         after two consecutive iterations with |dE| <= tol */
   double mix;        /* CCGPU_ACCEL_ANDERSON: T_next = sum_j c_j (T_j + mix f_j);
                         (0, 1], 0 selects 1 */
+  int tail;          /* contractions after the low-precision phase:
+                        CCGPU_PREC_FP64 (0, default) or CCGPU_PREC_V32 (FP64
+                        arithmetic on the complex64 copy of V_pppp, a fixed
+                        rounding of the Hamiltonian by at most 2^-24 relative);
+                        the stopping test runs in the tail precision */
 } ccgpu_solve_opts;
 
 /* convergence accelerators (ccgpu_solve_opts.accel) */
