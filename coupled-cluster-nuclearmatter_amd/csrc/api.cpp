@@ -1289,6 +1289,13 @@ int fp32_families() {
   return e ? atoi(e) : 15;
 }
 
+// CCGPU_TAIL_LOW=mask (precision studies): in FP64 and V32 residuals, the
+// families of the mask (bits as CCGPU_FP32_FAMILIES) run in BF16x3
+int tail_low_mask() {
+  const char *e = getenv("CCGPU_TAIL_LOW");
+  return e ? atoi(e) : 0;
+}
+
 bool fork_enabled() {
   const char *e = getenv("CCGPU_FORK");
   return !(e && atoi(e) == 0);
@@ -1301,8 +1308,12 @@ void residual_launches(Batch &B, int prec, bool timing) {
   if (prec != CCGPU_PREC_FP64 && !B.has_f32) throw Error("FP32/V32 residual without the FP32 copy of Vpppp");
   const bool low = prec == CCGPU_PREC_FP32 || prec == CCGPU_PREC_BF16X3;
   const int fam = low ? fp32_families() : 0;
-  auto pf = [&](int bit) { return (fam >> bit) & 1 ? prec : CCGPU_PREC_FP64; };
-  const int pp_prec = prec == CCGPU_PREC_V32 ? CCGPU_PREC_V32 : pf(1);
+  const int tlow = low ? 0 : tail_low_mask();
+  auto pf = [&](int bit) {
+    if ((tlow >> bit) & 1) return (int)CCGPU_PREC_BF16X3;
+    return (fam >> bit) & 1 ? prec : (int)CCGPU_PREC_FP64;
+  };
+  const int pp_prec = (tlow >> 1) & 1 ? CCGPU_PREC_BF16X3 : prec == CCGPU_PREC_V32 ? CCGPU_PREC_V32 : pf(1);
   // the BF16x3 kernel runs its own lists (wider tiles)
   auto list = [&](GemmList &L64, GemmList &Lbf, int bit) -> const GemmList & {
     return pf(bit) == CCGPU_PREC_BF16X3 ? Lbf : L64;
