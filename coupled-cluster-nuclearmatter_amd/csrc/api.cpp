@@ -2761,6 +2761,20 @@ void build_triples(Solve &S) {
     td.vvals = dupload(vals, S.owned);
     td.vmask = cap - 1;
   }
+  // parity: with no twist, n -> -n maps the lattice onto itself with k -> -k
+  // bit for bit and the closed Fermi sea onto itself; the built-in elements
+  // are even under it (CCGPU_T3_PARITY=0 evaluates every element)
+  td.parity = 0;
+  if (S.native_v && !(getenv("CCGPU_T3_PARITY") && atoi(getenv("CCGPU_T3_PARITY")) == 0)) {
+    bool ok = true;
+    for (int x = 0; x < P.nsp && ok; ++x) {
+      const size_t idx = (((size_t)(nm - o.nx[x]) * n + (nm - o.ny[x])) * n + (nm - o.nz[x])) * 4 +
+                         ((o.sz[x] + 1) / 2) * 2 + (o.tz[x] + 1) / 2;
+      const int y = orbitof[idx];
+      ok = y >= 0 && (x < P.A) == (y < P.A) && o.kx[y] == -o.kx[x] && o.ky[y] == -o.ky[x] && o.kz[y] == -o.kz[x];
+    }
+    td.parity = ok ? 1 : 0;
+  }
   // dense <cb||kd> and <jk||lc> tables, evaluated once per interaction, when
   // they fit in CCGPU_T3_TABLE_GB (default 16 GB); otherwise the kernel
   // evaluates the elements in place

@@ -108,6 +108,11 @@ struct TriplesDesc {
   // momentum and isospin follow from conservation); vhhhp holds <jk||lc> at
   // (j A + k) np + (c-A), times 2 for the spin of l
   const double2 *vhppp, *vhhhp;
+  // 1: the lattice maps onto itself under n -> -n with k -> -k exactly (no
+  // twist) and holes onto holes, so <cb||kd> = <Pc Pb||Pk Pd> (the NN and 3NF
+  // elements are even under parity) and the table fill evaluates one of each
+  // parity pair
+  int parity;
   // normal-ordered 3NF in <cb||kd> (the reference adds it to vnn_hppp, not to
   // vnn_hhhp: ccm_iter.f90:1641-1690)
   int tnf;

@@ -161,31 +161,53 @@ __device__ __forceinline__ double2 herm_no2b(const Chiral3Dev &c, int A, int p, 
   return make_double2(v.x, -v.y);
 }
 
+// The Hermitian blocks (V_pppp, V_hhhh, W0) take herm_no2b, which evaluates
+// every element from the orientation pair_le puts first, so element (s, r) is
+// the conjugate of element (r, s) bit for bit.  One thread evaluates the pair
+// r <= s and adds to both.
+__device__ __forceinline__ void add_herm_pair(double2 *blk, int n, int r, int s, double2 v, double sg) {
+  double2 &o = blk[(int64_t)r * n + s];
+  o.x += sg * v.x;
+  o.y += sg * v.y;
+  if (r != s) {
+    double2 &t = blk[(int64_t)s * n + r];
+    t.x += sg * v.x;
+    t.y -= sg * v.y;
+  }
+}
+
+// upper-triangle index t = s (s + 1) / 2 + r, r <= s, to (r, s)
+__device__ __forceinline__ void tri_rs(int64_t t, int &r, int &s) {
+  int64_t x = (int64_t)((sqrt(8.0 * (double)t + 1.0) - 1.0) * 0.5);
+  while (x * (x + 1) / 2 > t) --x;
+  while ((x + 1) * (x + 2) / 2 <= t) ++x;
+  s = (int)x;
+  r = (int)(t - x * (x + 1) / 2);
+}
+
 __global__ __launch_bounds__(256) void k_no2b_vpppp(SolveDesc d, Chiral3Dev c) {
   const int ch = blockIdx.y;
   const int n = d.npp[ch];
-  const int64_t ne = (int64_t)n * n;
+  const int64_t ne = (int64_t)n * (n + 1) / 2;
   const int *pa = d.pp_a + d.off_pp[ch], *pb = d.pp_b + d.off_pp[ch];
   for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < ne; e += (int64_t)gridDim.x * blockDim.x) {
-    const int r = (int)(e / n), s = (int)(e % n);
+    int r, s;
+    tri_rs(e, r, s);
     const double2 v = herm_no2b(c, d.A, pb[r], pa[r], pb[s], pa[s]);   // descending orientation
-    double2 &o = d.Vpppp[d.off_vpp[ch] + e];
-    o.x += v.x;
-    o.y += v.y;
+    add_herm_pair(d.Vpppp + d.off_vpp[ch], n, r, s, v, 1.0);
   }
 }
 
 __global__ __launch_bounds__(256) void k_no2b_vhhhh(SolveDesc d, Chiral3Dev c) {
   const int ch = blockIdx.y;
   const int nh = d.nhh[ch];
-  const int64_t ne = (int64_t)nh * nh;
+  const int64_t ne = (int64_t)nh * (nh + 1) / 2;
   const int *hi = d.hh_i + d.off_hh[ch], *hj = d.hh_j + d.off_hh[ch];
   for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < ne; e += (int64_t)gridDim.x * blockDim.x) {
-    const int s = (int)(e / nh), t = (int)(e % nh);
+    int s, t;
+    tri_rs(e, s, t);
     const double2 v = herm_no2b(c, d.A, hj[s], hi[s], hj[t], hi[t]);
-    double2 &o = d.Vhhhh[d.off_vhh[ch] + e];
-    o.x += v.x;
-    o.y += v.y;
+    add_herm_pair(d.Vhhhh + d.off_vhh[ch], nh, s, t, v, 1.0);
   }
 }
 
@@ -208,15 +230,14 @@ __global__ __launch_bounds__(256) void k_no2b_vhhpp(SolveDesc d, Chiral3Dev c) {
 __global__ __launch_bounds__(256) void k_no2b_w0(SolveDesc d, Chiral3Dev c) {
   const int K = blockIdx.y;
   const int n = d.nhp[K];
-  const int64_t ne = (int64_t)n * n;
+  const int64_t ne = (int64_t)n * (n + 1) / 2;
   const int *hj = d.hp_j + d.off_hp[K], *hb = d.hp_b + d.off_hp[K];
   for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < ne; e += (int64_t)gridDim.x * blockDim.x) {
-    const int r = (int)(e / n), s = (int)(e % n);
+    int r, s;
+    tri_rs(e, r, s);
     const int i = hj[r], a = hb[r], j = hj[s], b = hb[s];
     const double2 v = herm_no2b(c, d.A, i, b, j, a);
-    double2 &o = d.W0[d.off_i4[K] + e];
-    o.x -= v.x;
-    o.y -= v.y;
+    add_herm_pair(d.W0 + d.off_i4[K], n, r, s, v, -1.0);
   }
 }
 
@@ -1773,10 +1794,10 @@ void launch_3nf(const SolveDesc &d, const Chiral3Dev &c, int approx, int nch, in
   if (approx < 1) return;
   hipLaunchKernelGGL(k_3nf_fock, dim3((d.nsp + 255) / 256), dim3(256), 0, st, d, c);
   if (approx < 2) return;
-  hipLaunchKernelGGL(k_no2b_vpppp, dim3(gx((int64_t)max_npp * max_npp), nch), dim3(256), 0, st, d, c);
+  hipLaunchKernelGGL(k_no2b_vpppp, dim3(gx((int64_t)max_npp * (max_npp + 1) / 2), nch), dim3(256), 0, st, d, c);
   hipLaunchKernelGGL(k_no2b_vhhpp, dim3(gx((int64_t)max_npp * max_nhh), nch), dim3(256), 0, st, d, c);
-  hipLaunchKernelGGL(k_no2b_vhhhh, dim3(gx((int64_t)max_nhh * max_nhh), nch), dim3(256), 0, st, d, c);
-  hipLaunchKernelGGL(k_no2b_w0, dim3(gx((int64_t)max_nhp * max_nhp), nphc), dim3(256), 0, st, d, c);
+  hipLaunchKernelGGL(k_no2b_vhhhh, dim3(gx((int64_t)max_nhh * (max_nhh + 1) / 2), nch), dim3(256), 0, st, d, c);
+  hipLaunchKernelGGL(k_no2b_w0, dim3(gx((int64_t)max_nhp * (max_nhp + 1) / 2), nphc), dim3(256), 0, st, d, c);
 }
 
 void launch_derive(const SolveDesc &d, int nch, int nphc, int max_npp, int max_nhh, int max_nph,

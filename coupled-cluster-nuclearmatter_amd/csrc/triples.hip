@@ -159,6 +159,21 @@ __global__ __launch_bounds__(256) void k_t3_fill_hppp(SolveDesc d, TriplesDesc t
     while ((int64_t)(bx + 1) * bx / 2 <= q) ++bx;
     const int cx = (int)(q - (int64_t)bx * (bx - 1) / 2);
     const int b = bx + d.A, c = cx + d.A;
+    // parity image (Pc, Pb, Pk): the element with the smaller (pair, k) key of
+    // the two is evaluated and written to both places
+    int pbx = -1, pcx = -1, pk = -1;
+    if (td.parity) {
+      const int ib = orb_lookup(td, -d.nx[b], -d.ny[b], -d.nz[b], d.sz[b], d.tz[b]) - d.A;
+      const int ic = orb_lookup(td, -d.nx[c], -d.ny[c], -d.nz[c], d.sz[c], d.tz[c]) - d.A;
+      pk = orb_lookup(td, -d.nx[k], -d.ny[k], -d.nz[k], d.sz[k], d.tz[k]);
+      const int hi = max(ib, ic), lo = min(ib, ic);
+      const int64_t qi = (int64_t)hi * (hi - 1) / 2 + lo;
+      if (qi < q || (qi == q && pk < k)) continue;
+      if (qi != q || pk != k) {
+        pbx = ib;
+        pcx = ic;
+      }
+    }
     const int dd = orb_lookup(td, d.nx[c] + d.nx[b] - d.nx[k], d.ny[c] + d.ny[b] - d.ny[k],
                               d.nz[c] + d.nz[b] - d.nz[k], sd, d.tz[c] + d.tz[b] - d.tz[k]);
     double2 v = make_double2(0.0, 0.0);
@@ -173,6 +188,11 @@ __global__ __launch_bounds__(256) void k_t3_fill_hppp(SolveDesc d, TriplesDesc t
     const int64_t tail = (int64_t)k * 2 + (sd + 1) / 2;
     out[((int64_t)cx * np + bx) * per + tail] = v;
     out[((int64_t)bx * np + cx) * per + tail] = make_double2(-v.x, -v.y);
+    if (pbx >= 0) {
+      const int64_t ptail = (int64_t)pk * 2 + (sd + 1) / 2;
+      out[((int64_t)pcx * np + pbx) * per + ptail] = v;
+      out[((int64_t)pbx * np + pcx) * per + ptail] = make_double2(-v.x, -v.y);
+    }
   }
   // the diagonal c = b
   for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < (int64_t)np * per;

@@ -425,6 +425,27 @@ minnesota, 8, 8, 1e-8, 100, PREC, 1.0
     assert np.allclose(vals, ref, atol=2e-5), (vals, ref)
 
 
+def test_triples_parity_table_equals_full_table(ccnm, monkeypatch):
+    """The <cb||kd> table filled over one element of each parity pair (no
+    twist: n -> -n maps the lattice and the Fermi sea onto themselves) against
+    the table with every element evaluated, through (T) with the normal-ordered
+    3NF on SNM A=28 |n_i|<=2. The image's 3NF sum over holes runs in another
+    order, so the two agree to rounding, not bit for bit."""
+    out = {}
+    for par in ("0", "1"):
+        monkeypatch.setenv("CCGPU_T3_PARITY", par)
+        s = ccnm.Solver(A=28, matter=ccnm.SNM, rho=0.16, nmax=2)
+        s.set_chiral(ccnm.NNLO, 450.0)
+        s.set_3nf(2, 0.017, 0.79, 450.0, 3)
+        s.generate_interaction(ccnm.INT_CHIRAL)
+        s.setup_t_amplitudes()
+        assert s.ccd_iter(subspace=8, diis_step=8, tol=1e-8, maxit=100)["converged"]
+        out[par] = s.pert_triples2()
+        s.close()
+    assert out["0"] != 0
+    assert abs(out["1"] - out["0"]) <= 1e-13 * abs(out["0"]), out
+
+
 @pytest.mark.parametrize("matter", ["pnm", "snm"])
 def test_triples_staged_kernel_equals_table_kernel(ccnm, monkeypatch, matter):
     """(T) of the staged kernel (F-term amplitudes from a per-call table, H-term
