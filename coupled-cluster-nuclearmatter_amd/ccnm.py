@@ -44,7 +44,8 @@ class SolveOpts(C.Structure):
     _fields_ = [("subspace", C.c_int), ("diis_step", C.c_int), ("tol", C.c_double),
                 ("maxit", C.c_int), ("precision", C.c_int), ("refine_every", C.c_int),
                 ("use_graph", C.c_int), ("verbose", C.c_int), ("fp32_switch", C.c_double),
-                ("ccdt1", C.c_int), ("accel", C.c_int), ("mix", C.c_double), ("tail", C.c_int)]
+                ("ccdt1", C.c_int), ("accel", C.c_int), ("mix", C.c_double), ("tail", C.c_int),
+                ("confirm", C.c_int)]
 
 
 class SolveResult(C.Structure):
@@ -107,6 +108,7 @@ SIGNATURES = {
     "ccgpu_mlp_add_solve": [_I, _I, C.POINTER(C.c_longlong)],
     "ccgpu_mlp_train": [_I, _I, _I, _D, C.c_ulonglong, _DP],
     "ccgpu_mlp_warm_start": [_I, _I, _DP],
+    "ccgpu_mlp_warm_start_batch": [_I, _IP, _I],
     "ccgpu_mlp_forward": [_I, _I, _FP, _FP],
     "ccgpu_mlp_loss_grad": [_I, _I, _FP, _FP, _DP, _FP],
     "ccgpu_mlp_features": [_I, _I, _FP, _FP],
@@ -158,9 +160,9 @@ def plan(A, matter, rho, nmax, twist=(0.0, 0.0, 0.0)):
 
 def default_opts(subspace=6, diis_step=6, tol=1e-6, maxit=100, precision=PREC_FP64,
                  refine_every=0, use_graph=0, verbose=0, fp32_switch=1.0, ccdt1=0,
-                 accel=ACCEL_REFERENCE, mix=0.0, tail=PREC_FP64):
+                 accel=ACCEL_REFERENCE, mix=0.0, tail=PREC_FP64, confirm=0):
     return SolveOpts(subspace, diis_step, tol, maxit, precision, refine_every, use_graph, verbose,
-                     fp32_switch, int(ccdt1), int(accel), float(mix), int(tail))
+                     fp32_switch, int(ccdt1), int(accel), float(mix), int(tail), int(confirm))
 
 
 class Solver:
@@ -434,6 +436,10 @@ class WarmStart:
         x = np.zeros(2)
         _check(lib().ccgpu_mlp_warm_start(self.h, solver.h, _dp(x)))
         return complex(x[0], x[1])
+
+    def warm_start_batch(self, solvers):
+        hs = np.array([s.h for s in solvers], np.int32)
+        _check(lib().ccgpu_mlp_warm_start_batch(self.h, _ip(hs), len(hs)))
 
     def forward(self, x):
         x = np.ascontiguousarray(x, np.float32)

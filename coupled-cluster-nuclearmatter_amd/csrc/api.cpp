@@ -84,6 +84,8 @@ void launch_mlp_sample(int64_t *idx, int64_t B, int64_t ntot, uint64_t seed, hip
 void launch_mlp_features(const SolveDesc &d, float kf, float ef, const float tw[3], float scale, uint16_t *feat,
                          float *target, hipStream_t st);
 void launch_mlp_apply(const SolveDesc &d, const float *y, float scale, float ef, hipStream_t st);
+void launch_mlp_infer(const float *params, const SolveDesc &d, float kf, float ef, const float tw[3], float scale,
+                      hipStream_t st);
 }  // namespace ccgpu
 
 using namespace ccgpu;
@@ -705,8 +707,8 @@ struct Batch {
   hipStream_t stream = nullptr;
   // second stream (default priority) for the ring branch of the residual, and
   // the fork/join events
-  hipStream_t stream2 = nullptr;
-  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  hipStream_t stream2 = nullptr, stream3 = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr, ev_join3 = nullptr;
   hipGraph_t graph[4] = {nullptr, nullptr, nullptr, nullptr};        // per precision mode
   hipGraphExec_t gexec[4] = {nullptr, nullptr, nullptr, nullptr};
   ~Batch() {
@@ -716,7 +718,9 @@ struct Batch {
     }
     if (ev_fork) (void)hipEventDestroy(ev_fork);
     if (ev_join) (void)hipEventDestroy(ev_join);
+    if (ev_join3) (void)hipEventDestroy(ev_join3);
     if (stream2) (void)hipStreamDestroy(stream2);
+    if (stream3) (void)hipStreamDestroy(stream3);
     for (void *p : owned) (void)hipFree(p);
   }
 };
@@ -990,8 +994,10 @@ std::unique_ptr<Batch> make_batch(const std::vector<Solve *> &ss, hipStream_t st
     lists[k]->apply_env_shapes();
   }
   HIPCHK(hipStreamCreateWithFlags(&B->stream2, hipStreamNonBlocking));
+  HIPCHK(hipStreamCreateWithFlags(&B->stream3, hipStreamNonBlocking));
   HIPCHK(hipEventCreateWithFlags(&B->ev_fork, hipEventDisableTiming));
   HIPCHK(hipEventCreateWithFlags(&B->ev_join, hipEventDisableTiming));
+  HIPCHK(hipEventCreateWithFlags(&B->ev_join3, hipEventDisableTiming));
   B->d_slot = dalloc<int>(n, B->owned);
   B->d_slots = dalloc<int>((size_t)n * 16, B->owned);
   B->d_coef = dalloc<double2>((size_t)n * 16, B->owned);
@@ -1318,30 +1324,35 @@ void residual_launches(Batch &B, int prec, bool timing) {
   auto list = [&](GemmList &L64, GemmList &Lbf, int bit) -> const GemmList & {
     return pf(bit) == CCGPU_PREC_BF16X3 ? Lbf : L64;
   };
+  // The ring chain (T -> T~, I4, R~) reads only T and the ring blocks and
+  // touches R only in the final scatter, so outside the timing mode it runs on
+  // a second stream from the start of the residual (fork/join through events,
+  // also inside a captured graph): beside the one-body intermediates, whose
+  // small launches leave most of the chip idle, and beside the ladders, which
+  // stream V_pppp from HBM while the ring GEMMs keep the matrix cores busy.
+  // I6 (rows of V* against rows of T) needs neither X nor the ring either: a
+  // third stream, joined before the residual start
+  const bool fork = !timing && (terms & 8) && fork_enabled();
+  const bool fork6 = !timing && fork_enabled();
+  hipStream_t sr = fork ? B.stream2 : st;
+  hipStream_t s6 = fork6 ? B.stream3 : st;
+  if (fork || fork6) HIPCHK(hipEventRecord(B.ev_fork, st));
+  if (fork) HIPCHK(hipStreamWaitEvent(sr, B.ev_fork, 0));
+  if (fork6) HIPCHK(hipStreamWaitEvent(s6, B.ev_fork, 0));
+  // launch (and graph node) order: the one-body chain first, the ring after it
+  {
+    Timer t(timing, s6, "i6");
+    launch_i6(B.d_descs, n, B.maxNp, B.maxRows, s6);
+  }
+  if (fork6) HIPCHK(hipEventRecord(B.ev_join3, s6));
   {
     Timer t(timing, st, "gemm_x_vhhpp_t");
     run_gemm(list(B.g_x, B.g_xb, 0), B, pf(0), false, st);
   }
   {
-    Timer t(timing, st, "i7_i9_i6");
+    Timer t(timing, st, "i7_i9");
     launch_i7(B.d_descs, n, B.maxA, st);
     launch_i9(B.d_descs, n, B.maxVhh, st);
-    launch_i6(B.d_descs, n, B.maxNp, B.maxRows, st);
-  }
-  {
-    Timer t(timing, st, "resid_init_onebody");
-    launch_resid_init(B.d_descs, n, B.maxT, (terms & 1) != 0, st);
-  }
-  // The particle-particle and hole-hole ladders update R in place; the ring
-  // chain (T -> T~, I4, R~) touches R only in the final scatter.  Outside the
-  // timing mode the ring chain runs on a second stream beside the ladders
-  // (fork/join through events, also inside a captured graph): the ladder
-  // streams V_pppp from HBM while the ring GEMMs keep the matrix cores busy.
-  const bool fork = !timing && (terms & 8) && fork_enabled();
-  hipStream_t sr = fork ? B.stream2 : st;
-  if (fork) {
-    HIPCHK(hipEventRecord(B.ev_fork, st));
-    HIPCHK(hipStreamWaitEvent(sr, B.ev_fork, 0));
   }
   // BF16x3 ring on the FP32 ring blocks when every solve has them
   const bool ring32 = pf(3) == CCGPU_PREC_BF16X3 && B.ring32;
@@ -1360,6 +1371,11 @@ void residual_launches(Batch &B, int prec, bool timing) {
       if (ring32) run_gemm(B.g_rtf, B, pf(3), true, sr, false, 8);
       else run_gemm(list(B.g_rt, B.g_rtb, 3), B, pf(3), false, sr);
     }
+  }
+  if (fork6) HIPCHK(hipStreamWaitEvent(st, B.ev_join3, 0));
+  {
+    Timer t(timing, st, "resid_init_onebody");
+    launch_resid_init(B.d_descs, n, B.maxT, (terms & 1) != 0, st);
   }
   if ((terms & 2) && B.share.on) {
     // shared pp ladder: FP64 GEMMs on V_u, or BF16x3 on its FP32 copy in every
@@ -1655,6 +1671,7 @@ void run_batch(Batch &B, const ccgpu_solve_opts &op, std::vector<ccgpu_solve_res
   if (timing) { g_stats.clear(); g_stat_order.clear(); }
   if (op.accel != CCGPU_ACCEL_REFERENCE && op.accel != CCGPU_ACCEL_ANDERSON)
     throw Error(fmt("unknown accelerator %d", op.accel));
+  if (op.confirm < 0 || op.confirm > 4) throw Error(fmt("confirm %d: 0 (default) or 1-4", op.confirm));
   const bool aa = op.accel == CCGPU_ACCEL_ANDERSON;
   if (aa && op.ccdt1) throw Error("CCDT-1 runs with the reference update (accel 0)");
   if (aa && !(op.mix >= 0.0 && op.mix <= 1.0)) throw Error(fmt("mix %g outside [0, 1]", op.mix));
@@ -1724,10 +1741,10 @@ void run_batch(Batch &B, const ccgpu_solve_opts &op, std::vector<ccgpu_solve_res
   double prev_maxd = 1e300;
   std::vector<char> reached(n, 0), failed(n, 0);
   // stopping test: |dE| <= tol on `need` consecutive iterations.  The
-  // reference's update stops at the first (ccm_iter.f90:71-74); the Anderson
-  // update waits for a second one, since one small energy change can fall
-  // between two larger ones before the iterate has settled
-  const int need = aa ? 2 : 1;
+  // reference's update stops at the first (ccm_iter.f90:71-74); by default the
+  // Anderson update waits for a second one, since one small energy change can
+  // fall between two larger ones before the iterate has settled
+  const int need = op.confirm > 0 ? op.confirm : (aa ? 2 : 1);
   std::vector<int> small(n, 0);
   bool pending = false;   // Anderson: coefficients uploaded, next iterate not yet formed
   for (int it = 1; it <= op.maxit; ++it) {
@@ -3093,27 +3110,53 @@ int ccgpu_mlp_train(int mlp, int steps, int batch, double lr, unsigned long long
   });
 }
 
+// The learned start of one solve on stream st: one fused pass (features,
+// network, t = t_MBPT2 + scale y), or with CCGPU_MLP_FUSED=0 the three kernels
+// of the training path (same bits)
+void mlp_start(Mlp &M, Solve &S, hipStream_t st) {
+  ensure_ready(S);
+  float kf, ef, tw[3];
+  lattice_scalars(S, kf, ef, tw);
+  const char *e = getenv("CCGPU_MLP_FUSED");
+  if (e && atoi(e) == 0) {
+    const int64_t nT = S.plan.nT;
+    M.ensure_scratch(nT);
+    launch_mlp_features(S.desc, kf, ef, tw, kMlpScale, M.sfeat, nullptr, st);
+    launch_mlp_forward(M.params, M.sfeat, nT, M.sy, st);
+    launch_mlp_apply(S.desc, M.sy, kMlpScale, ef, st);
+  } else {
+    launch_mlp_infer(M.params, S.desc, kf, ef, tw, kMlpScale, st);
+  }
+  check_launch();
+  S.have_t = true;
+  S.nstep = 0;
+  S.count = 0;
+}
+
 int ccgpu_mlp_warm_start(int mlp, int handle, double *ecorr) {
   return guarded([&] {
     Mlp &M = get_mlp(mlp);
     Solve &S = get(handle);
-    ensure_ready(S);
-    const int64_t nT = S.plan.nT;
-    M.ensure_scratch(nT);
-    float kf, ef, tw[3];
-    lattice_scalars(S, kf, ef, tw);
     HIPCHK(hipStreamSynchronize(M.stream));
-    // on the solve's stream: features, network, t = t_MBPT2 + scale y
-    launch_mlp_features(S.desc, kf, ef, tw, kMlpScale, M.sfeat, nullptr, S.stream);
-    launch_mlp_forward(M.params, M.sfeat, nT, M.sy, S.stream);
-    launch_mlp_apply(S.desc, M.sy, kMlpScale, ef, S.stream);
-    check_launch();
+    // on the solve's stream
+    mlp_start(M, S, S.stream);
     Batch &B = single(handle);
     auto e = energies(B, false);
-    S.have_t = true;
-    S.nstep = 0;
-    S.count = 0;
     if (ecorr) { ecorr[0] = e[0].real(); ecorr[1] = e[0].imag(); }
+  });
+}
+
+int ccgpu_mlp_warm_start_batch(int mlp, const int *handles, int n) {
+  return guarded([&] {
+    if (n < 1 || !handles) throw Error("warm_start_batch: no solves");
+    Mlp &M = get_mlp(mlp);
+    std::vector<Solve *> ss;
+    for (int k = 0; k < n; ++k) ss.push_back(&get(handles[k]));
+    // a solve's own stream may still hold work on its T
+    for (Solve *S : ss) HIPCHK(hipStreamSynchronize(S->stream));
+    HIPCHK(hipStreamSynchronize(M.stream));
+    for (Solve *S : ss) mlp_start(M, *S, M.stream);
+    HIPCHK(hipStreamSynchronize(M.stream));
   });
 }
 

@@ -35,15 +35,29 @@ static_assert(kOffB4 + 1 == kMlpParamCount, "parameter layout");
 constexpr int kLdW = 68;   // padded LDS row (bf16 elements) of the 64-wide weight copies
 constexpr int kLdW1 = 20;  // padded LDS row of W1^T (16 inputs)
 
-__device__ __forceinline__ uint16_t f2bf(float f) {
-  uint32_t u = __float_as_uint(f);
-  u += 0x7FFFu + ((u >> 16) & 1u);
-  return (uint16_t)(u >> 16);
-}
+// round to nearest even (v_cvt_pk_bf16_f32 on gfx950)
+__device__ __forceinline__ uint16_t f2bf(float f) { return __builtin_bit_cast(uint16_t, (__bf16)f); }
 __device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float(((uint32_t)h) << 16); }
-__device__ __forceinline__ float gelu_f(float z) { return 0.5f * z * (1.0f + erff(z * 0.70710678118654752f)); }
+// erf(z / sqrt 2) and exp(-z^2 / 2) by Abramowitz and Stegun 7.1.26
+// (|erf error| <= 1.5e-7, below the BF16 rounding that follows every
+// activation): x = |z|/sqrt 2, t = 1 / (1 + p x),
+// erf(x) = 1 - t (a1 + t (a2 + t (a3 + t (a4 + t a5)))) exp(-x^2)
+__device__ __forceinline__ void erf_gauss(float z, float &erfv, float &ex) {
+  const float x = fabsf(z) * 0.70710678118654752f;
+  const float t = __builtin_amdgcn_rcpf(1.0f + 0.3275911f * x);
+  const float poly = t * (0.254829592f + t * (-0.284496736f + t * (1.421413741f + t * (-1.453152027f + t * 1.061405429f))));
+  ex = __expf(-x * x);
+  erfv = copysignf(1.0f - poly * ex, z);
+}
+__device__ __forceinline__ float gelu_f(float z) {
+  float e, ex;
+  erf_gauss(z, e, ex);
+  return 0.5f * z * (1.0f + e);
+}
 __device__ __forceinline__ float gelu_d(float z) {
-  return 0.5f * (1.0f + erff(z * 0.70710678118654752f)) + z * expf(-0.5f * z * z) * 0.39894228040143268f;
+  float e, ex;
+  erf_gauss(z, e, ex);
+  return 0.5f * (1.0f + e) + z * ex * 0.39894228040143268f;
 }
 // two BF16 values per 32-bit register: (lo, hi)
 __device__ __forceinline__ mbf4 pack4(uint32_t lo, uint32_t hi) {
@@ -147,6 +161,42 @@ __device__ __forceinline__ void mlp_back64(const uint16_t *w, const uint32_t (&d
   }
 }
 
+// Forward of one wave's 16 samples: lane l holds inputs 4(l/16)..4(l/16)+3 of
+// sample l%16; returns the output of the lane's sample (every lane of the
+// sample holds it).  Weights and biases in LDS as in MlpLds.
+__device__ __forceinline__ float mlp_forward16(const uint16_t *wt1, const float *b1, const uint16_t *wt2,
+                                               const float *b2, const uint16_t *wt3, const float *b3, const float *w4,
+                                               float b4, uint2 xin, int lane) {
+  const int r = lane & 15, q = lane >> 4;
+  uint32_t h1[4][2], h2[4][2], h3[4][2];
+  float gd[4][4];
+#pragma unroll
+  for (int mt = 0; mt < 4; ++mt) {
+    mf4 acc;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) acc[i] = b1[16 * mt + 4 * q + i];
+    acc = mfma_bf16(lds4(wt1 + (16 * mt + r) * kLdW1 + 4 * q), pack4(xin.x, xin.y), acc);
+    uint32_t o[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) o[i] = f2bf(gelu_f(acc[i]));
+    h1[mt][0] = o[0] | (o[1] << 16);
+    h1[mt][1] = o[2] | (o[3] << 16);
+  }
+  mlp_layer64<false>(wt2, b2, h1, h2, gd, lane);
+  mlp_layer64<false>(wt3, b3, h2, h3, gd, lane);
+  float part = 0.f;
+#pragma unroll
+  for (int mt = 0; mt < 4; ++mt) {
+    part += w4[16 * mt + 4 * q + 0] * bf2f((uint16_t)(h3[mt][0] & 0xFFFF));
+    part += w4[16 * mt + 4 * q + 1] * bf2f((uint16_t)(h3[mt][0] >> 16));
+    part += w4[16 * mt + 4 * q + 2] * bf2f((uint16_t)(h3[mt][1] & 0xFFFF));
+    part += w4[16 * mt + 4 * q + 3] * bf2f((uint16_t)(h3[mt][1] >> 16));
+  }
+  part += __shfl_xor(part, 16);
+  part += __shfl_xor(part, 32);
+  return part + b4;
+}
+
 // Forward (TRAIN: and backward) over samples at positions [0, npad), npad a
 // multiple of 64; position p reads sample idx[p] (or p), positions >= n are
 // padding (zero input, no loss, no gradient).  Inference writes y[p].
@@ -168,6 +218,11 @@ __global__ __launch_bounds__(256) void k_mlp_pass(const float *__restrict__ para
     // inputs: lane holds features 4q..4q+3 of its sample
     uint2 xin = make_uint2(0u, 0u);
     if (valid) xin = *reinterpret_cast<const uint2 *>(feat + smp * kMlpIn + 4 * q);
+    if constexpr (!TRAIN) {
+      const float y = mlp_forward16(L.wt1, L.b1, L.wt2, L.b2, L.wt3, L.b3, L.w4, L.b4, xin, lane);
+      if (q == 0 && valid) yout[pos] = y;
+      continue;
+    }
     float gd1[4][4], gd2[4][4], gd3[4][4];
     uint32_t h1[4][2], h2[4][2], h3[4][2];
 #pragma unroll
@@ -207,9 +262,7 @@ __global__ __launch_bounds__(256) void k_mlp_pass(const float *__restrict__ para
     part += __shfl_xor(part, 16);
     part += __shfl_xor(part, 32);
     const float y = part + L.b4;
-    if constexpr (!TRAIN) {
-      if (q == 0 && valid) yout[pos] = y;
-    } else {
+    {
       const float d = valid ? y - target[smp] : 0.f;
       if (q == 0) loss += d * d;
       const float g = 2.f * d * inv_n;
@@ -365,33 +418,91 @@ __device__ __forceinline__ void labels_of(const SolveDesc &d, int64_t u, int &a,
 //  11-13 twist angles / pi          14 25 Re t_MBPT2     15 D/e_F, D = f_ii+f_jj-f_aa-f_bb
 // With target != null also the regression target (Re t - Re t_MBPT2) w/scale of
 // the amplitudes currently in T (w = |D|/e_F).
+__device__ __forceinline__ void mlp_features_of(const SolveDesc &d, int64_t u, float kf, float ef, float tx,
+                                                float ty, float tz, float (&f)[16], double2 &t2, double &den) {
+  int a, b, i, j;
+  labels_of(d, u, a, b, i, j);
+  const double px = 0.5 * (d.kx[a] - d.kx[b]), py = 0.5 * (d.ky[a] - d.ky[b]), pz = 0.5 * (d.kz[a] - d.kz[b]);
+  const double qx = 0.5 * (d.kx[i] - d.kx[j]), qy = 0.5 * (d.ky[i] - d.ky[j]), qz = 0.5 * (d.kz[i] - d.kz[j]);
+  const double Kx = d.kx[a] + d.kx[b], Ky = d.ky[a] + d.ky[b], Kz = d.kz[a] + d.kz[b];
+  const double np_ = sqrt(px * px + py * py + pz * pz), nq = sqrt(qx * qx + qy * qy + qz * qz);
+  const double dx = px - qx, dy = py - qy, dz = pz - qz;
+  t2 = mbpt2_of(d, u, a, b, i, j);
+  den = -d.fdiag[a].x - d.fdiag[b].x + d.fdiag[i].x + d.fdiag[j].x;
+  f[0] = (float)(np_ / kf);
+  f[1] = (float)(nq / kf);
+  f[2] = (float)(sqrt(Kx * Kx + Ky * Ky + Kz * Kz) / kf);
+  f[3] = (np_ > 0.0 && nq > 0.0) ? (float)((px * qx + py * qy + pz * qz) / (np_ * nq)) : 0.f;
+  f[4] = (float)(sqrt(dx * dx + dy * dy + dz * dz) / kf);
+  f[5] = (float)d.sz[a]; f[6] = (float)d.sz[b]; f[7] = (float)d.sz[i]; f[8] = (float)d.sz[j];
+  f[9] = 0.5f * (float)(d.tz[a] + d.tz[b]);
+  f[10] = kf;
+  f[11] = tx; f[12] = ty; f[13] = tz;
+  f[14] = (float)(25.0 * t2.x);
+  f[15] = (float)(den / ef);
+}
+
 __global__ void k_mlp_features(SolveDesc d, float kf, float ef, float tx, float ty, float tz, float scale,
                                uint16_t *__restrict__ feat, float *__restrict__ target) {
   for (int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; u < d.nT; u += (int64_t)gridDim.x * blockDim.x) {
-    int a, b, i, j;
-    labels_of(d, u, a, b, i, j);
-    const double px = 0.5 * (d.kx[a] - d.kx[b]), py = 0.5 * (d.ky[a] - d.ky[b]), pz = 0.5 * (d.kz[a] - d.kz[b]);
-    const double qx = 0.5 * (d.kx[i] - d.kx[j]), qy = 0.5 * (d.ky[i] - d.ky[j]), qz = 0.5 * (d.kz[i] - d.kz[j]);
-    const double Kx = d.kx[a] + d.kx[b], Ky = d.ky[a] + d.ky[b], Kz = d.kz[a] + d.kz[b];
-    const double np_ = sqrt(px * px + py * py + pz * pz), nq = sqrt(qx * qx + qy * qy + qz * qz);
-    const double dx = px - qx, dy = py - qy, dz = pz - qz;
-    const double2 t2 = mbpt2_of(d, u, a, b, i, j);
-    const double den = -d.fdiag[a].x - d.fdiag[b].x + d.fdiag[i].x + d.fdiag[j].x;
     float f[16];
-    f[0] = (float)(np_ / kf);
-    f[1] = (float)(nq / kf);
-    f[2] = (float)(sqrt(Kx * Kx + Ky * Ky + Kz * Kz) / kf);
-    f[3] = (np_ > 0.0 && nq > 0.0) ? (float)((px * qx + py * qy + pz * qz) / (np_ * nq)) : 0.f;
-    f[4] = (float)(sqrt(dx * dx + dy * dy + dz * dz) / kf);
-    f[5] = (float)d.sz[a]; f[6] = (float)d.sz[b]; f[7] = (float)d.sz[i]; f[8] = (float)d.sz[j];
-    f[9] = 0.5f * (float)(d.tz[a] + d.tz[b]);
-    f[10] = kf;
-    f[11] = tx; f[12] = ty; f[13] = tz;
-    f[14] = (float)(25.0 * t2.x);
-    f[15] = (float)(den / ef);
+    double2 t2;
+    double den;
+    mlp_features_of(d, u, kf, ef, tx, ty, tz, f, t2, den);
 #pragma unroll
     for (int k = 0; k < 16; ++k) feat[u * 16 + k] = f2bf(f[k]);
     if (target) target[u] = (float)((d.T[u].x - t2.x) * corr_weight(den, ef) / scale);
+  }
+}
+
+// The learned start in one pass: features in registers, the network, and
+// t = t_MBPT2 + scale * y / w written to T (the bits of k_mlp_features,
+// k_mlp_pass<false> and k_mlp_apply in a row)
+struct MlpLdsF {
+  uint16_t wt1[kMlpH * kLdW1];
+  uint16_t wt2[kMlpH * kLdW], wt3[kMlpH * kLdW];
+  float b1[kMlpH], b2[kMlpH], b3[kMlpH], w4[kMlpH];
+  float b4;
+};
+__global__ __launch_bounds__(256) void k_mlp_infer(const float *__restrict__ p, SolveDesc d, float kf, float ef,
+                                                   float tx, float ty, float tz, float scale) {
+  __shared__ MlpLdsF L;
+  for (int e = threadIdx.x; e < kMlpIn * kMlpH; e += blockDim.x) {
+    const int k = e / kMlpH, o = e % kMlpH;
+    L.wt1[o * kLdW1 + k] = f2bf(p[kOffW1 + e]);
+  }
+  for (int e = threadIdx.x; e < kMlpH * kMlpH; e += blockDim.x) {
+    const int k = e / kMlpH, o = e % kMlpH;
+    L.wt2[o * kLdW + k] = f2bf(p[kOffW2 + e]);
+    L.wt3[o * kLdW + k] = f2bf(p[kOffW3 + e]);
+  }
+  for (int e = threadIdx.x; e < kMlpH; e += blockDim.x) {
+    L.b1[e] = p[kOffB1 + e]; L.b2[e] = p[kOffB2 + e]; L.b3[e] = p[kOffB3 + e]; L.w4[e] = p[kOffW4 + e];
+  }
+  if (threadIdx.x == 0) L.b4 = p[kOffB4];
+  __syncthreads();
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int r = lane & 15, q = lane >> 4;
+  const int64_t npad = (d.nT + 63) / 64 * 64;
+  for (int64_t base = (int64_t)blockIdx.x * 64; base < npad; base += (int64_t)gridDim.x * 64) {
+    const int64_t u = base + 16 * wave + r;
+    const bool valid = u < d.nT;
+    uint2 xin = make_uint2(0u, 0u);
+    double2 t2 = make_double2(0.0, 0.0);
+    double den = 0.0;
+    if (valid) {
+      float f[16];
+      mlp_features_of(d, u, kf, ef, tx, ty, tz, f, t2, den);
+      const uint32_t f0 = f2bf(f[4 * q]), f1 = f2bf(f[4 * q + 1]), f2 = f2bf(f[4 * q + 2]), f3 = f2bf(f[4 * q + 3]);
+      xin = make_uint2(f0 | (f1 << 16), f2 | (f3 << 16));
+    }
+    const float y = mlp_forward16(L.wt1, L.b1, L.wt2, L.b2, L.wt3, L.b3, L.w4, L.b4, xin, lane);
+    if (valid && q == 0) {
+      const double w = corr_weight(den, ef);
+      double2 t = t2;
+      if (w != 0.0) t.x += (double)scale * (double)y / w;
+      d.T[u] = t;
+    }
   }
 }
 
@@ -457,6 +568,13 @@ void launch_mlp_features(const SolveDesc &d, float kf, float ef, const float tw[
                          float *target, hipStream_t st) {
   const unsigned g = (unsigned)std::max<int64_t>(1, std::min<int64_t>((d.nT + 255) / 256, 8192));
   hipLaunchKernelGGL(k_mlp_features, dim3(g), dim3(256), 0, st, d, kf, ef, tw[0], tw[1], tw[2], scale, feat, target);
+}
+
+void launch_mlp_infer(const float *params, const SolveDesc &d, float kf, float ef, const float tw[3], float scale,
+                      hipStream_t st) {
+  const int64_t npad = (d.nT + 63) / 64 * 64;
+  hipLaunchKernelGGL(k_mlp_infer, dim3(mlp_grid(npad)), dim3(256), 0, st, params, d, kf, ef, tw[0], tw[1], tw[2],
+                     scale);
 }
 
 void launch_mlp_apply(const SolveDesc &d, const float *y, float scale, float ef, hipStream_t st) {

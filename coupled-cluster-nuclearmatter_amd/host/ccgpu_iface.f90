@@ -33,7 +33,7 @@ module ccgpu_iface
      real(c_double) :: fp32_switch
      integer(c_int) :: ccdt1 = 0, accel = 0
      real(c_double) :: mix = 0.d0
-     integer(c_int) :: tail = 0
+     integer(c_int) :: tail = 0, confirm = 0
   end type ccgpu_solve_opts
 
   type, bind(C) :: ccgpu_solve_result

@@ -152,8 +152,9 @@ typedef struct {
                         diis_step iterations over subspace iterates);
                         CCGPU_ACCEL_ANDERSON: Anderson/Pulay update every
                         iteration on the residual vectors f = R/D - T, with at
-                        most subspace held vectors (diis_step unused); it stops
-                        after two consecutive iterations with |dE| <= tol */
+                        most subspace held vectors (diis_step unused); by
+                        default it stops after two consecutive iterations with
+                        |dE| <= tol (confirm) */
   double mix;        /* CCGPU_ACCEL_ANDERSON: T_next = sum_j c_j (T_j + mix f_j);
                         (0, 1], 0 selects 1 */
   int tail;          /* contractions after the low-precision phase:
@@ -161,6 +162,12 @@ typedef struct {
                         arithmetic on the complex64 copy of V_pppp, a fixed
                         rounding of the Hamiltonian by at most 2^-24 relative);
                         the stopping test runs in the tail precision */
+  int confirm;       /* consecutive iterations with |dE| <= tol that end a
+                        solve, 1-4; 0 the default: 1 with
+                        CCGPU_ACCEL_REFERENCE (the reference's test,
+                        ccm_iter.f90:71-74), 2 with CCGPU_ACCEL_ANDERSON.  With
+                        a low-precision phase only FP64 (or V32 tail)
+                        iterations count */
 } ccgpu_solve_opts;
 
 /* convergence accelerators (ccgpu_solve_opts.accel) */
@@ -333,6 +340,10 @@ int ccgpu_mlp_train(int mlp, int steps, int batch, double lr, unsigned long long
 /* Start amplitudes of a handle from the network (instead of ccgpu_mbpt2_start);
  * returns the correlation energy of the start. */
 int ccgpu_mlp_warm_start(int mlp, int handle, double *ecorr);
+/* The same start for n handles in one stream, one synchronisation at the end
+ * (the start of setup_t_amplitudes, ccm_iter.f90:2329-2405, for every solve of
+ * a batch); the energies come with the solve. */
+int ccgpu_mlp_warm_start_batch(int mlp, const int *handles, int n);
 /* For checks: network output, mean-squared loss and its gradient on n host
  * samples (n x 16 features, rounded to BF16 on entry), and the features and
  * targets of a handle's amplitudes (nT x 16 and nT floats). */

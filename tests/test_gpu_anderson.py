@@ -129,3 +129,34 @@ def test_refine_every_trajectory(ccnm):
     assert r["fp32_iterations"] >= 4
     assert r["iterations"] - r["fp32_iterations"] >= r["fp32_iterations"] // 3
     assert abs(r["eccd"] - r64["eccd"]) / 66 < 1e-9
+
+
+def test_confirm_counts_consecutive_small_changes(ccnm):
+    """ccgpu_solve_opts.confirm: a solve ends at the first iteration that
+    completes `confirm` consecutive |dE| <= tol; the iterates do not depend on
+    it, so the shorter trajectories are prefixes of the longest. 0 is the
+    default (two for the Anderson update, one for the reference's)."""
+    s = native(ccnm, 66, "pnm", 0.16, 2)
+    tol = 1e-4
+    traj = {}
+    for c in (0, 1, 2, 3):
+        s.setup_t_amplitudes()
+        r = s.ccd_iter(subspace=8, diis_step=8, tol=tol, maxit=100, accel=ccnm.ACCEL_ANDERSON, mix=1.0,
+                       confirm=c)
+        assert r["converged"]
+        traj[c] = s.solve_energies()
+    longest = traj[3]
+    for c in (1, 2, 3):
+        en = traj[c]
+        assert np.array_equal(en, longest[:len(en)])
+        small = np.abs(np.diff(en)) <= tol
+        run = 0
+        for it, sm in enumerate(small, start=1):
+            run = run + 1 if sm else 0
+            if run == c:
+                break
+        assert len(en) - 1 == it, (c, len(en) - 1, it)
+    assert np.array_equal(traj[0], traj[2])
+    with pytest.raises(RuntimeError, match="confirm"):
+        s.ccd_iter(subspace=8, diis_step=8, tol=tol, maxit=100, accel=ccnm.ACCEL_ANDERSON, confirm=5)
+    s.close()

@@ -217,3 +217,38 @@ def test_warm_start_cuts_iterations_on_held_out_lattices(ccnm):
         s.close()
     print("iterations MBPT2", it_m, "learned", it_l)
     assert np.mean(it_l) < np.mean(it_m), (it_m, it_l)
+
+
+@pytest.mark.gpu
+def test_fused_and_batched_warm_start_equal_the_three_kernel_path(ccnm, monkeypatch):
+    """The fused start (features in registers, network, t = t_MBPT2 + scale y
+    in one kernel) writes the same bits as the three kernels of the training
+    path (features, network, apply; CCGPU_MLP_FUSED=0), and the batched entry
+    point the same bits as one call per solve."""
+    rng = np.random.default_rng(9)
+    net = ccnm.WarmStart(seed=4)
+    s = ccnm.Solver(A=66, matter=ccnm.PNM, rho=0.13, nmax=2, twist=(0.4, 1.3, 2.2))
+    s.generate_interaction()
+    s.setup_t_amplitudes()
+    assert s.ccd_iter(subspace=8, diis_step=8, tol=1e-8, maxit=100)["converged"]
+    net.add_solve(s)
+    net.train(100, batch=8192, lr=2e-3, seed=1)
+    sols = [ccnm.Solver(A=66, matter=ccnm.PNM, rho=rho, nmax=2, twist=tuple(rng.uniform(0, np.pi, 3)))
+            for rho in (0.09, 0.17, 0.21)]
+    for x in sols:
+        x.generate_interaction()
+    out = {}
+    for mode in ("0", "1", "batch"):
+        monkeypatch.setenv("CCGPU_MLP_FUSED", "0" if mode == "0" else "1")
+        for x in sols:
+            x.setup_t_amplitudes()   # every start overwrites the MBPT2 amplitudes
+        if mode == "batch":
+            net.warm_start_batch(sols)
+        else:
+            for x in sols:
+                net.warm_start(x)
+        out[mode] = [x.debug_fetch("T", x.dims["t2_unique"]) for x in sols]
+    for a, b, c in zip(out["0"], out["1"], out["batch"]):
+        assert np.array_equal(a, b)
+        assert np.array_equal(a, c)
+        assert np.any(a != 0)
