@@ -642,6 +642,11 @@ def main():
     roof.update({"kernel": "k_ppl<1,0> (pp ladder, FP64, one launch)", "avg_ms": round(ms64, 4), "launches": n64,
                  "alg_bytes_per_launch": bytes64, "alg_flops_per_launch": flops64,
                  "tflops": round(tfs64, 2), "arith_intensity": round(ai64, 2)})
+    # the practical streaming-read ceiling of this GPU's HBM, next to the peak
+    ceil_gbs = ccnm.hbm_read_probe(1 << 31, 10)
+    roof.update({"read_ceiling_gbs": round(ceil_gbs, 1),
+                 "read_ceiling_kernel": "k_read_probe: 16-byte loads, each workgroup a contiguous span of 2 GiB",
+                 "frac_of_read_ceiling": round(gbs64 / ceil_gbs, 4)})
     roof32 = {"kernel": "k_ppl<1,2> (pp ladder, FP32 MFMA on the FP32 copy of V)", "avg_ms": round(ms32, 4),
               "gbs": round(bytes32 / (ms32 * 1e-3) / 1e9, 1),
               "tflops": round(flops32 / (ms32 * 1e-3) / 1e12, 2)}

@@ -109,6 +109,7 @@ SIGNATURES = {
     "ccgpu_mlp_train": [_I, _I, _I, _D, C.c_ulonglong, _DP],
     "ccgpu_mlp_warm_start": [_I, _I, _DP],
     "ccgpu_mlp_warm_start_batch": [_I, _IP, _I],
+    "ccgpu_hbm_read_probe": [C.c_longlong, _I, _DP],
     "ccgpu_mlp_forward": [_I, _I, _FP, _FP],
     "ccgpu_mlp_loss_grad": [_I, _I, _FP, _FP, _DP, _FP],
     "ccgpu_mlp_features": [_I, _I, _FP, _FP],
@@ -360,6 +361,13 @@ class Solver:
         out = np.zeros((self.dims["t2_unique"], 4), np.int32)
         _check(lib().ccgpu_get_t2_labels(self.h, _ip(out)))
         return out
+
+
+def hbm_read_probe(nbytes=1 << 31, reps=10):
+    """Best streaming-read rate (GB/s) of this GPU's HBM over nbytes."""
+    x = np.zeros(1)
+    _check(lib().ccgpu_hbm_read_probe(int(nbytes), int(reps), _dp(x)))
+    return float(x[0])
 
 
 def batch_solve(solvers, **kw):

@@ -86,6 +86,7 @@ void launch_mlp_features(const SolveDesc &d, float kf, float ef, const float tw[
 void launch_mlp_apply(const SolveDesc &d, const float *y, float scale, float ef, hipStream_t st);
 void launch_mlp_infer(const float *params, const SolveDesc &d, float kf, float ef, const float tw[3], float scale,
                       hipStream_t st);
+void launch_read_probe(const double2 *a, int64_t n, double *out, int nblocks, hipStream_t st);
 }  // namespace ccgpu
 
 using namespace ccgpu;
@@ -3167,6 +3168,51 @@ int ccgpu_autotune_report(char *buf, int len) {
     buf[len - 1] = 0;
   }
   return (int)g_tune_report.size();
+}
+
+int ccgpu_hbm_read_probe(long long bytes, int reps, double *gbs) {
+  return guarded([&] {
+    if (bytes < (1ll << 20) || reps < 1 || !gbs) throw Error("hbm_read_probe: bytes >= 1 MiB, reps >= 1");
+    const int64_t n = bytes / (int64_t)sizeof(double2);
+    const int nblocks = 4096;
+    std::vector<void *> own;
+    double2 *a = dalloc<double2>((size_t)n, own);
+    double *out = dalloc<double>(nblocks, own);
+    hipStream_t st = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    double best = 1e300;
+    try {
+      HIPCHK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+      HIPCHK(hipEventCreate(&e0));
+      HIPCHK(hipEventCreate(&e1));
+      HIPCHK(hipMemsetAsync(a, 0, (size_t)n * sizeof(double2), st));
+      launch_read_probe(a, n, out, nblocks, st);   // warm-up
+      check_launch();
+      for (int r = 0; r < reps; ++r) {
+        HIPCHK(hipEventRecord(e0, st));
+        launch_read_probe(a, n, out, nblocks, st);
+        HIPCHK(hipEventRecord(e1, st));
+        HIPCHK(hipEventSynchronize(e1));
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+        best = std::min(best, (double)ms);
+      }
+      check_launch();
+    } catch (...) {
+      if (st) (void)hipStreamSynchronize(st);
+      for (void *p : own) (void)hipFree(p);
+      if (e0) (void)hipEventDestroy(e0);
+      if (e1) (void)hipEventDestroy(e1);
+      if (st) (void)hipStreamDestroy(st);
+      throw;
+    }
+    HIPCHK(hipStreamSynchronize(st));
+    for (void *p : own) (void)hipFree(p);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipStreamDestroy(st);
+    *gbs = (double)n * sizeof(double2) / (best * 1e-3) / 1e9;
+  });
 }
 
 int ccgpu_kernel_stats(char *names, int len, double *avg_ms, long long *launches, int max) {

@@ -1910,6 +1910,25 @@ void launch_scatter_values(double2 *dst, const int64_t *idx, const double2 *val,
   if (n <= 0) return;
   hipLaunchKernelGGL(k_scatter_values, egrid(n, 1), dim3(256), 0, st, dst, idx, val, n);
 }
+// Streaming read of n 16-byte elements: block b reads its contiguous span of
+// per elements, lanes 16 bytes apart; the sum is stored only when it takes an
+// impossible value, so the loads stay and nothing is written.
+__global__ __launch_bounds__(256) void k_read_probe(const double2 *__restrict__ a, int64_t n, int64_t per,
+                                                    double *out) {
+  const int64_t b0 = (int64_t)blockIdx.x * per;
+  const int64_t b1 = b0 + per < n ? b0 + per : n;
+  double acc = 0.0;
+  for (int64_t e = b0 + threadIdx.x; e < b1; e += blockDim.x) {
+    const double2 v = a[e];
+    acc += v.x + v.y;
+  }
+  if (acc == 123.456) out[blockIdx.x] = acc;
+}
+void launch_read_probe(const double2 *a, int64_t n, double *out, int nblocks, hipStream_t st) {
+  const int64_t per = (n + nblocks - 1) / nblocks;
+  hipLaunchKernelGGL(k_read_probe, dim3(nblocks), dim3(256), 0, st, a, n, per, out);
+}
+
 void launch_aa_update(const SolveDesc *dd, int nsolve, const int *slots, double beta, hipStream_t st) {
   constexpr int kAABlocks = 1024;   // partials: SolveDesc::red holds 1024 x 36
   hipLaunchKernelGGL(k_aa_update, dim3(kAABlocks, nsolve), dim3(256), 0, st, dd, slots, beta);

@@ -313,6 +313,12 @@ int ccgpu_debug_fetch(int handle, int what, double *out, long long n);
  * solve; names are written as a ';'-separated list. */
 int ccgpu_kernel_stats(char *names, int len, double *avg_ms, long long *launches, int max);
 
+/* Diagnostics, no counterpart in the reference: the streaming-read rate of
+ * this GPU's HBM (GB/s, best of reps passes of a plain 16-byte-load kernel
+ * over a fresh buffer of `bytes`, timed with HIP events), the practical
+ * ceiling next to the published peak for the HBM-bound kernels. */
+int ccgpu_hbm_read_probe(long long bytes, int reps, double *gbs);
+
 /* Tile shapes chosen by the autotuner so far, one entry per work list:
  * "famF/kindK: cC=MBxNB(time) ..."; returns the report length.  The tuner
  * times every candidate shape of every tile class of a batch's GEMM work lists

@@ -464,3 +464,12 @@ def test_triples_staged_kernel_equals_table_kernel(ccnm, monkeypatch, matter):
     s.close()
     assert out["1"] == out["3"], out
     assert out["1"] != 0
+
+
+def test_hbm_read_probe(ccnm):
+    """The HBM streaming-read probe returns a rate between a quarter of and
+    the published 8 TB/s peak, and rejects a buffer under 1 MiB."""
+    gbs = ccnm.hbm_read_probe(1 << 28, 3)
+    assert 2000.0 < gbs < 8000.0, gbs
+    with pytest.raises(RuntimeError, match="hbm_read_probe"):
+        ccnm.hbm_read_probe(1000, 1)
