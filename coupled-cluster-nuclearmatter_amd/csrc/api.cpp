@@ -48,6 +48,7 @@ void launch_i6(const SolveDesc *dd, int nsolve, int maxnp, int64_t maxrows, hipS
 void launch_i9(const SolveDesc *dd, int nsolve, int64_t maxvhh, hipStream_t st);
 void launch_resid_init(const SolveDesc *dd, int nsolve, int64_t maxT, bool onebody, hipStream_t st);
 void launch_ring_gather(const SolveDesc *dd, int nsolve, int64_t maxT, bool f32, hipStream_t st);
+void launch_t_split(const SolveDesc *dd, int nsolve, int64_t maxT, hipStream_t st);
 void launch_ring_scatter(const SolveDesc *dd, int nsolve, int64_t maxT, bool f32, hipStream_t st);
 void launch_update(const SolveDesc *dd, int nsolve, int64_t maxT, const int *slot, hipStream_t st);
 void launch_gram(const SolveDesc *dd, int nsolve, const int *slots, int nstep, hipStream_t st);
@@ -658,6 +659,8 @@ struct Batch {
   GemmList g_xb, g_hhb, g_i4b, g_rtb;   // the same groups for the BF16x3 kernel
   GemmList g_i4f, g_rtf;   // BF16x3 ring GEMMs on the FP32 ring blocks (when every solve has them)
   bool ring32 = false;
+  GemmList g_ppbf;         // BF16x3 pp ladder reading the split copy of T (when every solve has one)
+  bool ppbf = false;
   // Shared pp ladder of a batch of twists of one lattice and interaction: the
   // NN element <ab||cd> depends on the lattice labels of a, b, c, d only (the
   // twist cancels in every momentum difference), so one V_pppp over the union
@@ -928,6 +931,19 @@ std::unique_ptr<Batch> make_batch(const std::vector<Solve *> &ss, hipStream_t st
     B->g_pp32.groups[g].A = S.desc.Vpppp32 + S.off_vpp32[c];
     B->g_pp32.groups[g].lda = (S.plan.npp[c] + 1) & ~1;
   }
+  // the BF16x3 pp ladder on the split copy of T: the same groups with B moved
+  // to Tbf (16 bytes per amplitude as T, so the same offsets)
+  B->ppbf = B->has_f32;
+  for (Solve *S : ss) B->ppbf = B->ppbf && S->desc.Tbf;
+  if (B->ppbf) {
+    B->g_ppbf = B->g_pp32;
+    for (size_t g = 0; g < B->g_ppbf.groups.size(); ++g) {
+      const SolveDesc &d = ss[B->g_ppbf.gsolve[g]]->desc;
+      const double2 *q = B->g_ppbf.groups[g].B;
+      if (q < d.T || q >= d.T + ss[B->g_ppbf.gsolve[g]]->plan.nT) throw Error("pp ladder B outside T");
+      B->g_ppbf.groups[g].B = (const double2 *)(d.Tbf + (q - d.T));
+    }
+  }
   // BF16x3 lists: the groups of the X, hh and ring families, tiled for the
   // BF16x3 kernel (up to 128 columns, so a ring channel's Vt rows are read
   // once rather than once per 32 columns) and ordered channel-per-XCD
@@ -966,16 +982,16 @@ std::unique_ptr<Batch> make_batch(const std::vector<Solve *> &ss, hipStream_t st
     to32(B->g_i4f);
     to32(B->g_rtf);
   }
-  const int fams[12] = {0, 1, 2, 3, 4, 1, 0, 2, 3, 4, 3, 4};
-  GemmList *lists[12] = {&B->g_x, &B->g_pp, &B->g_hh, &B->g_i4, &B->g_rt, &B->g_pp32,
-                         &B->g_xb, &B->g_hhb, &B->g_i4b, &B->g_rtb, &B->g_i4f, &B->g_rtf};
+  const int fams[13] = {0, 1, 2, 3, 4, 1, 0, 2, 3, 4, 3, 4, 1};
+  GemmList *lists[13] = {&B->g_x, &B->g_pp, &B->g_hh, &B->g_i4, &B->g_rt, &B->g_pp32,
+                         &B->g_xb, &B->g_hhb, &B->g_i4b, &B->g_rtb, &B->g_i4f, &B->g_rtf, &B->g_ppbf};
   B->d_descs = dupload(B->h_descs, B->owned);
-  for (int k = 0; k < 12; ++k) {
+  for (int k = 0; k < 13; ++k) {
     lists[k]->fam = fams[k];
-    lists[k]->bf3 = k >= 6;
+    lists[k]->bf3 = k >= 6 && k < 12;
     lists[k]->xcd = fams[k] == 1 || k >= 6;
-    lists[k]->nosplit = lists[k]->nosplit || k >= 10;   // FP32 outputs: no FP64 partials
-    if (k >= 6) {
+    lists[k]->nosplit = lists[k]->nosplit || (k >= 10 && k < 12);   // FP32 outputs: no FP64 partials
+    if (k >= 6 && k < 12) {
       const GemmShape def[GemmList::NCLS] = {{1, 1}, {1, 2}, {1, 4}, {1, 8}, {1, 6}, {0, 4}};
       for (int c = 0; c < GemmList::NCLS; ++c) lists[k]->shape[c] = def[c];
     }
@@ -1205,15 +1221,16 @@ void refresh_shared(Batch &B) {
 // never contracts (a single-solve batch used only for its MBPT2 start or
 // energy, the FP32 ladder of an FP64 solve) are never tuned.
 void prepare_lists(Batch &B, int prec) {
-  const int kinds[12] = {0, 3, 0, 0, 0, 4, 6, 6, 6, 6, 8, 8};   // the kernel kind each list is tuned for
-  GemmList *lists[12] = {&B.g_x, &B.g_pp, &B.g_hh, &B.g_i4, &B.g_rt, &B.g_pp32, &B.g_xb, &B.g_hhb, &B.g_i4b, &B.g_rtb,
-                         &B.g_i4f, &B.g_rtf};
-  for (int k = 0; k < 12; ++k) {
+  const int kinds[13] = {0, 3, 0, 0, 0, 4, 6, 6, 6, 6, 8, 8, 10};   // the kernel kind each list is tuned for
+  GemmList *lists[13] = {&B.g_x, &B.g_pp, &B.g_hh, &B.g_i4, &B.g_rt, &B.g_pp32, &B.g_xb, &B.g_hhb, &B.g_i4b, &B.g_rtb,
+                         &B.g_i4f, &B.g_rtf, &B.g_ppbf};
+  for (int k = 0; k < 13; ++k) {
     GemmList &L = *lists[k];
     if (L.ready) continue;
     if (k == 5 && (prec == CCGPU_PREC_FP64 || !B.has_f32)) continue;
     if (k >= 6 && prec != CCGPU_PREC_BF16X3) continue;
-    if (k >= 10 && !B.ring32) continue;
+    if (k >= 10 && k < 12 && !B.ring32) continue;
+    if (k == 12 && !B.ppbf) continue;
     autotune(L, kinds[k], B.d_descs, B.stream);
     L.build_tiles();
     L.upload(B.owned);
@@ -1301,6 +1318,11 @@ int fp32_families() {
 int tail_low_mask() {
   const char *e = getenv("CCGPU_TAIL_LOW");
   return e ? atoi(e) : 0;
+}
+
+bool tsplit_enabled() {
+  const char *e = getenv("CCGPU_PPL_TSPLIT");
+  return !(e && atoi(e) == 0);
 }
 
 bool fork_enabled() {
@@ -1395,8 +1417,16 @@ void residual_launches(Batch &B, int prec, bool timing) {
       launch_share_scatter(B.d_descs, n, B.maxT, B.share.d_maps, B.share.Ru, st);
     }
   } else if (terms & 2) {
+    // BF16x3 reads T split once (k_t_split) rather than per wave and row
+    // tile; CCGPU_PPL_TSPLIT=0 keeps the in-kernel split (same bits)
+    const bool tsplit = pp_prec == CCGPU_PREC_BF16X3 && B.ppbf && tsplit_enabled();
+    if (tsplit) {
+      Timer t(timing, st, "pp_t_split");
+      launch_t_split(B.d_descs, n, B.maxT, st);
+    }
     Timer t(timing, st, "gemm_pp_ladder");
-    if (pp_prec != CCGPU_PREC_FP64) run_gemm(B.g_pp32, B, pp_prec, true, st, true);
+    if (tsplit) run_gemm(B.g_ppbf, B, pp_prec, true, st, true, 10);
+    else if (pp_prec != CCGPU_PREC_FP64) run_gemm(B.g_pp32, B, pp_prec, true, st, true);
     else run_gemm(B.g_pp, B, pp_prec, false, st, true);
   }
   if (terms & 4) {
@@ -1938,6 +1968,7 @@ void ensure_f32(Solve &S, int handle) {
   S.desc.Vt32 = dalloc<float2>(std::max<int64_t>(P.nRt, 1), S.owned);
   S.desc.I4_32 = dalloc<float2>(std::max<int64_t>(P.nI4, 1), S.owned);
   S.desc.W0_32 = dalloc<float2>(std::max<int64_t>(P.nI4, 1), S.owned);
+  S.desc.Tbf = dalloc<uint4>(std::max<int64_t>(P.nT, 1), S.owned);
   HIPCHK(hipMemset(S.desc.Tt32, 0, std::max<int64_t>(P.nRt, 1) * sizeof(float2)));
   fill_f32(S);
   HIPCHK(hipMemcpy(S.d_desc, &S.desc, sizeof(SolveDesc), hipMemcpyHostToDevice));

@@ -47,6 +47,9 @@ struct SolveDesc {
   // FP32 ring blocks of the BF16x3 residual (null until the FP32 copies are
   // made): T~, R~, I4 per iteration, V~ and W0 copied from the FP64 blocks
   float2 *Tt32, *Rt32, *I4_32, *W0_32, *Vt32;
+  // T as the BF16x3 pp ladder's B operand, split once per residual: per
+  // amplitude the bf16 hi and lo pairs of (re, -im) and of (im, re)
+  uint4 *Tbf;
   // one-body
   double2 *fdiag, *fpart;   // f(p,p) and f(p,partner(p))
   double2 *dh, *I7off, *dp, *I6off;

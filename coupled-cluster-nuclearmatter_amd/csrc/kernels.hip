@@ -1422,6 +1422,21 @@ __device__ __forceinline__ u32x4 bf3_im_pairs(u32x4 v) {
   for (int q = 0; q < 4; ++q) r[q] = ((v[q] >> 16) | (v[q] << 16)) ^ 0x8000u;
   return r;
 }
+// T split once per residual for the BF16x3 pp ladder (B operand (br, -bi)):
+// the bits the ladder's in-kernel split gives, computed once per amplitude
+// instead of once per wave and row tile
+__global__ void k_t_split(const SolveDesc *descs) {
+  const SolveDesc &d = descs[blockIdx.y];
+  if (!d.active) return;
+  for (int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; u < d.nT; u += (int64_t)gridDim.x * blockDim.x) {
+    const double2 b = d.T[u];
+    unsigned h, l;
+    bf3_split((float)b.x, -(float)b.y, h, l);
+    const u32x4 im = bf3_im_pairs(u32x4{h, l, 0u, 0u});
+    d.Tbf[u] = make_uint4(h, l, im[0], im[1]);
+  }
+}
+
 __device__ __forceinline__ void bf3_lds_sync() {
   asm volatile("" ::: "memory");
   __builtin_amdgcn_s_waitcnt((15) | (3 << 14) | (7 << 4) | (0 << 8));   // lgkmcnt(0) only
@@ -1870,6 +1885,9 @@ void launch_i9(const SolveDesc *dd, int nsolve, int64_t maxvhh, hipStream_t st) 
 void launch_resid_init(const SolveDesc *dd, int nsolve, int64_t maxT, bool onebody, hipStream_t st) {
   hipLaunchKernelGGL(k_resid_init, egrid(maxT, nsolve), dim3(256), 0, st, dd, onebody ? 1 : 0);
 }
+void launch_t_split(const SolveDesc *dd, int nsolve, int64_t maxT, hipStream_t st) {
+  hipLaunchKernelGGL(k_t_split, egrid(maxT, nsolve), dim3(256), 0, st, dd);
+}
 void launch_ring_gather(const SolveDesc *dd, int nsolve, int64_t maxT, bool f32, hipStream_t st) {
   if (f32) hipLaunchKernelGGL(k_ring_gather<true>, egrid(maxT, nsolve), dim3(256), 0, st, dd);
   else hipLaunchKernelGGL(k_ring_gather<false>, egrid(maxT, nsolve), dim3(256), 0, st, dd);
@@ -1946,7 +1964,8 @@ void launch_diis_axpy(const SolveDesc *dd, int nsolve, int64_t maxT, const int *
 
 // Kernel kinds: 0 FP64, 1 FP32 (A double), 2 FP32 (A float), 3 FP64 pp ladder,
 // 4 FP32 pp ladder (A float), 5 pp ladder with A float and FP64 arithmetic,
-// 9 pp ladder on BF16x3 split products (A float);
+// 9 pp ladder on BF16x3 split products (A float), 10 the same with B the
+// split copy of T (k_t_split);
 // 6 BF16x3 split products (A double; the thin N <= 4 class stays FP64), tile
 // shapes (1,1..6), (1,8), (2,1..4), (4,1); 7 the same with A float; 8 with A,
 // B, C and C0 float (the FP32 ring blocks).
@@ -1983,6 +2002,7 @@ static void launch_shape_lb(const GemmGroup *groups, const GemmTile *tiles, int 
 //   VK 1: FP32-stored V, FP64 MFMA
 //   VK 2: FP32-stored V, FP32 MFMA (4 real products)
 //   VK 3: FP32-stored V, BF16x3 split products on bf16 MFMA (FP32 sums)
+//   VK 4: as 3, with T read already split (k_t_split; same bits)
 // Waits are explicit: s_waitcnt vmcnt(n) for the chunk about to be read and
 // lgkmcnt(0) for this wave's LDS reads of the stage about to be refilled, then
 // a bare s_barrier (a workgroup fence would force vmcnt(0) and drain the
@@ -2020,7 +2040,7 @@ __global__ __launch_bounds__(256, 2) void k_ppl(const GemmGroup *__restrict__ gr
                                               const SolveDesc *__restrict__ descs,
                                               const int *__restrict__ gsolve) {
   typedef PplCfg<NB, VK> C;
-  constexpr bool VF32 = C::VF32, MF32 = VK == 2 || VK == 3, BF3 = VK == 3;
+  constexpr bool VF32 = C::VF32, MF32 = VK >= 2, BF3 = VK >= 3, TSPL = VK == 4;
   constexpr int KC = C::KC, NT = C::NT, S = C::S, IPW = C::IPW, BIW = C::BIW;
   static_assert(C::BBYTES % 4096 == 0, "T rows of a chunk in whole instructions per wave");
   typedef typename std::conditional<MF32, float, double>::type real;
@@ -2121,16 +2141,36 @@ __global__ __launch_bounds__(256, 2) void k_ppl(const GemmGroup *__restrict__ gr
         const bf16x8 xh = __builtin_bit_cast(bf16x8, ah), xl = __builtin_bit_cast(bf16x8, al);
 #pragma unroll
         for (int y = 0; y < NB; ++y) {
-          unsigned bhh[4], bll[4];
+          u32x4 bh, bl, ih, il;
+          if constexpr (TSPL) {
+            // masked k: the split of b = 0 as the in-kernel form makes it
+            unsigned zh, zl;
+            bf3_split(0.f, -0.f, zh, zl);
+            const u32x4 zi = bf3_im_pairs(u32x4{zh, zl, 0u, 0u});
 #pragma unroll
-          for (int q = 0; q < 4; ++q) {
-            const int kk = 16 * st + 4 * lg + q;
-            double2 b = *(const double2 *)(bs + (kk * NT + 16 * y + li) * 16);
-            if (kk >= kvalid) b = make_double2(0.0, 0.0);
-            bf3_split((float)b.x, -(float)b.y, bhh[q], bll[q]);
+            for (int q = 0; q < 4; ++q) {
+              const int kk = 16 * st + 4 * lg + q;
+              uint4 e = *(const uint4 *)(bs + (kk * NT + 16 * y + li) * 16);
+              if (kk >= kvalid) e = make_uint4(zh, zl, zi[0], zi[1]);
+              bh[q] = e.x;
+              bl[q] = e.y;
+              ih[q] = e.z;
+              il[q] = e.w;
+            }
+          } else {
+            unsigned bhh[4], bll[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+              const int kk = 16 * st + 4 * lg + q;
+              double2 b = *(const double2 *)(bs + (kk * NT + 16 * y + li) * 16);
+              if (kk >= kvalid) b = make_double2(0.0, 0.0);
+              bf3_split((float)b.x, -(float)b.y, bhh[q], bll[q]);
+            }
+            bh = u32x4{bhh[0], bhh[1], bhh[2], bhh[3]};
+            bl = u32x4{bll[0], bll[1], bll[2], bll[3]};
+            ih = bf3_im_pairs(bh);
+            il = bf3_im_pairs(bl);
           }
-          const u32x4 bh = {bhh[0], bhh[1], bhh[2], bhh[3]}, bl = {bll[0], bll[1], bll[2], bll[3]};
-          const u32x4 ih = bf3_im_pairs(bh), il = bf3_im_pairs(bl);
           cre[y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(xl, __builtin_bit_cast(bf16x8, bh), cre[y], 0, 0, 0);
           cre[y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(xh, __builtin_bit_cast(bf16x8, bl), cre[y], 0, 0, 0);
           cre[y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(xh, __builtin_bit_cast(bf16x8, bh), cre[y], 0, 0, 0);
@@ -2386,6 +2426,7 @@ static void launch_shape_ppl8(const GemmGroup *groups, const GemmTile *tiles, in
     case 4: hipLaunchKernelGGL((k_ppl8<2>), dim3(ntiles), dim3(512), 0, st, groups, tiles, dd, gsolve); break;
     // BF16x3: the four-wave kernel on the same 64 x 32 tiles
     case 9: hipLaunchKernelGGL((k_ppl<2, 3>), dim3(ntiles), dim3(256), 0, st, groups, tiles, dd, gsolve); break;
+    case 10: hipLaunchKernelGGL((k_ppl<2, 4>), dim3(ntiles), dim3(256), 0, st, groups, tiles, dd, gsolve); break;
     default: break;
   }
 }
@@ -2393,12 +2434,14 @@ static void launch_shape_ppl8(const GemmGroup *groups, const GemmTile *tiles, in
 template <int NB>
 static void launch_shape_ppl(const GemmGroup *groups, const GemmTile *tiles, int ntiles, const SolveDesc *dd,
                              const int *gsolve, int kind, hipStream_t st) {
-  // kinds of the pp ladder: 3 FP64, 4 FP32 MFMA on the FP32 copy, 5 FP32 copy with FP64 MFMA
+  // kinds of the pp ladder: 3 FP64, 4 FP32 MFMA on the FP32 copy, 5 FP32 copy with FP64 MFMA,
+  // 9 BF16x3, 10 BF16x3 on the split copy of T
   switch (kind) {
     case 3: hipLaunchKernelGGL((k_ppl<NB, 0>), dim3(ntiles), dim3(256), 0, st, groups, tiles, dd, gsolve); break;
     case 5: hipLaunchKernelGGL((k_ppl<NB, 1>), dim3(ntiles), dim3(256), 0, st, groups, tiles, dd, gsolve); break;
     case 4: hipLaunchKernelGGL((k_ppl<NB, 2>), dim3(ntiles), dim3(256), 0, st, groups, tiles, dd, gsolve); break;
     case 9: hipLaunchKernelGGL((k_ppl<NB, 3>), dim3(ntiles), dim3(256), 0, st, groups, tiles, dd, gsolve); break;
+    case 10: hipLaunchKernelGGL((k_ppl<NB, 4>), dim3(ntiles), dim3(256), 0, st, groups, tiles, dd, gsolve); break;
     default: break;
   }
 }

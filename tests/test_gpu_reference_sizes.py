@@ -339,3 +339,30 @@ def test_bf16x3_residual_within_split_product_bound(ccnm, oracle_n2, monkeypatch
     assert err.max() < 2.0**-12 * np.abs(r64).max()
     s.close()
 
+
+
+def test_bf16x3_ladder_split_copy_of_t_gives_the_same_bits(ccnm, monkeypatch):
+    """The BF16x3 pp ladder reading T split once per residual (k_t_split, kernel
+    kind 10) against the ladder that splits T itself (kind 9,
+    CCGPU_PPL_TSPLIT=0): the same residual bit for bit, whole residual and
+    ladder alone, and each run launched the form it names."""
+    s = ccnm.Solver(A=A, matter=ccnm.PNM, rho=0.16, nmax=2)
+    s.set_chiral(ccnm.NNLO, 450.0)
+    s.generate_interaction(ccnm.INT_CHIRAL)
+    s.setup_t_amplitudes()
+    s.ccd_iter(subspace=8, diis_step=8, tol=0.0, maxit=5, accel=ccnm.ACCEL_ANDERSON, mix=1.0)
+    t = s.t2_unique()
+    monkeypatch.setenv("CCGPU_TIMING", "2")
+    for terms in ("2", "15"):
+        monkeypatch.setenv("CCGPU_TERMS", terms)
+        out = {}
+        for split in ("0", "1"):
+            monkeypatch.setenv("CCGPU_PPL_TSPLIT", split)
+            s.set_t2_unique(t)
+            s.t2_intermediate(ccnm.PREC_BF16X3)
+            out[split] = s.debug_fetch("R", len(t))
+            launched = list(ccnm.kernel_stats())
+            assert any((".k10." if split == "1" else ".k9.") in k for k in launched), launched
+        assert np.array_equal(out["0"], out["1"])
+        assert np.any(out["1"] != 0)
+    s.close()
