@@ -532,7 +532,7 @@ def main():
         net = train_warm_start(ccnm)[0]
         for _ in range(3):
             t = time.perf_counter()
-            r = run_fast(ccnm, solvers, net, plan="bf16x3")
+            r = run_fast(ccnm, solvers, net, plan="bf16x3/v32")
             print(f"# profile fast arm: {(time.perf_counter() - t) * 1e3:.1f} ms, iterations "
                   f"{[x['iterations'] for x in r]}", file=sys.stderr, flush=True)
         return
