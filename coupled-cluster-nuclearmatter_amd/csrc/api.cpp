@@ -2288,6 +2288,28 @@ int ccgpu_generate_interaction(int handle, int kind) {
         c.kx = S.desc.kx; c.ky = S.desc.ky; c.kz = S.desc.kz;
         c.nx = S.desc.nx; c.ny = S.desc.ny; c.nz = S.desc.nz; c.sz = S.desc.sz; c.tz = S.desc.tz;
         c.scale = std::pow(197.326968, 6) / (P.orb.volume * P.orb.volume);
+        // parity images for the V_pppp fill: with no twist n -> -n maps the
+        // lattice onto itself with k -> -k bit for bit and the closed Fermi
+        // sea onto itself (CCGPU_3NF_PARITY=0 evaluates every element)
+        S.desc.pimg = nullptr;
+        if (!(getenv("CCGPU_3NF_PARITY") && atoi(getenv("CCGPU_3NF_PARITY")) == 0)) {
+          const Orbitals &o = P.orb;
+          std::map<std::tuple<int, int, int, int, int>, int> at;
+          for (int x = 0; x < o.nsp; ++x) at[std::make_tuple(o.nx[x], o.ny[x], o.nz[x], o.sz[x], o.tz[x])] = x;
+          std::vector<int> img(o.nsp, -1);
+          bool ok = true;
+          for (int x = 0; x < o.nsp && ok; ++x) {
+            auto it = at.find(std::make_tuple(-o.nx[x], -o.ny[x], -o.nz[x], o.sz[x], o.tz[x]));
+            if (it == at.end()) {
+              ok = false;
+              break;
+            }
+            const int y = it->second;
+            ok = (x < P.A) == (y < P.A) && o.kx[y] == -o.kx[x] && o.ky[y] == -o.ky[x] && o.kz[y] == -o.kz[x];
+            img[x] = y;
+          }
+          if (ok) S.desc.pimg = dupload(img, S.owned);
+        }
         check_launch();
         launch_3nf(S.desc, c, S.tnf, P.nch, P.nphc, P.max_npp, P.max_nhh, max_nhp, S.stream);
       }

@@ -35,6 +35,10 @@ struct SolveDesc {
   const int *i6_dptr, *i6_didx, *i6_optr, *i6_oidx;
   double2 *p6;
   const DPair *ppref, *hhref;
+  // parity image of each orbital, n -> -n (null unless the lattice maps onto
+  // itself with k -> -k bit for bit and holes onto holes): the 3NF fill of
+  // V_pppp evaluates one element of each parity pair
+  const int *pimg;
   // Pandya
   const int *nph, *nhp;
   const int64_t *off_ph, *off_hp, *off_rt, *off_i4, *off_vt;

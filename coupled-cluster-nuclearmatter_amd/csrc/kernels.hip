@@ -185,6 +185,10 @@ __device__ __forceinline__ void tri_rs(int64_t t, int &r, int &s) {
   r = (int)(t - x * (x + 1) / 2);
 }
 
+// With d.pimg the element's parity image <Pb Pa||Pd Pc> equals it (the 3NF
+// is even); the image of the stored pair r is sr times pair ri of channel
+// chi, so V(chi)[ri][si] = sr ss v.  Of the two, the element with the
+// smaller (channel, upper-triangle position) evaluates and writes both.
 __global__ __launch_bounds__(256) void k_no2b_vpppp(SolveDesc d, Chiral3Dev c) {
   const int ch = blockIdx.y;
   const int n = d.npp[ch];
@@ -193,8 +197,28 @@ __global__ __launch_bounds__(256) void k_no2b_vpppp(SolveDesc d, Chiral3Dev c) {
   for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < ne; e += (int64_t)gridDim.x * blockDim.x) {
     int r, s;
     tri_rs(e, r, s);
+    int chi = -1, ri = 0, si = 0;
+    double sg = 1.0;
+    if (d.pimg) {
+      const DPair pr = d.ppref[(int64_t)(d.pimg[pb[r]] - d.A) * d.np + (d.pimg[pa[r]] - d.A)];
+      const DPair ps = d.ppref[(int64_t)(d.pimg[pb[s]] - d.A) * d.np + (d.pimg[pa[s]] - d.A)];
+      const int pri = pr.v >> 1, psi = ps.v >> 1;   // pair index; bit 0: the orientation flips
+      const int lo = min(pri, psi), hi = max(pri, psi);
+      if (pr.ch < ch || (pr.ch == ch && (lo < r || (lo == r && hi < s)))) continue;
+      if (pr.ch != ch || lo != r || hi != s) {
+        chi = pr.ch;
+        ri = pri;
+        si = psi;
+        sg = ((pr.v ^ ps.v) & 1) ? -1.0 : 1.0;
+      }
+    }
     const double2 v = herm_no2b(c, d.A, pb[r], pa[r], pb[s], pa[s]);   // descending orientation
     add_herm_pair(d.Vpppp + d.off_vpp[ch], n, r, s, v, 1.0);
+    if (chi >= 0) {
+      const double2 w = make_double2(sg * v.x, sg * v.y);
+      if (ri <= si) add_herm_pair(d.Vpppp + d.off_vpp[chi], d.npp[chi], ri, si, w, 1.0);
+      else add_herm_pair(d.Vpppp + d.off_vpp[chi], d.npp[chi], si, ri, make_double2(w.x, -w.y), 1.0);
+    }
   }
 }
 

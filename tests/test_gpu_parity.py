@@ -446,6 +446,32 @@ def test_triples_parity_table_equals_full_table(ccnm, monkeypatch):
     assert abs(out["1"] - out["0"]) <= 1e-13 * abs(out["0"]), out
 
 
+def test_3nf_vpppp_parity_fill_equals_full_fill(ccnm, monkeypatch):
+    """V_pppp's normal-ordered 3NF filled over one element of each parity
+    pair against every element evaluated (SNM A=28 |n_i|<=2, no twist). The
+    image's sum over holes runs in another order, so the two agree to
+    rounding: the MBPT2 energy and the iterations before the first DIIS step
+    to 1e-12, and the converged energy within the stopping tolerance (the
+    DIIS system at depth 8 amplifies rounding in between)."""
+    out = {}
+    for par in ("0", "1"):
+        monkeypatch.setenv("CCGPU_3NF_PARITY", par)
+        s = ccnm.Solver(A=28, matter=ccnm.SNM, rho=0.16, nmax=2)
+        s.set_chiral(ccnm.NNLO, 450.0)
+        s.set_3nf(2, 0.017, 0.79, 450.0, 3)
+        s.generate_interaction(ccnm.INT_CHIRAL)
+        e2 = s.setup_t_amplitudes()
+        r = s.ccd_iter(subspace=8, diis_step=8, tol=1e-9, maxit=100)
+        assert r["converged"]
+        out[par] = (e2, r["iterations"], s.solve_energies(), r["eccd"])
+        s.close()
+    (m0, i0, t0, e0), (m1, i1, t1, e1) = out["0"], out["1"]
+    assert abs(m1 - m0) <= 1e-12 * abs(m0)
+    assert i1 == i0
+    assert np.max(np.abs(t1[:8] - t0[:8])) <= 1e-12 * np.max(np.abs(t0))
+    assert abs(e1 - e0) <= 1e-8
+
+
 @pytest.mark.parametrize("matter", ["pnm", "snm"])
 def test_triples_staged_kernel_equals_table_kernel(ccnm, monkeypatch, matter):
     """(T) of the staged kernel (F-term amplitudes from a per-call table, H-term
