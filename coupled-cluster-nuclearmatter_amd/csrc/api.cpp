@@ -994,6 +994,12 @@ std::unique_ptr<Batch> make_batch(const std::vector<Solve *> &ss, hipStream_t st
     if (k >= 6 && k < 12) {
       const GemmShape def[GemmList::NCLS] = {{1, 1}, {1, 2}, {1, 4}, {1, 8}, {1, 6}, {0, 4}};
       for (int c = 0; c < GemmList::NCLS; ++c) lists[k]->shape[c] = def[c];
+      // CCGPU_ONE_LAUNCH=1 (studies): every MFMA class of a BF16x3 family on
+      // 64 x 64 tiles in one launch (the thin class keeps its own)
+      if (getenv("CCGPU_ONE_LAUNCH") && atoi(getenv("CCGPU_ONE_LAUNCH")) == 1) {
+        for (int c = 0; c < GemmList::NCLS - 1; ++c) lists[k]->shape[c] = GemmShape{1, 4};
+        lists[k]->merged = true;
+      }
     }
     if (fams[k] == 1) {
       // the pp ladder on 64 x 16 tiles of the LDS-pipelined kernel (four
