@@ -5,7 +5,7 @@
 //   t(ab,ij) = t_MBPT2(ab,ij) + scale * y(features(ab,ij)),
 // and the solver then iterates to the FP64 fixed point as before.
 //
-// Network 16 -> 64 -> 64 -> 64 -> 1, GELU (erf form).  Parameters are one flat
+// Network 16 -> 64 -> 64 -> 64 -> 1, GELU (normal CDF as a polynomial, below).  Parameters are one flat
 // FP32 vector (row-major W[k][o]):
 //   W1[16][64] b1[64] W2[64][64] b2[64] W3[64][64] b3[64] w4[64] b4
 // The three hidden layers run on v_mfma_f32_16x16x16bf16 with BF16 weights and
@@ -38,26 +38,50 @@ constexpr int kLdW1 = 20;  // padded LDS row of W1^T (16 inputs)
 // round to nearest even (v_cvt_pk_bf16_f32 on gfx950)
 __device__ __forceinline__ uint16_t f2bf(float f) { return __builtin_bit_cast(uint16_t, (__bf16)f); }
 __device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float(((uint32_t)h) << 16); }
-// erf(z / sqrt 2) and exp(-z^2 / 2) by Abramowitz and Stegun 7.1.26
-// (|erf error| <= 1.5e-7, below the BF16 rounding that follows every
-// activation): x = |z|/sqrt 2, t = 1 / (1 + p x),
-// erf(x) = 1 - t (a1 + t (a2 + t (a3 + t (a4 + t a5)))) exp(-x^2)
-__device__ __forceinline__ void erf_gauss(float z, float &erfv, float &ex) {
-  const float x = fabsf(z) * 0.70710678118654752f;
-  const float t = __builtin_amdgcn_rcpf(1.0f + 0.3275911f * x);
-  const float poly = t * (0.254829592f + t * (-0.284496736f + t * (1.421413741f + t * (-1.453152027f + t * 1.061405429f))));
-  ex = __expf(-x * x);
-  erfv = copysignf(1.0f - poly * ex, z);
+// GELU(z) = z Phi(z), Phi the normal CDF, as an odd polynomial on [-4, 4]:
+// Phi(z) = 1/2 + zc P(zc^2), zc = z clamped to [-4, 4], P of degree 7.  Its
+// error is at most 5.3e-5 (at the clamp, where 1 - Phi(4) = 3.2e-5), far below
+// the BF16 rounding (2^-9 relative) that follows every activation, and it
+// takes full-rate FMAs only (no exponential or reciprocal).  Coefficients: a
+// least-squares fit on [0, 4] reweighted towards the largest error, rounded to
+// FP32 one at a time with the others refitted.
+#define PHI_C0 3.988475204e-01f
+#define PHI_C1 -6.617541611e-02f
+#define PHI_C2 9.664904326e-03f
+#define PHI_C3 -1.048214734e-03f
+#define PHI_C4 8.066917508e-05f
+#define PHI_C5 -4.101028480e-06f
+#define PHI_C6 1.217185570e-07f
+#define PHI_C7 -1.580922393e-09f
+__device__ __forceinline__ float phi_poly(float zc, float s) {
+  float p = PHI_C7;
+  p = fmaf(p, s, PHI_C6);
+  p = fmaf(p, s, PHI_C5);
+  p = fmaf(p, s, PHI_C4);
+  p = fmaf(p, s, PHI_C3);
+  p = fmaf(p, s, PHI_C2);
+  p = fmaf(p, s, PHI_C1);
+  p = fmaf(p, s, PHI_C0);
+  return p;
 }
 __device__ __forceinline__ float gelu_f(float z) {
-  float e, ex;
-  erf_gauss(z, e, ex);
-  return 0.5f * z * (1.0f + e);
+  const float zc = fminf(fmaxf(z, -4.f), 4.f);
+  return z * fmaf(zc, phi_poly(zc, zc * zc), 0.5f);
 }
+// d/dz [z Phi(z)] = Phi(z) + z Phi'(z), Phi'(z) = P(s) + 2 s P'(s) inside the
+// clamp, 0 outside
 __device__ __forceinline__ float gelu_d(float z) {
-  float e, ex;
-  erf_gauss(z, e, ex);
-  return 0.5f * (1.0f + e) + z * ex * 0.39894228040143268f;
+  const float zc = fminf(fmaxf(z, -4.f), 4.f), s = zc * zc;
+  const float p = phi_poly(zc, s);
+  float q = 7.f * PHI_C7;
+  q = fmaf(q, s, 6.f * PHI_C6);
+  q = fmaf(q, s, 5.f * PHI_C5);
+  q = fmaf(q, s, 4.f * PHI_C4);
+  q = fmaf(q, s, 3.f * PHI_C3);
+  q = fmaf(q, s, 2.f * PHI_C2);
+  q = fmaf(q, s, PHI_C1);
+  const float dphi = fabsf(z) < 4.f ? fmaf(2.f * s, q, p) : 0.f;
+  return fmaf(zc, p, 0.5f) + z * dphi;
 }
 // two BF16 values per 32-bit register: (lo, hi)
 __device__ __forceinline__ mbf4 pack4(uint32_t lo, uint32_t hi) {
@@ -455,14 +479,18 @@ __global__ void k_mlp_features(SolveDesc d, float kf, float ef, float tx, float 
   }
 }
 
-// The learned start in one pass: features in registers, the network, and
+// The learned start in one pass: features, the network, and
 // t = t_MBPT2 + scale * y / w written to T (the bits of k_mlp_features,
-// k_mlp_pass<false> and k_mlp_apply in a row)
+// k_mlp_pass<false> and k_mlp_apply in a row).  A wave takes 64 samples: lane l
+// computes the features of sample l once and leaves them in LDS; the network
+// then runs over the 64 in four groups of 16, and lane l = 16 q + r keeps the
+// output of group q (its own sample) and writes its amplitude.
 struct MlpLdsF {
   uint16_t wt1[kMlpH * kLdW1];
   uint16_t wt2[kMlpH * kLdW], wt3[kMlpH * kLdW];
   float b1[kMlpH], b2[kMlpH], b3[kMlpH], w4[kMlpH];
   float b4;
+  uint2 x[4][64][4];   // per wave: 64 samples x 16 BF16 inputs
 };
 __global__ __launch_bounds__(256) void k_mlp_infer(const float *__restrict__ p, SolveDesc d, float kf, float ef,
                                                    float tx, float ty, float tz, float scale) {
@@ -483,21 +511,33 @@ __global__ __launch_bounds__(256) void k_mlp_infer(const float *__restrict__ p, 
   __syncthreads();
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int r = lane & 15, q = lane >> 4;
-  const int64_t npad = (d.nT + 63) / 64 * 64;
-  for (int64_t base = (int64_t)blockIdx.x * 64; base < npad; base += (int64_t)gridDim.x * 64) {
-    const int64_t u = base + 16 * wave + r;
+  // every workgroup runs the same number of rounds (the barriers below)
+  for (int64_t base = (int64_t)blockIdx.x * 256; base < d.nT; base += (int64_t)gridDim.x * 256) {
+    const int64_t u = base + 64 * wave + lane;
     const bool valid = u < d.nT;
-    uint2 xin = make_uint2(0u, 0u);
     double2 t2 = make_double2(0.0, 0.0);
     double den = 0.0;
+    uint32_t fb[16];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) fb[k] = 0u;
     if (valid) {
       float f[16];
       mlp_features_of(d, u, kf, ef, tx, ty, tz, f, t2, den);
-      const uint32_t f0 = f2bf(f[4 * q]), f1 = f2bf(f[4 * q + 1]), f2 = f2bf(f[4 * q + 2]), f3 = f2bf(f[4 * q + 3]);
-      xin = make_uint2(f0 | (f1 << 16), f2 | (f3 << 16));
+#pragma unroll
+      for (int k = 0; k < 16; ++k) fb[k] = f2bf(f[k]);
     }
-    const float y = mlp_forward16(L.wt1, L.b1, L.wt2, L.b2, L.wt3, L.b3, L.w4, L.b4, xin, lane);
-    if (valid && q == 0) {
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+      L.x[wave][lane][k] = make_uint2(fb[4 * k] | (fb[4 * k + 1] << 16), fb[4 * k + 2] | (fb[4 * k + 3] << 16));
+    __syncthreads();
+    float y = 0.f;
+#pragma unroll
+    for (int g = 0; g < 4; ++g) {
+      const float yg = mlp_forward16(L.wt1, L.b1, L.wt2, L.b2, L.wt3, L.b3, L.w4, L.b4, L.x[wave][16 * g + r][q], lane);
+      if (g == q) y = yg;
+    }
+    __syncthreads();
+    if (valid) {
       const double w = corr_weight(den, ef);
       double2 t = t2;
       if (w != 0.0) t.x += (double)scale * (double)y / w;
@@ -572,9 +612,8 @@ void launch_mlp_features(const SolveDesc &d, float kf, float ef, const float tw[
 
 void launch_mlp_infer(const float *params, const SolveDesc &d, float kf, float ef, const float tw[3], float scale,
                       hipStream_t st) {
-  const int64_t npad = (d.nT + 63) / 64 * 64;
-  hipLaunchKernelGGL(k_mlp_infer, dim3(mlp_grid(npad)), dim3(256), 0, st, params, d, kf, ef, tw[0], tw[1], tw[2],
-                     scale);
+  const unsigned g = (unsigned)std::max<int64_t>(1, std::min<int64_t>((d.nT + 255) / 256, 2048));
+  hipLaunchKernelGGL(k_mlp_infer, dim3(g), dim3(256), 0, st, params, d, kf, ef, tw[0], tw[1], tw[2], scale);
 }
 
 void launch_mlp_apply(const SolveDesc &d, const float *y, float scale, float ef, hipStream_t st) {

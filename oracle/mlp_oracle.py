@@ -5,7 +5,7 @@ the product never imports it.  The network has no counterpart in the
 reference (whose start is MBPT2, setup_t_amplitudes, ccm_iter.f90:2329-2405):
 it predicts the correction of the CCD amplitude over its MBPT2 value.
 
-Network: 16 inputs -> 64 -> 64 -> 64 (GELU, exact erf form) -> 1.
+Network: 16 inputs -> 64 -> 64 -> 64 (GELU, the CDF as a polynomial) -> 1.
 Parameters are one flat float vector in this order (row-major, W[k][o] maps
 input k to output o):
     W1[16][64] b1[64] W2[64][64] b2[64] W3[64][64] b3[64] w4[64] b4[1]
@@ -48,17 +48,31 @@ def bf16(x):
     return u.astype(np.uint32).view(np.float32).astype(np.float64)
 
 
-def _erf(x):
-    from scipy.special import erf
-    return erf(x)
+# GELU(z) = z Phi(z) with the normal CDF as the kernels define it: an odd
+# polynomial on [-4, 4], Phi(z) = 1/2 + zc P(zc^2), zc = z clamped to [-4, 4]
+# (coefficients of P in FP32, lowest order first).  tests/test_mlp.py pins it
+# against the exact CDF (|error| <= 6e-5) and gelu_grad against differences.
+PHI_C = np.array([3.988475204e-01, -6.617541611e-02, 9.664904326e-03, -1.048214734e-03,
+                  8.066917508e-05, -4.101028480e-06, 1.217185570e-07, -1.580922393e-09],
+                 np.float32).astype(np.float64)
+
+
+def phi(z):
+    zc = np.clip(z, -4.0, 4.0)
+    return 0.5 + zc * np.polynomial.polynomial.polyval(zc * zc, PHI_C)
 
 
 def gelu(z):
-    return 0.5 * z * (1.0 + _erf(z / np.sqrt(2.0)))
+    return z * phi(z)
 
 
 def gelu_grad(z):
-    return 0.5 * (1.0 + _erf(z / np.sqrt(2.0))) + z * np.exp(-0.5 * z * z) / np.sqrt(2.0 * np.pi)
+    zc = np.clip(z, -4.0, 4.0)
+    s = zc * zc
+    p = np.polynomial.polynomial.polyval(s, PHI_C)
+    dp = np.polynomial.polynomial.polyval(s, PHI_C[1:] * np.arange(1, len(PHI_C)))
+    dphi = np.where(np.abs(z) < 4.0, p + 2.0 * s * dp, 0.0)
+    return phi(z) + z * dphi
 
 
 def forward(theta, x, bf=False):
