@@ -49,7 +49,8 @@ void launch_i9(const SolveDesc *dd, int nsolve, int64_t maxvhh, hipStream_t st);
 void launch_resid_init(const SolveDesc *dd, int nsolve, int64_t maxT, bool onebody, hipStream_t st);
 void launch_ring_gather(const SolveDesc *dd, int nsolve, int64_t maxT, bool f32, hipStream_t st);
 void launch_t_split(const SolveDesc *dd, int nsolve, int64_t maxT, hipStream_t st);
-void launch_ring_scatter(const SolveDesc *dd, int nsolve, int64_t maxT, bool f32, hipStream_t st);
+void launch_ring_scatter(const SolveDesc *dd, int nsolve, int64_t maxT, bool f32, bool ring, bool hh,
+                         hipStream_t st);
 void launch_update(const SolveDesc *dd, int nsolve, int64_t maxT, const int *slot, hipStream_t st);
 void launch_gram(const SolveDesc *dd, int nsolve, const int *slots, int nstep, hipStream_t st);
 void launch_aa_update(const SolveDesc *dd, int nsolve, const int *slots, double beta, hipStream_t st);
@@ -540,6 +541,7 @@ void build_device(Solve &S) {
   d.T = dalloc<double2>(P.nT, o);
   d.R = dalloc<double2>(P.nT, o);
   d.R0 = dalloc<double2>(P.nT, o);
+  d.Rh = dalloc<double2>(P.nT, o);
   d.Vhhpp = dalloc<double2>(P.nT, o);
   d.Vhhhh = dalloc<double2>(P.nVhh, o);
   d.X = dalloc<double2>(2 * P.nVhh, o);
@@ -562,6 +564,7 @@ void build_device(Solve &S) {
   d.hist = nullptr;
   S.scal = d.scal;
   HIPCHK(hipMemset(d.Tt, 0, P.nRt * sizeof(double2)));  // entries with a=b or i=j stay zero
+  HIPCHK(hipMemset(d.Rh, 0, P.nT * sizeof(double2)));
   HIPCHK(hipMemset(d.T, 0, P.nT * sizeof(double2)));
   HIPCHK(hipMemset(d.Vpppp, 0, P.nVpp * sizeof(double2)));
   HIPCHK(hipMemset(d.Vhhpp, 0, P.nT * sizeof(double2)));
@@ -581,7 +584,8 @@ void build_device(Solve &S) {
     // R += Vpppp T   (pp ladder, ccm_t2_eqn.f90:574-607)
     S.g_pp.add(mkgroup(d.Vpppp + P.off_vpp[c], d.T + P.off_t[c], d.R + P.off_t[c], d.R + P.off_t[c], np_, nh, np_, np_, nh, nh, nh, 1.0, 1.0), 0);
     // R += T I9      (hh ladder, ccm_t2_eqn.f90:614-647)
-    S.g_hh.add(mkgroup(d.T + P.off_t[c], d.X + P.nVhh + P.off_vhh[c], d.R + P.off_t[c], d.R + P.off_t[c], np_, nh, nh, nh, nh, nh, nh, 1.0, 1.0), 0);
+    // the hh ladder into Rh (beta 0), beside the pp ladder; the ring scatter adds it to R
+    S.g_hh.add(mkgroup(d.T + P.off_t[c], d.X + P.nVhh + P.off_vhh[c], d.Rh + P.off_t[c], nullptr, np_, nh, nh, nh, nh, nh, nh, 1.0, 0.0), 0);
   }
   for (int K = 0; K < P.nphc; ++K) {
     int nph = P.nph[K], nhp = P.nhp[K];
@@ -712,7 +716,7 @@ struct Batch {
   // second stream (default priority) for the ring branch of the residual, and
   // the fork/join events
   hipStream_t stream2 = nullptr, stream3 = nullptr;
-  hipEvent_t ev_fork = nullptr, ev_join = nullptr, ev_join3 = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr, ev_join3 = nullptr, ev_x = nullptr, ev_hh = nullptr;
   hipGraph_t graph[4] = {nullptr, nullptr, nullptr, nullptr};        // per precision mode
   hipGraphExec_t gexec[4] = {nullptr, nullptr, nullptr, nullptr};
   ~Batch() {
@@ -723,6 +727,8 @@ struct Batch {
     if (ev_fork) (void)hipEventDestroy(ev_fork);
     if (ev_join) (void)hipEventDestroy(ev_join);
     if (ev_join3) (void)hipEventDestroy(ev_join3);
+    if (ev_x) (void)hipEventDestroy(ev_x);
+    if (ev_hh) (void)hipEventDestroy(ev_hh);
     if (stream2) (void)hipStreamDestroy(stream2);
     if (stream3) (void)hipStreamDestroy(stream3);
     for (void *p : owned) (void)hipFree(p);
@@ -1026,6 +1032,8 @@ std::unique_ptr<Batch> make_batch(const std::vector<Solve *> &ss, hipStream_t st
   HIPCHK(hipEventCreateWithFlags(&B->ev_fork, hipEventDisableTiming));
   HIPCHK(hipEventCreateWithFlags(&B->ev_join, hipEventDisableTiming));
   HIPCHK(hipEventCreateWithFlags(&B->ev_join3, hipEventDisableTiming));
+  HIPCHK(hipEventCreateWithFlags(&B->ev_x, hipEventDisableTiming));
+  HIPCHK(hipEventCreateWithFlags(&B->ev_hh, hipEventDisableTiming));
   B->d_slot = dalloc<int>(n, B->owned);
   B->d_slots = dalloc<int>((size_t)n * 16, B->owned);
   B->d_coef = dalloc<double2>((size_t)n * 16, B->owned);
@@ -1358,18 +1366,25 @@ void residual_launches(Batch &B, int prec, bool timing) {
   auto list = [&](GemmList &L64, GemmList &Lbf, int bit) -> const GemmList & {
     return pf(bit) == CCGPU_PREC_BF16X3 ? Lbf : L64;
   };
-  // The ring chain (T -> T~, I4, R~) reads only T and the ring blocks and
-  // touches R only in the final scatter, so outside the timing mode it runs on
-  // a second stream from the start of the residual (fork/join through events,
-  // also inside a captured graph): beside the one-body intermediates, whose
-  // small launches leave most of the chip idle, and beside the ladders, which
-  // stream V_pppp from HBM while the ring GEMMs keep the matrix cores busy.
-  // I6 (rows of V* against rows of T) needs neither X nor the ring either: a
-  // third stream, joined before the residual start
+  // Three streams outside the timing mode (fork/join through events, also
+  // inside a captured graph).  The ring chain (T -> T~, I4, R~) reads only T
+  // and the ring blocks and touches R only in the final scatter: a second
+  // stream from the start of the residual, beside the one-body intermediates,
+  // whose small launches leave most of the chip idle, and beside the ladders,
+  // which stream V_pppp from HBM while the ring GEMMs keep the matrix cores
+  // busy.  A third stream takes I6 (rows of V* against rows of T) and the split
+  // copy of T for the BF16x3 pp ladder, both joined before the residual start,
+  // then the hh ladder into Rh once X is done, joined before the scatter that
+  // adds it: R = (Rh + R) + ring, the order of the single-stream sums.
   const bool fork = !timing && (terms & 8) && fork_enabled();
   const bool fork6 = !timing && fork_enabled();
   hipStream_t sr = fork ? B.stream2 : st;
   hipStream_t s6 = fork6 ? B.stream3 : st;
+  // BF16x3 ring on the FP32 ring blocks when every solve has them
+  const bool ring32 = pf(3) == CCGPU_PREC_BF16X3 && B.ring32;
+  // BF16x3 reads T split once (k_t_split) rather than per wave and row tile;
+  // CCGPU_PPL_TSPLIT=0 keeps the in-kernel split (same bits)
+  const bool tsplit = (terms & 2) && !B.share.on && pp_prec == CCGPU_PREC_BF16X3 && B.ppbf && tsplit_enabled();
   if (fork || fork6) HIPCHK(hipEventRecord(B.ev_fork, st));
   if (fork) HIPCHK(hipStreamWaitEvent(sr, B.ev_fork, 0));
   if (fork6) HIPCHK(hipStreamWaitEvent(s6, B.ev_fork, 0));
@@ -1377,6 +1392,10 @@ void residual_launches(Batch &B, int prec, bool timing) {
   {
     Timer t(timing, s6, "i6");
     launch_i6(B.d_descs, n, B.maxNp, B.maxRows, s6);
+  }
+  if (tsplit) {
+    Timer t(timing, s6, "pp_t_split");
+    launch_t_split(B.d_descs, n, B.maxT, s6);
   }
   if (fork6) HIPCHK(hipEventRecord(B.ev_join3, s6));
   {
@@ -1388,8 +1407,15 @@ void residual_launches(Batch &B, int prec, bool timing) {
     launch_i7(B.d_descs, n, B.maxA, st);
     launch_i9(B.d_descs, n, B.maxVhh, st);
   }
-  // BF16x3 ring on the FP32 ring blocks when every solve has them
-  const bool ring32 = pf(3) == CCGPU_PREC_BF16X3 && B.ring32;
+  if (terms & 4) {
+    if (fork6) {
+      HIPCHK(hipEventRecord(B.ev_x, st));
+      HIPCHK(hipStreamWaitEvent(s6, B.ev_x, 0));
+    }
+    Timer t(timing, s6, "gemm_hh_ladder");
+    run_gemm(list(B.g_hh, B.g_hhb, 2), B, pf(2), false, s6);
+  }
+  if (fork6) HIPCHK(hipEventRecord(B.ev_hh, s6));
   if (terms & 8) {
     {
       Timer t(timing, sr, "ring_gather");
@@ -1428,29 +1454,19 @@ void residual_launches(Batch &B, int prec, bool timing) {
       launch_share_scatter(B.d_descs, n, B.maxT, B.share.d_maps, B.share.Ru, st);
     }
   } else if (terms & 2) {
-    // BF16x3 reads T split once (k_t_split) rather than per wave and row
-    // tile; CCGPU_PPL_TSPLIT=0 keeps the in-kernel split (same bits)
-    const bool tsplit = pp_prec == CCGPU_PREC_BF16X3 && B.ppbf && tsplit_enabled();
-    if (tsplit) {
-      Timer t(timing, st, "pp_t_split");
-      launch_t_split(B.d_descs, n, B.maxT, st);
-    }
     Timer t(timing, st, "gemm_pp_ladder");
     if (tsplit) run_gemm(B.g_ppbf, B, pp_prec, true, st, true, 10);
     else if (pp_prec != CCGPU_PREC_FP64) run_gemm(B.g_pp32, B, pp_prec, true, st, true);
     else run_gemm(B.g_pp, B, pp_prec, false, st, true);
   }
-  if (terms & 4) {
-    Timer t(timing, st, "gemm_hh_ladder");
-    run_gemm(list(B.g_hh, B.g_hhb, 2), B, pf(2), false, st);
-  }
   if (fork) {
     HIPCHK(hipEventRecord(B.ev_join, sr));
     HIPCHK(hipStreamWaitEvent(st, B.ev_join, 0));
   }
-  if (terms & 8) {
+  if (fork6) HIPCHK(hipStreamWaitEvent(st, B.ev_hh, 0));
+  if (terms & 12) {
     Timer t(timing, st, "ring_scatter");
-    launch_ring_scatter(B.d_descs, n, B.maxT, ring32, st);
+    launch_ring_scatter(B.d_descs, n, B.maxT, ring32, (terms & 8) != 0, (terms & 4) != 0, st);
   }
 }
 

@@ -46,6 +46,7 @@ struct SolveDesc {
   const int *ring4;   // four Pandya positions per unique amplitude (every ring block below 2^31 elements)
   // amplitudes and blocks
   double2 *T, *R, *R0, *Vhhpp, *Vhhhh, *X, *Vpppp;
+  double2 *Rh;   // hh ladder T I9 of the residual, added to R by the ring scatter
   float2 *Vpppp32;
   double2 *Tt, *Rt, *I4, *W0, *Vt;
   // FP32 ring blocks of the BF16x3 residual (null until the FP32 copies are

@@ -662,28 +662,34 @@ __global__ void k_ring_gather(const SolveDesc *descs) {
 }
 
 // Ring scatter with P(ab)P(ij) (ccm_t2_eqn.f90:1133-1185), owner-computes;
-// F32: from the FP32 R~ of the BF16x3 residual (summed in FP64).
+// F32: from the FP32 R~ of the BF16x3 residual (summed in FP64).  With hh it
+// also adds the hh ladder's Rh first, R = (Rh + R) + ring: the order in which
+// the single-stream residual summed them.
 template <bool F32>
-__global__ void k_ring_scatter(const SolveDesc *descs) {
+__global__ void k_ring_scatter(const SolveDesc *descs, int ring, int hh) {
   const SolveDesc &d = descs[blockIdx.y];
   if (!d.active) return;
   for (int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; u < d.nT; u += (int64_t)gridDim.x * blockDim.x) {
-    const int4 p = *(const int4 *)(d.ring4 + 4 * u);
-    double2 r0, r1, r2, r3;
-    if constexpr (F32) {
-      const float2 a = d.Rt32[p.x], b = d.Rt32[p.y], c = d.Rt32[p.z], e = d.Rt32[p.w];
-      r0 = make_double2(a.x, a.y);
-      r1 = make_double2(b.x, b.y);
-      r2 = make_double2(c.x, c.y);
-      r3 = make_double2(e.x, e.y);
-    } else {
-      r0 = d.Rt[p.x];
-      r1 = d.Rt[p.y];
-      r2 = d.Rt[p.z];
-      r3 = d.Rt[p.w];
+    double2 r = d.R[u];
+    if (hh) r = cadd(d.Rh[u], r);
+    if (ring) {
+      const int4 p = *(const int4 *)(d.ring4 + 4 * u);
+      double2 r0, r1, r2, r3;
+      if constexpr (F32) {
+        const float2 a = d.Rt32[p.x], b = d.Rt32[p.y], c = d.Rt32[p.z], e = d.Rt32[p.w];
+        r0 = make_double2(a.x, a.y);
+        r1 = make_double2(b.x, b.y);
+        r2 = make_double2(c.x, c.y);
+        r3 = make_double2(e.x, e.y);
+      } else {
+        r0 = d.Rt[p.x];
+        r1 = d.Rt[p.y];
+        r2 = d.Rt[p.z];
+        r3 = d.Rt[p.w];
+      }
+      r = cadd(r, csub(csub(cadd(r0, r3), r1), r2));
     }
-    const double2 v = csub(csub(cadd(r0, r3), r1), r2);
-    d.R[u] = cadd(d.R[u], v);
+    d.R[u] = r;
   }
 }
 
@@ -1916,9 +1922,10 @@ void launch_ring_gather(const SolveDesc *dd, int nsolve, int64_t maxT, bool f32,
   if (f32) hipLaunchKernelGGL(k_ring_gather<true>, egrid(maxT, nsolve), dim3(256), 0, st, dd);
   else hipLaunchKernelGGL(k_ring_gather<false>, egrid(maxT, nsolve), dim3(256), 0, st, dd);
 }
-void launch_ring_scatter(const SolveDesc *dd, int nsolve, int64_t maxT, bool f32, hipStream_t st) {
-  if (f32) hipLaunchKernelGGL(k_ring_scatter<true>, egrid(maxT, nsolve), dim3(256), 0, st, dd);
-  else hipLaunchKernelGGL(k_ring_scatter<false>, egrid(maxT, nsolve), dim3(256), 0, st, dd);
+void launch_ring_scatter(const SolveDesc *dd, int nsolve, int64_t maxT, bool f32, bool ring, bool hh,
+                         hipStream_t st) {
+  if (f32) hipLaunchKernelGGL(k_ring_scatter<true>, egrid(maxT, nsolve), dim3(256), 0, st, dd, (int)ring, (int)hh);
+  else hipLaunchKernelGGL(k_ring_scatter<false>, egrid(maxT, nsolve), dim3(256), 0, st, dd, (int)ring, (int)hh);
 }
 void launch_update(const SolveDesc *dd, int nsolve, int64_t maxT, const int *slot, hipStream_t st) {
   hipLaunchKernelGGL(k_update, egrid(maxT, nsolve), dim3(256), 0, st, dd, slot);
