@@ -997,11 +997,6 @@ std::unique_ptr<Batch> make_batch(const std::vector<Solve *> &ss, hipStream_t st
     lists[k]->bf3 = k >= 6 && k < 12;
     lists[k]->xcd = fams[k] == 1 || k >= 6;
     lists[k]->nosplit = lists[k]->nosplit || (k >= 10 && k < 12);   // FP32 outputs: no FP64 partials
-    // the BF16x3 X GEMM sums its whole k range in FP32 in one tile (the error
-    // bound of the FP32 sum already counts every k): the split-K partials'
-    // reduce, thousands of tiny workgroups beside the ring gather and I6, held
-    // the residual's start back by up to 0.2 ms per batched iteration
-    if (k == 6) lists[k]->nosplit = true;
     if (k >= 6 && k < 12) {
       const GemmShape def[GemmList::NCLS] = {{1, 1}, {1, 2}, {1, 4}, {1, 8}, {1, 6}, {0, 4}};
       for (int c = 0; c < GemmList::NCLS; ++c) lists[k]->shape[c] = def[c];
