@@ -403,7 +403,7 @@ __device__ __forceinline__ void unique_labels(const SolveDesc &d, int64_t u, int
 // (row: the pair's union row, column: the solve's column range), with the
 // orientation sign of the pair there.  Positions no solve owns stay zero.
 __global__ void k_share_gather(const SolveDesc *descs, const uint32_t *const *maps, double2 *Tu) {
-  const SolveDesc &d = descs[blockIdx.y];
+  const SolveDesc d = descs[blockIdx.y];   // a copy: stores cannot change it, so no reloads
   const uint32_t *m = maps[blockIdx.y];
   for (int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; u < d.nT; u += (int64_t)gridDim.x * blockDim.x) {
     const uint32_t w = m[u];
@@ -413,7 +413,7 @@ __global__ void k_share_gather(const SolveDesc *descs, const uint32_t *const *ma
 }
 // R += the union product at each amplitude's position (active solves only)
 __global__ void k_share_scatter(const SolveDesc *descs, const uint32_t *const *maps, const double2 *Ru) {
-  const SolveDesc &d = descs[blockIdx.y];
+  const SolveDesc d = descs[blockIdx.y];   // a copy: stores cannot change it, so no reloads
   if (!d.active) return;
   const uint32_t *m = maps[blockIdx.y];
   for (int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; u < d.nT; u += (int64_t)gridDim.x * blockDim.x) {
@@ -432,7 +432,7 @@ __global__ void k_share_scatter(const SolveDesc *descs, const uint32_t *const *m
 }
 
 __global__ void k_mbpt2(const SolveDesc *descs) {
-  const SolveDesc &d = descs[blockIdx.y];
+  const SolveDesc d = descs[blockIdx.y];   // a copy: stores cannot change it, so no reloads
   for (int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; u < d.nT; u += (int64_t)gridDim.x * blockDim.x) {
     int a, b, i, j;
     amp_orbitals(d, u, a, b, i, j);
@@ -460,7 +460,7 @@ __device__ double2 block_sum(double2 v) {
 // Correlation energy 1/4 sum_ordered t(ab,ij) <ij||ab> = sum_unique t * conj(R0)
 // (ccd_energy_save, ccm_energy.f90:31-48).  Pass 1: block partials.
 __global__ void k_energy_partial(const SolveDesc *descs) {
-  const SolveDesc &d = descs[blockIdx.y];
+  const SolveDesc d = descs[blockIdx.y];   // a copy: stores cannot change it, so no reloads
   double2 acc = make_double2(0.0, 0.0);
   for (int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; u < d.nT; u += (int64_t)gridDim.x * blockDim.x)
     acc = cadd(acc, cmul(d.T[u], cconj(d.R0[u])));
@@ -470,7 +470,7 @@ __global__ void k_energy_partial(const SolveDesc *descs) {
 
 // Pass 2: fixed-order sum of nparts partials into scal[slot].
 __global__ void k_reduce_final(const SolveDesc *descs, int nparts, int stride, int nvals, int slot) {
-  const SolveDesc &d = descs[blockIdx.y];
+  const SolveDesc d = descs[blockIdx.y];   // a copy: stores cannot change it, so no reloads
   for (int v = 0; v < nvals; ++v) {
     double2 acc = make_double2(0.0, 0.0);
     for (int p = threadIdx.x; p < nparts; p += blockDim.x) acc = cadd(acc, d.red[(int64_t)p * stride + v]);
@@ -485,7 +485,7 @@ __global__ void k_reduce_final(const SolveDesc *descs, int nparts, int stride, i
 __device__ __forceinline__ double2 wave_sum(double2 v);
 
 __global__ __launch_bounds__(256) void k_i7(const SolveDesc *descs) {
-  const SolveDesc &d = descs[blockIdx.y];
+  const SolveDesc d = descs[blockIdx.y];   // a copy: stores cannot change it, so no reloads
   if (!d.active) return;
   const int lane = threadIdx.x & 63;
   const int t = blockIdx.x * 4 + (threadIdx.x >> 6);
@@ -533,7 +533,7 @@ __device__ __forceinline__ double2 cdotc(const double2 *v, const double2 *t, int
 }
 
 __global__ __launch_bounds__(256) void k_i6_rows(const SolveDesc *descs) {
-  const SolveDesc &d = descs[blockIdx.y];
+  const SolveDesc d = descs[blockIdx.y];   // a copy: stores cannot change it, so no reloads
   if (!d.active) return;
   const int l16 = threadIdx.x & 15;
   for (int64_t r = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 4; r < d.nrows;
@@ -565,7 +565,7 @@ __device__ __forceinline__ double2 wave_sum(double2 v) {
 }
 
 __global__ __launch_bounds__(256) void k_i6_reduce(const SolveDesc *descs) {
-  const SolveDesc &d = descs[blockIdx.y];
+  const SolveDesc d = descs[blockIdx.y];   // a copy: stores cannot change it, so no reloads
   if (!d.active) return;
   const int lane = threadIdx.x & 63;
   const int t = blockIdx.x * 4 + (threadIdx.x >> 6);
@@ -589,7 +589,7 @@ __global__ __launch_bounds__(256) void k_i6_reduce(const SolveDesc *descs) {
 // I9 = Vhhhh + 1/2 Vhhpp T (ccm_t2_eqn.f90:431-464, 558-560): X[0,n) holds
 // Vhhpp T (unique pairs absorb the 1/2), X[n,2n) receives I9.
 __global__ void k_i9(const SolveDesc *descs) {
-  const SolveDesc &d = descs[blockIdx.y];
+  const SolveDesc d = descs[blockIdx.y];   // a copy: stores cannot change it, so no reloads
   if (!d.active) return;
   int64_t n = d.off_vhh[d.nch];
   for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x)
@@ -600,7 +600,7 @@ __global__ void k_i9(const SolveDesc *descs) {
 //                 - P(ij) sum_k I7(k,j) t(ab,ik), diagonals removed
 // (ccm_t2_eqn.f90:81-211).  Owner-computes over unique elements.
 __global__ void k_resid_init(const SolveDesc *descs, int onebody) {
-  const SolveDesc &d = descs[blockIdx.y];
+  const SolveDesc d = descs[blockIdx.y];   // a copy: stores cannot change it, so no reloads
   if (!d.active) return;
   for (int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; u < d.nT; u += (int64_t)gridDim.x * blockDim.x) {
     int c, r, s;
@@ -640,7 +640,7 @@ __global__ void k_resid_init(const SolveDesc *descs, int onebody) {
 // are one 16-byte load.
 template <bool F32>
 __global__ void k_ring_gather(const SolveDesc *descs) {
-  const SolveDesc &d = descs[blockIdx.y];
+  const SolveDesc d = descs[blockIdx.y];   // a copy: stores cannot change it, so no reloads
   if (!d.active) return;
   for (int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; u < d.nT; u += (int64_t)gridDim.x * blockDim.x) {
     const double2 t = d.T[u];
@@ -667,7 +667,7 @@ __global__ void k_ring_gather(const SolveDesc *descs) {
 // the single-stream residual summed them.
 template <bool F32>
 __global__ void k_ring_scatter(const SolveDesc *descs, int ring, int hh) {
-  const SolveDesc &d = descs[blockIdx.y];
+  const SolveDesc d = descs[blockIdx.y];   // a copy: stores cannot change it, so no reloads
   if (!d.active) return;
   for (int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; u < d.nT; u += (int64_t)gridDim.x * blockDim.x) {
     double2 r = d.R[u];
@@ -696,7 +696,7 @@ __global__ void k_ring_scatter(const SolveDesc *descs, int ring, int hh) {
 // Jacobi step with mixing, t <- 0.75 t + 0.25 R/(dh_i+dh_j-dp_a-dp_b), and the
 // history write of diis_t1t2 (ccm_diis.f90:106-126).
 __global__ void k_update(const SolveDesc *descs, const int *slot) {
-  const SolveDesc &d = descs[blockIdx.y];
+  const SolveDesc d = descs[blockIdx.y];   // a copy: stores cannot change it, so no reloads
   if (!d.active) return;
   double2 *h = d.hist + (int64_t)slot[blockIdx.y] * d.nT;
   for (int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; u < d.nT; u += (int64_t)gridDim.x * blockDim.x) {
@@ -713,7 +713,7 @@ __global__ void k_update(const SolveDesc *descs, const int *slot) {
 // red[block*36 + idx].
 template <int NE>
 __global__ __launch_bounds__(256) void k_gram_partial(const SolveDesc *descs, const int *slots) {
-  const SolveDesc &d = descs[blockIdx.y];
+  const SolveDesc d = descs[blockIdx.y];   // a copy: stores cannot change it, so no reloads
   if (!d.active) return;
   constexpr int NP = NE * (NE + 1) / 2;
   double2 acc[NP];
@@ -797,7 +797,7 @@ __global__ __launch_bounds__(256) void k_gram_partial(const SolveDesc *descs, co
 // iteration and extrapolates every diis_step iterations.
 constexpr int kAAMax = 9;   // held vectors + the new one
 __global__ __launch_bounds__(256) void k_aa_update(const SolveDesc *descs, const int *slots, double beta) {
-  const SolveDesc &d = descs[blockIdx.y];
+  const SolveDesc d = descs[blockIdx.y];   // a copy: stores cannot change it, so no reloads
   if (!d.active) return;
   const int *sl = slots + blockIdx.y * 16;
   const int m = sl[15];
@@ -851,7 +851,7 @@ __global__ __launch_bounds__(256) void k_aa_update(const SolveDesc *descs, const
 // Next iterate of the Anderson update: T = sum_j c_j y_j over the m = sl[15]
 // held vectors (slots sl[0..m-1]); real coefficients coef[16 s + j].
 __global__ void k_aa_combine(const SolveDesc *descs, const int *slots, const double *coef) {
-  const SolveDesc &d = descs[blockIdx.y];
+  const SolveDesc d = descs[blockIdx.y];   // a copy: stores cannot change it, so no reloads
   if (!d.active) return;
   const int *sl = slots + blockIdx.y * 16;
   const double *cf = coef + blockIdx.y * 16;
@@ -865,7 +865,7 @@ __global__ void k_aa_combine(const SolveDesc *descs, const int *slots, const dou
 
 // t = sum_p c_p t_p over the first nstep-1 history vectors (ccm_diis.f90:186-212).
 __global__ void k_diis_axpy(const SolveDesc *descs, const int *slots, const double2 *coef, int nvec) {
-  const SolveDesc &d = descs[blockIdx.y];
+  const SolveDesc d = descs[blockIdx.y];   // a copy: stores cannot change it, so no reloads
   if (!d.active) return;
   const int *sl = slots + blockIdx.y * 16;
   const double2 *cf = coef + blockIdx.y * 16;
@@ -1456,7 +1456,7 @@ __device__ __forceinline__ u32x4 bf3_im_pairs(u32x4 v) {
 // the bits the ladder's in-kernel split gives, computed once per amplitude
 // instead of once per wave and row tile
 __global__ void k_t_split(const SolveDesc *descs) {
-  const SolveDesc &d = descs[blockIdx.y];
+  const SolveDesc d = descs[blockIdx.y];   // a copy: stores cannot change it, so no reloads
   if (!d.active) return;
   for (int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; u < d.nT; u += (int64_t)gridDim.x * blockDim.x) {
     const double2 b = d.T[u];
