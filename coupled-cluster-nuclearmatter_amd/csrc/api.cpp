@@ -719,7 +719,24 @@ struct Batch {
   hipEvent_t ev_fork = nullptr, ev_join = nullptr, ev_join3 = nullptr, ev_x = nullptr, ev_hh = nullptr;
   hipGraph_t graph[4] = {nullptr, nullptr, nullptr, nullptr};        // per precision mode
   hipGraphExec_t gexec[4] = {nullptr, nullptr, nullptr, nullptr};
+  // pinned staging of the small copies of every iteration, one buffer per
+  // call site; each site's copy is followed by a stream synchronisation
+  // before the host writes its buffer again
+  void *pin[8] = {};
+  size_t pin_bytes[8] = {};
+  void *pinned(int site, size_t bytes) {
+    if (pin_bytes[site] < bytes) {
+      if (pin[site]) (void)hipHostFree(pin[site]);
+      pin[site] = nullptr;
+      pin_bytes[site] = 0;
+      HIPCHK(hipHostMalloc(&pin[site], bytes, hipHostMallocDefault));
+      pin_bytes[site] = bytes;
+    }
+    return pin[site];
+  }
   ~Batch() {
+    for (void *p : pin)
+      if (p) (void)hipHostFree(p);
     for (int q = 0; q < 4; ++q) {
       if (gexec[q]) (void)hipGraphExecDestroy(gexec[q]);
       if (graph[q]) (void)hipGraphDestroy(graph[q]);
@@ -897,6 +914,21 @@ bool share_enabled();
 bool share_eligible(const std::vector<Solve *> &ss);
 void build_shared(Batch &B);
 void check_launch();
+
+// Copies of the solve loop through the batch's pinned buffers (site: one per
+// call site): host to device without a pageable staging copy; device to host
+// completes with the stream
+void h2d_pinned(Batch &B, int site, void *dst, const void *src, size_t bytes, hipStream_t st) {
+  void *p = B.pinned(site, bytes);
+  memcpy(p, src, bytes);
+  HIPCHK(hipMemcpyAsync(dst, p, bytes, hipMemcpyHostToDevice, st));
+}
+void d2h_pinned_sync(Batch &B, int site, void *dst, const void *src, size_t bytes, hipStream_t st) {
+  void *p = B.pinned(site, bytes);
+  HIPCHK(hipMemcpyAsync(p, src, bytes, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  memcpy(dst, p, bytes);
+}
 
 std::unique_ptr<Batch> make_batch(const std::vector<Solve *> &ss, hipStream_t st) {
   auto B = std::make_unique<Batch>();
@@ -1506,15 +1538,14 @@ void update_launches(Batch &B, int &count, int &nstep, int subspace, int diis_st
   if (nstep < 2) return;
   for (int s = 0; s < n; ++s)
     for (int k = 0; k < nstep; ++k) slots[(size_t)s * 16 + k] = B.solves[s]->ring[k];
-  HIPCHK(hipMemcpyAsync(B.d_slots, slots.data(), slots.size() * sizeof(int), hipMemcpyHostToDevice, st));
+  h2d_pinned(B, 4, B.d_slots, slots.data(), slots.size() * sizeof(int), st);
   std::vector<double2> &sc = B.h_scal;
   sc.assign((size_t)n * 64, double2{0, 0});
   {
     Timer t(timing, st, "diis_gram");
     launch_gram(B.d_descs, n, B.d_slots, nstep, st);
     check_launch();
-    HIPCHK(hipMemcpyAsync(sc.data(), B.d_scal, sc.size() * sizeof(double2), hipMemcpyDeviceToHost, st));
-    HIPCHK(hipStreamSynchronize(st));
+    d2h_pinned_sync(B, 3, sc.data(), B.d_scal, sc.size() * sizeof(double2), st);
   }
   std::vector<double2> &coef = B.h_coef;
   coef.assign((size_t)n * 16, double2{0, 0});
@@ -1547,8 +1578,8 @@ void update_launches(Batch &B, int &count, int &nstep, int subspace, int diis_st
     }
     for (int p = 0; p < ne; ++p) coef[(size_t)s * 16 + p] = double2{sol[p].real(), sol[p].imag()};
   }
-  HIPCHK(hipMemcpyAsync(B.d_slots, slots.data(), slots.size() * sizeof(int), hipMemcpyHostToDevice, st));
-  HIPCHK(hipMemcpyAsync(B.d_coef, coef.data(), coef.size() * sizeof(double2), hipMemcpyHostToDevice, st));
+  h2d_pinned(B, 5, B.d_slots, slots.data(), slots.size() * sizeof(int), st);
+  h2d_pinned(B, 6, B.d_coef, coef.data(), coef.size() * sizeof(double2), st);
   {
     Timer t(timing, st, "diis_axpy");
     launch_diis_axpy(B.d_descs, n, B.maxT, B.d_slots, B.d_coef, ne, st);
@@ -1567,8 +1598,7 @@ std::vector<std::complex<double>> energies(Batch &B, bool timing) {
   check_launch();
   std::vector<double2> &sc = B.h_scal;
   sc.assign((size_t)n * 64, double2{0, 0});
-  HIPCHK(hipMemcpyAsync(sc.data(), B.d_scal, sc.size() * sizeof(double2), hipMemcpyDeviceToHost, B.stream));
-  HIPCHK(hipStreamSynchronize(B.stream));
+  d2h_pinned_sync(B, 3, sc.data(), B.d_scal, sc.size() * sizeof(double2), B.stream);
   std::vector<std::complex<double>> e(n);
   for (int s = 0; s < n; ++s) e[s] = std::complex<double>(sc[(size_t)s * 64].x, sc[(size_t)s * 64].y);
   return e;
@@ -1652,8 +1682,7 @@ std::vector<std::complex<double>> anderson_step(Batch &B, int depth, double beta
     B.h_aaslots[(size_t)s * 16 + 14] = fresh;
     B.h_aaslots[(size_t)s * 16 + 15] = m;
   }
-  HIPCHK(hipMemcpyAsync(B.d_aaslots, B.h_aaslots.data(), B.h_aaslots.size() * sizeof(int), hipMemcpyHostToDevice,
-                        st));
+  h2d_pinned(B, 0, B.d_aaslots, B.h_aaslots.data(), B.h_aaslots.size() * sizeof(int), st);
   {
     Timer t(timing, st, "anderson_update");
     launch_aa_update(B.d_descs, n, B.d_aaslots, beta, st);
@@ -1661,8 +1690,7 @@ std::vector<std::complex<double>> anderson_step(Batch &B, int depth, double beta
   check_launch();
   std::vector<double2> &sc = B.h_scal;
   sc.assign((size_t)n * 64, double2{0, 0});
-  HIPCHK(hipMemcpyAsync(sc.data(), B.d_scal, sc.size() * sizeof(double2), hipMemcpyDeviceToHost, st));
-  HIPCHK(hipStreamSynchronize(st));
+  d2h_pinned_sync(B, 3, sc.data(), B.d_scal, sc.size() * sizeof(double2), st);
   std::vector<std::complex<double>> e(n, 0.0);
   B.h_aacoef.assign((size_t)n * 16, 0.0);
   for (int s = 0; s < n; ++s) {
@@ -1710,10 +1738,8 @@ std::vector<std::complex<double>> anderson_step(Batch &B, int depth, double beta
     }
     B.h_aaslots[(size_t)s * 16 + 15] = mm;
   }
-  HIPCHK(hipMemcpyAsync(B.d_aaslots, B.h_aaslots.data(), B.h_aaslots.size() * sizeof(int), hipMemcpyHostToDevice,
-                        st));
-  HIPCHK(hipMemcpyAsync(B.d_aacoef, B.h_aacoef.data(), B.h_aacoef.size() * sizeof(double), hipMemcpyHostToDevice,
-                        st));
+  h2d_pinned(B, 1, B.d_aaslots, B.h_aaslots.data(), B.h_aaslots.size() * sizeof(int), st);
+  h2d_pinned(B, 2, B.d_aacoef, B.h_aacoef.data(), B.h_aacoef.size() * sizeof(double), st);
   return e;
 }
 
